@@ -1,0 +1,257 @@
+/*
+ * mycel_hip.h -- C ABI of libmycel_hip.so, the MI355X (gfx950) implementation of Mycel.Core's
+ * matrix-free Hex8 elasticity path.
+ *
+ * Each entry point replaces one managed member of Mycel.Core; the reference interface is cited
+ * beside it (paths relative to the reference repository root). A P/Invoke shim re-implements the
+ * managed class bodies on top of these calls.
+ *
+ * Conventions
+ *   - Every function returns an int32 status (MYCEL_OK = 0). The codes map to the exceptions the
+ *     reference throws; mycel_last_error() returns a thread-local message for the last failure.
+ *   - Host arrays are borrowed for the duration of the call only; nothing is retained.
+ *   - Vectors over DOFs are box-sized, interleaved [ux,uy,uz] per node, node index
+ *     n = i + (nx+1)(j + (ny+1)k), DOF = 3n + c (Voxelization/GridSpec.cs:109-153).
+ *     Design-variable arrays (rho, rho_bar, dc, s) are active-compact in ascending element order.
+ *   - Functions whose name ends in _d take device pointers (HBM-resident data) and a hipStream_t
+ *     passed as void* (NULL = the handle's own stream); they return when the work is enqueued
+ *     unless stated otherwise.
+ *   - One thread per handle at a time; distinct handles are independent, each on its own stream.
+ *     PCG breakdowns are not errors: they return converged = 0 with a message code, as PcgResult.
+ */
+#ifndef MYCEL_HIP_H
+#define MYCEL_HIP_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define MYCEL_API __attribute__((visibility("default")))
+#else
+#define MYCEL_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+#define MYCEL_OK 0
+#define MYCEL_E_ARGUMENT_NULL 1     /* ArgumentNullException */
+#define MYCEL_E_ARGUMENT 2          /* ArgumentException (length mismatch, bad warm start) */
+#define MYCEL_E_OUT_OF_RANGE 3      /* ArgumentOutOfRangeException (fixed DOF, h <= 0) */
+#define MYCEL_E_INVALID_OPERATION 4 /* InvalidOperationException (<6 fixed DOFs, empty patch) */
+#define MYCEL_E_NOT_CONVERGED 5     /* InvalidOperationException thrown on PCG non-convergence */
+#define MYCEL_E_CANCELED 6          /* OperationCanceledException */
+#define MYCEL_E_DEVICE 7            /* HIP runtime failure */
+#define MYCEL_E_OUT_OF_MEMORY 8     /* device allocation or HBM budget */
+
+/* PcgResult.Message codes (Fem/PcgSolver.cs:139-368). */
+#define MYCEL_PCG_CONVERGED 0
+#define MYCEL_PCG_RHS_ZERO 1
+#define MYCEL_PCG_INITIAL_CONVERGED 2
+#define MYCEL_PCG_CURVATURE_BREAKDOWN 3
+#define MYCEL_PCG_STEP_BREAKDOWN 4
+#define MYCEL_PCG_STAGNATION 5
+#define MYCEL_PCG_DIRECTION_BREAKDOWN 6
+#define MYCEL_PCG_MAX_ITERATIONS 7
+#define MYCEL_PCG_CANCELED 8
+
+/* Precision of the inner Krylov solve. FP64 is the reference's arithmetic throughout. The mixed
+ * modes run an FP32 (or BF16-operand) matrix-core K*u inside FP64 defect correction: the outer
+ * residual r = b - K u, its norm and the solution update stay FP64. */
+#define MYCEL_PRECISION_FP64 0
+#define MYCEL_PRECISION_MIXED_FP32 1
+#define MYCEL_PRECISION_MIXED_BF16 2
+
+typedef struct mycel_op mycel_op;
+typedef struct mycel_filter mycel_filter;
+
+/* GridSpec (Voxelization/GridSpec.cs:11-44). h is the float32 voxel pitch widened to double. */
+typedef struct {
+    int32_t nx, ny, nz;
+    double h;
+} mycel_grid;
+
+/* FeModel counters (Fem/FeModel.cs:21-29). */
+typedef struct {
+    int32_t total_elements, total_nodes, total_dofs;
+    int32_t active_elements, active_nodes, active_fixed_dofs;
+} mycel_op_info;
+
+/* PcgResult (Fem/PcgResult.cs:7-14); message is a MYCEL_PCG_* code. */
+typedef struct {
+    int32_t iterations;
+    double final_residual;
+    int32_t converged;
+    int32_t message;
+    double elapsed_ms;
+    int32_t outer_corrections; /* defect-correction sweeps (mixed precision only) */
+} mycel_pcg_result;
+
+/* IProgress<PcgProgress> (Fem/PcgResult.cs:5). Called on the calling thread between iterations. */
+typedef void (*mycel_pcg_progress_cb)(int32_t iteration, double relative_residual, void *user);
+
+/* ---- library ------------------------------------------------------------------------------ */
+MYCEL_API const char *mycel_last_error(void);
+MYCEL_API int32_t mycel_version(void);
+/* Number of visible HIP devices; selects device `ordinal` for handles created afterwards. */
+MYCEL_API int32_t mycel_device_count(int32_t *count);
+MYCEL_API int32_t mycel_set_device(int32_t ordinal);
+
+/* ---- Hex8 element (Fem/Hex8Element.cs) ---------------------------------------------------- */
+/* Hex8Element.CreateKe(h, nu) (Fem/Hex8Element.cs:98-178): 24x24 row-major, E = 1. */
+MYCEL_API int32_t mycel_hex8_ke(double h, double nu, double *ke576);
+/* Hex8Element.CreateBc(h) (Fem/Hex8Element.cs:220-241): 6x24 row-major. */
+MYCEL_API int32_t mycel_hex8_bc(double h, double *bc144);
+
+/* ---- FeModel helpers (Fem/FeModel.cs) ----------------------------------------------------- */
+/* FeModel.AssembleForces (Fem/FeModel.cs:263-321). Load case c owns nodes/areas
+ * [case_offsets[c], case_offsets[c+1]); force_xyz holds 3 float32 per case (System.Numerics.Vector3). */
+MYCEL_API int32_t mycel_assemble_forces(int32_t total_nodes, int32_t n_cases, const int32_t *case_offsets,
+                              const int32_t *nodes, const double *areas, const float *force_xyz,
+                              double *f_out);
+/* PassiveRegion.Build(grid, null, nodes, radius) (Voxelization/PassiveRegion.cs:13-62). */
+MYCEL_API int32_t mycel_passive_region(const mycel_grid *grid, const int32_t *nodes, int32_t n_nodes,
+                             int32_t radius, uint8_t *mask_box);
+
+/* ---- FemOperator (Fem/FemOperator.cs) ----------------------------------------------------- */
+/* new FeModel(grid, domain, FeMaterial(e0, nu, emin_ratio), fixedDofs, elementDensity, penal)
+ * followed by new FemOperator(model) (Fem/FeModel.cs:43-184, Fem/FemOperator.cs:33-108).
+ * density_box may be NULL (all 1.0, FeModel.cs:74-78). */
+MYCEL_API int32_t mycel_op_create(const mycel_grid *grid, const uint8_t *domain, const int32_t *fixed_dofs,
+                        int32_t n_fixed, double e0, double nu, double emin_ratio, double penal,
+                        const double *density_box, mycel_op **out);
+MYCEL_API int32_t mycel_op_destroy(mycel_op *op);
+MYCEL_API int32_t mycel_op_info_get(const mycel_op *op, mycel_op_info *info);
+/* FemOperator.UpdateDensities(densities, penal) (Fem/FemOperator.cs:114-138): box-sized densities. */
+MYCEL_API int32_t mycel_op_update_densities(mycel_op *op, const double *density_box, double penal);
+MYCEL_API int32_t mycel_op_update_densities_d(mycel_op *op, const double *d_density_box, double penal, void *stream);
+/* FemOperator.ApplyK(x, y) (Fem/FemOperator.cs:196-243). Synchronous. */
+MYCEL_API int32_t mycel_op_apply_k(mycel_op *op, const double *x, double *y);
+/* FemOperator.ApplyA(x, y, xIsAlreadyProjected) (Fem/FemOperator.cs:250-294). Synchronous. */
+MYCEL_API int32_t mycel_op_apply_a(mycel_op *op, const double *x, double *y, int32_t x_is_projected);
+/* Device-pointer forms of ApplyK / ApplyA. */
+MYCEL_API int32_t mycel_op_apply_k_d(mycel_op *op, const double *d_x, double *d_y, void *stream);
+MYCEL_API int32_t mycel_op_apply_a_d(mycel_op *op, const double *d_x, double *d_y, int32_t x_is_projected, void *stream);
+/* FemOperator.DiagA / Ke / ConstrainedDofs / ActiveElements properties. */
+MYCEL_API int32_t mycel_op_get_diag_a(mycel_op *op, double *diag);
+MYCEL_API int32_t mycel_op_get_ke(mycel_op *op, double *ke576);
+MYCEL_API int32_t mycel_op_get_constrained(mycel_op *op, uint8_t *mask_dofs);
+MYCEL_API int32_t mycel_op_get_active_elements(mycel_op *op, int32_t *active);
+/* Stream the handle enqueues on (hipStream_t as void*). */
+MYCEL_API void *mycel_op_stream(mycel_op *op);
+
+/* ---- PcgSolver (Fem/PcgSolver.cs) + JacobiPreconditioner (Fem/JacobiPreconditioner.cs) ---- */
+/* PcgSolver.Solve(op, b, x, new JacobiPreconditioner(), tol, maxIterations, progress, ct)
+ * (Fem/PcgSolver.cs:80-370). x is the warm start on entry and the solution on return.
+ * cancel: optional host flag polled between device batches (nonzero = cancel -> MYCEL_E_CANCELED).
+ * Returns MYCEL_E_INVALID_OPERATION when fewer than 6 fixed DOFs sit on active nodes. */
+MYCEL_API int32_t mycel_pcg_solve(mycel_op *op, const double *b, double *x, double tol, int32_t max_iter,
+                        int32_t precision, const volatile int32_t *cancel,
+                        mycel_pcg_progress_cb progress, void *user, mycel_pcg_result *result);
+/* Device-pointer form; synchronous on the handle's stream. */
+MYCEL_API int32_t mycel_pcg_solve_d(mycel_op *op, const double *d_b, double *d_x, double tol, int32_t max_iter,
+                          int32_t precision, mycel_pcg_result *result);
+
+/* ---- StaticAnalysis (Fem/StaticAnalysis.cs) ----------------------------------------------- */
+/* StaticAnalysis.Run(model, options) (Fem/StaticAnalysis.cs:35-142). f: assembled forces (box DOFs),
+ * u: warm start in (zeros for a cold start), displacements out. reactions (box DOFs) and
+ * von_mises (box elements) may be NULL. throw_on_nonconvergence -> MYCEL_E_NOT_CONVERGED. */
+MYCEL_API int32_t mycel_static_analysis(mycel_op *op, const double *f, double *u, double tol, int32_t max_iter,
+                              int32_t throw_on_nonconvergence, int32_t precision, double *compliance,
+                              double *max_displacement, double *reactions, double *von_mises,
+                              mycel_pcg_result *result);
+/* StaticAnalysis.ComputeElementVonMises (Fem/StaticAnalysis.cs:144-216). */
+MYCEL_API int32_t mycel_von_mises(mycel_op *op, const double *u, double *von_mises_box);
+/* StaticAnalysis.AverageElementToNode (Fem/StaticAnalysis.cs:222-272). */
+MYCEL_API int32_t mycel_average_element_to_node(mycel_op *op, const double *element_box, double *nodal);
+
+/* ---- DensityFilter (Optimization/DensityFilter.cs) ---------------------------------------- */
+/* new DensityFilter(grid, domain, activeElements, rmin) (Optimization/DensityFilter.cs:34-199). */
+MYCEL_API int32_t mycel_filter_create(const mycel_grid *grid, const uint8_t *domain, const int32_t *active,
+                            int32_t n_active, double rmin, mycel_filter **out);
+MYCEL_API int32_t mycel_filter_destroy(mycel_filter *f);
+/* DensityFilter.S (Optimization/DensityFilter.cs:20,187-198). */
+MYCEL_API int32_t mycel_filter_get_s(mycel_filter *f, double *s);
+MYCEL_API int32_t mycel_filter_nnz(mycel_filter *f, int64_t *nnz);
+/* ApplyFilter (:205-251) and ApplySensitivityFilter (:257-303). */
+MYCEL_API int32_t mycel_filter_apply(mycel_filter *f, const double *in, double *out);
+MYCEL_API int32_t mycel_filter_apply_adjoint(mycel_filter *f, const double *in, double *out);
+
+/* ---- SensitivityAnalysis (Optimization/SensitivityAnalysis.cs:13-146) --------------------- */
+MYCEL_API int32_t mycel_sensitivities(mycel_op *op, const double *u, const double *rho_phys_active,
+                            double *dc_active, double penal);
+
+/* ---- OcOptimizer.Update (Optimization/OcOptimizer.cs:12-175) ------------------------------ */
+/* passive may be NULL. rho_new may alias rho. lambda_out receives the selected multiplier. */
+MYCEL_API int32_t mycel_oc_update(const double *rho, const double *dc, const double *s, const uint8_t *passive,
+                        double *rho_new, int32_t n, double volume_fraction, double move_limit,
+                        double rho_min, double lambda_guess, int32_t max_bisect, double volume_tol,
+                        double *lambda_out);
+
+/* ---- TopologyOptimizer (Optimization/TopologyOptimizer.cs) -------------------------------- */
+/* OptimizerOptions (Optimization/OptimizerOptions.cs:5-26): the fields the SIMP loop reads.
+ * The material's Emin comes from the operator (FeMaterial.EminRatio), as in the reference. */
+typedef struct {
+    double volume_fraction;     /* 0.4 */
+    double filter_radius_mm;    /* 2.0 */
+    double penal;               /* 3.0 */
+    double move_limit;          /* 0.2 */
+    int32_t max_iterations;     /* 200 */
+    double solver_tol_initial;  /* 1e-4 */
+    double solver_tol_final;    /* 1e-6 */
+    int32_t solver_max_iterations; /* 5000 */
+    double rho_min;             /* 1e-3 */
+    int32_t protect_boundary_hinges; /* 1 */
+    int32_t start_iteration;    /* 1 */
+    int32_t enforce_min_filter_radius; /* 1 */
+    int32_t continuation;       /* 0 */
+    int32_t precision;          /* MYCEL_PRECISION_* for the linear solves */
+} mycel_simp_options;
+
+/* Fills the reference defaults (OptimizerOptions.cs:5-26). */
+MYCEL_API void mycel_simp_options_default(mycel_simp_options *o);
+
+/* IterationSnapshot + IterationTimings (Optimization/IterationSnapshot.cs:5-31), FP64 scalars. */
+typedef struct {
+    int32_t iteration;
+    double compliance;
+    double volume_fraction;
+    double max_change;
+    double grey_level;
+    int32_t solver_iterations;
+    double solver_residual;
+    double max_displacement;
+    double lambda;
+    double solve_ms, sensitivity_ms, filter_ms, update_ms, total_ms;
+} mycel_snapshot;
+
+/* progress.Report(snapshot) (TopologyOptimizer.cs:372). rho_bar: active-compact FP64, valid during
+ * the call only. Return nonzero to stop after this iteration (cooperative cancel). */
+typedef int32_t (*mycel_snapshot_cb)(const mycel_snapshot *snap, const double *rho_bar, void *user);
+
+typedef struct {
+    int32_t converged;
+    int32_t total_iterations;
+    double final_compliance;
+    double final_volume_fraction;
+    int32_t canceled;
+} mycel_simp_result;
+
+/* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407), Heaviside off.
+ * f: assembled forces; passive_mask_box: optional OptimizerOptions.PassiveMask; patch_nodes: the
+ * load-patch nodes (for ProtectBoundaryHinges); initial_density: optional float32, box- or
+ * active-sized; warm_u: optional box DOFs. rho_out / rho_bar_out (active) and u_out (box DOFs)
+ * may be NULL. Returns MYCEL_E_NOT_CONVERGED when a PCG solve fails (the reference throws). */
+MYCEL_API int32_t mycel_simp_run(mycel_op *op, const double *f, const mycel_simp_options *opt,
+                       const uint8_t *passive_mask_box, const int32_t *patch_nodes, int32_t n_patch_nodes,
+                       const float *initial_density, int32_t initial_density_len, const double *warm_u,
+                       mycel_snapshot_cb cb, void *user, mycel_simp_result *result,
+                       mycel_snapshot *snapshots, int32_t snapshots_capacity,
+                       double *rho_out, double *rho_bar_out, double *u_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,66 @@
+// Device-side helpers shared by the kernels: deterministic reductions, XCD-aware block mapping,
+// SIMP stiffness interpolation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mycel {
+
+// Hex8 local corner coordinates (Fem/Hex8Element.cs:8-16).
+__host__ __device__ constexpr int corner_x(int a) { return (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : 0; }
+__host__ __device__ constexpr int corner_y(int a) { return (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : 0; }
+__host__ __device__ constexpr int corner_z(int a) { return a >= 4 ? 1 : 0; }
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with a private L2.
+// Remap so that each XCD receives a contiguous range of logical tiles: neighbouring node tiles then
+// share halo lines in the same L2. Bijective on [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G) {
+    const int x = b & 7, q = G >> 3, r = G & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Butterfly sum over a 64-lane wavefront. lane l adds v[l] and v[l^off]; both partners compute the
+// same commutative sum, so every lane ends with the bit-identical total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Fixed-order sum over a 256-thread block (4 wavefronts). Deterministic for a fixed launch shape.
+__device__ __forceinline__ double block_sum256(double v, double *lds4) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    const double t = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+    __syncthreads();
+    return t;
+}
+
+// Every block that calls this computes the same value from the same partials in the same order.
+__device__ __forceinline__ double sum_partials256(const double *__restrict__ part, int n, double *lds4) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) v += part[i];
+    return block_sum256(v, lds4);
+}
+
+// rho^3 rounded once to nearest (up to double-rounding ties): rho^2 is split exactly into hi + lo
+// with an FMA, then (hi + lo) * rho is formed with one more error-free product.
+__device__ __forceinline__ double cube_rn(double r) {
+    const double hi = r * r;
+    const double lo = __fma_rn(r, r, -hi);
+    const double p1 = hi * r;
+    const double e1 = __fma_rn(hi, r, -p1);
+    return p1 + __fma_rn(lo, r, e1);
+}
+
+// rho^p as Math.Pow would give it for the penalties in use (p = 1, 2, 3 exactly; pow otherwise).
+__device__ __forceinline__ double pow_penal(double r, double p) {
+    if (p == 3.0) return cube_rn(r);
+    if (p == 2.0) return r * r;
+    if (p == 1.0) return r;
+    return pow(r, p);
+}
+
+}  // namespace mycel

@@ -1,0 +1,441 @@
+// FemOperator on gfx950: SIMP stiffness update, Jacobi diagonal, node-centric matrix-free K*u,
+// element post-processing (von Mises, sensitivities) and deterministic FP64 reductions.
+//
+// Layout: vectors over DOFs are the reference's box-sized interleaved [ux,uy,uz] per node.
+// E_e lives on a zero-padded (nx+2)(ny+2)(nz+2) element grid: 0 for inactive elements and for
+// the halo, so a node's 8 incident elements are always addressable and inactive ones add nothing
+// (Fem/FeModel.cs:103-107: inactive elements are simply absent).
+//
+// K*u is a gather: one thread owns one node and forms its three output rows from the 8 incident
+// elements, y_n = sum_a E_e(a) * Ke[3a:3a+3, :] * u_e(a). Each element row is computed exactly once
+// (by the node that owns it), so there is no redundant arithmetic, no colouring and no atomics;
+// the 8-colour lattice of the reference (Fem/FemOperator.cs:214-222) exists only to avoid write
+// races on a CPU. Ke is read with wave-uniform addresses and lands in scalar registers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "device_common.hpp"
+#include "mycel_internal.hpp"
+
+namespace mycel {
+
+// ------------------------------------------------------------------------------------------------
+// Hex8 element, host side (Fem/Hex8Element.cs:48-251). Computed once per handle.
+// ------------------------------------------------------------------------------------------------
+namespace {
+const double kXi[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+const double kEta[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+const double kZeta[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
+// Strain-displacement matrix at (xi, eta, zeta), Voigt order xx,yy,zz,yz,zx,xy (Hex8Element.cs:183-215).
+void strain_displacement(double xi, double eta, double zeta, double h, double *b) {
+    std::fill(b, b + 144, 0.0);
+    const double s = 0.25 * (1.0 / h);
+    for (int a = 0; a < 8; a++) {
+        const double gx = s * kXi[a] * (1.0 + kEta[a] * eta) * (1.0 + kZeta[a] * zeta);
+        const double gy = s * kEta[a] * (1.0 + kXi[a] * xi) * (1.0 + kZeta[a] * zeta);
+        const double gz = s * kZeta[a] * (1.0 + kXi[a] * xi) * (1.0 + kEta[a] * eta);
+        double *c = b + 3 * a;
+        c[0 * 24 + 0] = gx;
+        c[1 * 24 + 1] = gy;
+        c[2 * 24 + 2] = gz;
+        c[3 * 24 + 1] = gz;
+        c[3 * 24 + 2] = gy;
+        c[4 * 24 + 0] = gz;
+        c[4 * 24 + 2] = gx;
+        c[5 * 24 + 0] = gy;
+        c[5 * 24 + 1] = gx;
+    }
+}
+}  // namespace
+
+void constitutive(double e, double nu, double *D) {
+    const double f = e / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double c1 = f * (1.0 - nu), c2 = f * nu, g = e / (2.0 * (1.0 + nu));
+    std::fill(D, D + 36, 0.0);
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) D[r * 6 + c] = (r == c) ? c1 : c2;
+    D[21] = D[28] = D[35] = g;
+}
+
+// Ke = h * sum_q B^T D B / 8 over the 2x2x2 Gauss points +-1/sqrt(3) (Hex8Element.cs:98-178).
+void hex8_ke(double h, double nu, double *ke) {
+    const double gp[2] = {-1.0 / std::sqrt(3.0), 1.0 / std::sqrt(3.0)};
+    double D[36], B[144], DB[144];
+    constitutive(1.0, nu, D);
+    std::fill(ke, ke + 576, 0.0);
+    for (int ix = 0; ix < 2; ix++)
+        for (int iy = 0; iy < 2; iy++)
+            for (int iz = 0; iz < 2; iz++) {
+                strain_displacement(gp[ix], gp[iy], gp[iz], 1.0, B);
+                for (int r = 0; r < 6; r++)
+                    for (int c = 0; c < 24; c++) {
+                        double s = 0.0;
+                        for (int k = 0; k < 6; k++) s += D[r * 6 + k] * B[k * 24 + c];
+                        DB[r * 24 + c] = s;
+                    }
+                for (int i = 0; i < 24; i++)
+                    for (int j = 0; j < 24; j++) {
+                        double s = 0.0;
+                        for (int k = 0; k < 6; k++) s += B[k * 24 + i] * DB[k * 24 + j];
+                        ke[i * 24 + j] += s * (1.0 / 8.0);
+                    }
+            }
+    // The reference scales by h on its nu = 0.3 fast path always, and otherwise only when h != 1;
+    // both give quad * h.
+    if (std::fabs(nu - 0.3) < 1e-15 || std::fabs(h - 1.0) > 1e-15)
+        for (int i = 0; i < 576; i++) ke[i] *= h;
+}
+
+void hex8_bc(double h, double *bc) {
+    double base[144];
+    strain_displacement(0.0, 0.0, 0.0, 1.0, base);
+    const double inv = 1.0 / h;
+    for (int i = 0; i < 144; i++) bc[i] = base[i] * inv;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+// E_e = Emin + rho^p (E0 - Emin) on the padded grid (FemOperator.cs:114-138).
+__global__ __launch_bounds__(256) void k_update_e(Dims g, const double *__restrict__ density,
+                                                  const uint8_t *__restrict__ domain, double e0, double emin,
+                                                  double penal, double *__restrict__ ep, float *__restrict__ ep32) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= g.n_pad) return;
+    const int ip = p % g.px, t = p / g.px, jp = t % g.py, kp = t / g.py;
+    double e = 0.0;
+    if (ip >= 1 && ip <= g.nx && jp >= 1 && jp <= g.ny && kp >= 1 && kp <= g.nz) {
+        const int el = (ip - 1) + g.nx * ((jp - 1) + g.ny * (kp - 1));
+        if (domain[el]) e = emin + pow_penal(density[el], penal) * (e0 - emin);
+    }
+    ep[p] = e;
+    ep32[p] = static_cast<float>(e);
+}
+
+// DiagA in colour order (FemOperator.cs:143-172): DOF (n, c) receives E_e * DiagKe[3a+c] from the
+// element of colour 0..7 that contains node n as its corner a. Constrained DOFs get 1.
+__global__ __launch_bounds__(256) void k_diag(Dims g, const double *__restrict__ ke, const double *__restrict__ ep,
+                                              const uint8_t *__restrict__ flags, double *__restrict__ diag,
+                                              double *__restrict__ inv_diag, float *__restrict__ inv_diag32) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.n_nodes) return;
+    const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+    const int corner_of[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // (ax + 2ay + 4az) -> Hex8 corner index
+    const double *dke = ke + 576;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 8; c++) {
+        const int ax = (i ^ c) & 1, ay = (j ^ (c >> 1)) & 1, az = (k ^ (c >> 2)) & 1;
+        const double e = ep[(i - ax + 1) + g.px * ((j - ay + 1) + g.py * (k - az + 1))];
+        const int a = corner_of[ax + 2 * ay + 4 * az];
+        if (e != 0.0)
+            for (int d = 0; d < 3; d++) acc[d] += e * dke[3 * a + d];
+    }
+    for (int d = 0; d < 3; d++) {
+        const int dof = 3 * n + d;
+        const double v = (flags[dof] & kFlagConstrained) ? 1.0 : acc[d];
+        diag[dof] = v;
+        const double inv = fabs(v) > 1e-30 ? 1.0 / v : 1.0;  // JacobiPreconditioner.cs:25
+        inv_diag[dof] = inv;
+        inv_diag32[dof] = static_cast<float>(inv);
+    }
+}
+
+// Node-centric K*u, FP64. MODE: kApplyK (y = Kx), kApplyAProjected (y = Kx, y[fixed] = x[fixed]),
+// kApplyAGeneral (y = K(Px), y[constrained] = x). DOT adds a per-block partial of x.y.
+template <int MODE, bool DOT>
+__global__ __launch_bounds__(256, 2) void k_apply_k_f64(Dims g, const double *__restrict__ ke,
+                                                        const double *__restrict__ ep, const double *__restrict__ x,
+                                                        const uint8_t *__restrict__ flags, double *__restrict__ y,
+                                                        double *__restrict__ part, const int *__restrict__ stop) {
+    __shared__ double red[4];
+    if (DOT && stop && *stop) return;  // PCG already terminated: the rest of the batch is a no-op
+    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    double dotv = 0.0;
+    if (n < g.n_nodes) {
+        const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+        const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
+        double E[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) E[a] = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        const bool any = (E[0] != 0.0) | (E[1] != 0.0) | (E[2] != 0.0) | (E[3] != 0.0) | (E[4] != 0.0) |
+                         (E[5] != 0.0) | (E[6] != 0.0) | (E[7] != 0.0);
+        if (any) {
+            // 3x3x3 node neighbourhood; clamped coordinates only ever feed halo elements (E = 0).
+            const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
+            const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
+            const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
+            double u[27][3];
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++) {
+                    const int row = g.nx1 * (ys[dy] + g.ny1 * zs[dz]);
+#pragma unroll
+                    for (int dx = 0; dx < 3; dx++) {
+                        const int m = 3 * (row + xs[dx]);
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            double v = x[m + d];
+                            if (MODE == kApplyAGeneral && (flags[m + d] & kFlagConstrained)) v = 0.0;
+                            u[dx + 3 * dy + 9 * dz][d] = v;
+                        }
+                    }
+                }
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                if (E[a] != 0.0) {
+                    const double *kr = ke + 72 * a;  // rows 3a .. 3a+2
+                    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 8; b++) {
+                        const int s = (corner_x(b) - corner_x(a) + 1) + 3 * (corner_y(b) - corner_y(a) + 1) +
+                                      9 * (corner_z(b) - corner_z(a) + 1);
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            const double v = u[s][d];
+                            t0 = __fma_rn(kr[3 * b + d], v, t0);
+                            t1 = __fma_rn(kr[24 + 3 * b + d], v, t1);
+                            t2 = __fma_rn(kr[48 + 3 * b + d], v, t2);
+                        }
+                    }
+                    acc0 = __fma_rn(E[a], t0, acc0);
+                    acc1 = __fma_rn(E[a], t1, acc1);
+                    acc2 = __fma_rn(E[a], t2, acc2);
+                }
+            }
+        }
+        double out[3] = {acc0, acc1, acc2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int dof = 3 * n + d;
+            if (MODE == kApplyAProjected) {
+                if (flags[dof] & kFlagFixed) out[d] = x[dof];
+            } else if (MODE == kApplyAGeneral) {
+                if (flags[dof] & kFlagConstrained) out[d] = x[dof];
+            }
+            y[dof] = out[d];
+            if (DOT) dotv += x[dof] * out[d];
+        }
+    }
+    if (DOT) {
+        const double s = block_sum256(dotv, red);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
+// Per-block partial of sum a_i b_i (optionally only over unconstrained i).
+template <bool MASKED>
+__global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ a, const double *__restrict__ b,
+                                                     int n, const uint8_t *__restrict__ flags,
+                                                     double *__restrict__ part) {
+    __shared__ double red[4];
+    double v = 0.0;
+    const int base = blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int idx = base + 256 * q;
+        if (idx < n && (!MASKED || !(flags[idx] & kFlagConstrained))) v += a[idx] * b[idx];
+    }
+    const double s = block_sum256(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ part, int n,
+                                                      double *__restrict__ out) {
+    __shared__ double red[4];
+    const double s = sum_partials256(part, n, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// Element von Mises at the centroid (StaticAnalysis.cs:144-216). cst: Bc(1) [144] then D(1, nu) [36].
+__global__ __launch_bounds__(256) void k_von_mises(Dims g, const int32_t *__restrict__ active, int na,
+                                                   const double *__restrict__ cst, const double *__restrict__ ep,
+                                                   const double *__restrict__ u, double inv_h,
+                                                   double *__restrict__ vm) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= na) return;
+    const int e = active[q];
+    const int ix = e % g.nx, t = e / g.nx, iy = t % g.ny, iz = t / g.ny;
+    const int n0 = ix + g.nx1 * (iy + g.ny1 * iz);
+    double ue[24];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const int nd = n0 + corner_x(a) + g.nx1 * (corner_y(a) + g.ny1 * corner_z(a));
+#pragma unroll
+        for (int d = 0; d < 3; d++) ue[3 * a + d] = u[3 * nd + d];
+    }
+    const double ee = ep[(ix + 1) + g.px * ((iy + 1) + g.py * (iz + 1))];
+    double eps[6], sig[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 24; c++) s += cst[r * 24 + c] * ue[c];
+        eps[r] = s * inv_h;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; c++) s += cst[144 + r * 6 + c] * eps[c];
+        sig[r] = s * ee;
+    }
+    const double d1 = sig[0] - sig[1], d2 = sig[1] - sig[2], d3 = sig[2] - sig[0];
+    vm[e] = sqrt(0.5 * (d1 * d1 + d2 * d2 + d3 * d3) + 3.0 * (sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5]));
+}
+
+// dc_e = -p rho^(p-1) (E0 - Emin) u_e^T Ke u_e, in the reference's summation order
+// (SensitivityAnalysis.cs:80-95); no FMA contraction so the per-element value matches it.
+__global__ __launch_bounds__(256) void k_sensitivities(Dims g, const int32_t *__restrict__ active, int na,
+                                                       const double *__restrict__ ke, const double *__restrict__ u,
+                                                       const double *__restrict__ rho, double penal, double de,
+                                                       double *__restrict__ dc) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= na) return;
+    const int e = active[q];
+    const int ix = e % g.nx, t = e / g.nx, iy = t % g.ny, iz = t / g.ny;
+    const int n0 = ix + g.nx1 * (iy + g.ny1 * iz);
+    double ue[24];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const int nd = n0 + corner_x(a) + g.nx1 * (corner_y(a) + g.ny1 * corner_z(a));
+#pragma unroll
+        for (int d = 0; d < 3; d++) ue[3 * a + d] = u[3 * nd + d];
+    }
+    double se = 0.0;
+#pragma unroll
+    for (int r = 0; r < 24; r++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 24; c++) s += ke[r * 24 + c] * ue[c];
+        se += ue[r] * s;
+    }
+    const double pm1 = penal - 1.0;
+    const double pw = pm1 == 2.0 ? rho[q] * rho[q] : (pm1 == 1.0 ? rho[q] : (pm1 == 0.0 ? 1.0 : pow(rho[q], pm1)));
+    const double dedr = penal * pw * de;
+    dc[q] = -dedr * se;
+}
+
+// StaticAnalysis.AverageElementToNode (StaticAnalysis.cs:222-272): per node, active incident
+// elements summed in ascending element order.
+__global__ __launch_bounds__(256) void k_avg_e2n(Dims g, const double *__restrict__ ep,
+                                                 const double *__restrict__ elem, double *__restrict__ nodal) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.n_nodes) return;
+    const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+    double s = 0.0;
+    int cnt = 0;
+    for (int oz = 1; oz >= 0; oz--)
+        for (int oy = 1; oy >= 0; oy--)
+            for (int ox = 1; ox >= 0; ox--) {
+                const int ex = i - ox, ey = j - oy, ez = k - oz;
+                if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+                if (ep[(ex + 1) + g.px * ((ey + 1) + g.py * (ez + 1))] == 0.0) continue;
+                s += elem[ex + g.nx * (ey + g.ny * ez)];
+                cnt++;
+            }
+    nodal[n] = cnt > 0 ? s / cnt : 0.0;
+}
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------------------------------------
+void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStream_t s) {
+    static_assert(sizeof(uint8_t) == 1, "");
+    op->penal = penal;
+    hipLaunchKernelGGL(k_update_e, dim3(cdiv(op->d.n_pad, 256)), dim3(256), 0, s, op->d, d_density_box,
+                       op->d_domain.get(), op->e0, op->emin, penal,
+                       op->d_ep.get(), op->d_ep32.get());
+    MYCEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_diag, dim3(cdiv(op->d.n_nodes, 256)), dim3(256), 0, s, op->d, op->d_ke.get(),
+                       op->d_ep.get(), op->d_flags.get(), op->d_diag.get(), op->d_inv_diag.get(),
+                       op->d_inv_diag32.get());
+    MYCEL_LAUNCH_CHECK();
+}
+
+int apply_k_blocks(const mycel_op *op) { return cdiv(op->d.n_nodes, 256); }
+
+void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s, double *d_dot_partial,
+                    const int *d_stop) {
+    const dim3 grid(apply_k_blocks(op)), block(256);
+    const double *ke = op->d_ke.get(), *ep = op->d_ep.get();
+    const uint8_t *fl = op->d_flags.get();
+    if (d_dot_partial) {
+        if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
+        hipLaunchKernelGGL((k_apply_k_f64<kApplyAProjected, true>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
+                           d_dot_partial, d_stop);
+    } else if (mode == kApplyK) {
+        hipLaunchKernelGGL((k_apply_k_f64<kApplyK, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, nullptr, nullptr);
+    } else if (mode == kApplyAProjected) {
+        hipLaunchKernelGGL((k_apply_k_f64<kApplyAProjected, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
+                           nullptr, nullptr);
+    } else {
+        hipLaunchKernelGGL((k_apply_k_f64<kApplyAGeneral, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
+                           nullptr, nullptr);
+    }
+    MYCEL_LAUNCH_CHECK();
+}
+
+void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s) {
+    const int na = static_cast<int>(op->active.size());
+    MYCEL_HIP(hipMemsetAsync(d_vm, 0, sizeof(double) * op->d.n_elem, s));
+    if (!na) return;
+    hipLaunchKernelGGL(k_von_mises, dim3(cdiv(na, 256)), dim3(256), 0, s, op->d, op->d_active.get(), na,
+                       op->d_ke.get() + 600, op->d_ep.get(), d_u, 1.0 / op->h, d_vm);
+    MYCEL_LAUNCH_CHECK();
+}
+
+void launch_sensitivities(mycel_op *op, const double *d_u, const double *d_rho, double *d_dc, double penal,
+                          hipStream_t s) {
+    const int na = static_cast<int>(op->active.size());
+    if (!na) return;
+    hipLaunchKernelGGL(k_sensitivities, dim3(cdiv(na, 256)), dim3(256), 0, s, op->d, op->d_active.get(), na,
+                       op->d_ke.get(), d_u, d_rho, penal, op->e0 - op->emin, d_dc);
+    MYCEL_LAUNCH_CHECK();
+}
+
+void launch_avg_e2n(mycel_op *op, const double *d_elem, double *d_nodal, hipStream_t s) {
+    hipLaunchKernelGGL(k_avg_e2n, dim3(cdiv(op->d.n_nodes, 256)), dim3(256), 0, s, op->d, op->d_ep.get(), d_elem,
+                       d_nodal);
+    MYCEL_LAUNCH_CHECK();
+}
+
+static double reduce_impl(mycel_op *op, const double *a, const double *b, int n, const uint8_t *flags,
+                          hipStream_t s) {
+    const int nb = cdiv(n, 1024);
+    if (op->d_part.size() < static_cast<size_t>(nb) + 1) op->d_part.resize(nb + 1);
+    if (flags)
+        hipLaunchKernelGGL(k_dot_partial<true>, dim3(nb), dim3(256), 0, s, a, b, n, flags, op->d_part.get());
+    else
+        hipLaunchKernelGGL(k_dot_partial<false>, dim3(nb), dim3(256), 0, s, a, b, n, flags, op->d_part.get());
+    MYCEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, op->d_part.get(), nb, op->d_part.get() + nb);
+    MYCEL_LAUNCH_CHECK();
+    double out = 0.0;
+    MYCEL_HIP(hipMemcpyAsync(&out, op->d_part.get() + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+    MYCEL_HIP(hipStreamSynchronize(s));
+    return out;
+}
+
+double reduce_dot(mycel_op *op, const double *a, const double *b, int n, hipStream_t s) {
+    return reduce_impl(op, a, b, n, nullptr, s);
+}
+
+double reduce_dot_masked(mycel_op *op, const double *a, const double *b, int n, const uint8_t *flags,
+                         hipStream_t s) {
+    return reduce_impl(op, a, b, n, flags, s);
+}
+
+}  // namespace mycel

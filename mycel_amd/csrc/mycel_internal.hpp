@@ -1,0 +1,186 @@
+// Internal declarations shared by the translation units of libmycel_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mycel_hip.h"
+
+namespace mycel {
+
+// ------------------------------------------------------------------------------------------------
+// Errors: internal code throws mycel::Error; every C entry point converts it into a status code and
+// a thread-local message (mycel_last_error).
+// ------------------------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int32_t status;
+    Error(int32_t s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define MYCEL_HIP(call)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw ::mycel::Error(e_ == hipErrorOutOfMemory ? MYCEL_E_OUT_OF_MEMORY : MYCEL_E_DEVICE, \
+                                 std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+#define MYCEL_LAUNCH_CHECK() MYCEL_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------------------------------------
+// Device memory owned by a handle. No copies, moves only.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+class DevBuf {
+   public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { resize(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void resize(size_t n) {
+        if (n == n_) return;
+        release();
+        if (n) MYCEL_HIP(hipMalloc(&p_, n * sizeof(T)));
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T *get() const { return p_; }
+    size_t size() const { return n_; }
+    void upload(const T *h, size_t n, hipStream_t s, size_t off = 0) {
+        MYCEL_HIP(hipMemcpyAsync(p_ + off, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *h, size_t n, hipStream_t s, size_t off = 0) const {
+        MYCEL_HIP(hipMemcpyAsync(h, p_ + off, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+
+   private:
+    T *p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// ------------------------------------------------------------------------------------------------
+// Structured grid as the kernels see it. Element data live on a padded (nx+2)(ny+2)(nz+2) grid
+// with a zero halo so that the node-centric gather never branches on the box boundary.
+// ------------------------------------------------------------------------------------------------
+struct Dims {
+    int nx, ny, nz;
+    int nx1, ny1, nz1;  // node counts per axis
+    int n_nodes, n_dofs, n_elem;
+    int px, py, pz;     // padded element grid
+    int n_pad;          // px*py*pz
+};
+
+inline Dims make_dims(int nx, int ny, int nz) {
+    Dims d;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.nx1 = nx + 1; d.ny1 = ny + 1; d.nz1 = nz + 1;
+    d.n_nodes = d.nx1 * d.ny1 * d.nz1;
+    d.n_dofs = 3 * d.n_nodes;
+    d.n_elem = nx * ny * nz;
+    d.px = nx + 2; d.py = ny + 2; d.pz = nz + 2;
+    d.n_pad = d.px * d.py * d.pz;
+    return d;
+}
+
+// DOF flag bits stored per DOF on the device.
+enum : uint8_t { kFlagConstrained = 1, kFlagFixed = 2 };
+
+// Apply modes of the K*u kernels (FemOperator.ApplyK / ApplyA).
+enum ApplyMode : int { kApplyK = 0, kApplyAProjected = 1, kApplyAGeneral = 2 };
+
+// Reduction partial-sum geometry: one FP64 partial per block, summed in a fixed order.
+constexpr int kBlock = 256;
+
+// Host-side Hex8 element (Fem/Hex8Element.cs).
+void hex8_ke(double h, double nu, double *ke576);
+void hex8_bc(double h, double *bc144);
+void constitutive(double e, double nu, double *d36);
+
+struct PcgWork;
+
+}  // namespace mycel
+
+// ------------------------------------------------------------------------------------------------
+// The operator handle: FeModel + FemOperator state, device-resident.
+// ------------------------------------------------------------------------------------------------
+struct mycel_op {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mycel::Dims d{};
+    double h = 1, e0 = 1, nu = 0.3, emin = 0, penal = 3;
+
+    // Host model description (FeModel): kept for the counters and the getters.
+    std::vector<uint8_t> domain;         // box elements
+    std::vector<int32_t> active;         // ascending box element indices
+    std::vector<uint8_t> active_nodes;   // box nodes
+    std::vector<uint8_t> dof_flags;      // box DOFs (kFlag*)
+    std::vector<int32_t> fixed;          // FixedDofs as given
+    int active_fixed = 0;
+    int n_active_nodes = 0;
+    double ke[576];
+    double diag_ke[24];
+
+    // Device state.
+    mycel::DevBuf<double> d_ke;        // [0,576) Ke, [576,600) DiagKe, [600,744) Bc(1), [744,780) D(1,nu)
+    mycel::DevBuf<float> d_ke32;       // FP32 Ke
+    mycel::DevBuf<double> d_ep;        // padded E_e (0 for inactive / halo)
+    mycel::DevBuf<float> d_ep32;       // FP32 copy of the same
+    mycel::DevBuf<uint8_t> d_flags;    // per DOF
+    mycel::DevBuf<uint8_t> d_domain;   // per box element (active mask)
+    mycel::DevBuf<int32_t> d_active;   // active list
+    mycel::DevBuf<double> d_density;   // box densities (ElementDensity)
+    mycel::DevBuf<double> d_diag;      // DiagA
+    mycel::DevBuf<double> d_inv_diag;  // Jacobi 1/DiagA
+    mycel::DevBuf<float> d_inv_diag32;
+    mycel::DevBuf<double> d_x, d_y;    // staging for the host-pointer entry points
+    mycel::DevBuf<double> d_part;      // block partials for one-off reductions
+    mycel::PcgWork *pcg = nullptr;
+};
+
+namespace mycel {
+
+// ---- fem.hip ---------------------------------------------------------------------------------
+void op_upload_model(mycel_op *op);
+void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStream_t s);
+void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s,
+                    double *d_dot_partial = nullptr, const int *d_stop = nullptr);
+int apply_k_blocks(const mycel_op *op);
+void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, int precision, hipStream_t s,
+                        double *d_dot_partial = nullptr);
+void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s);
+void launch_sensitivities(mycel_op *op, const double *d_u, const double *d_rho, double *d_dc, double penal,
+                          hipStream_t s);
+void launch_avg_e2n(mycel_op *op, const double *d_elem, double *d_nodal, hipStream_t s);
+
+// Deterministic FP64 reductions (fixed block partition, fixed tree) on the op's geometry.
+double reduce_dot(mycel_op *op, const double *d_a, const double *d_b, int n, hipStream_t s);
+double reduce_dot_masked(mycel_op *op, const double *d_a, const double *d_b, int n, const uint8_t *d_flags,
+                         hipStream_t s);
+
+// ---- pcg.hip ---------------------------------------------------------------------------------
+struct PcgOutcome {
+    mycel_pcg_result r;
+};
+int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
+                         const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
+                         mycel_pcg_result *res);
+void pcg_free(PcgWork *w);
+
+}  // namespace mycel

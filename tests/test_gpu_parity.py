@@ -1,0 +1,255 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle on identical seeded inputs.
+
+Bars: integer/index work and the kernels that keep the reference's evaluation order (filter, S,
+sensitivities, OC densities, DiagA) are compared bit-for-bit or to 1e-15; the K*u gather uses FMAs and
+a per-node corner order instead of colour order, so it is compared at 1e-12 relative (north-star bar
+1e-10); solutions are compared against the oracle solved to the same tolerance and against the
+reference's own known answers (V3 deflections, V10 iteration counts) and identities (V4, V5).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import oracle_forces, oracle_model, random_field, rel
+from mycel_amd import (DensityFilter, FemOperator, InvalidOperationError, JacobiPreconditioner, OcOptimizer,
+                       OperationCanceledError, OptimizerOptions, PcgSolver, SensitivityAnalysis, StaticAnalysis,
+                       StaticAnalysisOptions, TopologyOptimizer)
+from mycel_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    tiny = W.Problem(5, 4, 3, domain=W.l_shape_domain(5, 4, 3), emin_ratio=1e-3)
+    tiny.fixed_dofs = W.fix_all(W.face_nodes_x(tiny, 0))
+    tiny.density = random_field(3, tiny.n_elem, 0.01, 1.0)
+    c1 = W.cantilever_simp()
+    c1.density = random_field(4, c1.n_elem, 0.01, 1.0)
+    c2 = W.matvec_pcg_workstation()
+    return [tiny, c1, c2]
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2], ids=["lshape-5x4x3", "cantilever-60x20x20", "holed-80x40x40"])
+def test_apply_k_and_a_parity(gpu, idx):
+    p = _problems()[idx]
+    print(f"[parity] apply_k {p.name or 'tiny'} dofs={p.n_dofs}", flush=True)
+    m = oracle_model(p)
+    op = FemOperator(p.fe_model())
+    try:
+        x = random_field(10 + idx, p.n_dofs, -1, 1)
+        y = op.ApplyK(x)
+        assert rel(y, m.apply_k(x)) < 1e-12
+        assert np.array_equal(y, op.ApplyK(x))  # deterministic: no atomics, fixed order
+        ya = op.ApplyA(x)
+        assert rel(ya, m.apply_a(x, projected=False)) < 1e-12
+        xp = np.where(m.constrained, 0.0, x)
+        assert rel(op.ApplyA(xp, xIsAlreadyProjected=True), m.apply_a(xp, projected=True)) < 1e-12
+        assert rel(op.DiagA, m.diag_a) < 1e-15
+    finally:
+        op.close()
+
+
+def test_update_densities_parity(gpu):
+    p = W.cantilever_simp(20, 8, 6)
+    m = oracle_model(p)
+    op = FemOperator(p.fe_model())
+    try:
+        for seed, pen in ((1, 3.0), (2, 1.0), (3, 2.0), (4, 3.5)):
+            d = random_field(seed, p.n_elem, 0.001, 1.0)
+            m.update_densities(d, pen)
+            op.UpdateDensities(d, pen)
+            x = random_field(seed + 50, p.n_dofs, -1, 1)
+            assert rel(op.ApplyK(x), m.apply_k(x)) < 1e-12
+            assert rel(op.DiagA, m.diag_a) < 1e-14
+    finally:
+        op.close()
+
+
+def test_pcg_parity_random_density(gpu):
+    p = W.cantilever_simp(30, 10, 10)
+    p.density = random_field(21, p.n_elem, 0.01, 1.0)
+    m = oracle_model(p)
+    b = np.where(m.constrained, 0.0, oracle_forces(p))
+    u_ref, r_ref = m.pcg(b, tol=1e-10, max_iter=20000)
+    op = FemOperator(p.fe_model())
+    try:
+        u = np.zeros(p.n_dofs)
+        r = PcgSolver().Solve(op, b, u, JacobiPreconditioner(), tol=1e-10, maxIterations=20000)
+        print(f"[parity] pcg iters gpu={r.Iterations} oracle={r_ref.iterations}", flush=True)
+        assert r.Converged and r_ref.converged
+        assert abs(r.Iterations - r_ref.iterations) <= max(2, r_ref.iterations // 100)
+        assert rel(u, u_ref) < 1e-10
+        f = oracle_forces(p)
+        assert abs(f @ u - f @ u_ref) / abs(f @ u_ref) < 1e-10
+    finally:
+        op.close()
+
+
+def test_v10_iteration_counts_on_gpu(gpu):
+    """The reference's published Jacobi-PCG counts (docs/STEP3_BENCH.md:25): 436 to 1e-4, 481 to 1e-6."""
+    p = W.benchmark_cantilever(50_000)
+    fm = p.fe_model()
+    for tol, known in ((1e-4, 436), (1e-6, 481)):
+        r = StaticAnalysis.Run(fm, StaticAnalysisOptions(Tolerance=tol))
+        print(f"[parity] V10 tol={tol} iters={r.SolverInfo.Iterations} (reference {known})", flush=True)
+        assert abs(r.SolverInfo.Iterations - known) <= 1
+
+
+def test_v3_v4_v5_on_gpu(gpu):
+    known = {1: 86.3730, 2: 116.7719, 4: 128.6479, 8: 132.1529}
+    for nz in (1, 2, 4, 8):
+        p, tip, areas = W.bar_model(100.0, 10.0, 10.0, 10.0 / nz, 3000.0, 0.3)
+        p.loads = [(tip, areas, np.array([0, 0, -1000], np.float32))]
+        r = StaticAnalysis.Run(p.fe_model(), StaticAnalysisOptions(Tolerance=1e-8, MaxIterations=10000))
+        delta = (np.abs(r.U[3 * tip + 2]) * areas).sum() / areas.sum()
+        assert round(delta, 4) == known[nz]
+    p, tip, areas = W.bar_model(100.0, 10.0, 10.0, 2.5, 3000.0, 0.3)
+    p.loads = [(tip, areas, np.array([0, 0, -1000], np.float32))]
+    fm = p.fe_model()
+    r = StaticAnalysis.Run(fm, StaticAnalysisOptions(Tolerance=1e-10))
+    f = fm.AssembleForces()
+    R = r.Reactions.reshape(-1, 3).sum(0)
+    assert np.abs(R + f.reshape(-1, 3).sum(0)).max() / 1000 < 1e-8  # V4 sum R = -sum F
+    op = FemOperator(fm)
+    try:
+        utku = r.U @ op.ApplyK(r.U)
+    finally:
+        op.close()
+    assert abs(r.Compliance - utku) / abs(r.Compliance) < 1e-8  # V5 F^T u = u^T K u
+
+
+def test_static_analysis_parity(gpu):
+    p = W.matvec_pcg_workstation(seed=8, nx=30, ny=14, nz=14)
+    m = oracle_model(p)
+    f = oracle_forces(p)
+    ref = m.static_analysis(f, tol=1e-10, max_iter=20000)
+    r = StaticAnalysis.Run(p.fe_model(), StaticAnalysisOptions(Tolerance=1e-10, MaxIterations=20000))
+    assert rel(r.U, ref["u"]) < 1e-10
+    assert abs(r.Compliance - ref["compliance"]) / abs(ref["compliance"]) < 1e-10
+    assert abs(r.MaxDisplacement - ref["max_displacement"]) / ref["max_displacement"] < 1e-10
+    assert rel(r.Reactions, ref["reactions"]) < 1e-9
+    assert rel(r.VonMisesPerElement, ref["von_mises"]) < 1e-9
+
+
+def test_solver_special_cases(gpu):
+    p, tip, areas = W.bar_model(20.0, 5.0, 5.0, 2.5, 3000.0, 0.3)
+    fm = p.fe_model()
+    op = FemOperator(fm)
+    try:
+        x = np.zeros(p.n_dofs)
+        r = PcgSolver().Solve(op, np.zeros(p.n_dofs), x, tol=1e-8)  # V8 zero load
+        assert r.Converged and r.Iterations == 0 and r.MessageCode == 1
+        f = np.zeros(p.n_dofs)
+        f[3 * tip + 2] = -1.0
+        b = np.where(fm.ConstrainedDofs, 0.0, f)
+        u = np.zeros(p.n_dofs)
+        PcgSolver().Solve(op, b, u, tol=1e-12)
+        r2 = PcgSolver().Solve(op, b, u.copy(), tol=1e-8)  # V13 exact warm start
+        assert r2.Iterations == 0 and r2.MessageCode == 2
+        r3 = PcgSolver().Solve(op, b, np.zeros(p.n_dofs), tol=1e-14, maxIterations=5)
+        assert not r3.Converged and r3.Iterations == 5 and r3.MessageCode == 7
+        flag = C.c_int32(1)
+        with pytest.raises(OperationCanceledError):
+            PcgSolver().Solve(op, b, np.zeros(p.n_dofs), tol=1e-12, cancel=flag)
+    finally:
+        op.close()
+    few = W.Problem(3, 2, 2)
+    few.fixed_dofs = np.array([0, 1, 2, 3, 4], np.int32)
+    op = FemOperator(few.fe_model())
+    try:
+        with pytest.raises(InvalidOperationError):
+            PcgSolver().Solve(op, np.ones(few.n_dofs), np.zeros(few.n_dofs))
+    finally:
+        op.close()
+
+
+@pytest.mark.parametrize("case", ["lshape-r2.5", "cantilever-r1.5", "box-r2.0"])
+def test_filter_bitwise(gpu, oracle, case):
+    if case == "lshape-r2.5":
+        p, r = W.Problem(8, 6, 5, domain=W.l_shape_domain(8, 6, 5)), 2.5
+    elif case == "cantilever-r1.5":
+        p, r = W.cantilever_simp(), 1.5
+    else:
+        p, r = W.matvec_pcg_workstation(nx=40, ny=20, nz=20), 2.0
+    act = p.active
+    ref = oracle.Filter(p.nx, p.ny, p.nz, p.domain, act, r)
+    from mycel_amd import GridSpec
+
+    f = DensityFilter(GridSpec(p.nx, p.ny, p.nz), p.domain, act, r)
+    try:
+        x = random_field(1, act.size)
+        assert np.array_equal(f.S, ref.s)
+        assert np.array_equal(f.ApplyFilter(x), ref.apply(x))
+        assert np.array_equal(f.ApplySensitivityFilter(x), ref.apply_adjoint(x))
+        assert f.NonZeros == ref.nnz
+    finally:
+        f.close()
+
+
+def test_sensitivities_parity(gpu):
+    p = W.cantilever_simp(20, 10, 8)
+    p.density = random_field(6, p.n_elem, 0.01, 1.0)
+    m = oracle_model(p)
+    fm = p.fe_model()
+    op = FemOperator(fm)
+    try:
+        u = random_field(7, p.n_dofs, -1, 1)
+        rho = p.density[fm.ActiveElements]
+        for pen in (3.0, 1.0, 2.5):
+            dc = SensitivityAnalysis.ComputeComplianceSensitivities(fm, op, u, rho, penal=pen)
+            assert rel(dc, m.sensitivities(u, rho, pen)) < 1e-15
+    finally:
+        op.close()
+
+
+def test_oc_parity_and_fuzz(gpu, oracle):
+    n = 5000
+    rho = random_field(1, n, 0.05, 0.95)
+    dc = -(10.0 ** (-3 + 6 * random_field(2, n)))
+    s = random_field(3, n, 0.5, 1.5)
+    passive = (random_field(4, n) < 0.05).astype(np.uint8)
+    for guess in (1.0, 1e-4, 1e4, float("nan")):
+        lam, new = OcOptimizer.Update(rho, dc, s, passive, None, 0.35, 0.2, 1e-3, guess, 60, 1e-8)
+        ref, lam_ref, _ = oracle.oc_update(rho, dc, s, passive, 0.35, 0.2, 1e-3, guess, 60, 1e-8)
+        assert abs(lam - lam_ref) <= 1e-10 * lam_ref
+        assert np.abs(new - ref).max() <= 1e-10
+    # T3 fuzz (OptimizationTests.cs:297-387) on the GPU.
+    rng = W.SplitMix64(98765)
+    n = 200
+    s = 0.5 + rng.uniform(n)
+    s *= n / s.sum()
+    passive = np.zeros(n, np.uint8)
+    passive[rng.integers(20, n)] = 1
+    rho = np.where(passive == 1, 1.0, 0.35)
+    for _ in range(100):
+        d = -(10.0 ** (-3.0 + 8.0 * rng.uniform(n)))
+        lam, new = OcOptimizer.Update(rho, d, s, passive, None, 0.35, 0.2, 1e-3, 1.0, 60, 1e-8)
+        assert np.isfinite(lam) and lam > 0 and np.isfinite(new).all()
+        assert (np.abs(new - rho) <= 0.2 + 1e-12).all() and (new[passive == 1] == 1).all()
+        assert abs((s * new).sum() / n - 0.35) < 1e-6
+        rho = new
+
+
+def test_simp_parity_small(gpu):
+    """The SIMP loop on the GPU against the oracle: compliance history, filtered and design densities."""
+    p = W.cantilever_simp(24, 8, 8)
+    m = oracle_model(p)
+    f = oracle_forces(p)
+    kw = dict(volume_fraction=0.3, filter_radius_mm=1.5, penal=3.0, move_limit=0.2, max_iterations=12,
+              solver_tol_initial=1e-8, solver_tol_final=1e-8, solver_max_iterations=20000,
+              protect_boundary_hinges=False)
+    ref = m.simp(f, **kw)
+    assert ref["status"] == 0
+    o = OptimizerOptions(VolumeFraction=0.3, FilterRadiusMm=1.5, Penal=3.0, MoveLimit=0.2, MaxIterations=12,
+                         SolverToleranceInitial=1e-8, SolverToleranceFinal=1e-8, SolverMaxIterations=20000,
+                         ProtectBoundaryHinges=False)
+    res = TopologyOptimizer.Run(p.fe_model(), o)
+    c = np.array([s.Compliance for s in res.Snapshots])
+    c_ref = np.array([s["compliance"] for s in ref["snapshots"]])
+    print(f"[parity] simp max rel compliance diff {np.max(np.abs(c - c_ref) / c_ref):.3e}", flush=True)
+    assert len(c) == len(c_ref)
+    assert np.max(np.abs(c - c_ref) / c_ref) < 1e-8
+    assert np.abs(res.FinalPhysical64 - ref["rho_bar"]).max() < 1e-8
+    assert np.abs(res.FinalDesign - ref["rho"]).max() < 1e-8
+    assert [s.SolverIterations for s in res.Snapshots][0] == ref["snapshots"][0]["solver_iterations"]
