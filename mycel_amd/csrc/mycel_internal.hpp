@@ -162,8 +162,8 @@ void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStr
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s,
                     double *d_dot_partial = nullptr, const int *d_stop = nullptr);
 int apply_k_blocks(const mycel_op *op);
-void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, int precision, hipStream_t s,
-                        double *d_dot_partial = nullptr);
+void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s,
+                        double *d_dot_partial = nullptr, const int *d_stop = nullptr);
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s);
 void launch_sensitivities(mycel_op *op, const double *d_u, const double *d_rho, double *d_dc, double penal,
                           hipStream_t s);

@@ -1,5 +1,5 @@
 // Device-resident Jacobi-preconditioned CG on the projected operator A = PKP + (I - P)
-// (Fem/PcgSolver.cs:80-553, Fem/JacobiPreconditioner.cs:15-35).
+// (Fem/PcgSolver.cs:80-553, Fem/JacobiPreconditioner.cs:15-35), in two precisions.
 //
 // One iteration is three kernels with no host involvement:
 //   1. ap = A p fused with per-block partials of p.ap                       (node-centric K*u)
@@ -8,10 +8,20 @@
 //   3. relres, convergence, stagnation, beta = rz' / rz (breakdown check); p = z + beta p
 // Every block of kernels 2 and 3 re-sums the previous kernel's partials in the same fixed order, so
 // all blocks agree bit-for-bit on alpha, beta and the stopping decision without a grid barrier.
-// The scalar state lives on the device in two slots indexed by iteration parity: iteration t reads
-// slot (t-1)&1 and writes slot t&1, so a batch of an even number of iterations is a fixed sequence
-// of launches that is captured once into a hipGraph and replayed. After termination every kernel
-// returns at entry, so the host polls the state once per batch, not once per iteration.
+// The scalar state lives on the device in two slots indexed by iteration parity: an iteration reads
+// slot s and writes slot s^1, so a batch of an even number of iterations is a fixed sequence of launches
+// that is captured once into a hipGraph and replayed. Once the solve halts every kernel returns at
+// entry, so the host polls the state once per batch, not once per iteration.
+//
+// FP64 (MYCEL_PRECISION_FP64): the reference's arithmetic; it stops on the recursive residual as
+// PcgSolver does.
+// Mixed (MYCEL_PRECISION_MIXED_FP32): the iteration runs on FP32 vectors and the FP32 operator, with
+// FP64 dot products and scalars. The solution is accumulated in FP64 by reliable updates (residual
+// replacement, van der Vorst & Ye 2000): whenever the recursive residual has fallen by a factor delta
+// since the last update, or claims convergence, the iteration pauses; the FP32 increment is folded
+// into the FP64 solution, the true residual r = P(b - A x) is formed with the FP64 operator, and the
+// iteration resumes from it with the same search direction. Convergence is declared on the true FP64
+// residual only.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -23,69 +33,81 @@
 namespace mycel {
 
 struct PcgState {
-    double rz[2];        // rho0 at the start of the iteration in this slot
+    double rz[2];        // rho0 at the start of the iteration that reads this slot
     double prev_rel[2];  // relres of the previous iteration (stagnation test, breakdown report)
     int stag[2];
     int iter[2];         // iteration number t that reads this slot
-    double tol, bnorm;
+    double tol, bnorm, delta, maxrel, beta;
     double bnorm_sq, r_norm_sq, rz0;
-    int done, message, iterations, nonzero_x;
-    double final_rel;
-    double last_rel;
+    int max_iter, mixed;
+    int halt, done, pause, pause_slot;
+    int message, iterations, nonzero_x, outer;
+    double final_rel, last_rel;
     int last_iter, pad;
 };
 
 struct PcgWork {
     int n = 0;
     int nb_mv = 0, nb_vec = 0;
-    DevBuf<double> r, z, p, ap;
+    DevBuf<double> r, z, p, ap;              // FP64 iteration vectors (r / ap double as r64 / A x64 when mixed)
+    DevBuf<float> r32, z32, p32, ap32, x32;  // FP32 iteration vectors of the mixed path
     DevBuf<double> part_pap, part2, part1;
     DevBuf<PcgState> state;
-    PcgState *h_state = nullptr;  // pinned
-    hipGraphExec_t graph = nullptr;
-    int graph_iters = 0;
-    const double *graph_x = nullptr;  // solution vector the graph was captured with
+    PcgState *h_state = nullptr;             // pinned
+    hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+    const double *graph64_x = nullptr;       // solution vector the FP64 graph was captured with
 };
 
 namespace {
 
 constexpr int kVecPerBlock = 1024;  // DOFs per block in the vector kernels (4 per thread)
+constexpr int kGraphIters = 32;     // iterations per captured batch (even: the slot parity repeats)
+constexpr double kReliableDelta = 0.1;
 
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __fma_rn(a, b, c); }
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+
+__device__ __forceinline__ void halt_with(PcgState *st, int message, int iterations, double rel) {
+    st->message = message;
+    st->iterations = iterations;
+    st->final_rel = rel;
+    st->done = 1;
+    st->halt = 1;
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState *__restrict__ st,
                                                        const double *__restrict__ part_pap, int nb_mv,
-                                                       double *__restrict__ x, double *__restrict__ r,
-                                                       const double *__restrict__ p, const double *__restrict__ ap,
-                                                       double *__restrict__ z, const double *__restrict__ inv,
-                                                       double *__restrict__ part2) {
+                                                       T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+                                                       const T *__restrict__ ap, T *__restrict__ z,
+                                                       const T *__restrict__ inv, double *__restrict__ part2) {
     __shared__ double red[4];
-    if (st->done) return;
+    if (st->halt) return;
     const double pap = sum_partials256(part_pap, nb_mv, red);
     const double rz0 = st->rz[slot];
     const bool curv = !(pap > 1e-30) || !isfinite(pap);  // PcgSolver.cs:245
     const double alpha = rz0 / pap;
     const bool step = !curv && (!isfinite(alpha) || alpha <= 0.0);  // :260
     if (curv || step) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            st->message = curv ? MYCEL_PCG_CURVATURE_BREAKDOWN : MYCEL_PCG_STEP_BREAKDOWN;
-            st->iterations = st->iter[slot];
-            st->final_rel = st->prev_rel[slot];
-            st->done = 1;
-        }
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            halt_with(st, curv ? MYCEL_PCG_CURVATURE_BREAKDOWN : MYCEL_PCG_STEP_BREAKDOWN, st->iter[slot],
+                      st->prev_rel[slot]);
         return;
     }
+    const T a = static_cast<T>(alpha);
     double rr = 0.0, rz = 0.0;
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
         if (i < n) {
-            x[i] = __fma_rn(alpha, p[i], x[i]);
-            const double ri = __fma_rn(-alpha, ap[i], r[i]);
+            x[i] = fmad(a, p[i], x[i]);
+            const T ri = fmad(-a, ap[i], r[i]);
             r[i] = ri;
-            const double zi = ri * inv[i];
+            const T zi = ri * inv[i];
             z[i] = zi;
-            rr = __fma_rn(ri, ri, rr);
-            rz = __fma_rn(ri, zi, rz);
+            rr = __fma_rn(static_cast<double>(ri), static_cast<double>(ri), rr);
+            rz = __fma_rn(static_cast<double>(ri), static_cast<double>(zi), rz);
         }
     }
     rr = block_sum256(rr, red);
@@ -102,50 +124,49 @@ __device__ __forceinline__ double sum_strided2(const double *__restrict__ part, 
     return block_sum256(v, lds4);
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState *__restrict__ st,
                                                       const double *__restrict__ part2, int nb_vec,
-                                                      double *__restrict__ p, const double *__restrict__ z) {
+                                                      T *__restrict__ p, const T *__restrict__ z) {
     __shared__ double red[4];
-    if (st->done) return;
+    if (st->halt) return;
     const double rr = sum_strided2(part2, nb_vec, 0, red);
     const double rzn = sum_strided2(part2, nb_vec, 1, red);
     const double rel = sqrt(rr) / st->bnorm;
     const int t = st->iter[slot];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (rel <= st->tol) {  // PcgSolver.cs:279
-        if (lead) {
-            st->message = MYCEL_PCG_CONVERGED;
-            st->iterations = t;
-            st->final_rel = rel;
-            st->done = 1;
+    if (st->mixed) {
+        // Reliable update: the true residual decides convergence; refresh it after every delta drop.
+        if (rel <= st->tol || rel < st->delta * st->maxrel) {
+            if (lead) {
+                st->pause = 1;
+                st->pause_slot = slot;
+                st->iterations = t;
+                st->last_rel = rel;
+                st->halt = 1;
+            }
+            return;
         }
+    } else if (rel <= st->tol) {  // PcgSolver.cs:279
+        if (lead) halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
         return;
     }
     const int stag = (fabs(rel - st->prev_rel[slot]) < 1e-15 * rel) ? st->stag[slot] + 1 : 0;  // :293-313
     if (stag > 50) {
-        if (lead) {
-            st->message = MYCEL_PCG_STAGNATION;
-            st->iterations = t;
-            st->final_rel = rel;
-            st->done = 1;
-        }
+        if (lead) halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
         return;
     }
     const double beta = rzn / st->rz[slot];
     if (!isfinite(beta) || beta < 0.0) {  // :334
-        if (lead) {
-            st->message = MYCEL_PCG_DIRECTION_BREAKDOWN;
-            st->iterations = t;
-            st->final_rel = rel;
-            st->done = 1;
-        }
+        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
         return;
     }
+    const T b = static_cast<T>(beta);
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
-        if (i < n) p[i] = __fma_rn(beta, p[i], z[i]);
+        if (i < n) p[i] = fmad(b, p[i], z[i]);
     }
     if (lead) {
         const int o = slot ^ 1;
@@ -155,6 +176,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
         st->iter[o] = t + 1;
         st->last_rel = rel;
         st->last_iter = t;
+        if (t >= st->max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);  // :359-368
     }
 }
 
@@ -203,10 +225,11 @@ __global__ __launch_bounds__(256) void k_pcg_init_r(int n, const uint8_t *__rest
     if (threadIdx.x == 0) part1[blockIdx.x] = v;
 }
 
-// z = r / diag; p = z; partials of r.z.
-__global__ __launch_bounds__(256) void k_pcg_init_z(int n, const double *__restrict__ r,
-                                                    const double *__restrict__ inv, double *__restrict__ z,
-                                                    double *__restrict__ p, double *__restrict__ part1) {
+// z = r / diag; p = z; partials of r.z. With T = float the FP64 residual is rounded into r32 first.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pcg_init_z(int n, const double *__restrict__ r64, T *__restrict__ r,
+                                                    const T *__restrict__ inv, T *__restrict__ z, T *__restrict__ p,
+                                                    double *__restrict__ part1) {
     __shared__ double red[4];
     double v = 0.0;
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
@@ -214,14 +237,78 @@ __global__ __launch_bounds__(256) void k_pcg_init_z(int n, const double *__restr
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
         if (i < n) {
-            const double zi = r[i] * inv[i];
+            const T ri = static_cast<T>(r64[i]);
+            if (r != reinterpret_cast<const T *>(r64)) r[i] = ri;
+            const T zi = ri * inv[i];
             z[i] = zi;
-            p[i] = zi;
-            v = __fma_rn(r[i], zi, v);
+            if (p) p[i] = zi;
+            v = __fma_rn(static_cast<double>(ri), static_cast<double>(zi), v);
         }
     }
     v = block_sum256(v, red);
     if (threadIdx.x == 0) part1[blockIdx.x] = v;
+}
+
+// Reliable update, part 1: x64 += x32; x32 = 0.
+__global__ __launch_bounds__(256) void k_mixed_flush(int n, double *__restrict__ x, float *__restrict__ x32) {
+    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = base + 256 * q;
+        if (i < n) {
+            x[i] += static_cast<double>(x32[i]);
+            x32[i] = 0.0f;
+        }
+    }
+}
+
+// Reliable update, part 3 (single block): true relres, convergence, beta with the refreshed r, and
+// the state for the next iteration written to slot 0 (the slot a fresh batch starts with).
+__global__ __launch_bounds__(256) void k_mixed_resume(PcgState *__restrict__ st, const double *__restrict__ part_rr,
+                                                      const double *__restrict__ part_rz, int nb) {
+    __shared__ double red[4];
+    const double rr = sum_partials256(part_rr, nb, red);
+    const double rzn = sum_partials256(part_rz, nb, red);
+    if (threadIdx.x != 0) return;
+    const double rel = sqrt(rr) / st->bnorm;
+    const int t = st->iterations;
+    st->outer += 1;
+    st->pause = 0;
+    if (rel <= st->tol) {
+        halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
+        return;
+    }
+    if (t >= st->max_iter) {
+        halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);
+        return;
+    }
+    const double beta = rzn / st->rz[st->pause_slot];
+    if (!isfinite(beta) || beta < 0.0) {
+        halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        return;
+    }
+    st->beta = beta;
+    st->rz[0] = rzn;
+    st->prev_rel[0] = rel;
+    st->stag[0] = 0;
+    st->iter[0] = t + 1;
+    st->maxrel = rel;
+    st->last_rel = rel;
+    st->last_iter = t;
+    st->halt = 0;
+}
+
+// Reliable update, part 4: p = z + beta p with the beta of k_mixed_resume.
+__global__ __launch_bounds__(256) void k_mixed_update_p(int n, const PcgState *__restrict__ st, float *__restrict__ p,
+                                                        const float *__restrict__ z) {
+    if (st->done) return;
+    const float b = static_cast<float>(st->beta);
+    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = base + 256 * q;
+        if (i < n) p[i] = __fmaf_rn(b, p[i], z[i]);
+    }
 }
 
 // Sums the partials into one state field (single block).
@@ -231,14 +318,22 @@ __global__ __launch_bounds__(256) void k_pcg_sum_to(const double *__restrict__ p
     if (threadIdx.x == 0) *out = s;
 }
 
-__global__ void k_pcg_start(PcgState *__restrict__ st, double tol, double bnorm, double rel) {
+__global__ void k_pcg_start(PcgState *__restrict__ st, double tol, double bnorm, double rel, int max_iter, int mixed,
+                            double delta) {
     st->tol = tol;
     st->bnorm = bnorm;
+    st->delta = delta;
+    st->maxrel = rel;
+    st->max_iter = max_iter;
+    st->mixed = mixed;
     st->rz[0] = st->rz0;
     st->prev_rel[0] = rel;
     st->stag[0] = 0;
     st->iter[0] = 1;
     st->done = 0;
+    st->halt = 0;
+    st->pause = 0;
+    st->outer = 0;
     st->last_rel = rel;
     st->last_iter = 0;
 }
@@ -258,19 +353,38 @@ void ensure_work(mycel_op *op) {
     w->ap.resize(w->n);
     w->part_pap.resize(w->nb_mv);
     w->part2.resize(2 * w->nb_vec);
-    w->part1.resize(w->nb_vec);
+    w->part1.resize(2 * w->nb_vec);
     w->state.resize(1);
     MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_state), sizeof(PcgState), hipHostMallocDefault));
 }
 
-void enqueue_iteration(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
+void ensure_mixed(PcgWork *w) {
+    if (w->x32.size()) return;
+    w->r32.resize(w->n);
+    w->z32.resize(w->n);
+    w->p32.resize(w->n);
+    w->ap32.resize(w->n);
+    w->x32.resize(w->n);
+}
+
+void enqueue_iteration64(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
     PcgState *st = w->state.get();
-    launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->done);
-    hipLaunchKernelGGL(k_pcg_update_xr, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+    launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
+    hipLaunchKernelGGL(k_pcg_update_xr<double>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                        w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->z.get(), op->d_inv_diag.get(),
                        w->part2.get());
-    hipLaunchKernelGGL(k_pcg_update_p, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(), w->nb_vec,
-                       w->p.get(), w->z.get());
+    hipLaunchKernelGGL(k_pcg_update_p<double>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(),
+                       w->nb_vec, w->p.get(), w->z.get());
+}
+
+void enqueue_iteration32(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
+    PcgState *st = w->state.get();
+    launch_apply_k_f32(op, w->p32.get(), w->ap32.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
+    hipLaunchKernelGGL(k_pcg_update_xr<float>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+                       w->nb_mv, w->x32.get(), w->r32.get(), w->p32.get(), w->ap32.get(), w->z32.get(),
+                       op->d_inv_diag32.get(), w->part2.get());
+    hipLaunchKernelGGL(k_pcg_update_p<float>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(),
+                       w->nb_vec, w->p32.get(), w->z32.get());
 }
 
 void read_state(PcgWork *w, hipStream_t s) {
@@ -278,24 +392,39 @@ void read_state(PcgWork *w, hipStream_t s) {
     MYCEL_HIP(hipStreamSynchronize(s));
 }
 
-constexpr int kGraphIters = 32;  // iterations per captured batch (even: slot parity repeats)
+template <typename F>
+hipGraphExec_t capture(hipStream_t s, F &&enqueue) {
+    hipGraph_t g;
+    hipGraphExec_t ex;
+    MYCEL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    MYCEL_HIP(hipStreamEndCapture(s, &g));
+    MYCEL_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    MYCEL_HIP(hipGraphDestroy(g));
+    return ex;
+}
 
 }  // namespace
 
 void pcg_free(PcgWork *w) {
     if (!w) return;
-    if (w->graph) (void)hipGraphExecDestroy(w->graph);
+    if (w->graph64) (void)hipGraphExecDestroy(w->graph64);
+    if (w->graph32) (void)hipGraphExecDestroy(w->graph32);
     if (w->h_state) (void)hipHostFree(w->h_state);
     delete w;
 }
 
-int32_t pcg_solve_fp64(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter,
-                       const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user, mycel_pcg_result *res) {
+int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
+                         const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
+                         mycel_pcg_result *res) {
     const auto t0 = std::chrono::steady_clock::now();
     auto elapsed = [&] {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
     *res = mycel_pcg_result{};
+    if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32)
+        throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
+    const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32;
     if (op->active_fixed < 6)  // PcgSolver.cs:95-100
         throw Error(MYCEL_E_INVALID_OPERATION,
                     "Model has fewer than 6 unique fixed DOFs on active nodes (found " +
@@ -303,22 +432,21 @@ int32_t pcg_solve_fp64(mycel_op *op, const double *d_b, double *d_x, double tol,
                         "). At least 6 fixed DOFs are required to prevent rigid body modes in 3D.");
     ensure_work(op);
     PcgWork *w = op->pcg;
+    if (mixed) ensure_mixed(w);
     hipStream_t s = op->stream;
     PcgState *st = w->state.get();
     const int n = w->n;
+    const dim3 gv(w->nb_vec), b256(256);
 
     MYCEL_HIP(hipMemsetAsync(st, 0, sizeof(PcgState), s));
-    hipLaunchKernelGGL(k_pcg_init, dim3(w->nb_vec), dim3(256), 0, s, n, op->d_flags.get(), d_b, d_x, st,
-                       w->part1.get());
-    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), dim3(256), 0, s, w->part1.get(), w->nb_vec, &st->bnorm_sq);
+    hipLaunchKernelGGL(k_pcg_init, gv, b256, 0, s, n, op->d_flags.get(), d_b, d_x, st, w->part1.get());
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->bnorm_sq);
     MYCEL_LAUNCH_CHECK();
     read_state(w, s);
     const double bnorm = std::sqrt(w->h_state->bnorm_sq);
     if (bnorm <= 1e-30) {  // PcgSolver.cs:139-151
         MYCEL_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, s));
         MYCEL_HIP(hipStreamSynchronize(s));
-        res->iterations = 0;
-        res->final_residual = 0.0;
         res->converged = 1;
         res->message = MYCEL_PCG_RHS_ZERO;
         res->elapsed_ms = elapsed();
@@ -326,92 +454,105 @@ int32_t pcg_solve_fp64(mycel_op *op, const double *d_b, double *d_x, double tol,
     }
     const bool warm = w->h_state->nonzero_x != 0;
     if (warm) launch_apply_k(op, d_x, w->ap.get(), kApplyAProjected, s);
-    hipLaunchKernelGGL(k_pcg_init_r, dim3(w->nb_vec), dim3(256), 0, s, n, op->d_flags.get(), d_b,
-                       warm ? w->ap.get() : nullptr, w->r.get(), w->part1.get());
-    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), dim3(256), 0, s, w->part1.get(), w->nb_vec, &st->r_norm_sq);
-    hipLaunchKernelGGL(k_pcg_init_z, dim3(w->nb_vec), dim3(256), 0, s, n, w->r.get(), op->d_inv_diag.get(),
-                       w->z.get(), w->p.get(), w->part1.get());
-    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), dim3(256), 0, s, w->part1.get(), w->nb_vec, &st->rz0);
+    hipLaunchKernelGGL(k_pcg_init_r, gv, b256, 0, s, n, op->d_flags.get(), d_b, warm ? w->ap.get() : nullptr,
+                       w->r.get(), w->part1.get());
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->r_norm_sq);
+    if (mixed) {
+        hipLaunchKernelGGL(k_pcg_init_z<float>, gv, b256, 0, s, n, w->r.get(), w->r32.get(), op->d_inv_diag32.get(),
+                           w->z32.get(), w->p32.get(), w->part1.get());
+        MYCEL_HIP(hipMemsetAsync(w->x32.get(), 0, sizeof(float) * n, s));
+    } else {
+        hipLaunchKernelGGL(k_pcg_init_z<double>, gv, b256, 0, s, n, w->r.get(), w->r.get(), op->d_inv_diag.get(),
+                           w->z.get(), w->p.get(), w->part1.get());
+    }
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->rz0);
     MYCEL_LAUNCH_CHECK();
     read_state(w, s);
     // A cold start uses ||Pb||^2 itself as ||r||^2 (PcgSolver.cs:185-192).
     const double rr0 = warm ? w->h_state->r_norm_sq : w->h_state->bnorm_sq;
     const double rel0 = std::sqrt(rr0) / bnorm;
     if (rel0 <= tol) {  // :197-209
-        res->iterations = 0;
         res->final_residual = rel0;
         res->converged = 1;
         res->message = MYCEL_PCG_INITIAL_CONVERGED;
         res->elapsed_ms = elapsed();
         return MYCEL_OK;
     }
-    hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1), 0, s, st, tol, bnorm, rel0);
+    if (max_iter <= 0) {  // the reference's loop body never runs (PcgSolver.cs:235,359-368)
+        res->final_residual = rel0;
+        res->message = MYCEL_PCG_MAX_ITERATIONS;
+        res->elapsed_ms = elapsed();
+        return MYCEL_OK;
+    }
+    hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1), 0, s, st, tol, bnorm, rel0, max_iter, mixed ? 1 : 0,
+                       kReliableDelta);
     MYCEL_LAUNCH_CHECK();
 
-    // The captured batch bakes in the solution pointer; recapture if the caller's x moved.
-    if (w->graph && w->graph_x != d_x) {
-        MYCEL_HIP(hipGraphExecDestroy(w->graph));
-        w->graph = nullptr;
-    }
-    if (!w->graph) {
-        hipGraph_t g;
-        MYCEL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int t = 0; t < kGraphIters; t++) enqueue_iteration(op, w, d_x, t & 1, s);
-        MYCEL_HIP(hipStreamEndCapture(s, &g));
-        MYCEL_HIP(hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0));
-        MYCEL_HIP(hipGraphDestroy(g));
-        w->graph_iters = kGraphIters;
-        w->graph_x = d_x;
+    hipGraphExec_t graph;
+    if (mixed) {
+        if (!w->graph32)
+            w->graph32 = capture(s, [&] {
+                for (int t = 0; t < kGraphIters; t++) enqueue_iteration32(op, w, t & 1, s);
+            });
+        graph = w->graph32;
+    } else {
+        // The captured FP64 batch bakes in the solution pointer; recapture if the caller's x moved.
+        if (w->graph64 && w->graph64_x != d_x) {
+            MYCEL_HIP(hipGraphExecDestroy(w->graph64));
+            w->graph64 = nullptr;
+        }
+        if (!w->graph64) {
+            w->graph64 = capture(s, [&] {
+                for (int t = 0; t < kGraphIters; t++) enqueue_iteration64(op, w, d_x, t & 1, s);
+            });
+            w->graph64_x = d_x;
+        }
+        graph = w->graph64;
     }
 
-    int launched = 0;
-    bool done = false;
-    while (launched < max_iter) {
+    // Every batch advances the iteration count by at least one or halts the solve, so max_iter bounds
+    // the number of batches (a guard against a logic error, not a limit).
+    const long long max_batches = static_cast<long long>(max_iter) + 64;
+    for (long long batch = 0;; batch++) {
+        if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "PCG batch loop did not terminate");
         if (cancel && *cancel) {
             MYCEL_HIP(hipStreamSynchronize(s));
-            res->message = MYCEL_PCG_CANCELED;
-            res->elapsed_ms = elapsed();
             throw Error(MYCEL_E_CANCELED, "The operation was canceled.");
         }
-        const int remaining = max_iter - launched;
-        if (remaining >= w->graph_iters) {
-            MYCEL_HIP(hipGraphLaunch(w->graph, s));
-            launched += w->graph_iters;
-        } else {
-            for (int t = 0; t < remaining; t++) enqueue_iteration(op, w, d_x, (launched + t) & 1, s);
-            MYCEL_LAUNCH_CHECK();
-            launched = max_iter;
-        }
+        MYCEL_HIP(hipGraphLaunch(graph, s));
         read_state(w, s);
-        if (cb) cb(w->h_state->done ? w->h_state->iterations : w->h_state->last_iter,
-                   w->h_state->done ? w->h_state->final_rel : w->h_state->last_rel, user);
-        if (w->h_state->done) {
-            done = true;
-            break;
+        const PcgState &h = *w->h_state;
+        if (cb) cb(h.done ? h.iterations : h.last_iter, h.done ? h.final_rel : h.last_rel, user);
+        if (h.done) break;
+        if (h.pause) {  // reliable update with the FP64 operator
+            hipLaunchKernelGGL(k_mixed_flush, gv, b256, 0, s, n, d_x, w->x32.get());
+            launch_apply_k(op, d_x, w->ap.get(), kApplyAProjected, s);
+            hipLaunchKernelGGL(k_pcg_init_r, gv, b256, 0, s, n, op->d_flags.get(), d_b, w->ap.get(), w->r.get(),
+                               w->part1.get());
+            hipLaunchKernelGGL(k_pcg_init_z<float>, gv, b256, 0, s, n, w->r.get(), w->r32.get(),
+                               op->d_inv_diag32.get(), w->z32.get(), static_cast<float *>(nullptr),
+                               w->part1.get() + w->nb_vec);
+            hipLaunchKernelGGL(k_mixed_resume, dim3(1), b256, 0, s, st, w->part1.get(), w->part1.get() + w->nb_vec,
+                               w->nb_vec);
+            hipLaunchKernelGGL(k_mixed_update_p, gv, b256, 0, s, n, st, w->p32.get(), w->z32.get());
+            MYCEL_LAUNCH_CHECK();
+            read_state(w, s);
+            if (w->h_state->done) break;
         }
+    }
+    if (mixed) {  // fold the last FP32 increment (zero after a converging update)
+        hipLaunchKernelGGL(k_mixed_flush, gv, b256, 0, s, n, d_x, w->x32.get());
+        MYCEL_LAUNCH_CHECK();
+        MYCEL_HIP(hipStreamSynchronize(s));
     }
     const PcgState &h = *w->h_state;
-    if (done) {
-        res->iterations = h.iterations;
-        res->final_residual = h.final_rel;
-        res->message = h.message;
-        res->converged = h.message == MYCEL_PCG_CONVERGED;
-    } else {  // PcgSolver.cs:359-368
-        res->iterations = max_iter;
-        res->final_residual = h.last_rel;
-        res->message = MYCEL_PCG_MAX_ITERATIONS;
-        res->converged = 0;
-    }
+    res->iterations = h.iterations;
+    res->final_residual = h.final_rel;
+    res->message = h.message;
+    res->converged = h.message == MYCEL_PCG_CONVERGED;
+    res->outer_corrections = h.outer;
     res->elapsed_ms = elapsed();
     return MYCEL_OK;
-}
-
-int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
-                         const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
-                         mycel_pcg_result *res) {
-    if (precision != MYCEL_PRECISION_FP64)
-        throw Error(MYCEL_E_ARGUMENT, "precision mode not available in this build");
-    return pcg_solve_fp64(op, d_b, d_x, tol, max_iter, cancel, cb, user, res);
 }
 
 }  // namespace mycel
