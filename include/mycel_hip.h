@@ -250,6 +250,67 @@ MYCEL_API int32_t mycel_simp_run(mycel_op *op, const double *f, const mycel_simp
                        mycel_snapshot *snapshots, int32_t snapshots_capacity,
                        double *rho_out, double *rho_bar_out, double *u_out);
 
+
+/* ---- Job server (continuous batching; no counterpart in the reference, whose only job machinery is
+ * one run at a time behind a RAM guard: Optimization/OptimizationCoordinator.cs:214-353,
+ * Diagnostics/MemoryGuard.cs:434-495) ------------------------------------------------------- */
+typedef struct mycel_server mycel_server;
+
+#define MYCEL_JOB_FEA 0  /* StaticAnalysis.Run */
+#define MYCEL_JOB_SIMP 1 /* TopologyOptimizer.Run */
+
+#define MYCEL_JOB_QUEUED 0
+#define MYCEL_JOB_RUNNING 1
+#define MYCEL_JOB_DONE 2
+#define MYCEL_JOB_FAILED 3
+
+/* One job. Arrays are copied at submit time. forces: assembled box DOFs; patch_nodes: the load-patch
+ * nodes (ProtectBoundaryHinges); density_box: optional (FEA). tol/max_iter/precision drive FEA jobs,
+ * simp drives SIMP jobs (its precision field selects the solver precision). */
+typedef struct {
+    int32_t kind;
+    mycel_grid grid;
+    const uint8_t *domain;
+    const int32_t *fixed_dofs;
+    int32_t n_fixed;
+    const double *forces;
+    const int32_t *patch_nodes;
+    int32_t n_patch_nodes;
+    double e0, nu, emin_ratio, penal;
+    const double *density_box;
+    double tol;
+    int32_t max_iter;
+    int32_t precision;
+    mycel_simp_options simp;
+} mycel_job_desc;
+
+typedef struct {
+    int32_t state;          /* MYCEL_JOB_* */
+    int32_t status;         /* MYCEL_OK or the failure status */
+    double compliance;
+    double max_displacement;
+    int32_t pcg_iterations; /* total over the job */
+    int32_t simp_iterations;
+    int32_t converged;
+    double footprint_bytes; /* HBM charged against the budget */
+    double queue_ms, run_ms, latency_ms;
+} mycel_job_info;
+
+/* hbm_budget_bytes <= 0 uses 95% of the device's free memory; workers = jobs resident at once
+ * (each on its own HIP stream). */
+MYCEL_API int32_t mycel_server_create(double hbm_budget_bytes, int32_t workers, mycel_server **out);
+/* Waits for every submitted job to finish, then frees the server. */
+MYCEL_API int32_t mycel_server_destroy(mycel_server *server);
+MYCEL_API int32_t mycel_server_submit(mycel_server *server, const mycel_job_desc *job, int64_t *job_id);
+MYCEL_API int32_t mycel_server_poll(mycel_server *server, int64_t job_id, mycel_job_info *info);
+/* timeout_s < 0 waits without limit. */
+MYCEL_API int32_t mycel_server_wait(mycel_server *server, int64_t job_id, double timeout_s, mycel_job_info *info);
+/* u_out: box DOFs; rho_bar_out: active elements (SIMP jobs). Either may be NULL. */
+MYCEL_API int32_t mycel_server_fetch(mycel_server *server, int64_t job_id, double *u_out, double *rho_bar_out);
+MYCEL_API int32_t mycel_server_release(mycel_server *server, int64_t job_id);
+MYCEL_API int32_t mycel_server_stats(mycel_server *server, double *budget_bytes, double *in_use_bytes,
+                                     int32_t *queued, int32_t *active);
+
 #ifdef __cplusplus
 }
 #endif

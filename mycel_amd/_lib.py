@@ -123,6 +123,21 @@ class SimpResultC(C.Structure):
                 ("final_volume_fraction", C.c_double), ("canceled", C.c_int32)]
 
 
+class JobDescC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("grid", Grid), ("domain", C.c_void_p), ("fixed_dofs", C.c_void_p),
+                ("n_fixed", C.c_int32), ("forces", C.c_void_p), ("patch_nodes", C.c_void_p),
+                ("n_patch_nodes", C.c_int32), ("e0", C.c_double), ("nu", C.c_double), ("emin_ratio", C.c_double),
+                ("penal", C.c_double), ("density_box", C.c_void_p), ("tol", C.c_double), ("max_iter", C.c_int32),
+                ("precision", C.c_int32), ("simp", SimpOptionsC)]
+
+
+class JobInfoC(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("compliance", C.c_double),
+                ("max_displacement", C.c_double), ("pcg_iterations", C.c_int32), ("simp_iterations", C.c_int32),
+                ("converged", C.c_int32), ("footprint_bytes", C.c_double), ("queue_ms", C.c_double),
+                ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
 PROGRESS_CB = C.CFUNCTYPE(None, C.c_int32, C.c_double, C.c_void_p)
 SNAPSHOT_CB = C.CFUNCTYPE(C.c_int32, C.POINTER(SnapshotC), C.POINTER(C.c_double), C.c_void_p)
 
@@ -169,6 +184,14 @@ SIGNATURES = {
     "mycel_sensitivities": (_i, [_vp, _vp, _vp, _vp, _d]),
     "mycel_oc_update": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _d, C.POINTER(_d)]),
     "mycel_simp_options_default": (None, [C.POINTER(SimpOptionsC)]),
+    "mycel_server_create": (_i, [_d, _i, C.POINTER(_vp)]),
+    "mycel_server_destroy": (_i, [_vp]),
+    "mycel_server_submit": (_i, [_vp, C.POINTER(JobDescC), C.POINTER(C.c_int64)]),
+    "mycel_server_poll": (_i, [_vp, C.c_int64, C.POINTER(JobInfoC)]),
+    "mycel_server_wait": (_i, [_vp, C.c_int64, _d, C.POINTER(JobInfoC)]),
+    "mycel_server_fetch": (_i, [_vp, C.c_int64, _vp, _vp]),
+    "mycel_server_release": (_i, [_vp, C.c_int64]),
+    "mycel_server_stats": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), C.POINTER(_i)]),
     "mycel_simp_run": (_i, [_vp, _vp, C.POINTER(SimpOptionsC), _vp, _vp, _i, _vp, _i, _vp, SNAPSHOT_CB, _vp,
                             C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
 }

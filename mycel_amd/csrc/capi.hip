@@ -69,7 +69,7 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 // ------------------------------------------------------------------------------------------------
 // FeModel + FemOperator construction (Fem/FeModel.cs:43-184, Fem/FemOperator.cs:33-108)
 // ------------------------------------------------------------------------------------------------
-static mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,
+mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,
                            double nu, double emin_ratio, double penal, const double *density) {
     require(g.nx > 0 && g.ny > 0 && g.nz > 0, MYCEL_E_OUT_OF_RANGE, "Grid dimensions must be strictly positive.");
     require(g.h > 0, MYCEL_E_OUT_OF_RANGE, "Element edge length h must be positive.");
@@ -159,7 +159,7 @@ static mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int
     return op.release();
 }
 
-static void op_destroy(mycel_op *op) {
+void op_destroy(mycel_op *op) {
     if (!op) return;
     (void)hipSetDevice(op->device);
     if (op->stream) (void)hipStreamSynchronize(op->stream);
@@ -172,7 +172,7 @@ static void op_destroy(mycel_op *op) {
 // ------------------------------------------------------------------------------------------------
 // SIMP driver (Optimization/TopologyOptimizer.cs:16-407), Heaviside off.
 // ------------------------------------------------------------------------------------------------
-static void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const uint8_t *passive_mask,
+void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const uint8_t *passive_mask,
                      const int32_t *patch_nodes, int n_patch, const float *init_density, int init_len,
                      const double *warm_u, mycel_snapshot_cb cb, void *user, mycel_simp_result *result,
                      mycel_snapshot *snaps, int snaps_cap, double *rho_out, double *rho_bar_out, double *u_out) {
@@ -333,6 +333,48 @@ static void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o,
     if (rho_out && na) d_rho.download(rho_out, na, s);
     if (rho_bar_out && na) d_rho_bar.download(rho_bar_out, na, s);
     if (u_out) d_u.download(u_out, d.n_dofs, s);
+    MYCEL_HIP(hipStreamSynchronize(s));
+}
+
+// StaticAnalysis.Run body (Fem/StaticAnalysis.cs:35-142): b = P F, PCG, C = F.u, R = (K u)|fixed,
+// max |u| over active nodes, element von Mises.
+void static_analysis_run(mycel_op *op, const double *f, double *u, double tol, int max_iter, bool throw_on_nonconv,
+                         int precision, double *compliance, double *max_disp, double *reactions, double *von_mises,
+                         mycel_pcg_result *result) {
+    if (op->active_fixed < 6)
+        throw Error(MYCEL_E_INVALID_OPERATION,
+                    "Model has fewer than 6 unique fixed DOFs on active nodes (found " +
+                        std::to_string(op->active_fixed) + ").");
+    hipStream_t s = op->stream;
+    const Dims &d = op->d;
+    DevBuf<double> d_f(d.n_dofs), d_b(d.n_dofs), d_u(d.n_dofs), d_part;
+    d_f.upload(f, d.n_dofs, s);
+    d_u.upload(u, d.n_dofs, s);
+    launch_project(d.n_dofs, op->d_flags.get(), d_f.get(), d_b.get(), s);
+    pcg_solve_device(op, d_b.get(), d_u.get(), tol, max_iter, precision, nullptr, nullptr, nullptr, result);
+    if (throw_on_nonconv && !result->converged)
+        throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " + std::to_string(result->iterations) +
+                                               " iterations. Is the model not sufficiently constrained?");
+    DevBuf<uint8_t> d_an(d.n_nodes);
+    d_an.upload(op->active_nodes.data(), d.n_nodes, s);
+    double cm[2];
+    compliance_maxdisp(d.n_nodes, d_an.get(), d_f.get(), d_u.get(), d_part, cm, s);
+    *compliance = cm[0];
+    *max_disp = std::sqrt(cm[1]);
+    if (reactions) {  // R = K u sampled at FixedDofs (:103-110)
+        launch_apply_k(op, d_u.get(), op->d_y.get(), kApplyK, s);
+        hipLaunchKernelGGL(k_select_fixed, dim3(cdiv(d.n_dofs, 256)), dim3(256), 0, s, d.n_dofs, op->d_flags.get(),
+                           op->d_y.get(), d_b.get());
+        MYCEL_LAUNCH_CHECK();
+        d_b.download(reactions, d.n_dofs, s);
+    }
+    if (von_mises) {
+        DevBuf<double> d_vm(d.n_elem);
+        launch_von_mises(op, d_u.get(), d_vm.get(), s);
+        d_vm.download(von_mises, d.n_elem, s);
+        MYCEL_HIP(hipStreamSynchronize(s));
+    }
+    d_u.download(u, d.n_dofs, s);
     MYCEL_HIP(hipStreamSynchronize(s));
 }
 
@@ -594,42 +636,9 @@ int32_t mycel_static_analysis(mycel_op *op, const double *f, double *u, double t
                               double *reactions, double *von_mises, mycel_pcg_result *result) {
     return guarded([&] {
         require(op && f && u && compliance && max_disp && result, MYCEL_E_ARGUMENT_NULL, "null argument");
-        if (op->active_fixed < 6)
-            throw Error(MYCEL_E_INVALID_OPERATION,
-                        "Model has fewer than 6 unique fixed DOFs on active nodes (found " +
-                            std::to_string(op->active_fixed) + ").");
         DeviceGuard dg(op->device);
-        hipStream_t s = op->stream;
-        const Dims &d = op->d;
-        DevBuf<double> d_f(d.n_dofs), d_b(d.n_dofs), d_u(d.n_dofs), d_part;
-        d_f.upload(f, d.n_dofs, s);
-        d_u.upload(u, d.n_dofs, s);
-        launch_project(d.n_dofs, op->d_flags.get(), d_f.get(), d_b.get(), s);
-        pcg_solve_device(op, d_b.get(), d_u.get(), tol, max_iter, precision, nullptr, nullptr, nullptr, result);
-        if (throw_on_nonconv && !result->converged)
-            throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " + std::to_string(result->iterations) +
-                                                   " iterations. Is the model not sufficiently constrained?");
-        DevBuf<uint8_t> d_an(d.n_nodes);
-        d_an.upload(op->active_nodes.data(), d.n_nodes, s);
-        double cm[2];
-        compliance_maxdisp(d.n_nodes, d_an.get(), d_f.get(), d_u.get(), d_part, cm, s);
-        *compliance = cm[0];
-        *max_disp = std::sqrt(cm[1]);
-        if (reactions) {  // R = K u sampled at FixedDofs (:103-110)
-            launch_apply_k(op, d_u.get(), op->d_y.get(), kApplyK, s);
-            hipLaunchKernelGGL(k_select_fixed, dim3(cdiv(d.n_dofs, 256)), dim3(256), 0, s, d.n_dofs,
-                               op->d_flags.get(), op->d_y.get(), d_b.get());
-            MYCEL_LAUNCH_CHECK();
-            d_b.download(reactions, d.n_dofs, s);
-        }
-        if (von_mises) {
-            DevBuf<double> d_vm(d.n_elem);
-            launch_von_mises(op, d_u.get(), d_vm.get(), s);
-            d_vm.download(von_mises, d.n_elem, s);
-            MYCEL_HIP(hipStreamSynchronize(s));
-        }
-        d_u.download(u, d.n_dofs, s);
-        MYCEL_HIP(hipStreamSynchronize(s));
+        static_analysis_run(op, f, u, tol, max_iter, throw_on_nonconv != 0, precision, compliance, max_disp,
+                            reactions, von_mises, result);
     });
 }
 

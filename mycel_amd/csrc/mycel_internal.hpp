@@ -183,4 +183,16 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
                          mycel_pcg_result *res);
 void pcg_free(PcgWork *w);
 
+// ---- capi.hip (internal forms of the entry points, used by the job server) -------------------
+mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,
+                    double nu, double emin_ratio, double penal, const double *density);
+void op_destroy(mycel_op *op);
+void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const uint8_t *passive_mask,
+              const int32_t *patch_nodes, int n_patch, const float *init_density, int init_len,
+              const double *warm_u, mycel_snapshot_cb cb, void *user, mycel_simp_result *result,
+              mycel_snapshot *snaps, int snaps_cap, double *rho_out, double *rho_bar_out, double *u_out);
+void static_analysis_run(mycel_op *op, const double *f, double *u, double tol, int max_iter, bool throw_on_nonconv,
+                         int precision, double *compliance, double *max_disp, double *reactions, double *von_mises,
+                         mycel_pcg_result *result);
+
 }  // namespace mycel

@@ -274,3 +274,86 @@ def l_shape_domain(nx=8, ny=6, nz=5) -> np.ndarray:
     iy = np.arange(ny)[None, :, None]
     d[:] = ((ix < nx // 2) | (iy < ny // 2)).astype(np.uint8)
     return d.reshape(-1)
+
+
+# --------------------------------------------------------------------------------------------
+# Serving stream
+# --------------------------------------------------------------------------------------------
+@dataclass
+class StreamJob:
+    kind: str  # "fea" or "simp"
+    problem: Problem
+    arrival_s: float
+    volume_fraction: float = 0.3
+    rmin: float = 1.5
+    simp_iterations: int = 40
+
+
+_FACES = ("-x", "+x", "-y", "+y", "-z", "+z")
+
+
+def _face_nodes(p: Problem, face: str) -> np.ndarray:
+    nx, ny, nz = p.nx, p.ny, p.nz
+    if face in ("-x", "+x"):
+        i = 0 if face == "-x" else nx
+        return np.array([p.node(i, j, k) for k in range(nz + 1) for j in range(ny + 1)], np.int32)
+    if face in ("-y", "+y"):
+        j = 0 if face == "-y" else ny
+        return np.array([p.node(i, j, k) for k in range(nz + 1) for i in range(nx + 1)], np.int32)
+    k = 0 if face == "-z" else nz
+    return np.array([p.node(i, j, k) for j in range(ny + 1) for i in range(nx + 1)], np.int32)
+
+
+def _patch_on_face(p: Problem, face: str, rng: SplitMix64, size: int = 4) -> np.ndarray:
+    axis = face[1]
+    dims = {"x": (p.ny, p.nz), "y": (p.nx, p.nz), "z": (p.nx, p.ny)}[axis]
+    a0 = int(rng.integers(1, max(1, dims[0] - size + 2))[0])
+    b0 = int(rng.integers(1, max(1, dims[1] - size + 2))[0])
+    fix = 0 if face[0] == "-" else {"x": p.nx, "y": p.ny, "z": p.nz}[axis]
+    nodes = []
+    for b in range(size):
+        for a in range(size):
+            u, v = min(a0 + a, dims[0]), min(b0 + b, dims[1])
+            if axis == "x":
+                nodes.append(p.node(fix, u, v))
+            elif axis == "y":
+                nodes.append(p.node(u, fix, v))
+            else:
+                nodes.append(p.node(u, v, fix))
+    return np.unique(np.array(nodes, np.int32))
+
+
+def stream_problem(rng: SplitMix64, min_el: int, max_el: int) -> Problem:
+    """A full box grid with log-uniform element count, aspect 1:1:1, 2:1:1 or 3:1:1, one random face
+    clamped and 1-4 random 4x4-node patch loads with N(0, I3) forces on the other faces."""
+    n = float(np.exp(np.log(min_el) + (np.log(max_el) - np.log(min_el)) * rng.uniform(1)[0]))
+    aspect = 1 + int(rng.integers(1, 3)[0])
+    s = max(2, int(round((n / aspect) ** (1.0 / 3.0))))
+    p = Problem(aspect * s, s, s, name=f"stream box {aspect * s}x{s}x{s}")
+    fixed_face = _FACES[int(rng.integers(1, 6)[0])]
+    p.fixed_dofs = fix_all(_face_nodes(p, fixed_face))
+    others = [f for f in _FACES if f != fixed_face]
+    loads = []
+    for _ in range(1 + int(rng.integers(1, 4)[0])):
+        face = others[int(rng.integers(1, len(others))[0])]
+        nodes = _patch_on_face(p, face, rng)
+        loads.append((nodes, np.ones(nodes.size), rng.normal(3).astype(np.float32)))
+    p.loads = loads
+    return p
+
+
+def serving_stream(n_jobs: int = 512, seed: int = 5, min_el: int = 20_000, max_el: int = 250_000,
+                   rate_hz: float = 1.0, simp_iterations: int = 40):
+    """Jobs on fresh full-box grids, half single FEA solves and half SIMP runs (volume fraction U[0.2, 0.4],
+    rmin U[1.5, 2.0]), with Poisson arrivals at rate_hz."""
+    rng = SplitMix64(seed)
+    t = 0.0
+    jobs = []
+    for q in range(n_jobs):
+        t += float(-np.log1p(-rng.uniform(1)[0]) / rate_hz)
+        p = stream_problem(rng, min_el, max_el)
+        kind = "fea" if q % 2 == 0 else "simp"
+        vf = 0.2 + 0.2 * float(rng.uniform(1)[0])
+        rmin = 1.5 + 0.5 * float(rng.uniform(1)[0])
+        jobs.append(StreamJob(kind, p, t, vf, rmin, simp_iterations))
+    return jobs
