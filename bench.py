@@ -1,12 +1,16 @@
-"""Benchmark: Jacobi-PCG solves/s on the ~350k-DOF holed box, on one MI355X.
+"""Benchmark: Jacobi-PCG solves/s on the ~350k-DOF holed box, on one MI355X, FP64 and mixed precision.
 
 Workload (the workstation matvec/PCG configuration): 80x40x40 Hex8 grid with a radius-12 through-hole
 (110,080 active elements, 359,160 active / 408,483 box DOFs), i.i.d. densities U[0.01, 1], x = 0
 clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
 
-A step is one cold PCG solve (u = 0) to relative residual 1e-8 with all inputs resident in HBM.
-Prints one JSON line (metric, value = solves/s, roofline of the dominant K*u kernel measured live with
-HIP events on the operator's stream, and the oracle CPU baseline timed on a bounded sample).
+--mode solve (default): a step is one cold PCG solve (u = 0) to relative residual 1e-8 with all inputs
+  resident in HBM, in the mixed-precision solver (FP32 iteration, FP64 reliable updates, converged on the
+  true FP64 residual); the plain FP64 solver is timed the same way beside it. Prints one JSON line.
+--mode kernels: a short fixed run of each kernel (for rocprofv3 kernel traces and PMC counters),
+  including a device copy of known size that calibrates the FETCH_SIZE / WRITE_SIZE counters.
+--mode serve: a stream of FEA and SIMP jobs with Poisson arrivals through the job server, against the
+  same stream run one job at a time in plain FP64; reports jobs/s and p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -21,6 +25,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+METRIC = "PCG solves/s @350k DOF, mixed vs FP64 at compliance <=1e-8 rel; p99 job latency"
+WORKLOAD = "holed-box-80x40x40"
 
 
 def parse():
@@ -28,29 +34,32 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=("solve", "kernels", "serve"), default="solve")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--jobs", type=int, default=48, help="serve mode: jobs in the stream")
+    ap.add_argument("--workers", type=int, default=4, help="serve mode: resident jobs")
+    ap.add_argument("--simp-iterations", type=int, default=10, help="serve mode: SIMP iterations per job")
+    ap.add_argument("--max-elements", type=int, default=250_000, help="serve mode: largest job")
     return ap.parse_args()
 
 
 def traffic_from_profiles(kernel: str, workload: str):
     """HBM bytes per launch measured with rocprofv3 PMC counters (profiles/traffic.json), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        with open(path) as fh:
-            data = json.load(fh)
-        return data.get(f"{kernel}@{workload}")
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            return json.load(fh).get(f"{kernel}@{workload}")
     except (OSError, ValueError):
         return None
 
 
 def cpu_baseline(p, gpu_iters: int, budget_s: float):
-    """The FP64 oracle (reference-algorithm CPU restatement: 8-colour K*u in 256-element chunks,
-    8,192-DOF PCG reduction blocks) on this host's cores: matvecs and a bounded run of PCG iterations,
-    scaled to solves/s with the iteration count of the same solve."""
+    """The FP64 oracle (reference-algorithm CPU restatement: 8-colour K*u in 256-element chunks, 8,192-DOF
+    PCG reduction blocks) on this host's cores: K*u and a bounded run of PCG iterations of the same
+    problem, scaled to solves/s with the iteration count of the same solve."""
     import numpy as np
 
     import oracle as O
@@ -78,126 +87,265 @@ def cpu_baseline(p, gpu_iters: int, budget_s: float):
         "cores": threads,
         "kind": "port",
         "sample": f"{n_mv} K*u + {res.iterations} Jacobi-PCG iterations of the same problem on the FP64 oracle "
-                  f"({threads} threads); solves/s = 1 / (s per iteration x {gpu_iters} iterations)",
+                  f"({threads} threads); solves/s = 1 / (s per iteration x {gpu_iters} iterations of the FP64 "
+                  f"GPU solve)",
         "ms_per_matvec": mv_ms,
         "ms_per_pcg_iteration": per_it * 1e3,
     }
+
+
+def setup(seed):
+    import numpy as np
+    import torch
+
+    from mycel_amd import FemOperator
+    from mycel_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    p = W.matvec_pcg_workstation(seed=seed)
+    fm = p.fe_model()
+    op = FemOperator(fm)
+    f = fm.AssembleForces()
+    b = torch.tensor(np.where(fm.ConstrainedDofs, 0.0, f), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    return p, fm, op, f, b
+
+
+def time_matvecs(op, n, fp32=False):
+    """Average K*u launch time from HIP events recorded on the operator's stream."""
+    import torch
+
+    dofs = op.Model.TotalDofs
+    dt = torch.float32 if fp32 else torch.float64
+    x = torch.randn(dofs, dtype=dt, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    run = op.apply_k_device_f32 if fp32 else op.apply_k_device
+    stream = torch.cuda.ExternalStream(op.stream())
+    for _ in range(3):
+        run(x, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(n):
+        run(x, y)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def timed_solves(op, b, steps, warmup, tol, precision):
+    import torch
+
+    from mycel_amd import PcgSolver
+
+    x = torch.zeros_like(b)
+    solver = PcgSolver()
+    res = None
+    for _ in range(warmup):
+        x.zero_()
+        torch.cuda.synchronize()
+        res = solver.solve_device(op, b, x, tol=tol, maxIterations=500_000, precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x.zero_()
+        res = solver.solve_device(op, b, x, tol=tol, maxIterations=500_000, precision=precision)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    if not res.Converged:
+        raise SystemExit(f"PCG did not converge: {res.Message}")
+    return el, res, x
+
+
+def mode_solve(args):
+    import numpy as np
+    import torch
+
+    import mycel_amd as M
+
+    p, fm, op, f, b = setup(args.seed)
+    info = op.info()
+    mv_ms = time_matvecs(op, args.matvecs)
+    mv32_ms = time_matvecs(op, args.matvecs, fp32=True)
+    el64, r64, x64 = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_FP64)
+    elmx, rmx, xmx = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_FP32)
+    # Accuracy against a tight FP64 reference of the same problem (untimed).
+    xr = torch.zeros_like(b)
+    M.PcgSolver().solve_device(op, b, xr, tol=1e-12, maxIterations=1_000_000)
+    ft = torch.tensor(f, dtype=torch.float64, device="cuda:0")
+    c_ref = float(ft @ xr)
+    c64, cmx = float(ft @ x64), float(ft @ xmx)
+    u_err = float(torch.linalg.norm(xmx - xr) / torch.linalg.norm(xr))
+    solves64, solvesmx = args.steps / el64, args.steps / elmx
+    na, nn = info.active_elements, info.active_nodes
+    bytes_mv = 2 * 3 * 8 * nn + 8 * na
+    flops_mv = 1152 * na
+    achieved = bytes_mv / (mv_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": solvesmx,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elmx / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": WORKLOAD,
+            "grid": [p.nx, p.ny, p.nz],
+            "active_elements": na,
+            "active_dofs": 3 * nn,
+            "box_dofs": p.n_dofs,
+            "density": f"iid U[0.01,1] (SplitMix64 seed {args.seed})",
+            "emin_ratio": p.emin_ratio,
+            "tol": args.tol,
+            "precision": "mixed: FP32 iteration, FP64 reliable updates, true FP64 residual",
+            "parallelism": "single-gpu",
+        },
+        "mixed": {"solves_per_s": solvesmx, "pcg_iterations": rmx.Iterations,
+                  "reliable_updates": rmx.OuterCorrections, "ms_per_solve": elmx / args.steps * 1e3,
+                  "compliance_rel_err_vs_fp64_1e-12": abs(cmx - c_ref) / abs(c_ref), "u_rel_err": u_err},
+        "fp64": {"solves_per_s": solves64, "pcg_iterations": r64.Iterations, "ms_per_solve": el64 / args.steps * 1e3,
+                 "ms_per_pcg_iteration": el64 / args.steps * 1e3 / r64.Iterations,
+                 "compliance_rel_err_vs_fp64_1e-12": abs(c64 - c_ref) / abs(c_ref)},
+        "mixed_speedup_vs_fp64": solvesmx / solves64,
+        "ms_per_matvec": {"fp64": mv_ms, "fp32": mv32_ms},
+        "gdof_per_s_matvec": {"active": 3 * nn / (mv_ms * 1e-3) / 1e9, "box": p.n_dofs / (mv_ms * 1e-3) / 1e9},
+        "roofline": {
+            "kernel": "k_apply_k<double,kApplyK>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("k_apply_k<double>", WORKLOAD),
+            "algorithmic_bytes_per_launch": bytes_mv,
+            "avg_launch_ms": mv_ms,
+            "fp64": {"achieved_tflops": flops_mv / (mv_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
+                     "frac": flops_mv / (mv_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops_per_launch": flops_mv},
+        },
+        "cpu_baseline": None,
+    }
+    op.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, r64.Iterations, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def mode_kernels(args):
+    """Fixed short run for profilers: 20 FP64 K*u, 20 FP32 K*u, one FP64 and one mixed solve, and a
+    256 MiB device-to-device copy (calibrates the byte counters)."""
+    import torch
+
+    import mycel_amd as M
+
+    p, fm, op, f, b = setup(args.seed)
+    print(f"[kernels] fp64 K*u {time_matvecs(op, 20):.4f} ms", flush=True)
+    print(f"[kernels] fp32 K*u {time_matvecs(op, 20, fp32=True):.4f} ms", flush=True)
+    src = torch.ones(32 << 20, dtype=torch.float64, device="cuda:0")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print("[kernels] copy 256 MiB x3 done", flush=True)
+    for prec in (M.PRECISION_FP64, M.PRECISION_MIXED_FP32):
+        el, r, _ = timed_solves(op, b, 1, 0, args.tol, prec)
+        print(f"[kernels] solve precision={prec} {el * 1e3:.2f} ms, {r.Iterations} it", flush=True)
+    op.close()
+
+
+def mode_serve(args):
+    import numpy as np
+
+    from mycel_amd import OptimizerOptions, StaticAnalysis, StaticAnalysisOptions, TopologyOptimizer
+    from mycel_amd import workloads as W
+    from mycel_amd.serve import DONE, JobServer
+
+    def options(j):
+        return OptimizerOptions(VolumeFraction=j.volume_fraction, FilterRadiusMm=j.rmin,
+                                MaxIterations=j.simp_iterations, SolverToleranceInitial=1e-8,
+                                SolverToleranceFinal=1e-8, SolverMaxIterations=500_000, ProtectBoundaryHinges=False)
+
+    stream = W.serving_stream(args.jobs, seed=args.seed, max_el=args.max_elements,
+                              simp_iterations=args.simp_iterations)
+    models = [j.problem.fe_model() for j in stream]
+    forces = [m.AssembleForces() for m in models]
+    # 1. One job at a time, plain FP64 (the baseline): measures service times.
+    service = []
+    for j, m, f in zip(stream, models, forces):
+        t0 = time.perf_counter()
+        if j.kind == "fea":
+            StaticAnalysis.Run(m, StaticAnalysisOptions(Tolerance=1e-8, MaxIterations=500_000))
+        else:
+            TopologyOptimizer.Run(m, options(j))
+        service.append(time.perf_counter() - t0)
+    capacity = len(service) / sum(service)  # jobs/s one at a time
+    rate = 0.8 * capacity * args.workers / 2.0  # offered load: 80% of the measured server capacity estimate
+    rng = W.SplitMix64(args.seed + 1)
+    gaps = -np.log1p(-rng.uniform(len(stream))) / rate
+    arrivals = np.cumsum(gaps)
+    # Sequential replay of the same arrivals: latency = queueing behind earlier jobs + service.
+    t_free, seq_lat = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, t_free)
+        t_free = start + s
+        seq_lat.append(t_free - a)
+    seq_makespan = t_free
+    # 2. Continuous batching through the job server with the same arrivals.
+    ids, lat = [], []
+    with JobServer(workers=args.workers) as sv:
+        t0 = time.perf_counter()
+        for j, m, f, a in zip(stream, models, forces, arrivals):
+            now = time.perf_counter() - t0
+            if a > now:
+                time.sleep(a - now)
+            ids.append(sv.submit_fea(m, tol=1e-8, max_iter=500_000, forces=f) if j.kind == "fea" else
+                       sv.submit_simp(m, options(j), forces=f))
+        for jid in ids:
+            info = sv.wait(jid)
+            if info.state != DONE:
+                raise SystemExit(f"job {jid} failed with status {info.status}")
+            lat.append(info.latency_ms / 1e3)
+        makespan = time.perf_counter() - t0
+    out = {
+        "metric": METRIC,
+        "value": len(stream) / makespan,
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": len(stream),
+        "warmup": 0,
+        "ms_per_step": makespan / len(stream) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving-stream-{len(stream)}-jobs", "workers": args.workers,
+                   "elements": [20_000, args.max_elements], "simp_iterations": args.simp_iterations,
+                   "offered_rate_jobs_per_s": rate},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "one_at_a_time": {"jobs_per_s": len(stream) / seq_makespan,
+                          "latency_s": {"p50": float(np.percentile(seq_lat, 50)),
+                                        "p99": float(np.percentile(seq_lat, 99))},
+                          "mean_service_s": float(np.mean(service))},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "Independent jobs would shard across GPUs with no communication; see DESIGN.md.", file=sys.stderr)
+              "Independent jobs would shard across GPUs with no communication (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
 
-    from mycel_amd import FemOperator, PcgSolver
-    from mycel_amd import workloads as W
-
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
-    dev = torch.device("cuda:0")
-    p = W.matvec_pcg_workstation(seed=args.seed)
-    fm = p.fe_model()
-    op = FemOperator(fm)
-    info = op.info()
-    f = fm.AssembleForces()
-    b_t = torch.tensor(np.where(fm.ConstrainedDofs, 0.0, f), dtype=torch.float64, device=dev)
-    x_t = torch.zeros(p.n_dofs, dtype=torch.float64, device=dev)
-    y_t = torch.empty_like(x_t)
-    xin = torch.tensor(np.random.default_rng(1).standard_normal(p.n_dofs), dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    # ---- K*u: timed with HIP events on the operator's own stream --------------------------------
-    stream = torch.cuda.ExternalStream(op.stream())
-    for _ in range(3):
-        op.apply_k_device(xin, y_t)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.matvecs):
-        op.apply_k_device(xin, y_t)
-    ev1.record(stream)
-    ev1.synchronize()
-    mv_ms = ev0.elapsed_time(ev1) / args.matvecs
-
-    # ---- PCG solves --------------------------------------------------------------------------
-    solver = PcgSolver()
-    res = None
-    for _ in range(args.warmup):
-        x_t.zero_()
-        torch.cuda.synchronize()
-        res = solver.solve_device(op, b_t, x_t, tol=args.tol, maxIterations=200_000)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    iters = []
-    for _ in range(args.steps):
-        x_t.zero_()
-        res = solver.solve_device(op, b_t, x_t, tol=args.tol, maxIterations=200_000)
-        iters.append(res.Iterations)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    if not res.Converged:
-        raise SystemExit(f"PCG did not converge: {res.Message}")
-    ms_per_step = elapsed / args.steps * 1e3
-    solves_per_s = args.steps / elapsed
-    n_it = iters[-1]
-
-    # ---- roofline of the dominant kernel (K*u) ---------------------------------------------------
-    na, nn = info.active_elements, info.active_nodes
-    bytes_mv = 2 * 3 * 8 * nn + 8 * na  # read x, write y over active nodes + one FP64 E per element
-    flops_mv = 1152 * na
-    achieved = bytes_mv / (mv_ms * 1e-3) / 1e9
-    workload = f"holed-box-80x40x40"
-    out = {
-        "metric": "PCG solves/s @350k DOF, mixed vs FP64 at compliance <=1e-8 rel; p99 job latency",
-        "value": solves_per_s,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": workload,
-            "grid": [p.nx, p.ny, p.nz],
-            "active_elements": na,
-            "active_dofs": 3 * nn,
-            "box_dofs": p.n_dofs,
-            "density": "iid U[0.01,1] (SplitMix64 seed %d)" % args.seed,
-            "emin_ratio": p.emin_ratio,
-            "tol": args.tol,
-            "precision": "fp64",
-            "parallelism": "single-gpu",
-        },
-        "pcg_iterations": n_it,
-        "ms_per_pcg_iteration": ms_per_step / n_it,
-        "ms_per_matvec": mv_ms,
-        "gdof_per_s_matvec": {"active": 3 * nn / (mv_ms * 1e-3) / 1e9, "box": p.n_dofs / (mv_ms * 1e-3) / 1e9},
-        "roofline": {
-            "kernel": "k_apply_k_f64<kApplyK>",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("k_apply_k_f64", workload),
-            "algorithmic_bytes_per_launch": bytes_mv,
-            "avg_launch_ms": mv_ms,
-            "fp64": {"achieved_tflops": flops_mv / (mv_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
-                     "frac": flops_mv / (mv_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                     "flops_per_launch": flops_mv},
-        },
-        "cpu_baseline": None,
-    }
-    op.close()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, n_it, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve}[args.mode](args)
 
 
 if __name__ == "__main__":

@@ -133,6 +133,8 @@ MYCEL_API int32_t mycel_op_apply_a(mycel_op *op, const double *x, double *y, int
 /* Device-pointer forms of ApplyK / ApplyA. */
 MYCEL_API int32_t mycel_op_apply_k_d(mycel_op *op, const double *d_x, double *d_y, void *stream);
 MYCEL_API int32_t mycel_op_apply_a_d(mycel_op *op, const double *d_x, double *d_y, int32_t x_is_projected, void *stream);
+/* The FP32 operator of the mixed-precision solver (FP32 Ke and E_e), device pointers, FP32 DOF vectors. */
+MYCEL_API int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *d_y, void *stream);
 /* FemOperator.DiagA / Ke / ConstrainedDofs / ActiveElements properties. */
 MYCEL_API int32_t mycel_op_get_diag_a(mycel_op *op, double *diag);
 MYCEL_API int32_t mycel_op_get_ke(mycel_op *op, double *ke576);

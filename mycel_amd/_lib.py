@@ -164,6 +164,7 @@ SIGNATURES = {
     "mycel_op_apply_a": (_i, [_vp, _vp, _vp, _i]),
     "mycel_op_apply_k_d": (_i, [_vp, _vp, _vp, _vp]),
     "mycel_op_apply_a_d": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "mycel_op_apply_k_f32_d": (_i, [_vp, _vp, _vp, _vp]),
     "mycel_op_get_diag_a": (_i, [_vp, _vp]),
     "mycel_op_get_ke": (_i, [_vp, _vp]),
     "mycel_op_get_constrained": (_i, [_vp, _vp]),

@@ -574,6 +574,13 @@ int32_t mycel_op_apply_a_d(mycel_op *op, const double *d_x, double *d_y, int32_t
     });
 }
 
+int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *d_y, void *stream) {
+    return guarded([&] {
+        require(op && d_x && d_y, MYCEL_E_ARGUMENT_NULL, "op/x/y");
+        launch_apply_k_f32(op, d_x, d_y, kApplyK, pick(op, stream));
+    });
+}
+
 int32_t mycel_op_get_diag_a(mycel_op *op, double *diag) {
     return guarded([&] {
         require(op && diag, MYCEL_E_ARGUMENT_NULL, "op/diag");

@@ -299,6 +299,10 @@ class FemOperator:
     def apply_k_device(self, x_t, y_t, stream=None):
         L.check(L.load().mycel_op_apply_k_d(self._h, L.device_ptr(x_t), L.device_ptr(y_t), stream))
 
+    def apply_k_device_f32(self, x_t, y_t, stream=None):
+        """The FP32 operator of the mixed-precision solver on float32 device tensors."""
+        L.check(L.load().mycel_op_apply_k_f32_d(self._h, L.device_ptr(x_t), L.device_ptr(y_t), stream))
+
     def stream(self):
         return L.load().mycel_op_stream(self._h)
 
