@@ -85,7 +85,8 @@ typedef struct {
     int32_t converged;
     int32_t message;
     double elapsed_ms;
-    int32_t outer_corrections; /* defect-correction sweeps (mixed precision only) */
+    int32_t outer_corrections;   /* reliable updates with the FP64 operator (mixed precision only) */
+    int32_t fallback_iterations; /* FP64 iterations after an FP32 breakdown (mixed precision only) */
 } mycel_pcg_result;
 
 /* IProgress<PcgProgress> (Fem/PcgResult.cs:5). Called on the calling thread between iterations. */

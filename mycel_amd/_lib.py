@@ -99,7 +99,8 @@ class OpInfo(C.Structure):
 
 class PcgResultC(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("final_residual", C.c_double), ("converged", C.c_int32),
-                ("message", C.c_int32), ("elapsed_ms", C.c_double), ("outer_corrections", C.c_int32)]
+                ("message", C.c_int32), ("elapsed_ms", C.c_double), ("outer_corrections", C.c_int32),
+                ("fallback_iterations", C.c_int32)]
 
 
 class SimpOptionsC(C.Structure):

@@ -136,8 +136,10 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
     const int t = st->iter[slot];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (st->mixed) {
-        // Reliable update: the true residual decides convergence; refresh it after every delta drop.
-        if (rel <= st->tol || rel < st->delta * st->maxrel) {
+        // Reliable update: the true residual decides convergence; refresh it once the recursive residual
+        // has fallen by delta below its running maximum since the last refresh.
+        const double mx = fmax(st->maxrel, rel);
+        if (rel <= st->tol || rel < st->delta * mx) {
             if (lead) {
                 st->pause = 1;
                 st->pause_slot = slot;
@@ -176,6 +178,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
         st->iter[o] = t + 1;
         st->last_rel = rel;
         st->last_iter = t;
+        if (st->mixed) st->maxrel = fmax(st->maxrel, rel);
         if (t >= st->max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);  // :359-368
     }
 }
@@ -540,17 +543,32 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             if (w->h_state->done) break;
         }
     }
-    if (mixed) {  // fold the last FP32 increment (zero after a converging update)
-        hipLaunchKernelGGL(k_mixed_flush, gv, b256, 0, s, n, d_x, w->x32.get());
-        MYCEL_LAUNCH_CHECK();
-        MYCEL_HIP(hipStreamSynchronize(s));
-    }
-    const PcgState &h = *w->h_state;
+    const PcgState h = *w->h_state;
     res->iterations = h.iterations;
     res->final_residual = h.final_rel;
     res->message = h.message;
     res->converged = h.message == MYCEL_PCG_CONVERGED;
     res->outer_corrections = h.outer;
+    if (mixed) {
+        const bool broke = h.message == MYCEL_PCG_CURVATURE_BREAKDOWN || h.message == MYCEL_PCG_STEP_BREAKDOWN ||
+                           h.message == MYCEL_PCG_DIRECTION_BREAKDOWN || h.message == MYCEL_PCG_STAGNATION;
+        if (broke && h.iterations < max_iter) {
+            // The FP32 recurrences lost conjugacy (extreme stiffness contrast): drop the FP32 increment since
+            // the last verified reliable update and finish in FP64 from that solution.
+            mycel_pcg_result r64{};
+            pcg_solve_device(op, d_b, d_x, tol, max_iter - h.iterations, MYCEL_PRECISION_FP64, cancel, cb, user,
+                             &r64);
+            res->iterations = h.iterations + r64.iterations;
+            res->final_residual = r64.final_residual;
+            res->message = r64.message;
+            res->converged = r64.converged;
+            res->fallback_iterations = r64.iterations;
+        } else {  // fold the last FP32 increment (zero after a converging update)
+            hipLaunchKernelGGL(k_mixed_flush, gv, b256, 0, s, n, d_x, w->x32.get());
+            MYCEL_LAUNCH_CHECK();
+            MYCEL_HIP(hipStreamSynchronize(s));
+        }
+    }
     res->elapsed_ms = elapsed();
     return MYCEL_OK;
 }

@@ -323,11 +323,12 @@ class PcgResult:
     Message: Optional[str] = None
     MessageCode: int = 0
     OuterCorrections: int = 0
+    FallbackIterations: int = 0
 
 
 def _pcg_result(r: L.PcgResultC) -> PcgResult:
     return PcgResult(r.iterations, r.final_residual, bool(r.converged), r.elapsed_ms, L.PCG_MESSAGES.get(r.message),
-                     r.message, r.outer_corrections)
+                     r.message, r.outer_corrections, r.fallback_iterations)
 
 
 class PcgSolver:

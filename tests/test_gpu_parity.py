@@ -277,6 +277,7 @@ def test_mixed_precision_defect_correction(gpu):
         op.close()
     e64_c, e_c = abs(f @ u64 - c_ref) / abs(c_ref), abs(f @ u - c_ref) / abs(c_ref)
     print(f"[mixed] fp64 it={r64.Iterations} mixed it={r.Iterations} outer={r.OuterCorrections} "
+          f"fp64 fallback it={r.FallbackIterations} "
           f"compliance err fp64={e64_c:.2e} mixed={e_c:.2e} u err fp64={rel(u64, u_ref):.2e} mixed={rel(u, u_ref):.2e}",
           flush=True)
     assert r.Converged and r64.Converged
