@@ -1,0 +1,111 @@
+// Node-centric Hex8 K*u for one node, shared by the stand-alone K*u kernels and the PCG.
+//
+// One thread owns one node and forms its three output rows from its <= 8 incident elements,
+// y_n = sum_a E_e(a) Ke[3a:3a+3, :] u_e(a) (a = the node's corner in element e). Every element row is
+// computed exactly once, by the node that owns it: no colouring, no atomics, deterministic.
+//
+// Ke never leaves scalar registers: by the cube's symmetry every entry is +-v[k] for one of 14 class
+// values (ke_sym.hpp), which arrive as kernel arguments; class and sign are compile-time constants of the
+// fully unrolled loops, so each of the 576 products is one FMA with an SGPR operand and a sign modifier,
+// and no load of Ke ever stalls the wave. The 27-node neighbourhood is held in registers. E lives on a
+// zero-padded element grid, so inactive and out-of-box elements contribute nothing and the boundary needs
+// no branches; neighbour coordinates are clamped and only ever feed such elements.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "ke_sym.hpp"
+#include "mycel_internal.hpp"
+
+namespace mycel {
+
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __fma_rn(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+
+// Rows 3A..3A+2 of Ke times the element vector of the element that holds the node as corner A.
+template <int A, typename T>
+__device__ __forceinline__ void corner_rows(const KeSym<T> &ks, const T (&u)[27][3], T E, T &acc0, T &acc1,
+                                            T &acc2) {
+    T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int s = (corner_x(b) - corner_x(A) + 1) + 3 * (corner_y(b) - corner_y(A) + 1) +
+                      9 * (corner_z(b) - corner_z(A) + 1);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int c = 3 * b + d;
+            const T v = u[s][d];
+            const T k0 = ks.v[kKeClass[3 * A + 0][c]], k1 = ks.v[kKeClass[3 * A + 1][c]],
+                    k2 = ks.v[kKeClass[3 * A + 2][c]];
+            t0 = fma_t(kKeSign[3 * A + 0][c] > 0 ? k0 : -k0, v, t0);
+            t1 = fma_t(kKeSign[3 * A + 1][c] > 0 ? k1 : -k1, v, t1);
+            t2 = fma_t(kKeSign[3 * A + 2][c] > 0 ? k2 : -k2, v, t2);
+        }
+    }
+    acc0 = fma_t(E, t0, acc0);
+    acc1 = fma_t(E, t1, acc1);
+    acc2 = fma_t(E, t2, acc2);
+}
+
+template <int A, typename T>
+__device__ __forceinline__ void all_corners(const KeSym<T> &ks, const T (&u)[27][3], const T (&E)[8], T &acc0, T &acc1,
+                                            T &acc2) {
+    if (E[A] != T(0)) corner_rows<A>(ks, u, E[A], acc0, acc1, acc2);
+    if constexpr (A < 7) all_corners<A + 1>(ks, u, E, acc0, acc1, acc2);
+}
+
+// Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op.
+template <typename T, int MODE>
+__device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
+                                               const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                               T *__restrict__ y, int n) {
+    double dotv = 0.0;
+    if (n >= g.n_nodes) return dotv;
+    const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+    const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
+    T E[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) E[a] = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    const bool any = (E[0] != T(0)) | (E[1] != T(0)) | (E[2] != T(0)) | (E[3] != T(0)) | (E[4] != T(0)) |
+                     (E[5] != T(0)) | (E[6] != T(0)) | (E[7] != T(0));
+    if (any) {
+        const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
+        const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
+        const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
+        T u[27][3];
+#pragma unroll
+        for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int row = g.nx1 * (ys[dy] + g.ny1 * zs[dz]);
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) {
+                    const int m = 3 * (row + xs[dx]);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        T v = x[m + d];
+                        if (MODE == kApplyAGeneral && (flags[m + d] & kFlagConstrained)) v = T(0);
+                        u[dx + 3 * dy + 9 * dz][d] = v;
+                    }
+                }
+            }
+        all_corners<0>(ks, u, E, acc0, acc1, acc2);
+    }
+    T out[3] = {acc0, acc1, acc2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int dof = 3 * n + d;
+        if (MODE == kApplyAProjected) {
+            if (flags[dof] & kFlagFixed) out[d] = x[dof];
+        } else if (MODE == kApplyAGeneral) {
+            if (flags[dof] & kFlagConstrained) out[d] = x[dof];
+        }
+        y[dof] = out[d];
+        dotv += static_cast<double>(x[dof]) * static_cast<double>(out[d]);
+    }
+    return dotv;
+}
+
+}  // namespace mycel

@@ -119,6 +119,17 @@ mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *f
 
     hex8_ke(g.h, nu, op->ke);
     for (int k = 0; k < 24; k++) op->diag_ke[k] = op->ke[k * 24 + k];
+    // Ke as symmetry-class values; the identity Ke = sign * v[class] is checked on this Ke.
+    double kmax = 0.0;
+    for (int i = 0; i < 576; i++) kmax = std::max(kmax, std::fabs(op->ke[i]));
+    for (int c = 0; c < kKeClasses; c++) {
+        op->ks64.v[c] = op->ke[kKeRep[c][0] * 24 + kKeRep[c][1]];
+        op->ks32.v[c] = static_cast<float>(op->ks64.v[c]);
+    }
+    for (int r = 0; r < 24; r++)
+        for (int c = 0; c < 24; c++)
+            if (std::fabs(op->ke[r * 24 + c] - kKeSign[r][c] * op->ks64.v[kKeClass[r][c]]) > 1e-14 * kmax)
+                throw Error(MYCEL_E_ARGUMENT, "element stiffness does not have the isotropic cube symmetry");
     double cst[780];
     std::copy(op->ke, op->ke + 576, cst);
     std::copy(op->diag_ke, op->diag_ke + 24, cst + 576);

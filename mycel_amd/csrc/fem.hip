@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "apply_k.hpp"
 #include "device_common.hpp"
 #include "mycel_internal.hpp"
 
@@ -145,89 +146,19 @@ __global__ __launch_bounds__(256) void k_diag(Dims g, const double *__restrict__
     }
 }
 
-__device__ __forceinline__ double fma_t(double a, double b, double c) { return __fma_rn(a, b, c); }
-__device__ __forceinline__ float fma_t(float a, float b, float c) { return __fmaf_rn(a, b, c); }
-
 // Node-centric K*u in storage type T (double: the FP64 path; float: the inner operator of the mixed
 // path). MODE: kApplyK (y = Kx), kApplyAProjected (y = Kx, y[fixed] = x[fixed]), kApplyAGeneral
 // (y = K(Px), y[constrained] = x). DOT adds a per-block FP64 partial of x.y; `stop` (PCG state) turns
 // the launch into a no-op once the solve has halted.
 template <typename T, int MODE, bool DOT>
-__global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const T *__restrict__ ke, const T *__restrict__ ep,
+__global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const KeSym<T> ks, const T *__restrict__ ep,
                                                     const T *__restrict__ x, const uint8_t *__restrict__ flags,
                                                     T *__restrict__ y, double *__restrict__ part,
                                                     const int *__restrict__ stop) {
     __shared__ double red[4];
     if (DOT && stop && *stop) return;
     const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    double dotv = 0.0;
-    if (n < g.n_nodes) {
-        const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
-        const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
-        T E[8];
-#pragma unroll
-        for (int a = 0; a < 8; a++) E[a] = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
-        T acc0 = 0, acc1 = 0, acc2 = 0;
-        const bool any = (E[0] != T(0)) | (E[1] != T(0)) | (E[2] != T(0)) | (E[3] != T(0)) | (E[4] != T(0)) |
-                         (E[5] != T(0)) | (E[6] != T(0)) | (E[7] != T(0));
-        if (any) {
-            // 3x3x3 node neighbourhood; clamped coordinates only ever feed halo elements (E = 0).
-            const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
-            const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
-            const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
-            T u[27][3];
-#pragma unroll
-            for (int dz = 0; dz < 3; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 3; dy++) {
-                    const int row = g.nx1 * (ys[dy] + g.ny1 * zs[dz]);
-#pragma unroll
-                    for (int dx = 0; dx < 3; dx++) {
-                        const int m = 3 * (row + xs[dx]);
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            T v = x[m + d];
-                            if (MODE == kApplyAGeneral && (flags[m + d] & kFlagConstrained)) v = T(0);
-                            u[dx + 3 * dy + 9 * dz][d] = v;
-                        }
-                    }
-                }
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                if (E[a] != T(0)) {
-                    const T *kr = ke + 72 * a;  // rows 3a .. 3a+2
-                    T t0 = 0, t1 = 0, t2 = 0;
-#pragma unroll
-                    for (int b = 0; b < 8; b++) {
-                        const int s = (corner_x(b) - corner_x(a) + 1) + 3 * (corner_y(b) - corner_y(a) + 1) +
-                                      9 * (corner_z(b) - corner_z(a) + 1);
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            const T v = u[s][d];
-                            t0 = fma_t(kr[3 * b + d], v, t0);
-                            t1 = fma_t(kr[24 + 3 * b + d], v, t1);
-                            t2 = fma_t(kr[48 + 3 * b + d], v, t2);
-                        }
-                    }
-                    acc0 = fma_t(E[a], t0, acc0);
-                    acc1 = fma_t(E[a], t1, acc1);
-                    acc2 = fma_t(E[a], t2, acc2);
-                }
-            }
-        }
-        T out[3] = {acc0, acc1, acc2};
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const int dof = 3 * n + d;
-            if (MODE == kApplyAProjected) {
-                if (flags[dof] & kFlagFixed) out[d] = x[dof];
-            } else if (MODE == kApplyAGeneral) {
-                if (flags[dof] & kFlagConstrained) out[d] = x[dof];
-            }
-            y[dof] = out[d];
-            if (DOT) dotv += static_cast<double>(x[dof]) * static_cast<double>(out[d]);
-        }
-    }
+    const double dotv = apply_k_node<T, MODE>(g, ks, ep, x, flags, y, n);
     if (DOT) {
         const double s = block_sum256(dotv, red);
         if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -373,8 +304,8 @@ void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStr
 int apply_k_blocks(const mycel_op *op) { return cdiv(op->d.n_nodes, 256); }
 
 template <typename T>
-static void launch_apply_k_t(mycel_op *op, const T *ke, const T *ep, const T *d_x, T *d_y, int mode, hipStream_t s,
-                             double *d_dot_partial, const int *d_stop) {
+static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const T *ep, const T *d_x, T *d_y, int mode,
+                             hipStream_t s, double *d_dot_partial, const int *d_stop) {
     const dim3 grid(apply_k_blocks(op)), block(256);
     const uint8_t *fl = op->d_flags.get();
     if (d_dot_partial) {
@@ -396,12 +327,12 @@ static void launch_apply_k_t(mycel_op *op, const T *ke, const T *ep, const T *d_
 
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s, double *d_dot_partial,
                     const int *d_stop) {
-    launch_apply_k_t<double>(op, op->d_ke.get(), op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop);
+    launch_apply_k_t<double>(op, op->ks64, op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop);
 }
 
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s, double *d_dot_partial,
                         const int *d_stop) {
-    launch_apply_k_t<float>(op, op->d_ke32.get(), op->d_ep32.get(), d_x, d_y, mode, s, d_dot_partial, d_stop);
+    launch_apply_k_t<float>(op, op->ks32, op->d_ep32.get(), d_x, d_y, mode, s, d_dot_partial, d_stop);
 }
 
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s) {

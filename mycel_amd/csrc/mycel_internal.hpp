@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/mycel_hip.h"
+#include "ke_sym.hpp"
 
 namespace mycel {
 
@@ -136,6 +137,8 @@ struct mycel_op {
     int n_active_nodes = 0;
     double ke[576];
     double diag_ke[24];
+    mycel::KeSym<double> ks64;  // Ke as 14 symmetry-class values (kernel arguments)
+    mycel::KeSym<float> ks32;
 
     // Device state.
     mycel::DevBuf<double> d_ke;        // [0,576) Ke, [576,600) DiagKe, [600,744) Bc(1), [744,780) D(1,nu)

@@ -82,6 +82,19 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
                                                        const T *__restrict__ ap, T *__restrict__ z,
                                                        const T *__restrict__ inv, double *__restrict__ part2) {
     __shared__ double red[4];
+    // Issue this block's vector loads first: they do not depend on alpha, so their latency overlaps the
+    // halt check and the partial-sum reduction below.
+    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+    T xv[4], pv[4], rv[4], av[4], iv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = min(base + 256 * q, n - 1);
+        xv[q] = x[i];
+        pv[q] = p[i];
+        rv[q] = r[i];
+        av[q] = ap[i];
+        iv[q] = inv[i];
+    }
     if (st->halt) return;
     const double pap = sum_partials256(part_pap, nb_mv, red);
     const double rz0 = st->rz[slot];
@@ -96,15 +109,14 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
     }
     const T a = static_cast<T>(alpha);
     double rr = 0.0, rz = 0.0;
-    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
         if (i < n) {
-            x[i] = fmad(a, p[i], x[i]);
-            const T ri = fmad(-a, ap[i], r[i]);
+            x[i] = fmad(a, pv[q], xv[q]);
+            const T ri = fmad(-a, av[q], rv[q]);
             r[i] = ri;
-            const T zi = ri * inv[i];
+            const T zi = ri * iv[q];
             z[i] = zi;
             rr = __fma_rn(static_cast<double>(ri), static_cast<double>(ri), rr);
             rz = __fma_rn(static_cast<double>(ri), static_cast<double>(zi), rz);
@@ -129,6 +141,14 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
                                                       const double *__restrict__ part2, int nb_vec,
                                                       T *__restrict__ p, const T *__restrict__ z) {
     __shared__ double red[4];
+    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+    T pv[4], zv[4];  // prefetched before the dependent reductions
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = min(base + 256 * q, n - 1);
+        pv[q] = p[i];
+        zv[q] = z[i];
+    }
     if (st->halt) return;
     const double rr = sum_strided2(part2, nb_vec, 0, red);
     const double rzn = sum_strided2(part2, nb_vec, 1, red);
@@ -164,11 +184,10 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
         return;
     }
     const T b = static_cast<T>(beta);
-    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
-        if (i < n) p[i] = fmad(b, p[i], z[i]);
+        if (i < n) p[i] = fmad(b, pv[q], zv[q]);
     }
     if (lead) {
         const int o = slot ^ 1;
