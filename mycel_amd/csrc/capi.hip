@@ -80,6 +80,7 @@ mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *f
     auto op = std::make_unique<mycel_op>();
     op->device = g_device;
     op->d = make_dims(g.nx, g.ny, g.nz);
+    op->tile = choose_tile(op->d);
     op->h = g.h;
     op->e0 = e0;
     op->nu = nu;

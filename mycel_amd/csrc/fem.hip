@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 #include "apply_k.hpp"
 #include "device_common.hpp"
@@ -165,6 +167,23 @@ __global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const KeSym<T> ks, c
     }
 }
 
+// LDS-tiled K*u: one block per brick of nodes (apply_k_tile); dynamic LDS holds the halo and E.
+template <typename T, int MODE, bool DOT>
+__global__ __launch_bounds__(256, 2) void k_apply_k_tiled(Dims g, Tile tl, const KeSym<T> ks, const T *__restrict__ ep,
+                                                          const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                                          T *__restrict__ y, double *__restrict__ part,
+                                                          const int *__restrict__ stop) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double red[4];
+    if (DOT && stop && *stop) return;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const double dotv = apply_k_tile<T, MODE, DOT>(g, tl, ks, ep, x, flags, y, tile, reinterpret_cast<T *>(lds_raw));
+    if (DOT) {
+        const double s = block_sum256(dotv, red);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
 // Per-block partial of sum a_i b_i (optionally only over unconstrained i).
 template <bool MASKED>
 __global__ __launch_bounds__(256) void k_dot_partial(const double *__restrict__ a, const double *__restrict__ b,
@@ -301,13 +320,70 @@ void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStr
     MYCEL_LAUNCH_CHECK();
 }
 
-int apply_k_blocks(const mycel_op *op) { return cdiv(op->d.n_nodes, 256); }
+// Brick shape for the tiled K*u: minimise thread slots (idle lanes cost like busy ones) plus the halo
+// staging, over bricks of 192..256 nodes.
+Tile choose_tile(const Dims &d) {
+    Tile best{};
+    double best_cost = 1e300;
+    for (int tx = 8; tx <= 64; tx++)
+        for (int ty = 1; ty <= 16; ty++)
+            for (int tz = 1; tz <= 16; tz++) {
+                const int nt = tx * ty * tz;
+                if (nt > 256 || nt < 192) continue;
+                const int ntx = cdiv(d.nx1, tx), nty = cdiv(d.ny1, ty), ntz = cdiv(d.nz1, tz);
+                const double tiles = static_cast<double>(ntx) * nty * ntz;
+                const double cost = tiles * (256.0 + 0.35 * tile_halo_nodes(tx, ty, tz));
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best = Tile{tx, ty, tz, ntx, nty, ntz};
+                }
+            }
+    return best;
+}
+
+static bool use_tiled() {
+    static const bool tiled = [] {
+        const char *v = std::getenv("MYCEL_KU_KERNEL");
+        return !(v && std::string(v) == "node");
+    }();
+    return tiled;
+}
+
+int apply_k_blocks(const mycel_op *op) {
+    if (use_tiled()) return op->tile.ntx * op->tile.nty * op->tile.ntz;
+    return cdiv(op->d.n_nodes, 256);
+}
+
+template <typename T>
+static size_t tile_lds_bytes(const Tile &t) {
+    return sizeof(T) * (3 * tile_halo_nodes(t.tx, t.ty, t.tz) + tile_elems(t.tx, t.ty, t.tz));
+}
 
 template <typename T>
 static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const T *ep, const T *d_x, T *d_y, int mode,
                              hipStream_t s, double *d_dot_partial, const int *d_stop) {
     const dim3 grid(apply_k_blocks(op)), block(256);
     const uint8_t *fl = op->d_flags.get();
+    if (use_tiled()) {
+        const Tile tl = op->tile;
+        const size_t lds = tile_lds_bytes<T>(tl);
+        if (d_dot_partial) {
+            if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
+            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyAProjected, true>), grid, block, lds, s, op->d, tl, ke, ep,
+                               d_x, fl, d_y, d_dot_partial, d_stop);
+        } else if (mode == kApplyK) {
+            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyK, false>), grid, block, lds, s, op->d, tl, ke, ep, d_x, fl,
+                               d_y, nullptr, nullptr);
+        } else if (mode == kApplyAProjected) {
+            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyAProjected, false>), grid, block, lds, s, op->d, tl, ke, ep,
+                               d_x, fl, d_y, nullptr, nullptr);
+        } else {
+            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyAGeneral, false>), grid, block, lds, s, op->d, tl, ke, ep,
+                               d_x, fl, d_y, nullptr, nullptr);
+        }
+        MYCEL_LAUNCH_CHECK();
+        return;
+    }
     if (d_dot_partial) {
         if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
         hipLaunchKernelGGL((k_apply_k<T, kApplyAProjected, true>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,

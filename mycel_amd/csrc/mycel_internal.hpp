@@ -116,6 +116,12 @@ void constitutive(double e, double nu, double *d36);
 
 struct PcgWork;
 
+struct Tile {
+    int tx, ty, tz;     // brick extent in nodes (tx * ty * tz <= 256; the other threads only stage data)
+    int ntx, nty, ntz;  // bricks per axis
+};
+Tile choose_tile(const Dims &d);
+
 }  // namespace mycel
 
 // ------------------------------------------------------------------------------------------------
@@ -138,6 +144,7 @@ struct mycel_op {
     double ke[576];
     double diag_ke[24];
     mycel::KeSym<double> ks64;  // Ke as 14 symmetry-class values (kernel arguments)
+    mycel::Tile tile{};         // brick shape of the tiled K*u
     mycel::KeSym<float> ks32;
 
     // Device state.
