@@ -11,6 +11,11 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   including a device copy of known size that calibrates the FETCH_SIZE / WRITE_SIZE counters.
 --mode serve: a stream of FEA and SIMP jobs with Poisson arrivals through the job server, against the
   same stream run one job at a time in plain FP64; reports jobs/s and p50/p99 latency.
+--mode summarize: no GPU; reads rocprofv3 CSV output (--stats-dir, --fetch-dir, --write-dir) of a
+  `--mode kernels` run and prints per-kernel durations and HBM bytes per launch. FETCH_SIZE and WRITE_SIZE
+  (KiB per dispatch) are each scaled by the factor that makes the largest device copy of the run read and
+  write exactly its 256 MiB; --traffic-json writes the K*u entries that --mode solve reports as
+  roofline.traffic.
 """
 from __future__ import annotations
 
@@ -34,7 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=("solve", "kernels", "serve"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "summarize"), default="solve")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
@@ -44,6 +49,10 @@ def parse():
     ap.add_argument("--workers", type=int, default=4, help="serve mode: resident jobs")
     ap.add_argument("--simp-iterations", type=int, default=10, help="serve mode: SIMP iterations per job")
     ap.add_argument("--max-elements", type=int, default=250_000, help="serve mode: largest job")
+    ap.add_argument("--stats-dir", help="summarize: rocprofv3 --kernel-trace --stats output directory")
+    ap.add_argument("--fetch-dir", help="summarize: rocprofv3 --pmc FETCH_SIZE output directory")
+    ap.add_argument("--write-dir", help="summarize: rocprofv3 --pmc WRITE_SIZE output directory")
+    ap.add_argument("--traffic-json", help="summarize: write K*u HBM bytes per launch to this file")
     return ap.parse_args()
 
 
@@ -336,11 +345,91 @@ def mode_serve(args):
     print(json.dumps(out), flush=True)
 
 
+COPY_BYTES = 256 << 20  # the device copy in --mode kernels
+
+
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True)):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def _short(name: str) -> str:
+    """Kernel name without namespaces and argument lists: k_apply_k<double, 1, true>."""
+    name = name.replace("mycel::(anonymous namespace)::", "").replace("void ", "")
+    depth, out = 0, []
+    for ch in name:
+        if ch == "(" and depth == 0:
+            break
+        depth += ch == "<"
+        depth -= ch == ">"
+        out.append(ch)
+    return "".join(out).strip()
+
+
+def _pmc_bytes(directory, counter):
+    """Median KiB per dispatch for each kernel, and the scale that makes the largest copy exact."""
+    import statistics
+    from collections import defaultdict
+
+    per = defaultdict(list)
+    copy_kib = 0.0
+    for r in _csv_rows(directory, "counter_collection.csv"):
+        if r["Counter_Name"] != counter:
+            continue
+        v = float(r["Counter_Value"])
+        if "copyBuffer" in r["Kernel_Name"]:
+            copy_kib = max(copy_kib, v)
+        else:
+            per[_short(r["Kernel_Name"])].append(v)
+    scale = (COPY_BYTES / 1024.0) / copy_kib if copy_kib > 0 else float("nan")
+    return {k: (statistics.median(v), len(v)) for k, v in per.items()}, scale, copy_kib
+
+
+def mode_summarize(args):
+    lines = []
+    if args.stats_dir:
+        lines.append("kernel | calls | avg us | min us | max us | % of GPU time")
+        lines.append("--- | ---: | ---: | ---: | ---: | ---:")
+        for r in _csv_rows(args.stats_dir, "kernel_stats.csv")[:24]:
+            lines.append(f"{_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.3f} | "
+                         f"{float(r['MinNs']) / 1e3:.3f} | {float(r['MaxNs']) / 1e3:.3f} | {float(r['Percentage']):.2f}")
+    traffic = {}
+    if args.fetch_dir and args.write_dir:
+        fetch, fs, fcopy = _pmc_bytes(args.fetch_dir, "FETCH_SIZE")
+        write, ws, wcopy = _pmc_bytes(args.write_dir, "WRITE_SIZE")
+        lines.append("")
+        lines.append(f"calibration: 256 MiB device copy read FETCH_SIZE {fcopy:.1f} KiB -> scale {fs:.4f}; "
+                     f"WRITE_SIZE {wcopy:.1f} KiB -> scale {ws:.4f}")
+        lines.append("")
+        lines.append("kernel | dispatches | FETCH KiB (raw median) | WRITE KiB (raw median) | HBM bytes/launch (scaled)")
+        lines.append("--- | ---: | ---: | ---: | ---:")
+        for k in sorted(set(fetch) & set(write)):
+            b = fetch[k][0] * 1024 * fs + write[k][0] * 1024 * ws
+            lines.append(f"{k} | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | {b:.0f}")
+            if k.startswith("k_apply_k<"):
+                traffic[k] = b
+    print("\n".join(lines))
+    if args.traffic_json and traffic:
+        ku = {f"k_apply_k<{t}>@{WORKLOAD}": traffic[f"k_apply_k<{t}, 0, false>"]
+              for t in ("double", "float") if f"k_apply_k<{t}, 0, false>" in traffic}
+        with open(args.traffic_json, "w") as fh:
+            json.dump(ku, fh, indent=1, sort_keys=True)
+
+
 def main():
     args = parse()
+    if args.mode == "summarize":
+        mode_summarize(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "Independent jobs would shard across GPUs with no communication (see DESIGN.md).", file=sys.stderr)
+              "Independent jobs would shard across GPUs with no communication.", file=sys.stderr)
         sys.exit(2)
     import torch
 

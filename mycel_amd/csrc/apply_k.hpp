@@ -55,7 +55,59 @@ __device__ __forceinline__ void all_corners(const KeSym<T> &ks, const T (&u)[27]
     if constexpr (A < 7) all_corners<A + 1>(ks, u, E, acc0, acc1, acc2);
 }
 
+// Rows 3A..3A+2 of Ke times the element vector of the element that holds the node as corner A, with the
+// element's nodes taken from two 3x3 node layers: `lo` at the element's lower z, `up` at its upper z. The
+// product joins the accumulators only when E != 0, so an element outside the domain contributes nothing even
+// if x holds non-finite values there (a select, not a branch: it does not split the basic block).
+template <int A, typename T>
+__device__ __forceinline__ void corner_rows_slab(const KeSym<T> &ks, const T (&lo)[9][3], const T (&up)[9][3], T E,
+                                                 T &acc0, T &acc1, T &acc2) {
+    T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int s = (corner_x(b) - corner_x(A) + 1) + 3 * (corner_y(b) - corner_y(A) + 1);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int c = 3 * b + d;
+            const T v = corner_z(b) ? up[s][d] : lo[s][d];
+            const T k0 = ks.v[kKeClass[3 * A + 0][c]], k1 = ks.v[kKeClass[3 * A + 1][c]],
+                    k2 = ks.v[kKeClass[3 * A + 2][c]];
+            t0 = fma_t(kKeSign[3 * A + 0][c] > 0 ? k0 : -k0, v, t0);
+            t1 = fma_t(kKeSign[3 * A + 1][c] > 0 ? k1 : -k1, v, t1);
+            t2 = fma_t(kKeSign[3 * A + 2][c] > 0 ? k2 : -k2, v, t2);
+        }
+    }
+    const bool on = E != T(0);
+    acc0 = on ? fma_t(E, t0, acc0) : acc0;
+    acc1 = on ? fma_t(E, t1, acc1) : acc1;
+    acc2 = on ? fma_t(E, t2, acc2) : acc2;
+}
+
+// One 3x3 layer of node values at node-row offsets rows[dy] (already including z) and columns xs[dx].
+template <typename T, int MODE>
+__device__ __forceinline__ void load_layer(const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                           const int (&xs)[3], const int (&rows)[3], T (&L)[9][3]) {
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            const int m = 3 * (rows[dy] + xs[dx]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                T v = x[m + d];
+                if (MODE == kApplyAGeneral && (flags[m + d] & kFlagConstrained)) v = T(0);
+                L[dx + 3 * dy][d] = v;
+            }
+        }
+}
+
 // Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op.
+//
+// The gather runs in two z-slabs so that only two node layers are live at a time: the four elements below
+// the node (corners 4..7, node layers k-1 and k), then the four above it (corners 0..3, layers k and k+1),
+// whose upper layer reuses the registers of the first slab's lower one. The E values, the node's own flags
+// and both first layers are requested together up front, so the memory latencies overlap instead of
+// chaining (no branch on the moduli before the gather).
 template <typename T, int MODE>
 __device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
                                                const T *__restrict__ x, const uint8_t *__restrict__ flags,
@@ -67,43 +119,47 @@ __device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks
     T E[8];
 #pragma unroll
     for (int a = 0; a < 8; a++) E[a] = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
-    T acc0 = 0, acc1 = 0, acc2 = 0;
-    const bool any = (E[0] != T(0)) | (E[1] != T(0)) | (E[2] != T(0)) | (E[3] != T(0)) | (E[4] != T(0)) |
-                     (E[5] != T(0)) | (E[6] != T(0)) | (E[7] != T(0));
-    if (any) {
-        const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
-        const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
-        const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
-        T u[27][3];
+    uint8_t fl[3] = {0, 0, 0};
+    if (MODE != kApplyK) {
 #pragma unroll
-        for (int dz = 0; dz < 3; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 3; dy++) {
-                const int row = g.nx1 * (ys[dy] + g.ny1 * zs[dz]);
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++) {
-                    const int m = 3 * (row + xs[dx]);
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        T v = x[m + d];
-                        if (MODE == kApplyAGeneral && (flags[m + d] & kFlagConstrained)) v = T(0);
-                        u[dx + 3 * dy + 9 * dz][d] = v;
-                    }
-                }
-            }
-        all_corners<0>(ks, u, E, acc0, acc1, acc2);
+        for (int d = 0; d < 3; d++) fl[d] = flags[3 * n + d];
     }
+    const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
+    const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
+    const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
+    int rows[3];
+    T lo[9][3], mid[9][3];
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) rows[dy] = g.nx1 * (ys[dy] + g.ny1 * zs[0]);
+    load_layer<T, MODE>(x, flags, xs, rows, lo);
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) rows[dy] = g.nx1 * (ys[dy] + g.ny1 * zs[1]);
+    load_layer<T, MODE>(x, flags, xs, rows, mid);
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    corner_rows_slab<4>(ks, lo, mid, E[4], acc0, acc1, acc2);
+    corner_rows_slab<5>(ks, lo, mid, E[5], acc0, acc1, acc2);
+    corner_rows_slab<6>(ks, lo, mid, E[6], acc0, acc1, acc2);
+    corner_rows_slab<7>(ks, lo, mid, E[7], acc0, acc1, acc2);
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) rows[dy] = g.nx1 * (ys[dy] + g.ny1 * zs[2]);
+    load_layer<T, MODE>(x, flags, xs, rows, lo);  // layer k+1
+    corner_rows_slab<0>(ks, mid, lo, E[0], acc0, acc1, acc2);
+    corner_rows_slab<1>(ks, mid, lo, E[1], acc0, acc1, acc2);
+    corner_rows_slab<2>(ks, mid, lo, E[2], acc0, acc1, acc2);
+    corner_rows_slab<3>(ks, mid, lo, E[3], acc0, acc1, acc2);
     T out[3] = {acc0, acc1, acc2};
+    const int c = 4;  // the node itself within a layer
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int dof = 3 * n + d;
+        const T xd = mid[c][d];
         if (MODE == kApplyAProjected) {
-            if (flags[dof] & kFlagFixed) out[d] = x[dof];
+            if (fl[d] & kFlagFixed) out[d] = xd;
         } else if (MODE == kApplyAGeneral) {
-            if (flags[dof] & kFlagConstrained) out[d] = x[dof];
+            if (fl[d] & kFlagConstrained) out[d] = x[dof];
         }
         y[dof] = out[d];
-        dotv += static_cast<double>(x[dof]) * static_cast<double>(out[d]);
+        dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xd) * static_cast<double>(out[d]);
     }
     return dotv;
 }

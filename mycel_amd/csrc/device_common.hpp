@@ -45,6 +45,41 @@ __device__ __forceinline__ double sum_partials256(const double *__restrict__ par
     return block_sum256(v, lds4);
 }
 
+// The same fixed-order sum, formed by each wavefront on its own (no LDS, no barrier): lane l adds the
+// partials l, l + 64, ... and the lanes are then combined by the butterfly. Every wave of every block
+// gets the bit-identical value.
+__device__ __forceinline__ double wave_sum_partials(const double *__restrict__ part, int n) {
+    double v = 0.0;
+    for (int i = threadIdx.x & 63; i < n; i += 64) v += part[i];
+    return wave_sum(v);
+}
+
+// Two interleaved partial arrays (part[2i], part[2i + 1]) summed per wave as above.
+__device__ __forceinline__ void wave_sum_partials2(const double *__restrict__ part, int n, double &s0, double &s1) {
+    double v0 = 0.0, v1 = 0.0;
+    for (int i = threadIdx.x & 63; i < n; i += 64) {
+        v0 += part[2 * i];
+        v1 += part[2 * i + 1];
+    }
+    s0 = wave_sum(v0);
+    s1 = wave_sum(v1);
+}
+
+// Two fixed-order block sums with one LDS round trip.
+__device__ __forceinline__ void block_sum256x2(double &a, double &b, double *lds8) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds8[w] = a;
+        lds8[4 + w] = b;
+    }
+    __syncthreads();
+    a = (lds8[0] + lds8[1]) + (lds8[2] + lds8[3]);
+    b = (lds8[4] + lds8[5]) + (lds8[6] + lds8[7]);
+    __syncthreads();
+}
+
 // rho^3 rounded once to nearest (up to double-rounding ties): rho^2 is split exactly into hi + lo
 // with an FMA, then (hi + lo) * rho is formed with one more error-free product.
 __device__ __forceinline__ double cube_rn(double r) {

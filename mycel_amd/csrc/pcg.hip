@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
                                                        T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
                                                        const T *__restrict__ ap, T *__restrict__ z,
                                                        const T *__restrict__ inv, double *__restrict__ part2) {
-    __shared__ double red[4];
+    __shared__ double red[8];
     // Issue this block's vector loads first: they do not depend on alpha, so their latency overlaps the
     // halt check and the partial-sum reduction below.
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
         iv[q] = inv[i];
     }
     if (st->halt) return;
-    const double pap = sum_partials256(part_pap, nb_mv, red);
+    const double pap = wave_sum_partials(part_pap, nb_mv);
     const double rz0 = st->rz[slot];
     const bool curv = !(pap > 1e-30) || !isfinite(pap);  // PcgSolver.cs:245
     const double alpha = rz0 / pap;
@@ -122,25 +122,17 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
             rz = __fma_rn(static_cast<double>(ri), static_cast<double>(zi), rz);
         }
     }
-    rr = block_sum256(rr, red);
-    rz = block_sum256(rz, red);
+    block_sum256x2(rr, rz, red);
     if (threadIdx.x == 0) {
         part2[2 * blockIdx.x] = rr;
         part2[2 * blockIdx.x + 1] = rz;
     }
 }
 
-__device__ __forceinline__ double sum_strided2(const double *__restrict__ part, int n, int which, double *lds4) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) v += part[2 * i + which];
-    return block_sum256(v, lds4);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState *__restrict__ st,
                                                       const double *__restrict__ part2, int nb_vec,
                                                       T *__restrict__ p, const T *__restrict__ z) {
-    __shared__ double red[4];
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
     T pv[4], zv[4];  // prefetched before the dependent reductions
 #pragma unroll
@@ -150,8 +142,8 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
         zv[q] = z[i];
     }
     if (st->halt) return;
-    const double rr = sum_strided2(part2, nb_vec, 0, red);
-    const double rzn = sum_strided2(part2, nb_vec, 1, red);
+    double rr, rzn;
+    wave_sum_partials2(part2, nb_vec, rr, rzn);
     const double rel = sqrt(rr) / st->bnorm;
     const int t = st->iter[slot];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
