@@ -56,12 +56,19 @@ extern "C" {
 #define MYCEL_PCG_MAX_ITERATIONS 7
 #define MYCEL_PCG_CANCELED 8
 
-/* Precision of the inner Krylov solve. FP64 is the reference's arithmetic throughout. The mixed
- * modes run an FP32 (or BF16-operand) matrix-core K*u inside FP64 defect correction: the outer
- * residual r = b - K u, its norm and the solution update stay FP64. */
+/* Precision / preconditioner of the solve.
+ * FP64: the reference's Jacobi-PCG arithmetic throughout (Fem/PcgSolver.cs:80-370).
+ * MIXED_FP32: Jacobi-PCG iterating on FP32 vectors with the FP32 operator and FP64 dot products, with FP64
+ *   reliable updates (the true residual r = P(b - K u) in FP64) and convergence on that true residual.
+ * MIXED_BF16: reserved; rejected with MYCEL_E_ARGUMENT.
+ * MIXED_GMG: FP64 flexible PCG (FP64 operator, residual, dot products, convergence test and update)
+ *   preconditioned by one FP32 geometric-multigrid V-cycle per iteration in place of Jacobi: a second
+ *   IPreconditioner (Fem/IPreconditioner.cs:9-20). Needs a grid that coarsens by 2 per axis down to
+ *   <= 140 DOFs; otherwise MYCEL_E_ARGUMENT. */
 #define MYCEL_PRECISION_FP64 0
 #define MYCEL_PRECISION_MIXED_FP32 1
 #define MYCEL_PRECISION_MIXED_BF16 2
+#define MYCEL_PRECISION_MIXED_GMG 3
 
 typedef struct mycel_op mycel_op;
 typedef struct mycel_filter mycel_filter;

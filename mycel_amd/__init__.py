@@ -4,19 +4,20 @@ The compute path is the HIP library libmycel_hip.so (C ABI in include/mycel_hip.
 the host-side mirror of the reference's managed API (Mycel.Core.Fem / Mycel.Core.Optimization)
 over that ABI. There is no CPU fallback: importing the API without the built library raises.
 """
-from ._lib import (PRECISION_FP64, PRECISION_MIXED_BF16, PRECISION_MIXED_FP32, ArgumentError,
+from ._lib import (PRECISION_FP64, PRECISION_MIXED_BF16, PRECISION_MIXED_FP32, PRECISION_MIXED_GMG, ArgumentError,
                    ArgumentNullError, ArgumentOutOfRangeError, DeviceError, InvalidOperationError, MycelError,
                    OperationCanceledError, OutOfMemoryError)
 from .fem import (FeMaterial, FemOperator, FeModel, GridSpec, Hex8Element, JacobiPreconditioner, LoadCase,
-                  PcgResult, PcgSolver, StaticAnalysis, StaticAnalysisOptions, StaticAnalysisResult, SurfacePatch)
+                  MultigridPreconditioner, PcgResult, PcgSolver, StaticAnalysis, StaticAnalysisOptions, StaticAnalysisResult, SurfacePatch)
 from .optimization import (DensityFilter, IterationSnapshot, OcOptimizer, OptimizerOptions, SensitivityAnalysis,
                            TopologyOptimizationResult, TopologyOptimizer)
 
 __all__ = [
-    "PRECISION_FP64", "PRECISION_MIXED_FP32", "PRECISION_MIXED_BF16",
+    "PRECISION_FP64", "PRECISION_MIXED_FP32", "PRECISION_MIXED_BF16", "PRECISION_MIXED_GMG",
     "ArgumentError", "ArgumentNullError", "ArgumentOutOfRangeError", "DeviceError", "InvalidOperationError",
     "MycelError", "OperationCanceledError", "OutOfMemoryError",
     "FeMaterial", "FemOperator", "FeModel", "GridSpec", "Hex8Element", "JacobiPreconditioner", "LoadCase",
+    "MultigridPreconditioner",
     "PcgResult", "PcgSolver", "StaticAnalysis", "StaticAnalysisOptions", "StaticAnalysisResult", "SurfacePatch",
     "DensityFilter", "IterationSnapshot", "OcOptimizer", "OptimizerOptions", "SensitivityAnalysis",
     "TopologyOptimizationResult", "TopologyOptimizer",

@@ -26,6 +26,7 @@ E_OUT_OF_MEMORY = 8
 PRECISION_FP64 = 0
 PRECISION_MIXED_FP32 = 1
 PRECISION_MIXED_BF16 = 2
+PRECISION_MIXED_GMG = 3
 
 PCG_MESSAGES = {
     0: None,

@@ -101,29 +101,22 @@ __device__ __forceinline__ void load_layer(const T *__restrict__ x, const uint8_
         }
 }
 
-// Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op.
+// Raw (K x) at node n = (i, j, k) into kx[3], and the node's own x values into xc[3] (masked like the
+// neighbours in kApplyAGeneral mode).
 //
 // The gather runs in two z-slabs so that only two node layers are live at a time: the four elements below
 // the node (corners 4..7, node layers k-1 and k), then the four above it (corners 0..3, layers k and k+1),
-// whose upper layer reuses the registers of the first slab's lower one. The E values, the node's own flags
-// and both first layers are requested together up front, so the memory latencies overlap instead of
-// chaining (no branch on the moduli before the gather).
+// whose upper layer reuses the registers of the first slab's lower one. The E values and both first layers
+// are requested together up front, so the memory latencies overlap instead of chaining (no branch on the
+// moduli before the gather).
 template <typename T, int MODE>
-__device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
-                                               const T *__restrict__ x, const uint8_t *__restrict__ flags,
-                                               T *__restrict__ y, int n) {
-    double dotv = 0.0;
-    if (n >= g.n_nodes) return dotv;
-    const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+__device__ __forceinline__ void ku_gather(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
+                                          const T *__restrict__ x, const uint8_t *__restrict__ flags, int i, int j,
+                                          int k, T (&kx)[3], T (&xc)[3]) {
     const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
     T E[8];
 #pragma unroll
     for (int a = 0; a < 8; a++) E[a] = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
-    uint8_t fl[3] = {0, 0, 0};
-    if (MODE != kApplyK) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) fl[d] = flags[3 * n + d];
-    }
     const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
     const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
     const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
@@ -147,19 +140,38 @@ __device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks
     corner_rows_slab<1>(ks, mid, lo, E[1], acc0, acc1, acc2);
     corner_rows_slab<2>(ks, mid, lo, E[2], acc0, acc1, acc2);
     corner_rows_slab<3>(ks, mid, lo, E[3], acc0, acc1, acc2);
-    T out[3] = {acc0, acc1, acc2};
-    const int c = 4;  // the node itself within a layer
+    kx[0] = acc0;
+    kx[1] = acc1;
+    kx[2] = acc2;
+#pragma unroll
+    for (int d = 0; d < 3; d++) xc[d] = mid[4][d];
+}
+
+// Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op.
+template <typename T, int MODE>
+__device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
+                                               const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                               T *__restrict__ y, int n) {
+    double dotv = 0.0;
+    if (n >= g.n_nodes) return dotv;
+    const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+    uint8_t fl[3] = {0, 0, 0};
+    if (MODE != kApplyK) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) fl[d] = flags[3 * n + d];
+    }
+    T out[3], xc[3];
+    ku_gather<T, MODE>(g, ks, ep, x, flags, i, j, k, out, xc);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int dof = 3 * n + d;
-        const T xd = mid[c][d];
         if (MODE == kApplyAProjected) {
-            if (fl[d] & kFlagFixed) out[d] = xd;
+            if (fl[d] & kFlagFixed) out[d] = xc[d];
         } else if (MODE == kApplyAGeneral) {
             if (fl[d] & kFlagConstrained) out[d] = x[dof];
         }
         y[dof] = out[d];
-        dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xd) * static_cast<double>(out[d]);
+        dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
     }
     return dotv;
 }

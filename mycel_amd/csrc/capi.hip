@@ -177,6 +177,8 @@ void op_destroy(mycel_op *op) {
     if (op->stream) (void)hipStreamSynchronize(op->stream);
     pcg_free(op->pcg);
     op->pcg = nullptr;
+    gmg_free(op->gmg);
+    op->gmg = nullptr;
     if (op->stream) (void)hipStreamDestroy(op->stream);
     delete op;
 }

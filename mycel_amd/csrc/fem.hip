@@ -310,6 +310,7 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStream_t s) {
     static_assert(sizeof(uint8_t) == 1, "");
     op->penal = penal;
+    op->epoch++;
     hipLaunchKernelGGL(k_update_e, dim3(cdiv(op->d.n_pad, 256)), dim3(256), 0, s, op->d, d_density_box,
                        op->d_domain.get(), op->e0, op->emin, penal,
                        op->d_ep.get(), op->d_ep32.get());

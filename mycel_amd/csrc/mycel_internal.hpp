@@ -115,6 +115,7 @@ void hex8_bc(double h, double *bc144);
 void constitutive(double e, double nu, double *d36);
 
 struct PcgWork;
+struct Gmg;
 
 struct Tile {
     int tx, ty, tz;     // brick extent in nodes (tx * ty * tz <= 256; the other threads only stage data)
@@ -162,6 +163,8 @@ struct mycel_op {
     mycel::DevBuf<double> d_x, d_y;    // staging for the host-pointer entry points
     mycel::DevBuf<double> d_part;      // block partials for one-off reductions
     mycel::PcgWork *pcg = nullptr;
+    mycel::Gmg *gmg = nullptr;         // multigrid hierarchy (built on first use)
+    uint64_t epoch = 0;                // bumped whenever the element moduli change
 };
 
 namespace mycel {
@@ -192,6 +195,16 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
                          const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
                          mycel_pcg_result *res);
 void pcg_free(PcgWork *w);
+
+// ---- gmg.hip ---------------------------------------------------------------------------------
+// Builds the hierarchy on first use and refreshes its moduli, constraints and coarsest inverse when
+// op->epoch changed (enqueued on s, no host synchronisation).
+void gmg_prepare(mycel_op *op, hipStream_t s);
+// z = M(r) (one FP32 V-cycle) plus per-block partials of z.r and z.ap, gmg_partials(op) of each.
+void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
+                        double *part_zap, hipStream_t s);
+int gmg_partials(const mycel_op *op);
+void gmg_free(Gmg *g);
 
 // ---- capi.hip (internal forms of the entry points, used by the job server) -------------------
 mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,

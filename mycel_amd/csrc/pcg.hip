@@ -22,6 +22,12 @@
 // into the FP64 solution, the true residual r = P(b - A x) is formed with the FP64 operator, and the
 // iteration resumes from it with the same search direction. Convergence is declared on the true FP64
 // residual only.
+// Multigrid (MYCEL_PRECISION_MIXED_GMG): the FP64 iteration above with the Jacobi step z = D^-1 r replaced
+// by one FP32 geometric-multigrid V-cycle (gmg.hip). Because the FP32 V-cycle is not exactly a fixed
+// linear map, beta is the flexible (Polak-Ribiere) form z'.(r' - r) / (r.z) = -alpha z'.Ap / (r.z), clamped
+// at 0 (a restart) if rounding makes it negative; the operator, residual, dot products, convergence test
+// and solution update are the FP64 ones of the reference path. One iteration: K*u (+ p.Ap partials),
+// alpha and the x, r updates (+ r.r partials), the V-cycle (+ z.r and z.Ap partials), beta and p.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -33,6 +39,7 @@
 namespace mycel {
 
 struct PcgState {
+    double alpha[2];     // step length of the iteration that reads this slot (flexible beta)
     double rz[2];        // rho0 at the start of the iteration that reads this slot
     double prev_rel[2];  // relres of the previous iteration (stagnation test, breakdown report)
     int stag[2];
@@ -56,12 +63,18 @@ struct PcgWork {
     PcgState *h_state = nullptr;             // pinned
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
     const double *graph64_x = nullptr;       // solution vector the FP64 graph was captured with
+    DevBuf<double> part_zr, part_zap;        // V-cycle partials of z.r and z.Ap (multigrid)
+    int nb_z = 0;
+    hipGraphExec_t graph_gmg = nullptr;
+    const double *graph_gmg_x = nullptr;
+    const Gmg *graph_gmg_h = nullptr;        // hierarchy the multigrid graph was captured with
 };
 
 namespace {
 
 constexpr int kVecPerBlock = 1024;  // DOFs per block in the vector kernels (4 per thread)
 constexpr int kGraphIters = 32;     // iterations per captured batch (even: the slot parity repeats)
+constexpr int kGmgGraphIters = 4;   // multigrid iterations are ~30 launches each and few in number
 constexpr double kReliableDelta = 0.1;
 
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __fma_rn(a, b, c); }
@@ -192,6 +205,112 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
         if (st->mixed) st->maxrel = fmax(st->maxrel, rel);
         if (t >= st->max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);  // :359-368
     }
+}
+
+// Flexible PCG, step 2: alpha = rz / p.Ap (the breakdown tests of PcgSolver.cs:245-270); x += alpha p;
+// r -= alpha Ap; per-block partials of r.r. alpha is kept in the state for the flexible beta.
+__global__ __launch_bounds__(256) void k_fpcg_update_xr(int n, int slot, PcgState *__restrict__ st,
+                                                        const double *__restrict__ part_pap, int nb_mv,
+                                                        double *__restrict__ x, double *__restrict__ r,
+                                                        const double *__restrict__ p, const double *__restrict__ ap,
+                                                        double *__restrict__ part_rr) {
+    __shared__ double red[4];
+    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+    double xv[4], pv[4], rv[4], av[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = min(base + 256 * q, n - 1);
+        xv[q] = x[i];
+        pv[q] = p[i];
+        rv[q] = r[i];
+        av[q] = ap[i];
+    }
+    if (st->halt) return;
+    const double pap = wave_sum_partials(part_pap, nb_mv);
+    const double rz0 = st->rz[slot];
+    const bool curv = !(pap > 1e-30) || !isfinite(pap);
+    const double alpha = rz0 / pap;
+    const bool step = !curv && (!isfinite(alpha) || alpha <= 0.0);
+    if (curv || step) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            halt_with(st, curv ? MYCEL_PCG_CURVATURE_BREAKDOWN : MYCEL_PCG_STEP_BREAKDOWN, st->iter[slot],
+                      st->prev_rel[slot]);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[slot] = alpha;
+    double rr = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = base + 256 * q;
+        if (i < n) {
+            x[i] = fmad(alpha, pv[q], xv[q]);
+            const double ri = fmad(-alpha, av[q], rv[q]);
+            r[i] = ri;
+            rr = __fma_rn(ri, ri, rr);
+        }
+    }
+    rr = block_sum256(rr, red);
+    if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+}
+
+// Flexible PCG, step 4: relres, convergence and stagnation as PcgSolver.cs:279-313; beta = -alpha z.Ap / rz
+// (clamped at 0); p = z + beta p.
+__global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState *__restrict__ st,
+                                                       const double *__restrict__ part_rr, int nb_vec,
+                                                       const double *__restrict__ part_zr,
+                                                       const double *__restrict__ part_zap, int nb_z,
+                                                       double *__restrict__ p, const double *__restrict__ z) {
+    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+    double pv[4], zv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = min(base + 256 * q, n - 1);
+        pv[q] = p[i];
+        zv[q] = z[i];
+    }
+    if (st->halt) return;
+    const double rr = wave_sum_partials(part_rr, nb_vec);
+    const double zr = wave_sum_partials(part_zr, nb_z);
+    const double zap = wave_sum_partials(part_zap, nb_z);
+    const double rel = sqrt(rr) / st->bnorm;
+    const int t = st->iter[slot];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (rel <= st->tol) {
+        if (lead) halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
+        return;
+    }
+    const int stag = (fabs(rel - st->prev_rel[slot]) < 1e-15 * rel) ? st->stag[slot] + 1 : 0;
+    if (stag > 50) {
+        if (lead) halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
+        return;
+    }
+    double beta = -st->alpha[slot] * zap / st->rz[slot];
+    if (!isfinite(beta)) {
+        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        return;
+    }
+    beta = fmax(beta, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = base + 256 * q;
+        if (i < n) p[i] = fmad(beta, pv[q], zv[q]);
+    }
+    if (lead) {
+        const int o = slot ^ 1;
+        st->rz[o] = zr;
+        st->prev_rel[o] = rel;
+        st->stag[o] = stag;
+        st->iter[o] = t + 1;
+        st->last_rel = rel;
+        st->last_iter = t;
+        if (t >= st->max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);
+    }
+}
+
+// p = z (the first search direction of the preconditioned iteration).
+__global__ __launch_bounds__(256) void k_copy(int n, const double *__restrict__ src, double *__restrict__ dst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // x[constrained] = 0; flag a nonzero projected x; partials of ||Pb||^2 (PcgSolver.cs:113-137).
@@ -401,6 +520,23 @@ void enqueue_iteration32(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
                        w->nb_vec, w->p32.get(), w->z32.get());
 }
 
+void ensure_gmg(mycel_op *op, PcgWork *w) {
+    if (w->part_zr.size()) return;
+    w->nb_z = gmg_partials(op);
+    w->part_zr.resize(w->nb_z);
+    w->part_zap.resize(w->nb_z);
+}
+
+void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
+    PcgState *st = w->state.get();
+    launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
+    hipLaunchKernelGGL(k_fpcg_update_xr, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+                       w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->part2.get());
+    gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s);
+    hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(), w->nb_vec,
+                       w->part_zr.get(), w->part_zap.get(), w->nb_z, w->p.get(), w->z.get());
+}
+
 void read_state(PcgWork *w, hipStream_t s) {
     MYCEL_HIP(hipMemcpyAsync(w->h_state, w->state.get(), sizeof(PcgState), hipMemcpyDeviceToHost, s));
     MYCEL_HIP(hipStreamSynchronize(s));
@@ -424,6 +560,7 @@ void pcg_free(PcgWork *w) {
     if (!w) return;
     if (w->graph64) (void)hipGraphExecDestroy(w->graph64);
     if (w->graph32) (void)hipGraphExecDestroy(w->graph32);
+    if (w->graph_gmg) (void)hipGraphExecDestroy(w->graph_gmg);
     if (w->h_state) (void)hipHostFree(w->h_state);
     delete w;
 }
@@ -436,9 +573,11 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
     *res = mycel_pcg_result{};
-    if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32)
+    if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32 &&
+        precision != MYCEL_PRECISION_MIXED_GMG)
         throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
     const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32;
+    const bool gmg = precision == MYCEL_PRECISION_MIXED_GMG;
     if (op->active_fixed < 6)  // PcgSolver.cs:95-100
         throw Error(MYCEL_E_INVALID_OPERATION,
                     "Model has fewer than 6 unique fixed DOFs on active nodes (found " +
@@ -447,6 +586,10 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     ensure_work(op);
     PcgWork *w = op->pcg;
     if (mixed) ensure_mixed(w);
+    if (gmg) {
+        gmg_prepare(op, op->stream);
+        ensure_gmg(op, w);
+    }
     hipStream_t s = op->stream;
     PcgState *st = w->state.get();
     const int n = w->n;
@@ -471,7 +614,11 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     hipLaunchKernelGGL(k_pcg_init_r, gv, b256, 0, s, n, op->d_flags.get(), d_b, warm ? w->ap.get() : nullptr,
                        w->r.get(), w->part1.get());
     hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->r_norm_sq);
-    if (mixed) {
+    if (gmg) {
+        gmg_enqueue_vcycle(op, w->r.get(), w->r.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s);
+        hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), b256, 0, s, n, w->z.get(), w->p.get());
+        hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part_zr.get(), w->nb_z, &st->rz0);
+    } else if (mixed) {
         hipLaunchKernelGGL(k_pcg_init_z<float>, gv, b256, 0, s, n, w->r.get(), w->r32.get(), op->d_inv_diag32.get(),
                            w->z32.get(), w->p32.get(), w->part1.get());
         MYCEL_HIP(hipMemsetAsync(w->x32.get(), 0, sizeof(float) * n, s));
@@ -503,7 +650,20 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     MYCEL_LAUNCH_CHECK();
 
     hipGraphExec_t graph;
-    if (mixed) {
+    if (gmg) {
+        if (w->graph_gmg && (w->graph_gmg_x != d_x || w->graph_gmg_h != op->gmg)) {
+            MYCEL_HIP(hipGraphExecDestroy(w->graph_gmg));
+            w->graph_gmg = nullptr;
+        }
+        if (!w->graph_gmg) {
+            w->graph_gmg = capture(s, [&] {
+                for (int t = 0; t < kGmgGraphIters; t++) enqueue_iteration_gmg(op, w, d_x, t & 1, s);
+            });
+            w->graph_gmg_x = d_x;
+            w->graph_gmg_h = op->gmg;
+        }
+        graph = w->graph_gmg;
+    } else if (mixed) {
         if (!w->graph32)
             w->graph32 = capture(s, [&] {
                 for (int t = 0; t < kGraphIters; t++) enqueue_iteration32(op, w, t & 1, s);
@@ -560,7 +720,22 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     res->message = h.message;
     res->converged = h.message == MYCEL_PCG_CONVERGED;
     res->outer_corrections = h.outer;
-    if (mixed) {
+    if (gmg) {
+        const bool broke = h.message == MYCEL_PCG_CURVATURE_BREAKDOWN || h.message == MYCEL_PCG_STEP_BREAKDOWN ||
+                           h.message == MYCEL_PCG_DIRECTION_BREAKDOWN || h.message == MYCEL_PCG_STAGNATION;
+        if (broke && h.iterations < max_iter) {
+            // The FP32 preconditioner lost positivity on this residual: x is a valid FP64 iterate, so finish
+            // with the FP64 Jacobi iteration warm-started from it.
+            mycel_pcg_result r64{};
+            pcg_solve_device(op, d_b, d_x, tol, max_iter - h.iterations, MYCEL_PRECISION_FP64, cancel, cb, user,
+                             &r64);
+            res->iterations = h.iterations + r64.iterations;
+            res->final_residual = r64.final_residual;
+            res->message = r64.message;
+            res->converged = r64.converged;
+            res->fallback_iterations = r64.iterations;
+        }
+    } else if (mixed) {
         const bool broke = h.message == MYCEL_PCG_CURVATURE_BREAKDOWN || h.message == MYCEL_PCG_STEP_BREAKDOWN ||
                            h.message == MYCEL_PCG_DIRECTION_BREAKDOWN || h.message == MYCEL_PCG_STAGNATION;
         if (broke && h.iterations < max_iter) {

@@ -296,6 +296,10 @@ class FemOperator:
         return out
 
     # Device-resident forms (torch tensors on the GPU; used by the benchmark and serving paths).
+    def update_densities_device(self, d_t, penal, stream=None):
+        """UpdateDensities from a box-sized float64 device tensor (enqueued on the operator's stream)."""
+        L.check(L.load().mycel_op_update_densities_d(self._h, L.device_ptr(d_t), float(penal), stream))
+
     def apply_k_device(self, x_t, y_t, stream=None):
         L.check(L.load().mycel_op_apply_k_d(self._h, L.device_ptr(x_t), L.device_ptr(y_t), stream))
 
@@ -310,6 +314,12 @@ class FemOperator:
 class JacobiPreconditioner:
     """Point Jacobi z = r / diag(A) (Fem/JacobiPreconditioner.cs:9-36). The GPU solver applies it fused
     into its vector kernels, as the reference special-cases Jacobi (Fem/PcgSolver.cs:213-226)."""
+
+
+class MultigridPreconditioner:
+    """A second IPreconditioner (Fem/IPreconditioner.cs:9-20): one FP32 geometric-multigrid V-cycle per
+    iteration (2x coarsening, Chebyshev-Jacobi smoothing, dense coarsest solve) inside an FP64 flexible PCG.
+    Passing it to PcgSolver.Solve selects PRECISION_MIXED_GMG."""
 
 
 @dataclass
@@ -331,8 +341,21 @@ def _pcg_result(r: L.PcgResultC) -> PcgResult:
                      r.message, r.outer_corrections, r.fallback_iterations)
 
 
+def _precision_for(preconditioner, precision):
+    """The solver mode a preconditioner object selects (None and Jacobi keep `precision`)."""
+    if preconditioner is None:
+        return precision
+    if isinstance(preconditioner, JacobiPreconditioner):
+        if precision == L.PRECISION_MIXED_GMG:
+            raise L.ArgumentError(L.E_ARGUMENT, "PRECISION_MIXED_GMG is the multigrid preconditioner, not Jacobi")
+        return precision
+    if isinstance(preconditioner, MultigridPreconditioner):
+        return L.PRECISION_MIXED_GMG
+    raise L.ArgumentError(L.E_ARGUMENT, "preconditioner must be JacobiPreconditioner or MultigridPreconditioner")
+
+
 class PcgSolver:
-    """Jacobi-preconditioned CG on A (Fem/PcgSolver.cs:16-554), device-resident."""
+    """Preconditioned CG on A (Fem/PcgSolver.cs:16-554), device-resident."""
 
     def Solve(self, op: FemOperator, b, x, preconditioner=None, tol=1e-6, maxIterations=5000, progress=None,
               cancel=None, precision=L.PRECISION_FP64):
@@ -340,8 +363,7 @@ class PcgSolver:
         between device batches; cancel is an optional ctypes c_int32 flag."""
         if op is None:
             raise L.ArgumentNullError(L.E_ARGUMENT_NULL, "op")
-        if preconditioner is not None and not isinstance(preconditioner, JacobiPreconditioner):
-            raise L.ArgumentError(L.E_ARGUMENT, "only the Jacobi preconditioner is implemented on the GPU")
+        precision = _precision_for(preconditioner, precision)
         b = op._vec(b)
         if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags["C_CONTIGUOUS"] and x.size == op._n):
             raise L.ArgumentError(L.E_ARGUMENT, "x must be a contiguous float64 array of length TotalDofs")
@@ -369,7 +391,7 @@ class StaticAnalysisOptions:
     Tolerance: float = 1e-6
     MaxIterations: int = 5000
     WarmStartDisplacements: Optional[np.ndarray] = None
-    Preconditioner: Optional[JacobiPreconditioner] = None
+    Preconditioner: Optional[object] = None  # JacobiPreconditioner or MultigridPreconditioner
     ThrowOnNonConvergence: bool = True
     Precision: int = L.PRECISION_FP64
 
@@ -416,7 +438,7 @@ class StaticAnalysis:
             res = L.PcgResultC()
             L.check(L.load().mycel_static_analysis(op._h, L.ptr(f), L.ptr(u), float(options.Tolerance),
                                                    int(options.MaxIterations), int(bool(options.ThrowOnNonConvergence)),
-                                                   int(options.Precision), C.byref(comp), C.byref(mx), L.ptr(react),
+                                                   int(_precision_for(options.Preconditioner, options.Precision)), C.byref(comp), C.byref(mx), L.ptr(react),
                                                    L.ptr(vm), C.byref(res)))
         finally:
             if own:
