@@ -48,21 +48,32 @@ __device__ __forceinline__ double sum_partials256(const double *__restrict__ par
 // The same fixed-order sum, formed by each wavefront on its own (no LDS, no barrier): lane l adds the
 // partials l, l + 64, ... and the lanes are then combined by the butterfly. Every wave of every block
 // gets the bit-identical value.
+// Four independent accumulators (lane l, stride 256, offsets 0/64/128/192) keep the dependent-add chain
+// short; they are combined as (a0 + a1) + (a2 + a3) before the butterfly.
 __device__ __forceinline__ double wave_sum_partials(const double *__restrict__ part, int n) {
-    double v = 0.0;
-    for (int i = threadIdx.x & 63; i < n; i += 64) v += part[i];
-    return wave_sum(v);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int i = threadIdx.x & 63; i < n; i += 256) {
+        a0 += part[i];
+        if (i + 64 < n) a1 += part[i + 64];
+        if (i + 128 < n) a2 += part[i + 128];
+        if (i + 192 < n) a3 += part[i + 192];
+    }
+    return wave_sum((a0 + a1) + (a2 + a3));
 }
 
 // Two interleaved partial arrays (part[2i], part[2i + 1]) summed per wave as above.
 __device__ __forceinline__ void wave_sum_partials2(const double *__restrict__ part, int n, double &s0, double &s1) {
-    double v0 = 0.0, v1 = 0.0;
-    for (int i = threadIdx.x & 63; i < n; i += 64) {
-        v0 += part[2 * i];
-        v1 += part[2 * i + 1];
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+    for (int i = threadIdx.x & 63; i < n; i += 128) {
+        a0 += part[2 * i];
+        b0 += part[2 * i + 1];
+        if (i + 64 < n) {
+            a1 += part[2 * (i + 64)];
+            b1 += part[2 * (i + 64) + 1];
+        }
     }
-    s0 = wave_sum(v0);
-    s1 = wave_sum(v1);
+    s0 = wave_sum(a0 + a1);
+    s1 = wave_sum(b0 + b1);
 }
 
 // Two fixed-order block sums with one LDS round trip.

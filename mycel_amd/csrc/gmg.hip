@@ -28,6 +28,7 @@
 // the inverse; each V-cycle then applies it as a dense product. Everything is deterministic: no atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 
@@ -42,6 +43,7 @@ namespace {
 constexpr int kCoarseMaxDofs = 140;  // 140^2 doubles = 156,800 B of LDS (gfx950: 160 KiB per workgroup)
 constexpr int kMaxLevels = 16;
 constexpr double kSmoothRange = 30.0;  // Chebyshev interval [lmax / 30, lmax]
+constexpr int kFineDegree = 3;          // Chebyshev degree on the finest level (2 on the coarser ones)
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -52,56 +54,32 @@ __device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, i
     k = t / g.ny1;
 }
 
-// One Chebyshev step on level g: r = b - K x; d' = cd d + cr inv r; x' = x + d'. FINAL (finest level, last
-// step) writes x' as the FP64 preconditioned residual z and per-block partials of z.r64 and z.ap64 (the
-// flexible-CG inner products); otherwise x' goes to x_out and d' back to d.
-template <bool FINAL>
-__global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, const float *__restrict__ ep,
-                                                     const float *__restrict__ x, const float *__restrict__ b,
-                                                     float *__restrict__ d, const float *__restrict__ inv, float cd,
-                                                     float cr, float *__restrict__ x_out, double *__restrict__ z64,
-                                                     const double *__restrict__ r64, const double *__restrict__ ap64,
-                                                     double *__restrict__ part_zr, double *__restrict__ part_zap) {
-    __shared__ double red[8];
-    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    double zr = 0.0, zap = 0.0;
-    if (n < g.n_nodes) {
-        int i, j, k;
-        node_ijk(g, n, i, j, k);
-        float kx[3], xc[3];
-        ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, kx, xc);
+// Per-node bodies of the V-cycle phases, shared by the one-launch-per-phase kernels of the large levels and
+// by the single-workgroup tail kernel of the small ones.
+
+// One Chebyshev step at node n: r = b - K x; d' = cd d + cr inv r; x' = x + d' (x' to x_out, d' back to d).
+__device__ __forceinline__ void cheb_node(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
+                                          const float *__restrict__ x, const float *__restrict__ b,
+                                          float *__restrict__ d, const float *__restrict__ inv, float cd, float cr,
+                                          float *__restrict__ x_out, int n) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    float kx[3], xc[3];
+    ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, kx, xc);
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int dof = 3 * n + c;
-            const float r = b[dof] - kx[c];
-            const float dn = cd * d[dof] + cr * (inv[dof] * r);
-            const float xn = xc[c] + dn;
-            if (FINAL) {
-                const double z = static_cast<double>(xn);
-                z64[dof] = z;
-                zr = __fma_rn(z, r64[dof], zr);
-                zap = __fma_rn(z, ap64[dof], zap);
-            } else {
-                x_out[dof] = xn;
-                d[dof] = dn;
-            }
-        }
-    }
-    if (FINAL) {
-        block_sum256x2(zr, zap, red);
-        if (threadIdx.x == 0) {
-            part_zr[blockIdx.x] = zr;
-            part_zap[blockIdx.x] = zap;
-        }
+    for (int c = 0; c < 3; c++) {
+        const int dof = 3 * n + c;
+        const float r = b[dof] - kx[c];
+        const float dn = cd * d[dof] + cr * (inv[dof] * r);
+        x_out[dof] = xc[c] + dn;
+        d[dof] = dn;
     }
 }
 
 // r = b - K x on unconstrained DOFs, 0 on constrained ones.
-__global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks, const float *__restrict__ ep,
-                                                         const float *__restrict__ x, const float *__restrict__ b,
-                                                         const float *__restrict__ inv, float *__restrict__ r) {
-    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (n >= g.n_nodes) return;
+__device__ __forceinline__ void residual_node(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
+                                              const float *__restrict__ x, const float *__restrict__ b,
+                                              const float *__restrict__ inv, float *__restrict__ r, int n) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
     float kx[3], xc[3];
@@ -113,28 +91,10 @@ __global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks
     }
 }
 
-// b0 = float(r64) on unconstrained DOFs; the finest pre-smoother's first step x = d = inv b0 / theta.
-__global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__restrict__ r64,
-                                                       const float *__restrict__ inv, float theta_inv,
-                                                       float *__restrict__ b, float *__restrict__ x,
-                                                       float *__restrict__ d) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float iv = inv[i];
-    const float bi = iv > 0.0f ? static_cast<float>(r64[i]) : 0.0f;
-    const float xi = (iv * bi) * theta_inv;
-    b[i] = bi;
-    x[i] = xi;
-    d[i] = xi;
-}
-
-// b_c = P^T r_f, then the coarse pre-smoother's first step x_c = d_c = inv_c b_c / theta.
-__global__ __launch_bounds__(256) void k_gmg_restrict(Dims gf, const float *__restrict__ rf, Dims gc,
-                                                      const float *__restrict__ invc, float theta_inv,
-                                                      float *__restrict__ bc, float *__restrict__ xcv,
-                                                      float *__restrict__ dc) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= gc.n_nodes) return;
+// b_c = P^T r_f at coarse node n, then the coarse pre-smoother's first step x_c = d_c = inv_c b_c / theta.
+__device__ __forceinline__ void restrict_node(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
+                                              const float *__restrict__ invc, float theta_inv, float *__restrict__ bc,
+                                              float *__restrict__ xcv, float *__restrict__ dc, int n) {
     int I, J, K;
     node_ijk(gc, n, I, J, K);
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
@@ -170,24 +130,20 @@ __global__ __launch_bounds__(256) void k_gmg_restrict(Dims gf, const float *__re
     }
 }
 
-// x_f += P x_c on unconstrained fine DOFs (trilinear: weight 1 at an even fine index, 1/2 from each
-// neighbour at an odd one).
-__global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict__ xf, const float *__restrict__ invf,
-                                                     Dims gc, const float *__restrict__ xcv) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= gf.n_nodes) return;
+// x_f += P x_c at fine node n, unconstrained DOFs only (trilinear: weight 1 at an even fine index, 1/2 from
+// each neighbour at an odd one).
+__device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__ xf, const float *__restrict__ invf,
+                                             const Dims &gc, const float *__restrict__ xcv, int n) {
     int i, j, k;
     node_ijk(gf, n, i, j, k);
-    const int i0 = i >> 1, i1 = (i + 1) >> 1, j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
-    const float wi = (i & 1) ? 0.5f : 1.0f, wj = (j & 1) ? 0.5f : 1.0f, wk = (k & 1) ? 0.5f : 1.0f;
-    const int is[2] = {i0, i1}, js[2] = {j0, j1}, ks_[2] = {k0, k1};
+    const int is[2] = {i >> 1, (i + 1) >> 1}, js[2] = {j >> 1, (j + 1) >> 1}, ks_[2] = {k >> 1, (k + 1) >> 1};
     const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    const float w = ((i & 1) ? 0.5f : 1.0f) * ((j & 1) ? 0.5f : 1.0f) * ((k & 1) ? 0.5f : 1.0f);
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
     for (int c = 0; c < nk; c++)
         for (int b = 0; b < nj; b++)
             for (int a = 0; a < ni; a++) {
                 const int m = 3 * (is[a] + gc.nx1 * (js[b] + gc.ny1 * ks_[c]));
-                const float w = wi * wj * wk;
                 s0 += w * xcv[m];
                 s1 += w * xcv[m + 1];
                 s2 += w * xcv[m + 2];
@@ -197,6 +153,151 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
     for (int c = 0; c < 3; c++) {
         const int dof = 3 * n + c;
         if (invf[dof] > 0.0f) xf[dof] += s[c];
+    }
+}
+
+// One Chebyshev step (one node per thread).
+__global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                     const float *__restrict__ x, const float *__restrict__ b,
+                                                     float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                     float cr, float *__restrict__ x_out) {
+    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (n < g.n_nodes) cheb_node(g, ks, ep, x, b, d, inv, cd, cr, x_out, n);
+}
+
+// The last post-smoothing step on the finest level: writes x' as the FP64 preconditioned residual z and
+// per-block partials of z.r64 and z.ap64 (the flexible-CG inner products).
+__global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                           const float *__restrict__ x, const float *__restrict__ b,
+                                                           const float *__restrict__ d, const float *__restrict__ inv,
+                                                           float cd, float cr, double *__restrict__ z64,
+                                                           const double *__restrict__ r64,
+                                                           const double *__restrict__ ap64,
+                                                           double *__restrict__ part_zr, double *__restrict__ part_zap) {
+    __shared__ double red[8];
+    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    double zr = 0.0, zap = 0.0;
+    if (n < g.n_nodes) {
+        int i, j, k;
+        node_ijk(g, n, i, j, k);
+        float kx[3], xc[3];
+        ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, kx, xc);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            const float r = b[dof] - kx[c];
+            const float dn = cd * d[dof] + cr * (inv[dof] * r);
+            const double z = static_cast<double>(xc[c] + dn);
+            z64[dof] = z;
+            zr = __fma_rn(z, r64[dof], zr);
+            zap = __fma_rn(z, ap64[dof], zap);
+        }
+    }
+    block_sum256x2(zr, zap, red);
+    if (threadIdx.x == 0) {
+        part_zr[blockIdx.x] = zr;
+        part_zap[blockIdx.x] = zap;
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                         const float *__restrict__ x, const float *__restrict__ b,
+                                                         const float *__restrict__ inv, float *__restrict__ r) {
+    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (n < g.n_nodes) residual_node(g, ks, ep, x, b, inv, r, n);
+}
+
+// b0 = float(r64) on unconstrained DOFs; the finest pre-smoother's first step x = d = inv b0 / theta.
+__global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__restrict__ r64,
+                                                       const float *__restrict__ inv, float theta_inv,
+                                                       float *__restrict__ b, float *__restrict__ x,
+                                                       float *__restrict__ d) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float iv = inv[i];
+    const float bi = iv > 0.0f ? static_cast<float>(r64[i]) : 0.0f;
+    const float xi = (iv * bi) * theta_inv;
+    b[i] = bi;
+    x[i] = xi;
+    d[i] = xi;
+}
+
+__global__ __launch_bounds__(256) void k_gmg_restrict(Dims gf, const float *__restrict__ rf, Dims gc,
+                                                      const float *__restrict__ invc, float theta_inv,
+                                                      float *__restrict__ bc, float *__restrict__ xcv,
+                                                      float *__restrict__ dc) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n < gc.n_nodes) restrict_node(gf, rf, gc, invc, theta_inv, bc, xcv, dc, n);
+}
+
+__global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict__ xf, const float *__restrict__ invf,
+                                                     Dims gc, const float *__restrict__ xcv) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n < gf.n_nodes) prolong_node(gf, xf, invf, gc, xcv, n);
+}
+
+// The small levels of the V-cycle in one workgroup: every phase is a loop over the level's nodes, phases
+// are separated by workgroup barriers (the waves of one workgroup share the CU's L1, so a barrier orders
+// their global writes and reads). lv[0] arrives with b, x = d = inv b / theta set by the restriction from
+// the level above; lv[count-1] is the coarsest level, solved with its dense inverse. On return the
+// corrected lv[0] iterate is in x2 (three buffer swaps: one pre- and two post-smoothing steps).
+constexpr int kMaxTail = 8;
+constexpr int kTailNodes = 512;  // levels up to this many nodes run in the tail
+struct TailLevel {
+    Dims d;
+    KeSym<float> ks;
+    const float *ep;
+    const float *inv;
+    float *b, *x, *x2, *dv, *r;
+};
+struct TailArgs {
+    TailLevel lv[kMaxTail];
+    int count;
+    const double *ainv;
+    float theta_inv, cd1, cr1;
+};
+
+__global__ __launch_bounds__(1024) void k_gmg_tail(TailArgs t) {
+    // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
+    unsigned flip = 0;
+    auto cur = [&](int l) { return ((flip >> l) & 1u) ? t.lv[l].x2 : t.lv[l].x; };
+    auto nxt = [&](int l) { return ((flip >> l) & 1u) ? t.lv[l].x : t.lv[l].x2; };
+    const int last = t.count - 1;
+    for (int l = 0; l < last; l++) {
+        const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
+            cheb_node(L.d, L.ks, L.ep, cur(l), L.b, L.dv, L.inv, t.cd1, t.cr1, nxt(l), n);
+        __syncthreads();
+        flip ^= 1u << l;
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
+            residual_node(L.d, L.ks, L.ep, cur(l), L.b, L.inv, L.r, n);
+        __syncthreads();
+        for (int n = threadIdx.x; n < C.d.n_nodes; n += blockDim.x)
+            restrict_node(L.d, L.r, C.d, C.inv, t.theta_inv, C.b, cur(l + 1), C.dv, n);
+        __syncthreads();
+    }
+    {
+        const TailLevel &T = t.lv[last];
+        const int n = T.d.n_dofs;
+        float *xo = cur(last);
+        for (int row = threadIdx.x; row < n; row += blockDim.x) {
+            double s = 0.0;
+            for (int j = 0; j < n; j++) s = __fma_rn(t.ainv[row * n + j], static_cast<double>(T.b[j]), s);
+            xo[row] = static_cast<float>(s);
+        }
+        __syncthreads();
+    }
+    for (int l = last - 1; l >= 0; l--) {
+        const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x) prolong_node(L.d, cur(l), L.inv, C.d, cur(l + 1), n);
+        __syncthreads();
+        for (int pass = 0; pass < 2; pass++) {
+            const float cd = pass == 0 ? 0.0f : t.cd1, cr = pass == 0 ? t.theta_inv : t.cr1;
+            for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
+                cheb_node(L.d, L.ks, L.ep, cur(l), L.b, L.dv, L.inv, cd, cr, nxt(l), n);
+            __syncthreads();
+            flip ^= 1u << l;
+        }
     }
 }
 
@@ -250,65 +351,163 @@ __global__ __launch_bounds__(256) void k_gmg_inv_coarse(Dims gc, const float *__
     }
 }
 
-// Coarsest level: assemble the dense matrix (identity rows and zero columns on constrained DOFs) into LDS,
-// invert it in place by Gauss-Jordan elimination and store the inverse. One workgroup of 1024 threads.
+// Galerkin blocks of the second-coarsest level: for each of its elements e, B_e = E_e P_e^T Ke P_e (24 x 24),
+// where P_e maps the 8 coarsest nodes around e (local index q = qx + 2 qy + 4 qz, origin (ex/2, ey/2, ez/2))
+// to e's 8 corners with the trilinear weights, and rows of constrained DOFs of e are zero. One block per
+// element.
+__global__ __launch_bounds__(256) void k_gmg_galerkin_blocks(Dims gs, const float *__restrict__ eps,
+                                                             const float *__restrict__ invs,
+                                                             const double *__restrict__ ke, double scale,
+                                                             double *__restrict__ blocks) {
+    __shared__ double w[8][8];  // w[a][q]: weight of coarse node q at corner a
+    __shared__ double T[576];
+    __shared__ double mask[24];
+    const int e = blockIdx.x;
+    const int ex = e % gs.nx, t = e / gs.nx, ey = t % gs.ny, ez = t / gs.ny;
+    const double E = static_cast<double>(eps[(ex + 1) + gs.px * ((ey + 1) + gs.py * (ez + 1))]);
+    double *out = blocks + static_cast<size_t>(e) * 576;
+    if (threadIdx.x < 64) {
+        const int a = threadIdx.x >> 3, q = threadIdx.x & 7;
+        const int d[3] = {ex + corner_x(a) - 2 * ((ex >> 1) + (q & 1)), ey + corner_y(a) - 2 * ((ey >> 1) + ((q >> 1) & 1)),
+                          ez + corner_z(a) - 2 * ((ez >> 1) + (q >> 2))};
+        double v = 1.0;
+        for (int k = 0; k < 3; k++) v *= d[k] == 0 ? 1.0 : (d[k] == 1 || d[k] == -1) ? 0.5 : 0.0;
+        w[a][q] = v;
+    } else if (threadIdx.x < 88) {
+        const int sdof = threadIdx.x - 64, a = sdof / 3, c = sdof % 3;
+        const int node = (ex + corner_x(a)) + gs.nx1 * ((ey + corner_y(a)) + gs.ny1 * (ez + corner_z(a)));
+        mask[sdof] = invs[3 * node + c] > 0.0f ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (E == 0.0) {
+        for (int k = threadIdx.x; k < 576; k += 256) out[k] = 0.0;
+        return;
+    }
+    for (int k = threadIdx.x; k < 576; k += 256) {  // T = Ke P_e
+        const int r = k / 24, col = k % 24, q = col / 3, c = col % 3;
+        double v = 0.0;
+        for (int b = 0; b < 8; b++) v += ke[r * 24 + 3 * b + c] * (w[b][q] * mask[3 * b + c]);
+        T[k] = v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 576; k += 256) {  // B = E P_e^T T
+        const int row = k / 24, col = k % 24, q = row / 3, c = row % 3;
+        double v = 0.0;
+        for (int a = 0; a < 8; a++) v += (w[a][q] * mask[3 * a + c]) * T[(3 * a + c) * 24 + col];
+        out[k] = E * scale * v;
+    }
+}
+
+// Coarsest level: assemble the dense matrix into LDS (one row per thread, fixed order; identity rows and zero
+// columns on constrained DOFs), invert it by Gauss-Jordan elimination (SPD, no pivoting) and store the
+// inverse. With `blocks` the matrix is the Galerkin product P^T A_s P summed from the second-coarsest
+// level's element blocks; without (a single-level hierarchy) it is assembled from Ke and the level's moduli.
+// The elimination keeps the matrix in registers: thread (ti, tj) = (t / 8, t % 8) owns rows ti + 128 s and
+// columns tj + 8 c; each step broadcasts the pivot column and the scaled pivot row through LDS.
 __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float *__restrict__ ep,
                                                             const double *__restrict__ ke, double scale,
-                                                            const float *__restrict__ inv, double *__restrict__ ainv) {
+                                                            const float *__restrict__ inv, Dims gs,
+                                                            const double *__restrict__ blocks,
+                                                            double *__restrict__ ainv) {
     extern __shared__ __align__(16) double A[];
-    __shared__ double col[kCoarseMaxDofs];
     const int n = g.n_dofs;
     for (int q = threadIdx.x; q < n * n; q += blockDim.x) A[q] = 0.0;
     __syncthreads();
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         const int node = r / 3, c = r % 3;
+        double *row = A + r * n;
         if (!(inv[r] > 0.0f)) {
-            A[r * n + r] = 1.0;
+            row[r] = 1.0;
             continue;
         }
         int I, J, K;
         node_ijk(g, node, I, J, K);
-        for (int a = 0; a < 8; a++) {  // the element that holds this node as its corner a
-            const int ox = I - corner_x(a), oy = J - corner_y(a), oz = K - corner_z(a);
-            const double E = static_cast<double>(ep[(ox + 1) + g.px * ((oy + 1) + g.py * (oz + 1))]);
-            if (E == 0.0) continue;
-            for (int b = 0; b < 8; b++) {
-                const int m = (ox + corner_x(b)) + g.nx1 * ((oy + corner_y(b)) + g.ny1 * (oz + corner_z(b)));
-                for (int d = 0; d < 3; d++) {
-                    const int col_dof = 3 * m + d;
-                    if (!(inv[col_dof] > 0.0f)) continue;
-                    A[r * n + col_dof] += E * scale * ke[(3 * a + c) * 24 + 3 * b + d];
+        if (blocks) {
+            for (int ez = max(2 * K - 2, 0); ez <= min(2 * K + 1, gs.nz - 1); ez++)
+                for (int ey = max(2 * J - 2, 0); ey <= min(2 * J + 1, gs.ny - 1); ey++)
+                    for (int ex = max(2 * I - 2, 0); ex <= min(2 * I + 1, gs.nx - 1); ex++) {
+                        const int bx = ex >> 1, by = ey >> 1, bz = ez >> 1;
+                        const int q1 = (I - bx) + 2 * (J - by) + 4 * (K - bz);
+                        const double *blk =
+                            blocks + static_cast<size_t>(ex + gs.nx * (ey + gs.ny * ez)) * 576 + (3 * q1 + c) * 24;
+                        for (int q2 = 0; q2 < 8; q2++) {
+                            const int m = (bx + (q2 & 1)) + g.nx1 * ((by + ((q2 >> 1) & 1)) + g.ny1 * (bz + (q2 >> 2)));
+                            for (int d = 0; d < 3; d++)
+                                if (inv[3 * m + d] > 0.0f) row[3 * m + d] += blk[3 * q2 + d];
+                        }
+                    }
+        } else {
+            for (int a = 0; a < 8; a++) {  // the element that holds this node as its corner a
+                const int ox = I - corner_x(a), oy = J - corner_y(a), oz = K - corner_z(a);
+                const double E = static_cast<double>(ep[(ox + 1) + g.px * ((oy + 1) + g.py * (oz + 1))]);
+                if (E == 0.0) continue;
+                for (int b = 0; b < 8; b++) {
+                    const int m = (ox + corner_x(b)) + g.nx1 * ((oy + corner_y(b)) + g.ny1 * (oz + corner_z(b)));
+                    for (int d = 0; d < 3; d++)
+                        if (inv[3 * m + d] > 0.0f) row[3 * m + d] += E * scale * ke[(3 * a + c) * 24 + 3 * b + d];
                 }
             }
         }
+        if (!(row[r] > 0.0)) row[r] = 1.0;  // a free DOF whose Galerkin row vanished: decouple it
     }
     __syncthreads();
+    constexpr int RS = 2, CS = (kCoarseMaxDofs + 7) / 8;
+    const int ti = threadIdx.x >> 3, tj = threadIdx.x & 7;
+    double a[RS][CS];
+#pragma unroll
+    for (int s = 0; s < RS; s++)
+#pragma unroll
+        for (int c = 0; c < CS; c++) {
+            const int i = ti + 128 * s, j = tj + 8 * c;
+            a[s][c] = (i < n && j < n) ? A[i * n + j] : 0.0;
+        }
+    __syncthreads();  // A's LDS now holds the broadcast buffers
+    double *colk = A, *rowk = A + 2 * kCoarseMaxDofs;
     for (int k = 0; k < n; k++) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = A[i * n + k];
-        __syncthreads();
-        const double piv = 1.0 / col[k];
-        for (int j = threadIdx.x; j < n; j += blockDim.x) A[k * n + j] = (j == k) ? piv : A[k * n + j] * piv;
-        __syncthreads();
-        for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
-            const int i = q / n, j = q % n;
-            if (i == k) continue;
-            A[q] = (j == k) ? -col[i] * piv : A[q] - col[i] * A[k * n + j];
+        const int p = k & 1, ck = k >> 3;
+        double *cb = colk + p * kCoarseMaxDofs, *rb = rowk + p * kCoarseMaxDofs;
+        if (tj == (k & 7)) {
+#pragma unroll
+            for (int s = 0; s < RS; s++)
+#pragma unroll
+                for (int c = 0; c < CS; c++)
+                    if (c == ck && ti + 128 * s < n) cb[ti + 128 * s] = a[s][c];
         }
         __syncthreads();
+        const double piv = 1.0 / cb[k];
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            if (ti + 128 * s != k) continue;
+#pragma unroll
+            for (int c = 0; c < CS; c++) {
+                const int j = tj + 8 * c;
+                if (j >= n) continue;
+                const double v = (j == k) ? piv : a[s][c] * piv;
+                a[s][c] = v;
+                rb[j] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < RS; s++) {
+            const int i = ti + 128 * s;
+            if (i >= n || i == k) continue;
+            const double f = cb[i];
+#pragma unroll
+            for (int c = 0; c < CS; c++) {
+                const int j = tj + 8 * c;
+                if (j >= n) continue;
+                a[s][c] = (j == k) ? -f * piv : a[s][c] - f * rb[j];
+            }
+        }
     }
-    for (int q = threadIdx.x; q < n * n; q += blockDim.x) ainv[q] = A[q];
-}
-
-// x = A^-1 b on the coarsest level: one wavefront per row, FP64 accumulation.
-__global__ __launch_bounds__(256) void k_gmg_coarse_solve(int n, const double *__restrict__ ainv,
-                                                          const float *__restrict__ b, float *__restrict__ x) {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    double s = 0.0;
-    for (int j = lane; j < n; j += 64) s = __fma_rn(ainv[row * n + j], static_cast<double>(b[j]), s);
-    s = wave_sum(s);
-    if (lane == 0) x[row] = static_cast<float>(s);
+#pragma unroll
+    for (int s = 0; s < RS; s++)
+#pragma unroll
+        for (int c = 0; c < CS; c++) {
+            const int i = ti + 128 * s, j = tj + 8 * c;
+            if (i < n && j < n) ainv[i * n + j] = a[s][c];
+        }
 }
 
 }  // namespace
@@ -327,8 +526,10 @@ struct Gmg {
     GmgLevel lv[kMaxLevels];
     int levels = 0;
     DevBuf<double> ainv;
+    DevBuf<double> blocks;  // Galerkin element blocks of the second-coarsest level
     uint64_t epoch = ~0ull;
-    float theta_inv = 0, cd1 = 0, cr1 = 0;
+    float theta_inv = 0;
+    float cd[kFineDegree], cr[kFineDegree];  // Chebyshev step k >= 1: d' = cd[k] d + cr[k] D^-1 r
 };
 
 void gmg_free(Gmg *g) { delete g; }
@@ -350,10 +551,17 @@ static void gmg_build(mycel_op *op) {
         if (std::fabs(op->ke[r * 25] - op->ke[0]) > 1e-12 * std::fabs(op->ke[0]))
             throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: element matrix diagonal is not uniform");
     const double lmin = lmax / kSmoothRange, theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-    const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+    const double sigma = theta / delta;
     g->theta_inv = static_cast<float>(1.0 / theta);
-    g->cd1 = static_cast<float>(rho1 * rho0);
-    g->cr1 = static_cast<float>(2.0 * rho1 / delta);
+    double rho = 1.0 / sigma;
+    g->cd[0] = 0.0f;
+    g->cr[0] = g->theta_inv;
+    for (int k = 1; k < kFineDegree; k++) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        g->cd[k] = static_cast<float>(rn * rho);
+        g->cr[k] = static_cast<float>(2.0 * rn / delta);
+        rho = rn;
+    }
 
     Dims d = op->d;
     double scale = 1.0;
@@ -389,6 +597,7 @@ static void gmg_build(mycel_op *op) {
     }
     const int nc = g->lv[g->levels - 1].d.n_dofs;
     g->ainv.resize(static_cast<size_t>(nc) * nc);
+    if (g->levels >= 2) g->blocks.resize(static_cast<size_t>(g->lv[g->levels - 2].d.n_elem) * 576);
     op->gmg = owner.release();
 }
 
@@ -407,11 +616,20 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
                            P.inv.get(), L.inv.get());
     }
     GmgLevel &C = g->lv[g->levels - 1];
-    const size_t lds = sizeof(double) * C.d.n_dofs * C.d.n_dofs;
+    const int nc = C.d.n_dofs;
+    const size_t lds = sizeof(double) * std::max(nc * nc, 4 * kCoarseMaxDofs);
     MYCEL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gmg_coarse_factor),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k_gmg_coarse_factor, dim3(1), dim3(1024), lds, s, C.d, C.ep, op->d_ke.get(), C.scale,
-                       C.inv.get(), g->ainv.get());
+    if (g->levels >= 2) {  // Galerkin coarsest operator from the second-coarsest level's elements
+        GmgLevel &S = g->lv[g->levels - 2];
+        hipLaunchKernelGGL(k_gmg_galerkin_blocks, dim3(S.d.n_elem), b256, 0, s, S.d, S.ep, S.inv.get(),
+                           op->d_ke.get(), S.scale, g->blocks.get());
+        hipLaunchKernelGGL(k_gmg_coarse_factor, dim3(1), dim3(1024), lds, s, C.d, C.ep, op->d_ke.get(), C.scale,
+                           C.inv.get(), S.d, static_cast<const double *>(g->blocks.get()), g->ainv.get());
+    } else {
+        hipLaunchKernelGGL(k_gmg_coarse_factor, dim3(1), dim3(1024), lds, s, C.d, C.ep, op->d_ke.get(), C.scale,
+                           C.inv.get(), C.d, static_cast<const double *>(nullptr), g->ainv.get());
+    }
     MYCEL_LAUNCH_CHECK();
     g->epoch = op->epoch;
 }
@@ -427,39 +645,53 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         cur[l] = g->lv[l].x.get();
         nxt[l] = g->lv[l].x2.get();
     }
-    auto cheb = [&](int l, float cd, float cr, bool final_step) {
+    auto cheb = [&](int l, float cd, float cr) {
         GmgLevel &L = g->lv[l];
-        if (final_step) {
-            hipLaunchKernelGGL(k_gmg_cheb<true>, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.dv.get(), L.inv.get(), cd, cr, nxt[l], z64, r64, ap64, part_zr, part_zap);
-        } else {
-            hipLaunchKernelGGL(k_gmg_cheb<false>, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.dv.get(), L.inv.get(), cd, cr, nxt[l], static_cast<double *>(nullptr),
-                               static_cast<const double *>(nullptr), static_cast<const double *>(nullptr),
-                               static_cast<double *>(nullptr), static_cast<double *>(nullptr));
-            std::swap(cur[l], nxt[l]);
-        }
+        hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(), L.dv.get(),
+                           L.inv.get(), cd, cr, nxt[l]);
+        std::swap(cur[l], nxt[l]);
     };
+    // Levels [t0, top] run in the single-workgroup tail kernel (the finest level never does: it has the FP64
+    // entry and exit kernels).
+    int t0 = top;
+    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= kTailNodes) t0--;
+    if (top == 0) t0 = 0;
+    TailArgs ta{};
+    ta.count = top - t0 + 1;
+    if (ta.count > kMaxTail) throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: tail too deep");
+    for (int l = t0; l <= top; l++) {
+        GmgLevel &L = g->lv[l];
+        ta.lv[l - t0] = TailLevel{L.d, L.ks, L.ep, L.inv.get(), L.b.get(), L.x.get(), L.x2.get(), L.dv.get(), L.r.get()};
+    }
+    ta.ainv = g->ainv.get();
+    ta.theta_inv = g->theta_inv;
+    ta.cd1 = g->cd[1];
+    ta.cr1 = g->cr[1];
+
     GmgLevel &F = g->lv[0];
     hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_dofs, 256)), b256, 0, s, F.d.n_dofs, r64, F.inv.get(),
                        g->theta_inv, F.b.get(), cur[0], F.dv.get());
-    for (int l = 0; l < top; l++) {
+    for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
-        cheb(l, g->cd1, g->cr1, false);
+        for (int k = 1; k < (l == 0 ? kFineDegree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
         hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                            L.inv.get(), L.r.get());
         hipLaunchKernelGGL(k_gmg_restrict, dim3(C.blocks), b256, 0, s, L.d, L.r.get(), C.d, C.inv.get(),
                            g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
-    GmgLevel &T = g->lv[top];
-    hipLaunchKernelGGL(k_gmg_coarse_solve, dim3(cdiv(T.d.n_dofs, 4)), b256, 0, s, T.d.n_dofs, g->ainv.get(),
-                       T.b.get(), cur[top]);
-    for (int l = top - 1; l >= 0; l--) {
+    hipLaunchKernelGGL(k_gmg_tail, dim3(1), dim3(1024), 0, s, ta);
+    if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
+    for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks), b256, 0, s, L.d, cur[l], L.inv.get(), C.d, cur[l + 1]);
-        cheb(l, 0.0f, g->theta_inv, false);
-        cheb(l, g->cd1, g->cr1, l == 0);
+        cheb(l, 0.0f, g->theta_inv);
+        for (int k = 1; k < (l == 0 ? kFineDegree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k]);
     }
+    // Finest level: the last post-smoothing step writes z (FP64) and the partials. A single-level hierarchy
+    // (a grid of <= kCoarseMaxDofs DOFs) was solved directly above: the step is then the identity (cd = cr = 0).
+    const float cd = top == 0 ? 0.0f : g->cd[kFineDegree - 1], cr = top == 0 ? 0.0f : g->cr[kFineDegree - 1];
+    hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(), F.dv.get(),
+                       F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
     MYCEL_LAUNCH_CHECK();
 }
 
