@@ -14,6 +14,10 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   including a device copy of known size that calibrates the FETCH_SIZE / WRITE_SIZE counters.
 --mode serve: a stream of FEA and SIMP jobs with Poisson arrivals through the job server, against the
   same stream run one job at a time in plain FP64; reports jobs/s and p50/p99 latency.
+--mode cases: the batched multi-RHS SIMP configuration: 100x50x50 grid (788,103 DOFs), x = 0 clamped, 16
+  independent load cases (1-4 seeded 4x4-node patches each), 20 SIMP iterations on the summed compliance
+  (vf 0.25, rmin 2.0, p = 3, solves to 1e-8 warm-started per case) with the multigrid solver; one SIMP
+  iteration with the plain FP64 Jacobi solver is timed beside it. Reports SIMP iterations/s.
 --mode summarize: no GPU; reads rocprofv3 CSV output (--stats-dir, --fetch-dir, --write-dir) of a
   `--mode kernels` run and prints per-kernel durations and HBM bytes per launch. FETCH_SIZE and WRITE_SIZE
   (KiB per dispatch) are each scaled by the factor that makes the largest device copy of the run read and
@@ -42,7 +46,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "summarize"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "summarize"), default="solve")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
@@ -52,6 +56,9 @@ def parse():
     ap.add_argument("--workers", type=int, default=4, help="serve mode: resident jobs")
     ap.add_argument("--simp-iterations", type=int, default=10, help="serve mode: SIMP iterations per job")
     ap.add_argument("--max-elements", type=int, default=250_000, help="serve mode: largest job")
+    ap.add_argument("--cases-iterations", type=int, default=20, help="cases mode: SIMP iterations")
+    ap.add_argument("--serve-precision", choices=("gmg", "fp64"), default="gmg",
+                    help="serve mode: solver of the job server (the one-at-a-time baseline is always plain FP64)")
     ap.add_argument("--stats-dir", help="summarize: rocprofv3 --kernel-trace --stats output directory")
     ap.add_argument("--fetch-dir", help="summarize: rocprofv3 --pmc FETCH_SIZE output directory")
     ap.add_argument("--write-dir", help="summarize: rocprofv3 --pmc WRITE_SIZE output directory")
@@ -292,10 +299,15 @@ def mode_serve(args):
     from mycel_amd import workloads as W
     from mycel_amd.serve import DONE, JobServer
 
-    def options(j):
+    import mycel_amd as M
+
+    served = {"fp64": M.PRECISION_FP64, "gmg": M.PRECISION_MIXED_GMG}[args.serve_precision]
+
+    def options(j, precision=M.PRECISION_FP64):
         return OptimizerOptions(VolumeFraction=j.volume_fraction, FilterRadiusMm=j.rmin,
                                 MaxIterations=j.simp_iterations, SolverToleranceInitial=1e-8,
-                                SolverToleranceFinal=1e-8, SolverMaxIterations=500_000, ProtectBoundaryHinges=False)
+                                SolverToleranceFinal=1e-8, SolverMaxIterations=500_000, ProtectBoundaryHinges=False,
+                                Precision=precision)
 
     stream = W.serving_stream(args.jobs, seed=args.seed, max_el=args.max_elements,
                               simp_iterations=args.simp_iterations)
@@ -330,8 +342,8 @@ def mode_serve(args):
             now = time.perf_counter() - t0
             if a > now:
                 time.sleep(a - now)
-            ids.append(sv.submit_fea(m, tol=1e-8, max_iter=500_000, forces=f) if j.kind == "fea" else
-                       sv.submit_simp(m, options(j), forces=f))
+            ids.append(sv.submit_fea(m, tol=1e-8, max_iter=500_000, precision=served, forces=f) if j.kind == "fea"
+                       else sv.submit_simp(m, options(j, served), forces=f))
         for jid in ids:
             info = sv.wait(jid)
             if info.state != DONE:
@@ -349,9 +361,12 @@ def mode_serve(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.serve_precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"serving-stream-{len(stream)}-jobs", "workers": args.workers,
+                   "server_solver": {"fp64": "plain FP64 Jacobi-PCG",
+                                     "gmg": "FP64 flexible PCG + FP32 multigrid V-cycle"}[args.serve_precision],
+                   "baseline_solver": "plain FP64 Jacobi-PCG, one job at a time",
                    "elements": [20_000, args.max_elements], "simp_iterations": args.simp_iterations,
                    "offered_rate_jobs_per_s": rate},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
@@ -359,6 +374,59 @@ def mode_serve(args):
                           "latency_s": {"p50": float(np.percentile(seq_lat, 50)),
                                         "p99": float(np.percentile(seq_lat, 99))},
                           "mean_service_s": float(np.mean(service))},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def mode_cases(args):
+    import numpy as np
+    import torch
+
+    import mycel_amd as M
+    from mycel_amd import OptimizerOptions, TopologyOptimizer
+    from mycel_amd import workloads as W
+
+    p, cases = W.multi_rhs(seed=99)
+    f = W.case_forces(p, cases)
+    n_it = args.cases_iterations
+
+    def run(precision, iters):
+        o = OptimizerOptions(VolumeFraction=0.25, FilterRadiusMm=2.0, Penal=3.0, MoveLimit=0.2, MaxIterations=iters,
+                             SolverToleranceInitial=1e-8, SolverToleranceFinal=1e-8, SolverMaxIterations=500_000,
+                             ProtectBoundaryHinges=False, Precision=precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = TopologyOptimizer.RunCases(p.fe_model(), f, o)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    run(M.PRECISION_MIXED_GMG, 1)  # warm-up: module load, graph capture
+    el, r = run(M.PRECISION_MIXED_GMG, n_it)
+    el64, r64 = run(M.PRECISION_FP64, 1)
+    snaps = r.Snapshots
+    out = {
+        "metric": "SIMP iterations/s, 16 load cases on 100x50x50 (summed compliance)",
+        "value": len(snaps) / el,
+        "unit": "simp_iterations/s",
+        "n_gpus": 1,
+        "steps": len(snaps),
+        "warmup": 1,
+        "ms_per_step": el / len(snaps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "multi-rhs-100x50x50-16-cases", "grid": [p.nx, p.ny, p.nz], "box_dofs": p.n_dofs,
+                   "load_cases": len(cases), "volume_fraction": 0.25, "rmin": 2.0, "penal": 3.0, "tol": 1e-8,
+                   "solver": "FP64 flexible PCG + FP32 multigrid V-cycle, warm-started per case"},
+        "pcg_iterations_per_simp_iteration": [s.SolverIterations for s in snaps],
+        "ms_per_solve": el * 1e3 / (len(snaps) * len(cases)),
+        "compliance_history": [s.Compliance for s in snaps],
+        "fp64_jacobi_first_simp_iteration": {"s": el64, "pcg_iterations": r64.Snapshots[0].SolverIterations,
+                                             "compliance": r64.Snapshots[0].Compliance},
+        "first_iteration_compliance_rel_diff_vs_fp64":
+            abs(snaps[0].Compliance - r64.Snapshots[0].Compliance) / abs(r64.Snapshots[0].Compliance),
     }
     print(json.dumps(out), flush=True)
 
@@ -452,7 +520,7 @@ def main():
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
-    {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve}[args.mode](args)
+    {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve, "cases": mode_cases}[args.mode](args)
 
 
 if __name__ == "__main__":

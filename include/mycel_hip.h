@@ -259,6 +259,19 @@ MYCEL_API int32_t mycel_simp_run(mycel_op *op, const double *f, const mycel_simp
                        mycel_snapshot_cb cb, void *user, mycel_simp_result *result,
                        mycel_snapshot *snapshots, int32_t snapshots_capacity,
                        double *rho_out, double *rho_bar_out, double *u_out);
+/* Several independent load cases on one mesh (the multi-RHS configuration): f, warm_u and u_out hold
+ * n_cases box-DOF vectors back to back, and the objective is the summed compliance sum_k f_k.u_k with
+ * dc = sum_k dc_k. Each case is solved on its own, warm-started from its previous solution; a snapshot's
+ * solver_iterations is the sum over the cases, solver_residual and max_displacement the largest. No
+ * counterpart in the reference, whose FeModel.AssembleForces sums load cases into one RHS
+ * (Fem/FeModel.cs:267-318); n_cases = 1 is mycel_simp_run. */
+MYCEL_API int32_t mycel_simp_run_cases(mycel_op *op, const double *f, int32_t n_cases, const mycel_simp_options *opt,
+                                       const uint8_t *passive_mask_box, const int32_t *patch_nodes,
+                                       int32_t n_patch_nodes, const float *initial_density, int32_t initial_density_len,
+                                       const double *warm_u, mycel_snapshot_cb cb, void *user,
+                                       mycel_simp_result *result, mycel_snapshot *snapshots,
+                                       int32_t snapshots_capacity, double *rho_out, double *rho_bar_out,
+                                       double *u_out);
 
 
 /* ---- Job server (continuous batching; no counterpart in the reference, whose only job machinery is

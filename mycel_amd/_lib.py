@@ -197,6 +197,8 @@ SIGNATURES = {
     "mycel_server_stats": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), C.POINTER(_i)]),
     "mycel_simp_run": (_i, [_vp, _vp, C.POINTER(SimpOptionsC), _vp, _vp, _i, _vp, _i, _vp, SNAPSHOT_CB, _vp,
                             C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
+    "mycel_simp_run_cases": (_i, [_vp, _vp, _i, C.POINTER(SimpOptionsC), _vp, _vp, _i, _vp, _i, _vp, SNAPSHOT_CB,
+                                  _vp, C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -186,7 +186,7 @@ void op_destroy(mycel_op *op) {
 // ------------------------------------------------------------------------------------------------
 // SIMP driver (Optimization/TopologyOptimizer.cs:16-407), Heaviside off.
 // ------------------------------------------------------------------------------------------------
-void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const uint8_t *passive_mask,
+void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_options &o, const uint8_t *passive_mask,
                      const int32_t *patch_nodes, int n_patch, const float *init_density, int init_len,
                      const double *warm_u, mycel_snapshot_cb cb, void *user, mycel_simp_result *result,
                      mycel_snapshot *snaps, int snaps_cap, double *rho_out, double *rho_bar_out, double *u_out) {
@@ -198,6 +198,8 @@ void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const 
     const int na = static_cast<int>(op->active.size());
     hipStream_t s = op->stream;
     *result = mycel_simp_result{};
+    if (n_cases < 1) throw Error(MYCEL_E_ARGUMENT, "at least one load case is required");
+    const size_t nd = static_cast<size_t>(d.n_dofs), ndc = nd * static_cast<size_t>(n_cases);
 
     double rmin = o.filter_radius_mm / op->h;  // TopologyOptimizer.cs:30-39
     if (rmin < 1.5 && o.enforce_min_filter_radius) rmin = 1.5;
@@ -238,15 +240,17 @@ void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const 
     DeviceFilter flt(d.nx, d.ny, d.nz, op->domain.data(), op->active.data(), na, rmin, s);
     const int nab = std::max(1, na);
     DevBuf<double> d_rho(nab), d_rho_bar(nab), d_rho_old(nab), d_dc_bar(nab), d_dc(nab), d_part;
+    DevBuf<double> d_dc_k(n_cases > 1 ? nab : 0);
     DevBuf<uint8_t> d_passive(nab), d_active_nodes(d.n_nodes);
-    DevBuf<double> d_u(d.n_dofs), d_f(d.n_dofs), d_b(d.n_dofs);
+    DevBuf<double> d_u(ndc), d_f(ndc), d_b(ndc);
     d_rho.upload(rho.data(), nab, s);
     d_passive.upload(passive.data(), nab, s);
     d_active_nodes.upload(op->active_nodes.data(), d.n_nodes, s);
-    d_f.upload(f, d.n_dofs, s);
-    if (warm_u) d_u.upload(warm_u, d.n_dofs, s);
-    else MYCEL_HIP(hipMemsetAsync(d_u.get(), 0, sizeof(double) * d.n_dofs, s));
-    launch_project(d.n_dofs, op->d_flags.get(), d_f.get(), d_b.get(), s);  // b = P F (:146-152)
+    d_f.upload(f, ndc, s);
+    if (warm_u) d_u.upload(warm_u, ndc, s);
+    else MYCEL_HIP(hipMemsetAsync(d_u.get(), 0, sizeof(double) * ndc, s));
+    for (int k = 0; k < n_cases; k++)  // b = P F (:146-152)
+        launch_project(d.n_dofs, op->d_flags.get(), d_f.get() + k * nd, d_b.get() + k * nd, s);
     OcWork oc;
     std::vector<double> h_rho_bar;
     if (cb) h_rho_bar.resize(nab);
@@ -276,21 +280,40 @@ void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const 
             const double lf = std::log10(std::max(o.solver_tol_final, 1e-15));
             tol = std::pow(10.0, lf + a * (li - lf));
         }
-        const auto t_solve = clock::now();
-        mycel_pcg_result pr;
-        pcg_solve_device(op, d_b.get(), d_u.get(), tol, o.solver_max_iterations, o.precision, nullptr, nullptr,
-                         nullptr, &pr);
-        const double solve_ms = ms_since(t_solve);
-        if (!pr.converged)
-            throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " + std::to_string(pr.iterations) +
-                                                   " iterations. Is the model not sufficiently constrained?");
-        double cm[2];
-        compliance_maxdisp(d.n_nodes, d_active_nodes.get(), d_f.get(), d_u.get(), d_part, cm, s);
+        // Load cases one after another (each warm-started from its own previous solution); compliance and
+        // solver iterations summed, residual and max|u| the largest; raw sensitivities summed in case order.
+        double solve_ms = 0.0, sens_ms = 0.0, cm[2] = {0.0, 0.0}, res_max = 0.0;
+        int solve_iters = 0;
+        for (int k = 0; k < n_cases; k++) {
+            double *uk = d_u.get() + k * nd;
+            const auto t_solve = clock::now();
+            mycel_pcg_result pr;
+            pcg_solve_device(op, d_b.get() + k * nd, uk, tol, o.solver_max_iterations, o.precision, nullptr,
+                             nullptr, nullptr, &pr);
+            solve_ms += ms_since(t_solve);
+            if (!pr.converged)
+                throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " +
+                                                       std::to_string(pr.iterations) +
+                                                       " iterations. Is the model not sufficiently constrained?");
+            solve_iters += pr.iterations;
+            res_max = std::max(res_max, pr.final_residual);
+            double cmk[2];
+            compliance_maxdisp(d.n_nodes, d_active_nodes.get(), d_f.get() + k * nd, uk, d_part, cmk, s);
+            cm[0] += cmk[0];
+            cm[1] = std::max(cm[1], cmk[1]);
+            const auto t_sens = clock::now();
+            if (k == 0) {
+                launch_sensitivities(op, uk, d_rho_bar.get(), d_dc_bar.get(), penal, s);
+            } else {
+                launch_sensitivities(op, uk, d_rho_bar.get(), d_dc_k.get(), penal, s);
+                launch_add_into(na, d_dc_k.get(), d_dc_bar.get(), s);
+            }
+            sens_ms += ms_since(t_sens);
+        }
         const auto t_sens = clock::now();
-        launch_sensitivities(op, d_u.get(), d_rho_bar.get(), d_dc_bar.get(), penal, s);
         flt.apply(d_dc_bar.get(), d_dc.get(), true, s);
         MYCEL_HIP(hipStreamSynchronize(s));
-        const double sens_ms = ms_since(t_sens);
+        sens_ms += ms_since(t_sens);
         const auto t_upd = clock::now();
         MYCEL_HIP(hipMemcpyAsync(d_rho_old.get(), d_rho.get(), sizeof(double) * nab, hipMemcpyDeviceToDevice, s));
         lambda = oc_update_device(oc, d_rho.get(), d_dc.get(), flt.d_s.get(), d_passive.get(), d_rho.get(), na,
@@ -305,8 +328,8 @@ void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const 
         sn.volume_fraction = met[1] / na;
         sn.max_change = max_change;
         sn.grey_level = (4.0 * met[2] / na) * 100.0;
-        sn.solver_iterations = pr.iterations;
-        sn.solver_residual = pr.final_residual;
+        sn.solver_iterations = solve_iters;
+        sn.solver_residual = res_max;
         sn.max_displacement = std::sqrt(cm[1]);
         sn.lambda = lambda;
         sn.solve_ms = solve_ms;
@@ -346,7 +369,7 @@ void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const 
     }
     if (rho_out && na) d_rho.download(rho_out, na, s);
     if (rho_bar_out && na) d_rho_bar.download(rho_bar_out, na, s);
-    if (u_out) d_u.download(u_out, d.n_dofs, s);
+    if (u_out) d_u.download(u_out, ndc, s);
     MYCEL_HIP(hipStreamSynchronize(s));
 }
 
@@ -832,8 +855,25 @@ int32_t mycel_simp_run(mycel_op *op, const double *f, const mycel_simp_options *
         DeviceGuard dg(op->device);
         // WarmStartDisplacements of the wrong length are ignored by the reference (:134-140); the C ABI
         // carries no length for warm_u, so a non-null pointer must hold TotalDofs values.
-        simp_run(op, f, *opt, passive_mask_box, patch_nodes, n_patch_nodes, initial_density, initial_density_len,
-                 warm_u, cb, user, result, snapshots, snapshots_capacity, rho_out, rho_bar_out, u_out);
+        simp_run(op, f, 1, *opt, passive_mask_box, patch_nodes, n_patch_nodes, initial_density,
+                 initial_density_len, warm_u, cb, user, result, snapshots, snapshots_capacity, rho_out, rho_bar_out,
+                 u_out);
+    });
+}
+
+int32_t mycel_simp_run_cases(mycel_op *op, const double *f, int32_t n_cases, const mycel_simp_options *opt,
+                             const uint8_t *passive_mask_box, const int32_t *patch_nodes, int32_t n_patch_nodes,
+                             const float *initial_density, int32_t initial_density_len, const double *warm_u,
+                             mycel_snapshot_cb cb, void *user, mycel_simp_result *result, mycel_snapshot *snapshots,
+                             int32_t snapshots_capacity, double *rho_out, double *rho_bar_out, double *u_out) {
+    return guarded([&] {
+        require(op && f && opt && result, MYCEL_E_ARGUMENT_NULL, "op/f/options/result");
+        require(n_patch_nodes == 0 || patch_nodes, MYCEL_E_ARGUMENT_NULL, "patch_nodes");
+        require(n_cases >= 1, MYCEL_E_ARGUMENT, "n_cases must be >= 1");
+        DeviceGuard dg(op->device);
+        simp_run(op, f, n_cases, *opt, passive_mask_box, patch_nodes, n_patch_nodes, initial_density,
+                 initial_density_len, warm_u, cb, user, result, snapshots, snapshots_capacity, rho_out, rho_bar_out,
+                 u_out);
     });
 }
 

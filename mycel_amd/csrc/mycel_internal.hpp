@@ -210,7 +210,7 @@ void gmg_free(Gmg *g);
 mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,
                     double nu, double emin_ratio, double penal, const double *density);
 void op_destroy(mycel_op *op);
-void simp_run(mycel_op *op, const double *f, const mycel_simp_options &o, const uint8_t *passive_mask,
+void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_options &o, const uint8_t *passive_mask,
               const int32_t *patch_nodes, int n_patch, const float *init_density, int init_len,
               const double *warm_u, mycel_snapshot_cb cb, void *user, mycel_simp_result *result,
               mycel_snapshot *snaps, int snaps_cap, double *rho_out, double *rho_bar_out, double *u_out);

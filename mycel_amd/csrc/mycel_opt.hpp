@@ -54,6 +54,7 @@ void launch_scatter_density(int na, const int32_t *d_active, const double *d_rho
                             hipStream_t s);
 void launch_force_passive(int na, const uint8_t *d_passive, double *d_rho, hipStream_t s);
 void launch_project(int n, const uint8_t *d_flags, const double *d_f, double *d_b, hipStream_t s);
+void launch_add_into(int n, const double *d_src, double *d_dst, hipStream_t s);
 void simp_metrics(int na, const double *d_rho, const double *d_rho_old, const double *d_rho_bar, DevBuf<double> &part,
                   double *h_out3, hipStream_t s);
 void compliance_maxdisp(int n_nodes, const uint8_t *d_active_nodes, const double *d_f, const double *d_u,

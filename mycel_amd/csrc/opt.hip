@@ -525,6 +525,18 @@ void launch_force_passive(int na, const uint8_t *d_passive, double *d_rho, hipSt
     MYCEL_LAUNCH_CHECK();
 }
 
+// dst += src (the per-load-case sensitivities summed in case order).
+__global__ __launch_bounds__(256) void k_add_into(int n, const double *__restrict__ src, double *__restrict__ dst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+void launch_add_into(int n, const double *d_src, double *d_dst, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_add_into, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d_src, d_dst);
+    MYCEL_LAUNCH_CHECK();
+}
+
 void launch_project(int n, const uint8_t *d_flags, const double *d_f, double *d_b, hipStream_t s) {
     hipLaunchKernelGGL(k_project, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d_flags, d_f, d_b);
     MYCEL_LAUNCH_CHECK();

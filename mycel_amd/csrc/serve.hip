@@ -59,7 +59,11 @@ double job_footprint(const mycel_job_desc &d) {
     const double nn = static_cast<double>(d.grid.nx + 1) * (d.grid.ny + 1) * (d.grid.nz + 1);
     const double npad = static_cast<double>(d.grid.nx + 2) * (d.grid.ny + 2) * (d.grid.nz + 2);
     const double ndof = 3 * nn;
-    return 12 * npad + ndof * (1 + 8 + 8 + 4 + 16 + 32 + 20 + 24) + ne * (1 + 8 + 4 + 4 + 8 * 8) + (8 << 20);
+    const int prec = d.kind == MYCEL_JOB_SIMP ? d.simp.precision : d.precision;
+    // Multigrid hierarchy: six FP32 vectors per level (the levels below the finest add < 1/7 of it), the
+    // coarse moduli, the Galerkin blocks and the coarsest inverse.
+    const double gmg = prec == MYCEL_PRECISION_MIXED_GMG ? ndof * 6 * 4 * (8.0 / 7.0) + npad * 4 / 7.0 + (1 << 20) : 0.0;
+    return 12 * npad + ndof * (1 + 8 + 8 + 4 + 16 + 32 + 20 + 24) + ne * (1 + 8 + 4 + 4 + 8 * 8) + gmg + (8 << 20);
 }
 
 }  // namespace mycel
@@ -101,7 +105,7 @@ void run_job(Job &j, int device) {
         const int na = static_cast<int>(op->active.size());
         j.rho_bar.assign(std::max(1, na), 0.0);
         std::vector<mycel_snapshot> snaps(std::max(1, d.simp.max_iterations - d.simp.start_iteration + 1));
-        simp_run(op.get(), j.forces.data(), d.simp, nullptr, j.patch.data(), static_cast<int>(j.patch.size()),
+        simp_run(op.get(), j.forces.data(), 1, d.simp, nullptr, j.patch.data(), static_cast<int>(j.patch.size()),
                  nullptr, 0, nullptr, nullptr, nullptr, &res, snaps.data(), static_cast<int>(snaps.size()), nullptr,
                  j.rho_bar.data(), j.u.data());
         j.rho_bar.resize(na);

@@ -198,6 +198,24 @@ class TopologyOptimizer:
         cancel() returning True stops after the current iteration."""
         if model is None:
             raise L.ArgumentNullError(L.E_ARGUMENT_NULL, "model")
+        return TopologyOptimizer._run(model, model.AssembleForces(), 1, options, progress, cancel, op)
+
+    @staticmethod
+    def RunCases(model: FeModel, forces, options: Optional[OptimizerOptions] = None,
+                 progress: Optional[Callable] = None, cancel: Optional[Callable[[], bool]] = None,
+                 op: Optional[FemOperator] = None):
+        """SIMP on the summed compliance of independent load cases (mycel_simp_run_cases): forces is an
+        (n_cases, TotalDofs) array of assembled RHS vectors; FinalDisplacements comes back with the same
+        shape. The reference instead sums load cases into one RHS (Fem/FeModel.cs:267-318)."""
+        if model is None:
+            raise L.ArgumentNullError(L.E_ARGUMENT_NULL, "model")
+        f = np.ascontiguousarray(forces, np.float64)
+        if f.ndim != 2 or f.shape[1] != model.TotalDofs or f.shape[0] < 1:
+            raise L.ArgumentError(L.E_ARGUMENT, "forces must be (n_cases, TotalDofs)")
+        return TopologyOptimizer._run(model, f, f.shape[0], options, progress, cancel, op)
+
+    @staticmethod
+    def _run(model, f, n_cases, options, progress, cancel, op):
         o = options or OptimizerOptions()
         if o.EnableHeaviside:
             raise L.ArgumentError(L.E_ARGUMENT, "Heaviside projection is not on the GPU path")
@@ -205,18 +223,18 @@ class TopologyOptimizer:
                             o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
                             int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
                             int(o.Continuation), int(o.Precision))
-        f = model.AssembleForces()
         patch = np.concatenate([lc.Patch.Nodes for lc in model.Loads]).astype(np.int32) if model.Loads else np.zeros(0, np.int32)
         pm = L.u8(o.PassiveMask) if o.PassiveMask is not None else None
         idn = np.ascontiguousarray(o.InitialDensity, np.float32) if o.InitialDensity is not None else None
         wu = None
-        if o.WarmStartDisplacements is not None and len(o.WarmStartDisplacements) == model.TotalDofs:
-            wu = L.f64(o.WarmStartDisplacements)
+        if o.WarmStartDisplacements is not None and np.size(o.WarmStartDisplacements) == model.TotalDofs * n_cases:
+            wu = L.f64(np.ravel(o.WarmStartDisplacements))
         n_it = max(1, o.MaxIterations - o.StartIteration + 1)
         snaps = (L.SnapshotC * n_it)()
         res = L.SimpResultC()
         na = model.ActiveElementCount
-        rho, rb, u = np.zeros(max(1, na)), np.zeros(max(1, na)), np.zeros(model.TotalDofs)
+        rho, rb = np.zeros(max(1, na)), np.zeros(max(1, na))
+        u = np.zeros((n_cases, model.TotalDofs)) if f.ndim == 2 else np.zeros(model.TotalDofs)
         collected: List[IterationSnapshot] = []
         act = model.ActiveElements
 
@@ -234,9 +252,10 @@ class TopologyOptimizer:
         op_own = op is None
         op = op or FemOperator(model)
         try:
-            L.check(L.load().mycel_simp_run(op._h, L.ptr(f), C.byref(co), L.ptr(pm), L.ptr(patch) if patch.size else None,
-                                            int(patch.size), L.ptr(idn), 0 if idn is None else int(idn.size), L.ptr(wu),
-                                            cb, None, C.byref(res), snaps, n_it, L.ptr(rho), L.ptr(rb), L.ptr(u)))
+            L.check(L.load().mycel_simp_run_cases(op._h, L.ptr(f), int(n_cases), C.byref(co), L.ptr(pm),
+                                                  L.ptr(patch) if patch.size else None, int(patch.size), L.ptr(idn),
+                                                  0 if idn is None else int(idn.size), L.ptr(wu), cb, None,
+                                                  C.byref(res), snaps, n_it, L.ptr(rho), L.ptr(rb), L.ptr(u)))
         finally:
             if op_own:
                 op.close()

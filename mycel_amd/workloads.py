@@ -218,6 +218,20 @@ def multi_rhs(seed: int = 99, nx=100, ny=50, nz=50, n_rhs=16) -> Tuple[Problem, 
     return p, cases
 
 
+def case_forces(p: Problem, cases) -> np.ndarray:
+    """(n_cases, n_dofs) assembled RHS vectors, one per load case of multi_rhs (FeModel.AssembleForces of
+    each case on its own)."""
+    keep = p.loads
+    try:
+        out = []
+        for c in cases:
+            p.loads = c
+            out.append(p.fe_model().AssembleForces())
+        return np.stack(out)
+    finally:
+        p.loads = keep
+
+
 # --------------------------------------------------------------------------------------------
 # Fixture models of the reference tests (tests/Mycel.Core.Tests)
 # --------------------------------------------------------------------------------------------

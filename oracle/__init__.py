@@ -118,6 +118,9 @@ def _declare(L):
     L.orc_simp_run.argtypes = [_vp, _dp, C.POINTER(SimpOptions), _vp, _vp, C.c_int, _vp, C.c_int, _vp,
                                C.c_int, C.POINTER(Snapshot), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                _dp, _dp, _dp]
+    L.orc_simp_run_cases.argtypes = [_vp, _dp, C.c_int, C.POINTER(SimpOptions), _vp, _vp, C.c_int, _vp, C.c_int,
+                                     _vp, C.c_int, C.POINTER(Snapshot), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     _dp, _dp, _dp]
 
 
 def _ptr(a):
@@ -289,14 +292,16 @@ class Model:
         snaps = (Snapshot * max(1, max_iterations - start_iteration + 1))()
         nsn, conv = C.c_int(), C.c_int()
         na = self.active_count
-        rho, rb, u = np.zeros(na), np.zeros(na), np.zeros(self.n_dofs)
+        f = np.ascontiguousarray(f, np.float64)
+        n_cases = 1 if f.ndim == 1 else f.shape[0]  # a 2-D f holds one load case per row (summed compliance)
+        rho, rb, u = np.zeros(na), np.zeros(na), np.zeros((n_cases, self.n_dofs) if f.ndim == 2 else self.n_dofs)
         pm = None if passive_mask is None else np.ascontiguousarray(passive_mask, np.uint8)
         pn = np.zeros(0, np.int32) if patch_nodes is None else np.ascontiguousarray(patch_nodes, np.int32)
         idn = None if initial_density is None else np.ascontiguousarray(initial_density, np.float32)
         wu = None if warm_u is None else np.ascontiguousarray(warm_u, np.float64)
-        st = lib().orc_simp_run(self._h, np.ascontiguousarray(f, np.float64), C.byref(o), _ptr(pm), _ptr(pn),
-                                int(pn.size), _ptr(idn), 0 if idn is None else int(idn.size), _ptr(wu),
-                                int(threads), snaps, C.byref(nsn), C.byref(conv), rho, rb, u)
+        st = lib().orc_simp_run_cases(self._h, f, int(n_cases), C.byref(o), _ptr(pm), _ptr(pn),
+                                      int(pn.size), _ptr(idn), 0 if idn is None else int(idn.size), _ptr(wu),
+                                      int(threads), snaps, C.byref(nsn), C.byref(conv), rho, rb, u)
         hist = [dict(iteration=s.iteration, compliance=s.compliance, volume_fraction=s.volume_fraction,
                      max_change=s.max_change, grey_level=s.grey_level, solver_iterations=s.solver_iterations,
                      solver_residual=s.solver_residual, max_displacement=s.max_displacement, lam=s.lambda_)

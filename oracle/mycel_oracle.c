@@ -980,12 +980,19 @@ void orc_passive_region(int nx, int ny, int nz, const int32_t *nodes, int n_node
 /* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407), Heaviside off             */
 /* ------------------------------------------------------------------------------------------ */
 
-int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
-                 const uint8_t *passive_mask_box, const int32_t *patch_nodes, int n_patch_nodes,
-                 const float *initial_density, int initial_density_len, const double *warm_u,
-                 int n_threads, orc_snapshot *snaps, int *n_snaps, int *converged_out,
-                 double *rho_out, double *rho_bar_out, double *u_out) {
+/* Several load cases (f, warm_u, u_out hold n_cases box-DOF vectors back to back): the objective is the
+ * summed compliance sum_k f_k.u_k. Each case is solved on its own (warm-started from its previous
+ * solution) and the raw sensitivities are summed in case order before the one adjoint filter; solver
+ * iterations are summed, the residual and max|u| are the largest over the cases. n_cases = 1 is
+ * TopologyOptimizer.Run itself. (The reference sums load cases into one RHS, Fem/FeModel.cs:267-318;
+ * independent cases are this restatement's extension for the multi-RHS configuration.) */
+int orc_simp_run_cases(orc_model *m, const double *f, int n_cases, const orc_simp_options *opt,
+                       const uint8_t *passive_mask_box, const int32_t *patch_nodes, int n_patch_nodes,
+                       const float *initial_density, int initial_density_len, const double *warm_u,
+                       int n_threads, orc_snapshot *snaps, int *n_snaps, int *converged_out,
+                       double *rho_out, double *rho_bar_out, double *u_out) {
     int na = m->n_active, ne = m->n_elem, nd = m->n_dofs;
+    if (n_cases < 1) return ORC_E_ARGUMENT;
     double h = m->h;
     double vf = opt->volume_fraction;
     double rmin = opt->filter_radius_mm / h; /* :30 */
@@ -1035,12 +1042,14 @@ int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
     double *rho_old = (double *)calloc((size_t)na, sizeof(double));
     double *dc_bar = (double *)calloc((size_t)na, sizeof(double));
     double *dc = (double *)calloc((size_t)na, sizeof(double));
-    double *u = (double *)calloc((size_t)nd, sizeof(double));
-    if (warm_u) memcpy(u, warm_u, (size_t)nd * sizeof(double));
+    double *dc_k = (double *)calloc((size_t)na, sizeof(double));
+    const size_t ndc = (size_t)nd * (size_t)n_cases;
+    double *u = (double *)calloc(ndc, sizeof(double));
+    if (warm_u) memcpy(u, warm_u, ndc * sizeof(double));
     /* new FemOperator(model): E from the model's current densities (:142). */
     orc_update_densities(m, m->density, m->penal);
-    double *b = (double *)malloc((size_t)nd * sizeof(double));
-    for (int i = 0; i < nd; i++) b[i] = m->constrained[i] ? 0.0 : f[i];
+    double *b = (double *)malloc(ndc * sizeof(double));
+    for (size_t i = 0; i < ndc; i++) b[i] = m->constrained[i % (size_t)nd] ? 0.0 : f[i];
 
     double lambda = 1.0, max_change = 1.0;
     double hist[6];
@@ -1063,20 +1072,34 @@ int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
             double lf = log10(fmax(opt->solver_tol_final, 1e-15));
             tol = pow(10.0, lf + a * (li - lf));
         }
-        orc_pcg_result pr;
-        int st = orc_pcg_solve(m, b, u, tol, opt->solver_max_iterations, n_threads, &pr);
-        if (st != ORC_OK) { status = st; break; }
-        if (!pr.converged) { status = ORC_E_NOT_CONVERGED; break; }
-        double c = 0.0;
-        for (int i = 0; i < nd; i++) c += f[i] * u[i];
-        double mds = 0.0;
-        for (int q = 0; q < m->n_nodes; q++) {
-            if (!m->active_nodes[q]) continue;
-            double ux = u[3 * q], uy = u[3 * q + 1], uz = u[3 * q + 2];
-            double ss = ux * ux + uy * uy + uz * uz;
-            if (ss > mds) mds = ss;
+        double c = 0.0, mds = 0.0, res_max = 0.0;
+        int iters = 0;
+        for (int k = 0; k < n_cases; k++) {
+            const double *fk = f + (size_t)k * nd;
+            double *uk = u + (size_t)k * nd;
+            orc_pcg_result pr;
+            int st = orc_pcg_solve(m, b + (size_t)k * nd, uk, tol, opt->solver_max_iterations, n_threads, &pr);
+            if (st != ORC_OK) { status = st; break; }
+            if (!pr.converged) { status = ORC_E_NOT_CONVERGED; break; }
+            iters += pr.iterations;
+            if (pr.final_residual > res_max) res_max = pr.final_residual;
+            double ck = 0.0;
+            for (int i = 0; i < nd; i++) ck += fk[i] * uk[i];
+            c += ck;
+            for (int q = 0; q < m->n_nodes; q++) {
+                if (!m->active_nodes[q]) continue;
+                double ux = uk[3 * q], uy = uk[3 * q + 1], uz = uk[3 * q + 2];
+                double ss = ux * ux + uy * uy + uz * uz;
+                if (ss > mds) mds = ss;
+            }
+            if (k == 0) {
+                orc_sensitivities(m, uk, rho_bar, dc_bar, penal);
+            } else {
+                orc_sensitivities(m, uk, rho_bar, dc_k, penal);
+                for (int i = 0; i < na; i++) dc_bar[i] += dc_k[i];
+            }
         }
-        orc_sensitivities(m, u, rho_bar, dc_bar, penal);
+        if (status != ORC_OK) break;
         orc_filter_apply_adjoint(flt, dc_bar, dc);
         memcpy(rho_old, rho, (size_t)na * sizeof(double));
         lambda = orc_oc_update(rho, dc, s, passive, rho, na, vf, opt->move_limit, opt->rho_min, lambda, 60, 1e-8, NULL);
@@ -1094,8 +1117,8 @@ int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
         sn->volume_fraction = vol / na;
         sn->max_change = max_change;
         sn->grey_level = (4.0 * grey / na) * 100.0;
-        sn->solver_iterations = pr.iterations;
-        sn->solver_residual = pr.final_residual;
+        sn->solver_iterations = iters;
+        sn->solver_residual = res_max;
         sn->max_displacement = sqrt(mds);
         sn->lambda = lambda;
         if (hist_n == 6) {
@@ -1114,8 +1137,18 @@ int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
     }
     if (rho_out) memcpy(rho_out, rho, (size_t)na * sizeof(double));
     if (rho_bar_out) memcpy(rho_bar_out, rho_bar, (size_t)na * sizeof(double));
-    if (u_out) memcpy(u_out, u, (size_t)nd * sizeof(double));
-    free(passive); free(rho); free(rho_bar); free(rho_old); free(dc_bar); free(dc); free(u); free(b);
+    if (u_out) memcpy(u_out, u, ndc * sizeof(double));
+    free(passive); free(rho); free(rho_bar); free(rho_old); free(dc_bar); free(dc); free(dc_k); free(u); free(b);
     orc_filter_destroy(flt);
     return status;
+}
+
+int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
+                 const uint8_t *passive_mask_box, const int32_t *patch_nodes, int n_patch_nodes,
+                 const float *initial_density, int initial_density_len, const double *warm_u,
+                 int n_threads, orc_snapshot *snaps, int *n_snaps, int *converged_out,
+                 double *rho_out, double *rho_bar_out, double *u_out) {
+    return orc_simp_run_cases(m, f, 1, opt, passive_mask_box, patch_nodes, n_patch_nodes, initial_density,
+                              initial_density_len, warm_u, n_threads, snaps, n_snaps, converged_out, rho_out,
+                              rho_bar_out, u_out);
 }

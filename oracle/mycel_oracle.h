@@ -160,6 +160,11 @@ typedef struct {
  * ProtectBoundaryHinges); initial_density: optional (active- or box-sized, float32 as the reference);
  * warm_u: optional box DOFs. Outputs: snapshots[max_iterations], rho/rho_bar active-sized FP64,
  * u box-sized. Returns ORC_E_NOT_CONVERGED if a PCG solve fails (the reference throws). */
+int orc_simp_run_cases(orc_model *m, const double *f, int n_cases, const orc_simp_options *opt,
+                       const uint8_t *passive_mask_box, const int32_t *patch_nodes, int n_patch_nodes,
+                       const float *initial_density, int initial_density_len, const double *warm_u,
+                       int n_threads, orc_snapshot *snapshots, int *n_snapshots, int *converged,
+                       double *rho_out, double *rho_bar_out, double *u_out);
 int orc_simp_run(orc_model *m, const double *f, const orc_simp_options *opt,
                  const uint8_t *passive_mask_box, const int32_t *patch_nodes, int n_patch_nodes,
                  const float *initial_density, int initial_density_len, const double *warm_u,

@@ -12,6 +12,12 @@ import numpy as np
 import pytest
 
 from helpers import oracle_forces, oracle_model, random_field, rel
+
+import oracle as _O
+
+
+def O_forces(p, case):
+    return _O.assemble_forces(p.n_nodes, case)
 from mycel_amd import (DensityFilter, FemOperator, InvalidOperationError, JacobiPreconditioner, OcOptimizer,
                        OperationCanceledError, OptimizerOptions, PcgSolver, SensitivityAnalysis, StaticAnalysis,
                        StaticAnalysisOptions, TopologyOptimizer)
@@ -287,3 +293,33 @@ def test_mixed_precision_defect_correction(gpu):
     # FP32 breakdown stops on its recursive residual, as the reference does).
     Ku = m.apply_a(u, projected=True)
     assert np.linalg.norm(np.where(m.constrained, 0.0, b - Ku)) / np.linalg.norm(b) <= 1e-7
+
+
+def test_simp_cases_parity(gpu):
+    """SIMP on the summed compliance of three independent load cases, GPU against the oracle: compliance
+    history, densities and every case's displacements."""
+    p, cases = W.multi_rhs(nx=24, ny=8, nz=8, n_rhs=3)
+    f = np.stack([O_forces(p, c) for c in cases])
+    m = oracle_model(p)
+    kw = dict(volume_fraction=0.25, filter_radius_mm=2.0, penal=3.0, move_limit=0.2, max_iterations=8,
+              solver_tol_initial=1e-8, solver_tol_final=1e-8, solver_max_iterations=20000,
+              protect_boundary_hinges=False)
+    ref = m.simp(f, **kw)
+    assert ref["status"] == 0
+    o = OptimizerOptions(VolumeFraction=0.25, FilterRadiusMm=2.0, Penal=3.0, MoveLimit=0.2, MaxIterations=8,
+                         SolverToleranceInitial=1e-8, SolverToleranceFinal=1e-8, SolverMaxIterations=20000,
+                         ProtectBoundaryHinges=False)
+    res = TopologyOptimizer.RunCases(p.fe_model(), f, o)
+    c = np.array([s.Compliance for s in res.Snapshots])
+    c_ref = np.array([s["compliance"] for s in ref["snapshots"]])
+    print(f"[parity] simp cases max rel compliance diff {np.max(np.abs(c - c_ref) / c_ref):.3e}", flush=True)
+    assert len(c) == len(c_ref)
+    assert np.max(np.abs(c - c_ref) / c_ref) < 1e-8
+    assert np.abs(res.FinalPhysical64 - ref["rho_bar"]).max() < 1e-8
+    assert np.abs(res.FinalDesign - ref["rho"]).max() < 1e-8
+    assert res.FinalDisplacements.shape == (3, p.n_dofs)
+    # Both sides stop each warm-started solve at a 1e-8 residual, so two correct solutions may differ in u
+    # by about that much times the condition of the solve; 1e-7 leaves that margin.
+    for k in range(3):
+        assert rel(res.FinalDisplacements[k], ref["u"][k]) < 1e-7
+    assert [s.SolverIterations for s in res.Snapshots][0] == ref["snapshots"][0]["solver_iterations"]

@@ -3,6 +3,7 @@ job's result is bit-identical to running it alone."""
 import numpy as np
 import pytest
 
+import mycel_amd as M
 from mycel_amd import OptimizerOptions, StaticAnalysis, StaticAnalysisOptions, TopologyOptimizer
 from mycel_amd import workloads as W
 from mycel_amd.serve import DONE, FAILED, JobServer
@@ -14,29 +15,30 @@ def _stream(n, seed=11):
     return W.serving_stream(n, seed=seed, min_el=1500, max_el=12000, simp_iterations=4)
 
 
-def _options(j):
+def _options(j, precision=M.PRECISION_FP64):
     return OptimizerOptions(VolumeFraction=j.volume_fraction, FilterRadiusMm=j.rmin, MaxIterations=j.simp_iterations,
                             SolverToleranceInitial=1e-8, SolverToleranceFinal=1e-8, SolverMaxIterations=50000,
-                            ProtectBoundaryHinges=False)
+                            ProtectBoundaryHinges=False, Precision=precision)
 
 
-def test_jobs_alone_equal_jobs_in_the_mix(gpu):
+@pytest.mark.parametrize("precision", [M.PRECISION_FP64, M.PRECISION_MIXED_GMG], ids=["fp64", "gmg"])
+def test_jobs_alone_equal_jobs_in_the_mix(gpu, precision):
     jobs = _stream(8)
     alone = []
     for j in jobs:
         fm = j.problem.fe_model()
         if j.kind == "fea":
-            r = StaticAnalysis.Run(fm, StaticAnalysisOptions(Tolerance=1e-8, MaxIterations=50000))
+            r = StaticAnalysis.Run(fm, StaticAnalysisOptions(Tolerance=1e-8, MaxIterations=50000, Precision=precision))
             alone.append((r.U, None, r.Compliance))
         else:
-            r = TopologyOptimizer.Run(fm, _options(j))
+            r = TopologyOptimizer.Run(fm, _options(j, precision))
             alone.append((r.FinalDisplacements, r.FinalPhysical64, r.FinalCompliance))
     with JobServer(workers=4) as sv:
         ids = []
         for j in jobs:  # submitted back to back: the later ones queue while the first ones run
             fm = j.problem.fe_model()
-            ids.append(sv.submit_fea(fm, tol=1e-8, max_iter=50000) if j.kind == "fea" else
-                       sv.submit_simp(fm, _options(j)))
+            ids.append(sv.submit_fea(fm, tol=1e-8, max_iter=50000, precision=precision) if j.kind == "fea" else
+                       sv.submit_simp(fm, _options(j, precision)))
         peak_active = 0
         for jid in ids:
             peak_active = max(peak_active, sv.stats()["active"])

@@ -399,3 +399,61 @@ def test_passive_region_r1(oracle):
     m = oracle.passive_region(4, 3, 2, np.array([0, 5 * 4 * 3 - 1], np.int32), 1)
     box = m.reshape(2, 3, 4)
     assert box[0, 0, 0] == 1 and box[1, 2, 3] == 1 and m.sum() == 2
+
+
+# ---------------------------------------------------------------------------------------------
+# SIMP on the summed compliance of independent load cases (the multi-RHS configuration)
+# ---------------------------------------------------------------------------------------------
+def _two_cases(p):
+    fa = oracle_forces(p)
+    fb = np.zeros(p.n_dofs)
+    for j in range(p.ny + 1):  # a y-load along the top edge at mid-span
+        fb[3 * p.node(p.nx // 2, j, p.nz) + 1] = 0.5 / (p.ny + 1)
+    return fa, fb
+
+
+def test_simp_cases_first_iteration_from_primitives(oracle):
+    """One iteration of the multi-case loop rebuilt from the oracle's own steps: filter, per-case cold
+    solves, summed compliance, summed raw sensitivities, adjoint filter, OC."""
+    p = W.cantilever_simp(12, 4, 4)
+    fa, fb = _two_cases(p)
+    kw = dict(volume_fraction=0.3, filter_radius_mm=1.5, max_iterations=1, solver_tol_initial=1e-10,
+              solver_tol_final=1e-10, solver_max_iterations=20000, protect_boundary_hinges=False)
+    r = oracle_model(p).simp(np.stack([fa, fb]), **kw)
+    assert r["status"] == 0 and r["u"].shape == (2, p.n_dofs)
+    m = oracle_model(p)
+    act = m.active_elements
+    flt = oracle.Filter(p.nx, p.ny, p.nz, p.domain, act, 1.5)
+    rho0 = np.full(act.size, 0.3)
+    rho_bar = flt.apply(rho0)
+    box = np.ones(p.n_elem)
+    box[act] = rho_bar
+    m.update_densities(box, 3.0)
+    b = [np.where(m.constrained, 0.0, f) for f in (fa, fb)]
+    u = [m.pcg(bk, tol=1e-10, max_iter=20000)[0] for bk in b]
+    c = fa @ u[0] + fb @ u[1]
+    assert abs(r["snapshots"][0]["compliance"] - c) <= 1e-12 * abs(c)
+    assert np.array_equal(r["u"][0], u[0]) and np.array_equal(r["u"][1], u[1])
+    dc = flt.apply_adjoint(m.sensitivities(u[0], rho_bar, 3.0) + m.sensitivities(u[1], rho_bar, 3.0))
+    rho1, _, _ = oracle.oc_update(rho0, dc, flt.s, vf=0.3, move=0.2, rho_min=1e-3, lambda_guess=1.0)
+    assert np.abs(r["rho"] - rho1).max() <= 1e-12
+    # A single case gives a different design (the second case is not ignored).
+    r1 = oracle_model(p).simp(fa, **kw)
+    assert np.abs(r1["rho"] - r["rho"]).max() > 1e-3
+
+
+def test_simp_cases_duplicate_case_doubles_compliance(oracle):
+    """Two copies of one load case: twice the compliance, and (OC is invariant to scaling dc) the same
+    design up to the bisection tolerance."""
+    p = W.cantilever_simp(12, 4, 4)
+    fa, _ = _two_cases(p)
+    kw = dict(volume_fraction=0.3, filter_radius_mm=1.5, max_iterations=4, solver_tol_initial=1e-10,
+              solver_tol_final=1e-10, solver_max_iterations=20000, protect_boundary_hinges=False)
+    r1 = oracle_model(p).simp(fa, **kw)
+    r2 = oracle_model(p).simp(np.stack([fa, fa]), **kw)
+    c1 = [s["compliance"] for s in r1["snapshots"]]
+    c2 = [s["compliance"] for s in r2["snapshots"]]
+    assert abs(c2[0] - 2 * c1[0]) <= 1e-13 * c2[0]
+    assert np.allclose(c2, 2 * np.array(c1), rtol=1e-6)
+    assert np.abs(r2["rho"] - r1["rho"]).max() <= 1e-6
+    assert [s["solver_iterations"] for s in r2["snapshots"]] == [2 * s["solver_iterations"] for s in r1["snapshots"]]
