@@ -222,12 +222,40 @@ __global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__re
     d[i] = xi;
 }
 
-__global__ __launch_bounds__(256) void k_gmg_restrict(Dims gf, const float *__restrict__ rf, Dims gc,
-                                                      const float *__restrict__ invc, float theta_inv,
-                                                      float *__restrict__ bc, float *__restrict__ xcv,
-                                                      float *__restrict__ dc) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n < gc.n_nodes) restrict_node(gf, rf, gc, invc, theta_inv, bc, xcv, dc, n);
+
+// Restriction with one thread per coarse DOF (the large levels: three times the parallelism of the node
+// form, the same sums in the same order).
+__global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *__restrict__ rf, Dims gc,
+                                                          const float *__restrict__ invc, float theta_inv,
+                                                          float *__restrict__ bc, float *__restrict__ xcv,
+                                                          float *__restrict__ dc) {
+    const int dof = blockIdx.x * 256 + threadIdx.x;
+    if (dof >= gc.n_dofs) return;
+    const int n = dof / 3, c = dof - 3 * n;
+    int I, J, K;
+    node_ijk(gc, n, I, J, K);
+    float sum = 0.0f;
+    for (int dk = -1; dk <= 1; dk++) {
+        const int k = 2 * K + dk;
+        if (k < 0 || k > gf.nz) continue;
+        const float wz = dk == 0 ? 1.0f : 0.5f;
+        for (int dj = -1; dj <= 1; dj++) {
+            const int j = 2 * J + dj;
+            if (j < 0 || j > gf.ny) continue;
+            const float wyz = wz * (dj == 0 ? 1.0f : 0.5f);
+            for (int di = -1; di <= 1; di++) {
+                const int i = 2 * I + di;
+                if (i < 0 || i > gf.nx) continue;
+                sum += (wyz * (di == 0 ? 1.0f : 0.5f)) * rf[3 * (i + gf.nx1 * (j + gf.ny1 * k)) + c];
+            }
+        }
+    }
+    const float iv = invc[dof];
+    const float bi = iv > 0.0f ? sum : 0.0f;
+    const float xi = (iv * bi) * theta_inv;
+    bc[dof] = bi;
+    xcv[dof] = xi;
+    dc[dof] = xi;
 }
 
 __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict__ xf, const float *__restrict__ invf,
@@ -242,7 +270,7 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
 // the level above; lv[count-1] is the coarsest level, solved with its dense inverse. On return the
 // corrected lv[0] iterate is in x2 (three buffer swaps: one pre- and two post-smoothing steps).
 constexpr int kMaxTail = 8;
-constexpr int kTailNodes = 512;  // levels up to this many nodes run in the tail
+constexpr int kTailNodes = 128;  // levels up to this many nodes run in the tail (larger ones are FMA-bound on one CU)
 struct TailLevel {
     Dims d;
     KeSym<float> ks;
@@ -413,30 +441,48 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
     const int n = g.n_dofs;
     for (int q = threadIdx.x; q < n * n; q += blockDim.x) A[q] = 0.0;
     __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        const int node = r / 3, c = r % 3;
-        double *row = A + r * n;
-        if (!(inv[r] > 0.0f)) {
-            row[r] = 1.0;
-            continue;
-        }
-        int I, J, K;
-        node_ijk(g, node, I, J, K);
-        if (blocks) {
-            for (int ez = max(2 * K - 2, 0); ez <= min(2 * K + 1, gs.nz - 1); ez++)
-                for (int ey = max(2 * J - 2, 0); ey <= min(2 * J + 1, gs.ny - 1); ey++)
-                    for (int ex = max(2 * I - 2, 0); ex <= min(2 * I + 1, gs.nx - 1); ex++) {
+    if (blocks) {
+        // Galerkin: one thread per (row node I, neighbour node m) pair sums the 3x3 block A[I][m] over the
+        // second-coarsest elements that hold both nodes, in a fixed element order (deterministic, no atomics).
+        const int nn = g.n_nodes;
+        for (int w = threadIdx.x; w < nn * 27; w += blockDim.x) {
+            const int node = w / 27, nb = w - 27 * (w / 27);
+            int I, J, K;
+            node_ijk(g, node, I, J, K);
+            const int mi = I + nb % 3 - 1, mj = J + (nb / 3) % 3 - 1, mk = K + nb / 9 - 1;
+            if (mi < 0 || mi > g.nx || mj < 0 || mj > g.ny || mk < 0 || mk > g.nz) continue;
+            const int m = mi + g.nx1 * (mj + g.ny1 * mk);
+            double acc[3][3] = {};
+            for (int ez = max(2 * max(K, mk) - 2, 0); ez <= min(2 * min(K, mk) + 1, gs.nz - 1); ez++)
+                for (int ey = max(2 * max(J, mj) - 2, 0); ey <= min(2 * min(J, mj) + 1, gs.ny - 1); ey++)
+                    for (int ex = max(2 * max(I, mi) - 2, 0); ex <= min(2 * min(I, mi) + 1, gs.nx - 1); ex++) {
                         const int bx = ex >> 1, by = ey >> 1, bz = ez >> 1;
                         const int q1 = (I - bx) + 2 * (J - by) + 4 * (K - bz);
-                        const double *blk =
-                            blocks + static_cast<size_t>(ex + gs.nx * (ey + gs.ny * ez)) * 576 + (3 * q1 + c) * 24;
-                        for (int q2 = 0; q2 < 8; q2++) {
-                            const int m = (bx + (q2 & 1)) + g.nx1 * ((by + ((q2 >> 1) & 1)) + g.ny1 * (bz + (q2 >> 2)));
-                            for (int d = 0; d < 3; d++)
-                                if (inv[3 * m + d] > 0.0f) row[3 * m + d] += blk[3 * q2 + d];
-                        }
+                        const int q2 = (mi - bx) + 2 * (mj - by) + 4 * (mk - bz);
+                        const double *blk = blocks + static_cast<size_t>(ex + gs.nx * (ey + gs.ny * ez)) * 576;
+                        for (int c = 0; c < 3; c++)
+                            for (int d = 0; d < 3; d++) acc[c][d] += blk[(3 * q1 + c) * 24 + 3 * q2 + d];
                     }
-        } else {
+            for (int c = 0; c < 3; c++) {
+                const int r = 3 * node + c;
+                if (!(inv[r] > 0.0f)) continue;
+                for (int d = 0; d < 3; d++)
+                    if (inv[3 * m + d] > 0.0f) A[r * n + 3 * m + d] = acc[c][d];
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += blockDim.x)
+            if (!(inv[r] > 0.0f) || !(A[r * n + r] > 0.0)) A[r * n + r] = 1.0;  // constrained or vanished row
+    } else {
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            const int node = r / 3, c = r % 3;
+            double *row = A + r * n;
+            if (!(inv[r] > 0.0f)) {
+                row[r] = 1.0;
+                continue;
+            }
+            int I, J, K;
+            node_ijk(g, node, I, J, K);
             for (int a = 0; a < 8; a++) {  // the element that holds this node as its corner a
                 const int ox = I - corner_x(a), oy = J - corner_y(a), oz = K - corner_z(a);
                 const double E = static_cast<double>(ep[(ox + 1) + g.px * ((oy + 1) + g.py * (oz + 1))]);
@@ -448,7 +494,6 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
                 }
             }
         }
-        if (!(row[r] > 0.0)) row[r] = 1.0;  // a free DOF whose Galerkin row vanished: decouple it
     }
     __syncthreads();
     constexpr int RS = 2, CS = (kCoarseMaxDofs + 7) / 8;
@@ -634,9 +679,15 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
     g->epoch = op->epoch;
 }
 
+GmgFineView gmg_fine_view(mycel_op *op) {
+    GmgLevel &F = op->gmg->lv[0];
+    return GmgFineView{F.b.get(), F.x.get(), F.dv.get(), F.inv.get(), op->gmg->theta_inv};
+}
+
 // z = M(r): one V-cycle. Also writes per-block partials of z.r and z.ap (gmg_partials(op) of each).
+// fine_ready: the caller has already written the finest level's b, x = d = D^-1 b / theta (gmg_fine_view).
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
-                        double *part_zap, hipStream_t s) {
+                        double *part_zap, hipStream_t s, bool fine_ready) {
     Gmg *g = op->gmg;
     const dim3 b256(256);
     const int top = g->levels - 1;
@@ -669,15 +720,16 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     ta.cr1 = g->cr[1];
 
     GmgLevel &F = g->lv[0];
-    hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_dofs, 256)), b256, 0, s, F.d.n_dofs, r64, F.inv.get(),
-                       g->theta_inv, F.b.get(), cur[0], F.dv.get());
+    if (!fine_ready)
+        hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_dofs, 256)), b256, 0, s, F.d.n_dofs, r64, F.inv.get(),
+                           g->theta_inv, F.b.get(), cur[0], F.dv.get());
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? kFineDegree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
         hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                            L.inv.get(), L.r.get());
-        hipLaunchKernelGGL(k_gmg_restrict, dim3(C.blocks), b256, 0, s, L.d, L.r.get(), C.d, C.inv.get(),
-                           g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
+        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256)), b256, 0, s, L.d, L.r.get(), C.d,
+                           C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
     hipLaunchKernelGGL(k_gmg_tail, dim3(1), dim3(1024), 0, s, ta);
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2

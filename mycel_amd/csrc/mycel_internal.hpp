@@ -201,8 +201,16 @@ void pcg_free(PcgWork *w);
 // op->epoch changed (enqueued on s, no host synchronisation).
 void gmg_prepare(mycel_op *op, hipStream_t s);
 // z = M(r) (one FP32 V-cycle) plus per-block partials of z.r and z.ap, gmg_partials(op) of each.
+// fine_ready: the finest level's b, x = d = D^-1 b / theta were written by the caller (gmg_fine_view).
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
-                        double *part_zap, hipStream_t s);
+                        double *part_zap, hipStream_t s, bool fine_ready = false);
+// The finest level's V-cycle inputs, for a caller that fuses their initialisation into its own kernel.
+struct GmgFineView {
+    float *b, *x, *d;
+    const float *inv;
+    float theta_inv;
+};
+GmgFineView gmg_fine_view(mycel_op *op);
 int gmg_partials(const mycel_op *op);
 void gmg_free(Gmg *g);
 

@@ -207,16 +207,20 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
     }
 }
 
-// Flexible PCG, step 2: alpha = rz / p.Ap (the breakdown tests of PcgSolver.cs:245-270); x += alpha p;
-// r -= alpha Ap; per-block partials of r.r. alpha is kept in the state for the flexible beta.
-__global__ __launch_bounds__(256) void k_fpcg_update_xr(int n, int slot, PcgState *__restrict__ st,
-                                                        const double *__restrict__ part_pap, int nb_mv,
-                                                        double *__restrict__ x, double *__restrict__ r,
-                                                        const double *__restrict__ p, const double *__restrict__ ap,
-                                                        double *__restrict__ part_rr) {
+// Flexible PCG, step 2, fused with the V-cycle's first step: alpha = rz / p.Ap (the breakdown tests of
+// PcgSolver.cs:245-270; alpha is kept in the state for the flexible beta); x += alpha p; r -= alpha Ap;
+// per-block partials of r.r; then, on the updated residual, b0 = float(r) (0 on constrained DOFs) and
+// x0 = d0 = D^-1 b0 / theta, the finest Chebyshev smoother's first step.
+__global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, PcgState *__restrict__ st,
+                                                            const double *__restrict__ part_pap, int nb_mv,
+                                                            double *__restrict__ x, double *__restrict__ r,
+                                                            const double *__restrict__ p,
+                                                            const double *__restrict__ ap,
+                                                            double *__restrict__ part_rr, GmgFineView fv) {
     __shared__ double red[4];
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
     double xv[4], pv[4], rv[4], av[4];
+    float iv[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = min(base + 256 * q, n - 1);
@@ -224,6 +228,7 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr(int n, int slot, PcgStat
         pv[q] = p[i];
         rv[q] = r[i];
         av[q] = ap[i];
+        iv[q] = fv.inv[i];
     }
     if (st->halt) return;
     const double pap = wave_sum_partials(part_pap, nb_mv);
@@ -247,6 +252,11 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr(int n, int slot, PcgStat
             const double ri = fmad(-alpha, av[q], rv[q]);
             r[i] = ri;
             rr = __fma_rn(ri, ri, rr);
+            const float bi = iv[q] > 0.0f ? static_cast<float>(ri) : 0.0f;
+            const float x0 = (iv[q] * bi) * fv.theta_inv;
+            fv.b[i] = bi;
+            fv.x[i] = x0;
+            fv.d[i] = x0;
         }
     }
     rr = block_sum256(rr, red);
@@ -530,9 +540,9 @@ void ensure_gmg(mycel_op *op, PcgWork *w) {
 void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
     PcgState *st = w->state.get();
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
-    hipLaunchKernelGGL(k_fpcg_update_xr, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
-                       w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->part2.get());
-    gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s);
+    hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+                       w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->part2.get(), gmg_fine_view(op));
+    gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true);
     hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(), w->nb_vec,
                        w->part_zr.get(), w->part_zap.get(), w->nb_z, w->p.get(), w->z.get());
 }
