@@ -217,6 +217,8 @@ typedef struct {
     int32_t enforce_min_filter_radius; /* 1 */
     int32_t continuation;       /* 0 */
     int32_t precision;          /* MYCEL_PRECISION_* for the linear solves */
+    int32_t enable_heaviside;   /* 0: smooth Heaviside projection of the filtered field, beta 1 -> 8
+                                   (Optimization/HeavisideProjection.cs, TopologyOptimizer.cs:193-201,281-289,315-318) */
 } mycel_simp_options;
 
 /* Fills the reference defaults (OptimizerOptions.cs:5-26). */

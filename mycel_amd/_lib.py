@@ -109,7 +109,8 @@ class SimpOptionsC(C.Structure):
                 ("move_limit", C.c_double), ("max_iterations", C.c_int32), ("solver_tol_initial", C.c_double),
                 ("solver_tol_final", C.c_double), ("solver_max_iterations", C.c_int32), ("rho_min", C.c_double),
                 ("protect_boundary_hinges", C.c_int32), ("start_iteration", C.c_int32),
-                ("enforce_min_filter_radius", C.c_int32), ("continuation", C.c_int32), ("precision", C.c_int32)]
+                ("enforce_min_filter_radius", C.c_int32), ("continuation", C.c_int32), ("precision", C.c_int32),
+                ("enable_heaviside", C.c_int32)]
 
 
 class SnapshotC(C.Structure):

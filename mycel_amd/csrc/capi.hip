@@ -184,7 +184,7 @@ void op_destroy(mycel_op *op) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// SIMP driver (Optimization/TopologyOptimizer.cs:16-407), Heaviside off.
+// SIMP driver (Optimization/TopologyOptimizer.cs:16-407), with optional p-continuation and Heaviside projection.
 // ------------------------------------------------------------------------------------------------
 void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_options &o, const uint8_t *passive_mask,
                      const int32_t *patch_nodes, int n_patch, const float *init_density, int init_len,
@@ -241,6 +241,9 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
     const int nab = std::max(1, na);
     DevBuf<double> d_rho(nab), d_rho_bar(nab), d_rho_old(nab), d_dc_bar(nab), d_dc(nab), d_part;
     DevBuf<double> d_dc_k(n_cases > 1 ? nab : 0);
+    const bool heav = o.enable_heaviside != 0;
+    DevBuf<double> d_rho_tilde(heav ? nab : 0);
+    double beta = 1.0;  // :156
     DevBuf<uint8_t> d_passive(nab), d_active_nodes(d.n_nodes);
     DevBuf<double> d_u(ndc), d_f(ndc), d_b(ndc);
     d_rho.upload(rho.data(), nab, s);
@@ -266,7 +269,12 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
         if (o.continuation) penal = iter < 20 ? 1.0 : (iter < 40 ? 2.0 : o.penal);
         launch_force_passive(na, d_passive.get(), d_rho.get(), s);
         const auto t_filter = clock::now();
-        flt.apply(d_rho.get(), d_rho_bar.get(), false, s);
+        if (heav) {  // :193-197
+            flt.apply(d_rho.get(), d_rho_tilde.get(), false, s);
+            launch_heaviside_project(na, d_rho_tilde.get(), d_rho_bar.get(), beta, s);
+        } else {
+            flt.apply(d_rho.get(), d_rho_bar.get(), false, s);
+        }
         MYCEL_HIP(hipStreamSynchronize(s));
         const double filter_ms = ms_since(t_filter);
         launch_scatter_density(na, op->d_active.get(), d_rho_bar.get(), op->d_density.get(), s);
@@ -311,6 +319,7 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
             sens_ms += ms_since(t_sens);
         }
         const auto t_sens = clock::now();
+        if (heav) launch_heaviside_chain(na, d_rho_tilde.get(), d_dc_bar.get(), beta, s);  // :281-289
         flt.apply(d_dc_bar.get(), d_dc.get(), true, s);
         MYCEL_HIP(hipStreamSynchronize(s));
         sens_ms += ms_since(t_sens);
@@ -319,8 +328,10 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
         lambda = oc_update_device(oc, d_rho.get(), d_dc.get(), flt.d_s.get(), d_passive.get(), d_rho.get(), na,
                                   o.volume_fraction, o.move_limit, o.rho_min, lambda, 60, 1e-8, s, nullptr);
         const double upd_ms = ms_since(t_upd);
+        if (heav && iter % 25 == 0 && beta < 8.0) beta = std::min(8.0, beta * 2.0);  // :315-318
         double met[3];
-        simp_metrics(na, d_rho.get(), d_rho_old.get(), d_rho_bar.get(), d_part, met, s);
+        simp_metrics(na, d_rho.get(), d_rho_old.get(), d_rho_bar.get(), heav ? d_rho_tilde.get() : nullptr, d_part,
+                     met, s);
         max_change = met[0];
         mycel_snapshot sn{};
         sn.iteration = iter;

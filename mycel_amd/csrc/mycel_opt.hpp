@@ -55,8 +55,10 @@ void launch_scatter_density(int na, const int32_t *d_active, const double *d_rho
 void launch_force_passive(int na, const uint8_t *d_passive, double *d_rho, hipStream_t s);
 void launch_project(int n, const uint8_t *d_flags, const double *d_f, double *d_b, hipStream_t s);
 void launch_add_into(int n, const double *d_src, double *d_dst, hipStream_t s);
-void simp_metrics(int na, const double *d_rho, const double *d_rho_old, const double *d_rho_bar, DevBuf<double> &part,
-                  double *h_out3, hipStream_t s);
+void simp_metrics(int na, const double *d_rho, const double *d_rho_old, const double *d_rho_bar,
+                  const double *d_grey_src, DevBuf<double> &part, double *h_out3, hipStream_t s);
+void launch_heaviside_project(int n, const double *d_rt, double *d_rb, double beta, hipStream_t s);
+void launch_heaviside_chain(int n, const double *d_rt, double *d_dc, double beta, hipStream_t s);
 void compliance_maxdisp(int n_nodes, const uint8_t *d_active_nodes, const double *d_f, const double *d_u,
                         DevBuf<double> &part, double *h_out2, hipStream_t s);
 

@@ -309,9 +309,12 @@ __global__ __launch_bounds__(256) void k_project(int n, const uint8_t *__restric
 }
 
 // Per-block partials: max|rho - rho_old|, sum rho_bar, sum rho_bar (1 - rho_bar) (TopologyOptimizer.cs:323-338).
+// grey_src: the field the grey level is measured on (rho_bar, or rho_tilde under Heaviside projection,
+// TopologyOptimizer.cs:333).
 __global__ __launch_bounds__(256) void k_simp_metrics(int na, const double *__restrict__ rho,
                                                       const double *__restrict__ rho_old,
-                                                      const double *__restrict__ rho_bar, double *__restrict__ part) {
+                                                      const double *__restrict__ rho_bar,
+                                                      const double *__restrict__ grey_src, double *__restrict__ part) {
     __shared__ double red[4];
     __shared__ double redm[4];
     double mx = 0.0, vol = 0.0, grey = 0.0;
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void k_simp_metrics(int na, const double *__re
         if (i < na) {
             mx = fmax(mx, fabs(rho[i] - rho_old[i]));
             vol += rho_bar[i];
-            grey += rho_bar[i] * (1.0 - rho_bar[i]);
+            grey += grey_src[i] * (1.0 - grey_src[i]);
         }
     }
 #pragma unroll
@@ -525,6 +528,41 @@ void launch_force_passive(int na, const uint8_t *d_passive, double *d_rho, hipSt
     MYCEL_LAUNCH_CHECK();
 }
 
+// HeavisideProjection.Project (Optimization/HeavisideProjection.cs:12-24):
+// rho_bar = clamp((tanh(beta eta) + tanh(beta (rt - eta))) * (1 / denom), 0, 1).
+__global__ __launch_bounds__(256) void k_heaviside_project(int n, const double *__restrict__ rt,
+                                                           double *__restrict__ rb, double beta, double eta) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double tbe = tanh(beta * eta);
+    const double inv_denom = 1.0 / (tbe + tanh(beta * (1.0 - eta)));
+    const double v = (tbe + tanh(beta * (rt[i] - eta))) * inv_denom;
+    rb[i] = fmin(fmax(v, 0.0), 1.0);
+}
+
+// dc *= max(beta (1 - t^2) / denom, 1e-12), t = tanh(beta (rt - eta)): the chain rule through the
+// projection (HeavisideProjection.cs:26-37, TopologyOptimizer.cs:281-289).
+__global__ __launch_bounds__(256) void k_heaviside_chain(int n, const double *__restrict__ rt, double *__restrict__ dc,
+                                                         double beta, double eta) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double inv_denom = 1.0 / (tanh(beta * eta) + tanh(beta * (1.0 - eta)));
+    const double t = tanh(beta * (rt[i] - eta));
+    dc[i] = dc[i] * fmax(beta * (1.0 - t * t) * inv_denom, 1e-12);
+}
+
+void launch_heaviside_project(int n, const double *d_rt, double *d_rb, double beta, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_heaviside_project, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d_rt, d_rb, beta, 0.5);
+    MYCEL_LAUNCH_CHECK();
+}
+
+void launch_heaviside_chain(int n, const double *d_rt, double *d_dc, double beta, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_heaviside_chain, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d_rt, d_dc, beta, 0.5);
+    MYCEL_LAUNCH_CHECK();
+}
+
 // dst += src (the per-load-case sensitivities summed in case order).
 __global__ __launch_bounds__(256) void k_add_into(int n, const double *__restrict__ src, double *__restrict__ dst) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -542,11 +580,12 @@ void launch_project(int n, const uint8_t *d_flags, const double *d_f, double *d_
     MYCEL_LAUNCH_CHECK();
 }
 
-void simp_metrics(int na, const double *d_rho, const double *d_rho_old, const double *d_rho_bar, DevBuf<double> &part,
-                  double *h_out3, hipStream_t s) {
+void simp_metrics(int na, const double *d_rho, const double *d_rho_old, const double *d_rho_bar,
+                  const double *d_grey_src, DevBuf<double> &part, double *h_out3, hipStream_t s) {
     const int nb = cdiv(na, 1024);
     if (part.size() < static_cast<size_t>(3 * nb + 3)) part.resize(3 * nb + 3);
-    hipLaunchKernelGGL(k_simp_metrics, dim3(nb), dim3(256), 0, s, na, d_rho, d_rho_old, d_rho_bar, part.get());
+    hipLaunchKernelGGL(k_simp_metrics, dim3(nb), dim3(256), 0, s, na, d_rho, d_rho_old, d_rho_bar,
+                       d_grey_src ? d_grey_src : d_rho_bar, part.get());
     hipLaunchKernelGGL(k_combine, dim3(1), dim3(256), 0, s, part.get(), nb, 3, 1, part.get() + 3 * nb);
     MYCEL_LAUNCH_CHECK();
     MYCEL_HIP(hipMemcpyAsync(h_out3, part.get() + 3 * nb, 3 * sizeof(double), hipMemcpyDeviceToHost, s));

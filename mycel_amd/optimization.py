@@ -217,12 +217,10 @@ class TopologyOptimizer:
     @staticmethod
     def _run(model, f, n_cases, options, progress, cancel, op):
         o = options or OptimizerOptions()
-        if o.EnableHeaviside:
-            raise L.ArgumentError(L.E_ARGUMENT, "Heaviside projection is not on the GPU path")
         co = L.SimpOptionsC(o.VolumeFraction, o.FilterRadiusMm, o.Penal, o.MoveLimit, o.MaxIterations,
                             o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
                             int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
-                            int(o.Continuation), int(o.Precision))
+                            int(o.Continuation), int(o.Precision), int(o.EnableHeaviside))
         patch = np.concatenate([lc.Patch.Nodes for lc in model.Loads]).astype(np.int32) if model.Loads else np.zeros(0, np.int32)
         pm = L.u8(o.PassiveMask) if o.PassiveMask is not None else None
         idn = np.ascontiguousarray(o.InitialDensity, np.float32) if o.InitialDensity is not None else None

@@ -32,9 +32,9 @@ PCG_MESSAGES = {
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+    """Compile liboracle.so with the committed Makefile (incremental: make rebuilds it when a source or
+    header is newer, so a stale library is never loaded)."""
+    subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
     return _LIB_PATH
 
 
@@ -61,7 +61,7 @@ class SimpOptions(C.Structure):
                 ("move_limit", C.c_double), ("max_iterations", C.c_int), ("solver_tol_initial", C.c_double),
                 ("solver_tol_final", C.c_double), ("solver_max_iterations", C.c_int), ("rho_min", C.c_double),
                 ("protect_boundary_hinges", C.c_int), ("start_iteration", C.c_int),
-                ("enforce_min_filter_radius", C.c_int), ("continuation", C.c_int)]
+                ("enforce_min_filter_radius", C.c_int), ("continuation", C.c_int), ("enable_heaviside", C.c_int)]
 
 
 class Snapshot(C.Structure):
@@ -114,6 +114,8 @@ def _declare(L):
     L.orc_oc_update.argtypes = [_dp, _dp, _dp, _vp, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
                                 C.c_double, C.c_int, C.c_double, C.POINTER(C.c_int)]
     L.orc_oc_update.restype = C.c_double
+    L.orc_heaviside_project.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double]
+    L.orc_heaviside_derivative.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double]
     L.orc_passive_region.argtypes = [C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.c_int, _up]
     L.orc_simp_run.argtypes = [_vp, _dp, C.POINTER(SimpOptions), _vp, _vp, C.c_int, _vp, C.c_int, _vp,
                                C.c_int, C.POINTER(Snapshot), C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -285,10 +287,10 @@ class Model:
              max_iterations=200, solver_tol_initial=1e-4, solver_tol_final=1e-6, solver_max_iterations=5000,
              rho_min=1e-3, protect_boundary_hinges=True, start_iteration=1, enforce_min_filter_radius=True,
              continuation=False, passive_mask=None, patch_nodes=None, initial_density=None, warm_u=None,
-             threads=0):
+             threads=0, enable_heaviside=False):
         o = SimpOptions(volume_fraction, filter_radius_mm, penal, move_limit, max_iterations, solver_tol_initial,
                         solver_tol_final, solver_max_iterations, rho_min, int(protect_boundary_hinges),
-                        start_iteration, int(enforce_min_filter_radius), int(continuation))
+                        start_iteration, int(enforce_min_filter_radius), int(continuation), int(enable_heaviside))
         snaps = (Snapshot * max(1, max_iterations - start_iteration + 1))()
         nsn, conv = C.c_int(), C.c_int()
         na = self.active_count
@@ -345,6 +347,20 @@ class Filter:
         y = np.zeros(self.n)
         lib().orc_filter_apply_adjoint(self._h, np.ascontiguousarray(x, np.float64), y)
         return y
+
+
+def heaviside_project(rho_tilde, beta, eta=0.5):
+    rt = np.ascontiguousarray(rho_tilde, np.float64)
+    out = np.zeros_like(rt)
+    lib().orc_heaviside_project(rt, out, rt.size, float(beta), float(eta))
+    return out
+
+
+def heaviside_derivative(rho_tilde, beta, eta=0.5):
+    rt = np.ascontiguousarray(rho_tilde, np.float64)
+    out = np.zeros_like(rt)
+    lib().orc_heaviside_derivative(rt, out, rt.size, float(beta), float(eta))
+    return out
 
 
 def oc_update(rho, dc, s, passive=None, vf=0.3, move=0.2, rho_min=1e-3, lambda_guess=1.0, max_bisect=60,

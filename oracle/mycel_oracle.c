@@ -977,7 +977,34 @@ void orc_passive_region(int nx, int ny, int nz, const int32_t *nodes, int n_node
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407), Heaviside off             */
+/* HeavisideProjection (Optimization/HeavisideProjection.cs:12-37): smooth threshold projection */
+/* ------------------------------------------------------------------------------------------ */
+
+/* rho_bar = clamp((tanh(beta eta) + tanh(beta (rt - eta))) * (1 / denom), 0, 1),
+ * denom = tanh(beta eta) + tanh(beta (1 - eta)) (:14-23). */
+void orc_heaviside_project(const double *rho_tilde, double *rho_bar, int n, double beta, double eta) {
+    const double denom = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+    const double inv_denom = 1.0 / denom;
+    const double tbe = tanh(beta * eta);
+    for (int i = 0; i < n; i++) {
+        const double num = tbe + tanh(beta * (rho_tilde[i] - eta));
+        rho_bar[i] = clampd(num * inv_denom, 0.0, 1.0);
+    }
+}
+
+/* d rho_bar / d rho_tilde = max(beta (1 - t^2) / denom, 1e-12), t = tanh(beta (rt - eta)) (:26-37). */
+void orc_heaviside_derivative(const double *rho_tilde, double *deriv, int n, double beta, double eta) {
+    const double denom = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+    const double inv_denom = 1.0 / denom;
+    for (int i = 0; i < n; i++) {
+        const double t = tanh(beta * (rho_tilde[i] - eta));
+        const double d = beta * (1.0 - t * t) * inv_denom;
+        deriv[i] = fmax(d, 1e-12);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407)                              */
 /* ------------------------------------------------------------------------------------------ */
 
 /* Several load cases (f, warm_u, u_out hold n_cases box-DOF vectors back to back): the objective is the
@@ -1039,6 +1066,10 @@ int orc_simp_run_cases(orc_model *m, const double *f, int n_cases, const orc_sim
         if (passive[i]) rho[i] = 1.0;
 
     double *rho_bar = (double *)calloc((size_t)na, sizeof(double));
+    const int heav = opt->enable_heaviside;
+    double *rho_tilde = (double *)calloc((size_t)na, sizeof(double));
+    double *dproj = (double *)calloc((size_t)na, sizeof(double));
+    double beta = 1.0; /* :156 */
     double *rho_old = (double *)calloc((size_t)na, sizeof(double));
     double *dc_bar = (double *)calloc((size_t)na, sizeof(double));
     double *dc = (double *)calloc((size_t)na, sizeof(double));
@@ -1061,7 +1092,12 @@ int orc_simp_run_cases(orc_model *m, const double *f, int n_cases, const orc_sim
         if (opt->continuation) penal = iter < 20 ? 1.0 : (iter < 40 ? 2.0 : opt->penal);
         for (int i = 0; i < na; i++)
             if (passive[i]) rho[i] = 1.0;
-        orc_filter_apply(flt, rho, rho_bar);
+        if (heav) { /* :193-197 */
+            orc_filter_apply(flt, rho, rho_tilde);
+            orc_heaviside_project(rho_tilde, rho_bar, na, beta, 0.5);
+        } else {
+            orc_filter_apply(flt, rho, rho_bar);
+        }
         for (int i = 0; i < na; i++) m->density[m->active[i]] = rho_bar[i];
         orc_update_densities(m, m->density, penal);
         double tol;
@@ -1100,16 +1136,22 @@ int orc_simp_run_cases(orc_model *m, const double *f, int n_cases, const orc_sim
             }
         }
         if (status != ORC_OK) break;
+        if (heav) { /* :281-289: chain rule through the projection, at the current beta */
+            orc_heaviside_derivative(rho_tilde, dproj, na, beta, 0.5);
+            for (int i = 0; i < na; i++) dc_bar[i] = dc_bar[i] * dproj[i];
+        }
         orc_filter_apply_adjoint(flt, dc_bar, dc);
         memcpy(rho_old, rho, (size_t)na * sizeof(double));
         lambda = orc_oc_update(rho, dc, s, passive, rho, na, vf, opt->move_limit, opt->rho_min, lambda, 60, 1e-8, NULL);
+        if (heav && iter % 25 == 0 && beta < 8.0) beta = fmin(8.0, beta * 2.0); /* :315-318 */
         max_change = 0.0;
         double vol = 0.0, grey = 0.0;
         for (int i = 0; i < na; i++) {
             double d = fabs(rho[i] - rho_old[i]);
             if (d > max_change) max_change = d;
             vol += rho_bar[i];
-            grey += rho_bar[i] * (1.0 - rho_bar[i]);
+            const double rb = heav ? rho_tilde[i] : rho_bar[i]; /* :333 */
+            grey += rb * (1.0 - rb);
         }
         orc_snapshot *sn = &snaps[(*n_snaps)++];
         sn->iteration = iter;
@@ -1139,6 +1181,7 @@ int orc_simp_run_cases(orc_model *m, const double *f, int n_cases, const orc_sim
     if (rho_bar_out) memcpy(rho_bar_out, rho_bar, (size_t)na * sizeof(double));
     if (u_out) memcpy(u_out, u, ndc * sizeof(double));
     free(passive); free(rho); free(rho_bar); free(rho_old); free(dc_bar); free(dc); free(dc_k); free(u); free(b);
+    free(rho_tilde); free(dproj);
     orc_filter_destroy(flt);
     return status;
 }

@@ -141,7 +141,12 @@ typedef struct {
     int start_iteration;
     int enforce_min_filter_radius;
     int continuation;
+    int enable_heaviside;
 } orc_simp_options;
+
+/* HeavisideProjection.Project / Derivative (Optimization/HeavisideProjection.cs:12-37). */
+void orc_heaviside_project(const double *rho_tilde, double *rho_bar, int n, double beta, double eta);
+void orc_heaviside_derivative(const double *rho_tilde, double *deriv, int n, double beta, double eta);
 
 typedef struct {
     int iteration;

@@ -325,3 +325,34 @@ def test_simp_cases_parity(gpu):
     # Each case's Jacobi-PCG may stop one iteration apart from the oracle's (reduction order on a residual
     # at the tolerance: the +-1 of the reference's own V10 counts), so the sum over 3 cases is held to +-3.
     assert abs(res.Snapshots[0].SolverIterations - ref["snapshots"][0]["solver_iterations"]) <= 3
+
+
+@pytest.mark.parametrize("iters", [1, 28], ids=["first-iteration", "through-beta-doubling"])
+def test_simp_heaviside_parity(gpu, iters):
+    """SIMP with the smooth Heaviside projection (beta 1, doubled at iteration 25), GPU against the oracle:
+    projected densities after one iteration at 1e-12 (device tanh vs the C library's differ by an ulp), the
+    compliance history and the densities through the beta step at 1e-8."""
+    p = W.cantilever_simp(24, 8, 8)
+    m = oracle_model(p)
+    f = oracle_forces(p)
+    kw = dict(volume_fraction=0.3, filter_radius_mm=1.5, penal=3.0, move_limit=0.2, max_iterations=iters,
+              solver_tol_initial=1e-8, solver_tol_final=1e-8, solver_max_iterations=20000,
+              protect_boundary_hinges=False, enable_heaviside=True)
+    ref = m.simp(f, **kw)
+    assert ref["status"] == 0
+    o = OptimizerOptions(VolumeFraction=0.3, FilterRadiusMm=1.5, Penal=3.0, MoveLimit=0.2, MaxIterations=iters,
+                         SolverToleranceInitial=1e-8, SolverToleranceFinal=1e-8, SolverMaxIterations=20000,
+                         ProtectBoundaryHinges=False, EnableHeaviside=True)
+    res = TopologyOptimizer.Run(p.fe_model(), o)
+    c = np.array([s.Compliance for s in res.Snapshots])
+    c_ref = np.array([s["compliance"] for s in ref["snapshots"]])
+    g = np.array([s.GreyLevel for s in res.Snapshots])
+    g_ref = np.array([s["grey_level"] for s in ref["snapshots"]])
+    print(f"[parity] heaviside {iters} it: compliance {np.max(np.abs(c - c_ref) / c_ref):.2e} "
+          f"rho_bar {np.abs(res.FinalPhysical64 - ref['rho_bar']).max():.2e}", flush=True)
+    assert len(c) == len(c_ref)
+    bar = 1e-12 if iters == 1 else 1e-8
+    assert np.max(np.abs(c - c_ref) / c_ref) < bar
+    assert np.abs(res.FinalPhysical64 - ref["rho_bar"]).max() < bar
+    assert np.abs(res.FinalDesign - ref["rho"]).max() < bar
+    assert np.max(np.abs(g - g_ref) / g_ref) < bar

@@ -457,3 +457,69 @@ def test_simp_cases_duplicate_case_doubles_compliance(oracle):
     assert np.allclose(c2, 2 * np.array(c1), rtol=1e-6)
     assert np.abs(r2["rho"] - r1["rho"]).max() <= 1e-6
     assert [s["solver_iterations"] for s in r2["snapshots"]] == [2 * s["solver_iterations"] for s in r1["snapshots"]]
+
+
+# ---------------------------------------------------------------------------------------------
+# Heaviside projection (HeavisideProjection.cs; OptimizationTests.cs PhaseB)
+# ---------------------------------------------------------------------------------------------
+def test_heaviside_finite_difference(oracle):
+    """PhaseB_HeavisideProjectionFiniteDifference (OptimizationTests.cs:806-850): central differences of
+    the projection against its derivative, beta 1..8, at 1e-5."""
+    rt = np.random.default_rng(54321).random(50)
+    h = 1e-6
+    for beta in (1.0, 2.0, 4.0, 8.0):
+        d = oracle.heaviside_derivative(rt, beta)
+        fd = (oracle.heaviside_project(rt + h, beta) - oracle.heaviside_project(rt - h, beta)) / (2 * h)
+        assert np.all(np.abs(fd - d) / np.maximum(np.abs(d), 1e-6) < 1e-5)
+
+
+def test_heaviside_closed_forms(oracle):
+    """Fixed points and symmetry of the eta = 1/2 projection: 0 -> 0, 1/2 -> 1/2, 1 -> 1, P(1/2 + t) +
+    P(1/2 - t) = 1 (tanh is odd); slope at 1/2 is beta / (2 tanh(beta / 2)); monotone; beta -> large is
+    the step."""
+    t = np.linspace(0.0, 0.5, 41)
+    for beta in (1.0, 2.0, 4.0, 8.0):
+        p0, ph, p1 = oracle.heaviside_project(np.array([0.0, 0.5, 1.0]), beta)
+        assert p0 == 0.0 and abs(ph - 0.5) < 1e-15 and abs(p1 - 1.0) < 1e-15
+        s = oracle.heaviside_project(0.5 + t, beta) + oracle.heaviside_project(0.5 - t, beta)
+        assert np.abs(s - 1.0).max() < 1e-14
+        assert abs(oracle.heaviside_derivative(np.array([0.5]), beta)[0] - beta / (2 * np.tanh(beta / 2))) < 1e-14
+        grid = oracle.heaviside_project(np.linspace(0, 1, 101), beta)
+        assert np.all(np.diff(grid) > 0)
+    steep = oracle.heaviside_project(np.array([0.3, 0.7]), 200.0)
+    assert steep[0] < 1e-12 and steep[1] > 1 - 1e-12
+    assert oracle.heaviside_derivative(np.array([0.0]), 200.0)[0] == 1e-12  # the floor
+
+
+def test_simp_heaviside_first_iteration_from_primitives(oracle):
+    """The Heaviside branch of one SIMP iteration rebuilt from the oracle's steps: rho_tilde = W rho,
+    rho_bar = P(rho_tilde), solve on rho_bar, dc chained through dP/drho_tilde before the adjoint filter,
+    OC; the grey level is measured on rho_tilde (TopologyOptimizer.cs:333)."""
+    p = W.cantilever_simp(12, 4, 4)
+    fa = oracle_forces(p)
+    kw = dict(volume_fraction=0.3, filter_radius_mm=1.5, max_iterations=1, solver_tol_initial=1e-10,
+              solver_tol_final=1e-10, solver_max_iterations=20000, protect_boundary_hinges=False)
+    r = oracle_model(p).simp(fa, enable_heaviside=True, **kw)
+    assert r["status"] == 0
+    m = oracle_model(p)
+    act = m.active_elements
+    flt = oracle.Filter(p.nx, p.ny, p.nz, p.domain, act, 1.5)
+    rho0 = np.full(act.size, 0.3)
+    rt = flt.apply(rho0)
+    rb = oracle.heaviside_project(rt, 1.0)
+    box = np.ones(p.n_elem)
+    box[act] = rb
+    m.update_densities(box, 3.0)
+    u = m.pcg(np.where(m.constrained, 0.0, fa), tol=1e-10, max_iter=20000)[0]
+    assert abs(r["snapshots"][0]["compliance"] - fa @ u) <= 1e-12 * abs(fa @ u)
+    dc = flt.apply_adjoint(m.sensitivities(u, rb, 3.0) * oracle.heaviside_derivative(rt, 1.0))
+    rho1, _, _ = oracle.oc_update(rho0, dc, flt.s, vf=0.3, move=0.2, rho_min=1e-3, lambda_guess=1.0)
+    assert np.abs(r["rho"] - rho1).max() <= 1e-12
+    assert np.abs(r["rho_bar"] - rb).max() == 0.0
+    grey = 4.0 * np.sum(rt * (1.0 - rt)) / act.size * 100.0
+    assert abs(r["snapshots"][0]["grey_level"] - grey) <= 1e-12 * grey
+    # From a uniform start rho_tilde is uniform, so the projection only rescales E everywhere: OC (invariant to
+    # scaling dc) gives the same first design, but the compliance of the projected field differs.
+    r_off = oracle_model(p).simp(fa, **kw)
+    c_on, c_off = r["snapshots"][0]["compliance"], r_off["snapshots"][0]["compliance"]
+    assert abs(c_on - c_off) / c_off > 1e-3
