@@ -163,6 +163,16 @@ MYCEL_API int32_t mycel_pcg_solve(mycel_op *op, const double *b, double *x, doub
 MYCEL_API int32_t mycel_pcg_solve_d(mycel_op *op, const double *d_b, double *d_x, double tol, int32_t max_iter,
                           int32_t precision, mycel_pcg_result *result);
 
+/* Several right-hand sides of one operator solved in lockstep (the multi-RHS form of PcgSolver.Solve; no
+ * counterpart in the reference, which solves one RHS at a time): d_b and d_x hold n_rhs box-DOF vectors
+ * back to back (x: warm starts in, solutions out); every RHS keeps its own stopping test and result. Kernels
+ * of all RHS share each launch. Host pointers (mycel_pcg_solve_batch) or device pointers (_d, synchronous on
+ * the handle's stream); precision must be MYCEL_PRECISION_MIXED_GMG. results: n_rhs entries. */
+MYCEL_API int32_t mycel_pcg_solve_batch(mycel_op *op, const double *b, double *x, int32_t n_rhs, double tol,
+                                        int32_t max_iter, int32_t precision, mycel_pcg_result *results);
+MYCEL_API int32_t mycel_pcg_solve_batch_d(mycel_op *op, const double *d_b, double *d_x, int32_t n_rhs, double tol,
+                                          int32_t max_iter, int32_t precision, mycel_pcg_result *results);
+
 /* ---- StaticAnalysis (Fem/StaticAnalysis.cs) ----------------------------------------------- */
 /* StaticAnalysis.Run(model, options) (Fem/StaticAnalysis.cs:35-142). f: assembled forces (box DOFs),
  * u: warm start in (zeros for a cold start), displacements out. reactions (box DOFs) and

@@ -198,6 +198,8 @@ SIGNATURES = {
     "mycel_server_stats": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), C.POINTER(_i)]),
     "mycel_simp_run": (_i, [_vp, _vp, C.POINTER(SimpOptionsC), _vp, _vp, _i, _vp, _i, _vp, SNAPSHOT_CB, _vp,
                             C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
+    "mycel_pcg_solve_batch": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, C.POINTER(PcgResultC)]),
+    "mycel_pcg_solve_batch_d": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, C.POINTER(PcgResultC)]),
     "mycel_simp_run_cases": (_i, [_vp, _vp, _i, C.POINTER(SimpOptionsC), _vp, _vp, _i, _vp, _i, _vp, SNAPSHOT_CB,
                                   _vp, C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
 }

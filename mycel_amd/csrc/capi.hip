@@ -241,6 +241,7 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
     const int nab = std::max(1, na);
     DevBuf<double> d_rho(nab), d_rho_bar(nab), d_rho_old(nab), d_dc_bar(nab), d_dc(nab), d_part;
     DevBuf<double> d_dc_k(n_cases > 1 ? nab : 0);
+    std::vector<mycel_pcg_result> case_res(n_cases);
     const bool heav = o.enable_heaviside != 0;
     DevBuf<double> d_rho_tilde(heav ? nab : 0);
     double beta = 1.0;  // :156
@@ -288,17 +289,29 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
             const double lf = std::log10(std::max(o.solver_tol_final, 1e-15));
             tol = std::pow(10.0, lf + a * (li - lf));
         }
-        // Load cases one after another (each warm-started from its own previous solution); compliance and
-        // solver iterations summed, residual and max|u| the largest; raw sensitivities summed in case order.
+        // Load cases (each warm-started from its own previous solution): with the multigrid solver all of them in
+        // one lockstep solve, otherwise one after another. Compliance and solver iterations summed, residual and
+        // max|u| the largest; raw sensitivities summed in case order.
         double solve_ms = 0.0, sens_ms = 0.0, cm[2] = {0.0, 0.0}, res_max = 0.0;
         int solve_iters = 0;
+        const bool batched = n_cases > 1 && o.precision == MYCEL_PRECISION_MIXED_GMG;
+        if (batched) {
+            const auto t_solve = clock::now();
+            pcg_solve_batch(op, d_b.get(), d_u.get(), n_cases, tol, o.solver_max_iterations, o.precision,
+                            case_res.data());
+            solve_ms += ms_since(t_solve);
+        }
         for (int k = 0; k < n_cases; k++) {
             double *uk = d_u.get() + k * nd;
-            const auto t_solve = clock::now();
             mycel_pcg_result pr;
-            pcg_solve_device(op, d_b.get() + k * nd, uk, tol, o.solver_max_iterations, o.precision, nullptr,
-                             nullptr, nullptr, &pr);
-            solve_ms += ms_since(t_solve);
+            if (batched) {
+                pr = case_res[k];
+            } else {
+                const auto t_solve = clock::now();
+                pcg_solve_device(op, d_b.get() + k * nd, uk, tol, o.solver_max_iterations, o.precision, nullptr,
+                                 nullptr, nullptr, &pr);
+                solve_ms += ms_since(t_solve);
+            }
             if (!pr.converged)
                 throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " +
                                                        std::to_string(pr.iterations) +
@@ -869,6 +882,32 @@ int32_t mycel_simp_run(mycel_op *op, const double *f, const mycel_simp_options *
         simp_run(op, f, 1, *opt, passive_mask_box, patch_nodes, n_patch_nodes, initial_density,
                  initial_density_len, warm_u, cb, user, result, snapshots, snapshots_capacity, rho_out, rho_bar_out,
                  u_out);
+    });
+}
+
+int32_t mycel_pcg_solve_batch(mycel_op *op, const double *b, double *x, int32_t n_rhs, double tol, int32_t max_iter,
+                              int32_t precision, mycel_pcg_result *results) {
+    return guarded([&] {
+        require(op && b && x && results, MYCEL_E_ARGUMENT_NULL, "op/b/x/results");
+        require(n_rhs >= 1, MYCEL_E_ARGUMENT, "n_rhs must be >= 1");
+        DeviceGuard dg(op->device);
+        const size_t n = static_cast<size_t>(op->d.n_dofs) * n_rhs;
+        DevBuf<double> d_b(n), d_x(n);
+        d_b.upload(b, n, op->stream);
+        d_x.upload(x, n, op->stream);
+        pcg_solve_batch(op, d_b.get(), d_x.get(), n_rhs, tol, max_iter, precision, results);
+        d_x.download(x, n, op->stream);
+        MYCEL_HIP(hipStreamSynchronize(op->stream));
+    });
+}
+
+int32_t mycel_pcg_solve_batch_d(mycel_op *op, const double *d_b, double *d_x, int32_t n_rhs, double tol,
+                                int32_t max_iter, int32_t precision, mycel_pcg_result *results) {
+    return guarded([&] {
+        require(op && d_b && d_x && results, MYCEL_E_ARGUMENT_NULL, "op/b/x/results");
+        require(n_rhs >= 1, MYCEL_E_ARGUMENT, "n_rhs must be >= 1");
+        DeviceGuard dg(op->device);
+        pcg_solve_batch(op, d_b, d_x, n_rhs, tol, max_iter, precision, results);
     });
 }
 

@@ -156,8 +156,13 @@ template <typename T, int MODE, bool DOT>
 __global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const KeSym<T> ks, const T *__restrict__ ep,
                                                     const T *__restrict__ x, const uint8_t *__restrict__ flags,
                                                     T *__restrict__ y, double *__restrict__ part,
-                                                    const int *__restrict__ stop) {
+                                                    const int *__restrict__ stop, Batch bt) {
     __shared__ double red[4];
+    const int rhs = blockIdx.y;  // Batch: one RHS per grid row
+    x += rhs * bt.vstride;
+    y += rhs * bt.vstride;
+    if (DOT) part += rhs * bt.pstride;
+    if (stop) stop += rhs * bt.sstride;
     if (DOT && stop && *stop) return;
     const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const double dotv = apply_k_node<T, MODE>(g, ks, ep, x, flags, y, n);
@@ -365,10 +370,11 @@ static size_t tile_lds_bytes(const Tile &t) {
 
 template <typename T>
 static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const T *ep, const T *d_x, T *d_y, int mode,
-                             hipStream_t s, double *d_dot_partial, const int *d_stop) {
-    const dim3 grid(apply_k_blocks(op)), block(256);
+                             hipStream_t s, double *d_dot_partial, const int *d_stop, const Batch &bt) {
+    const dim3 block(256);
     const uint8_t *fl = op->d_flags.get();
-    if (use_tiled()) {
+    if (use_tiled() && bt.count == 1) {
+        const dim3 grid(apply_k_blocks(op));
         const Tile tl = op->tile;
         const size_t lds = tile_lds_bytes<T>(tl);
         if (d_dot_partial) {
@@ -388,31 +394,32 @@ static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const T *ep, cons
         MYCEL_LAUNCH_CHECK();
         return;
     }
+    const dim3 grid(cdiv(op->d.n_nodes, 256), bt.count);
     if (d_dot_partial) {
         if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
         hipLaunchKernelGGL((k_apply_k<T, kApplyAProjected, true>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
-                           d_dot_partial, d_stop);
+                           d_dot_partial, d_stop, bt);
     } else if (mode == kApplyK) {
         hipLaunchKernelGGL((k_apply_k<T, kApplyK, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, nullptr,
-                           nullptr);
+                           nullptr, bt);
     } else if (mode == kApplyAProjected) {
         hipLaunchKernelGGL((k_apply_k<T, kApplyAProjected, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
-                           nullptr, nullptr);
+                           nullptr, nullptr, bt);
     } else {
         hipLaunchKernelGGL((k_apply_k<T, kApplyAGeneral, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
-                           nullptr, nullptr);
+                           nullptr, nullptr, bt);
     }
     MYCEL_LAUNCH_CHECK();
 }
 
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s, double *d_dot_partial,
-                    const int *d_stop) {
-    launch_apply_k_t<double>(op, op->ks64, op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop);
+                    const int *d_stop, const Batch &bt) {
+    launch_apply_k_t<double>(op, op->ks64, op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, bt);
 }
 
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s, double *d_dot_partial,
                         const int *d_stop) {
-    launch_apply_k_t<float>(op, op->ks32, op->d_ep32.get(), d_x, d_y, mode, s, d_dot_partial, d_stop);
+    launch_apply_k_t<float>(op, op->ks32, op->d_ep32.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, Batch{});
 }
 
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s) {

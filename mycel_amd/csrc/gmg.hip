@@ -157,12 +157,15 @@ __device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__
 }
 
 // One Chebyshev step (one node per thread).
+// Grid dimension y of every V-cycle kernel is the right-hand side: the level's vectors of RHS k start at
+// k * n_dofs (the moduli and D^-1 are shared).
 __global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, const float *__restrict__ ep,
                                                      const float *__restrict__ x, const float *__restrict__ b,
                                                      float *__restrict__ d, const float *__restrict__ inv, float cd,
                                                      float cr, float *__restrict__ x_out) {
     const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (n < g.n_nodes) cheb_node(g, ks, ep, x, b, d, inv, cd, cr, x_out, n);
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, n);
 }
 
 // The last post-smoothing step on the finest level: writes x' as the FP64 preconditioned residual z and
@@ -176,6 +179,15 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
                                                            double *__restrict__ part_zr, double *__restrict__ part_zap) {
     __shared__ double red[8];
     const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    d += o;
+    z64 += o;
+    r64 += o;
+    ap64 += o;
+    part_zr += blockIdx.y * gridDim.x;
+    part_zap += blockIdx.y * gridDim.x;
     double zr = 0.0, zap = 0.0;
     if (n < g.n_nodes) {
         int i, j, k;
@@ -204,7 +216,8 @@ __global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks
                                                          const float *__restrict__ x, const float *__restrict__ b,
                                                          const float *__restrict__ inv, float *__restrict__ r) {
     const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (n < g.n_nodes) residual_node(g, ks, ep, x, b, inv, r, n);
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    if (n < g.n_nodes) residual_node(g, ks, ep, x + o, b + o, inv, r + o, n);
 }
 
 // b0 = float(r64) on unconstrained DOFs; the finest pre-smoother's first step x = d = inv b0 / theta.
@@ -214,12 +227,13 @@ __global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__re
                                                        float *__restrict__ d) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const long long o = static_cast<long long>(blockIdx.y) * n;
     const float iv = inv[i];
-    const float bi = iv > 0.0f ? static_cast<float>(r64[i]) : 0.0f;
+    const float bi = iv > 0.0f ? static_cast<float>(r64[o + i]) : 0.0f;
     const float xi = (iv * bi) * theta_inv;
-    b[i] = bi;
-    x[i] = xi;
-    d[i] = xi;
+    b[o + i] = bi;
+    x[o + i] = xi;
+    d[o + i] = xi;
 }
 
 
@@ -231,6 +245,11 @@ __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *
                                                           float *__restrict__ dc) {
     const int dof = blockIdx.x * 256 + threadIdx.x;
     if (dof >= gc.n_dofs) return;
+    rf += static_cast<long long>(blockIdx.y) * gf.n_dofs;
+    const long long oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
+    bc += oc;
+    xcv += oc;
+    dc += oc;
     const int n = dof / 3, c = dof - 3 * n;
     int I, J, K;
     node_ijk(gc, n, I, J, K);
@@ -261,7 +280,9 @@ __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *
 __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict__ xf, const float *__restrict__ invf,
                                                      Dims gc, const float *__restrict__ xcv) {
     const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n < gf.n_nodes) prolong_node(gf, xf, invf, gc, xcv, n);
+    if (n < gf.n_nodes)
+        prolong_node(gf, xf + static_cast<long long>(blockIdx.y) * gf.n_dofs, invf, gc,
+                     xcv + static_cast<long long>(blockIdx.y) * gc.n_dofs, n);
 }
 
 // The small levels of the V-cycle in one workgroup: every phase is a loop over the level's nodes, phases
@@ -286,31 +307,38 @@ struct TailArgs {
 };
 
 __global__ __launch_bounds__(1024) void k_gmg_tail(TailArgs t) {
+    // One workgroup per right-hand side (blockIdx.x): level l's vectors of this RHS start at rhs * n_dofs(l).
     // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
+    const long long rhs = blockIdx.x;
+    auto off = [&](int l) { return rhs * t.lv[l].d.n_dofs; };
+    auto B = [&](int l) { return t.lv[l].b + off(l); };
+    auto D = [&](int l) { return t.lv[l].dv + off(l); };
+    auto Rv = [&](int l) { return t.lv[l].r + off(l); };
     unsigned flip = 0;
-    auto cur = [&](int l) { return ((flip >> l) & 1u) ? t.lv[l].x2 : t.lv[l].x; };
-    auto nxt = [&](int l) { return ((flip >> l) & 1u) ? t.lv[l].x : t.lv[l].x2; };
+    auto cur = [&](int l) { return (((flip >> l) & 1u) ? t.lv[l].x2 : t.lv[l].x) + off(l); };
+    auto nxt = [&](int l) { return (((flip >> l) & 1u) ? t.lv[l].x : t.lv[l].x2) + off(l); };
     const int last = t.count - 1;
     for (int l = 0; l < last; l++) {
         const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
         for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-            cheb_node(L.d, L.ks, L.ep, cur(l), L.b, L.dv, L.inv, t.cd1, t.cr1, nxt(l), n);
+            cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, t.cd1, t.cr1, nxt(l), n);
         __syncthreads();
         flip ^= 1u << l;
         for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-            residual_node(L.d, L.ks, L.ep, cur(l), L.b, L.inv, L.r, n);
+            residual_node(L.d, L.ks, L.ep, cur(l), B(l), L.inv, Rv(l), n);
         __syncthreads();
         for (int n = threadIdx.x; n < C.d.n_nodes; n += blockDim.x)
-            restrict_node(L.d, L.r, C.d, C.inv, t.theta_inv, C.b, cur(l + 1), C.dv, n);
+            restrict_node(L.d, Rv(l), C.d, C.inv, t.theta_inv, B(l + 1), cur(l + 1), D(l + 1), n);
         __syncthreads();
     }
     {
         const TailLevel &T = t.lv[last];
         const int n = T.d.n_dofs;
         float *xo = cur(last);
+        const float *bo = B(last);
         for (int row = threadIdx.x; row < n; row += blockDim.x) {
             double s = 0.0;
-            for (int j = 0; j < n; j++) s = __fma_rn(t.ainv[row * n + j], static_cast<double>(T.b[j]), s);
+            for (int j = 0; j < n; j++) s = __fma_rn(t.ainv[row * n + j], static_cast<double>(bo[j]), s);
             xo[row] = static_cast<float>(s);
         }
         __syncthreads();
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(1024) void k_gmg_tail(TailArgs t) {
         for (int pass = 0; pass < 2; pass++) {
             const float cd = pass == 0 ? 0.0f : t.cd1, cr = pass == 0 ? t.theta_inv : t.cr1;
             for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-                cheb_node(L.d, L.ks, L.ep, cur(l), L.b, L.dv, L.inv, cd, cr, nxt(l), n);
+                cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, cd, cr, nxt(l), n);
             __syncthreads();
             flip ^= 1u << l;
         }
@@ -575,7 +603,27 @@ struct Gmg {
     uint64_t epoch = ~0ull;
     float theta_inv = 0;
     float cd[kFineDegree], cr[kFineDegree];  // Chebyshev step k >= 1: d' = cd[k] d + cr[k] D^-1 r
+    int rhs_cap = 1;    // right-hand sides the level vectors hold
+    uint64_t gen = 0;   // bumped when the level vectors move (captured graphs must be rebuilt)
 };
+
+void gmg_reserve(mycel_op *op, int n_rhs) {
+    Gmg *g = op->gmg;
+    if (n_rhs <= g->rhs_cap) return;
+    for (int l = 0; l < g->levels; l++) {
+        GmgLevel &L = g->lv[l];
+        const size_t n = static_cast<size_t>(L.d.n_dofs) * n_rhs;
+        L.b.resize(n);
+        L.x.resize(n);
+        L.x2.resize(n);
+        L.dv.resize(n);
+        L.r.resize(n);
+    }
+    g->rhs_cap = n_rhs;
+    g->gen++;
+}
+
+uint64_t gmg_generation(const mycel_op *op) { return op->gmg ? op->gmg->gen : 0; }
 
 void gmg_free(Gmg *g) { delete g; }
 
@@ -687,9 +735,11 @@ GmgFineView gmg_fine_view(mycel_op *op) {
 // z = M(r): one V-cycle. Also writes per-block partials of z.r and z.ap (gmg_partials(op) of each).
 // fine_ready: the caller has already written the finest level's b, x = d = D^-1 b / theta (gmg_fine_view).
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
-                        double *part_zap, hipStream_t s, bool fine_ready) {
+                        double *part_zap, hipStream_t s, bool fine_ready, int n_rhs) {
     Gmg *g = op->gmg;
     const dim3 b256(256);
+    if (n_rhs < 1 || n_rhs > g->rhs_cap) throw Error(MYCEL_E_ARGUMENT, "multigrid: RHS count beyond the reserved");
+    const int R = n_rhs;
     const int top = g->levels - 1;
     float *cur[kMaxLevels], *nxt[kMaxLevels];
     for (int l = 0; l <= top; l++) {
@@ -698,7 +748,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     }
     auto cheb = [&](int l, float cd, float cr) {
         GmgLevel &L = g->lv[l];
-        hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(), L.dv.get(),
+        hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(), L.dv.get(),
                            L.inv.get(), cd, cr, nxt[l]);
         std::swap(cur[l], nxt[l]);
     };
@@ -721,28 +771,28 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
 
     GmgLevel &F = g->lv[0];
     if (!fine_ready)
-        hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_dofs, 256)), b256, 0, s, F.d.n_dofs, r64, F.inv.get(),
-                           g->theta_inv, F.b.get(), cur[0], F.dv.get());
+        hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_dofs, 256), R), b256, 0, s, F.d.n_dofs, r64,
+                           F.inv.get(), g->theta_inv, F.b.get(), cur[0], F.dv.get());
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? kFineDegree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
-        hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
+        hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                            L.inv.get(), L.r.get());
-        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256)), b256, 0, s, L.d, L.r.get(), C.d,
+        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
                            C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
-    hipLaunchKernelGGL(k_gmg_tail, dim3(1), dim3(1024), 0, s, ta);
+    hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(1024), 0, s, ta);
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
-        hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks), b256, 0, s, L.d, cur[l], L.inv.get(), C.d, cur[l + 1]);
+        hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv.get(), C.d, cur[l + 1]);
         cheb(l, 0.0f, g->theta_inv);
         for (int k = 1; k < (l == 0 ? kFineDegree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k]);
     }
     // Finest level: the last post-smoothing step writes z (FP64) and the partials. A single-level hierarchy
     // (a grid of <= kCoarseMaxDofs DOFs) was solved directly above: the step is then the identity (cd = cr = 0).
     const float cd = top == 0 ? 0.0f : g->cd[kFineDegree - 1], cr = top == 0 ? 0.0f : g->cr[kFineDegree - 1];
-    hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(), F.dv.get(),
+    hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(), F.dv.get(),
                        F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
     MYCEL_LAUNCH_CHECK();
 }

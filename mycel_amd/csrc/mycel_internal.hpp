@@ -117,6 +117,16 @@ void constitutive(double e, double nu, double *d36);
 struct PcgWork;
 struct Gmg;
 
+// Several right-hand sides in one launch: grid dimension y is the RHS index; vectors of RHS k start at
+// k * vstride, block partials at k * pstride, and the solver-state word that stops a launch at k * sstride
+// (in ints). count = 1 with zero strides is a single RHS.
+struct Batch {
+    int count = 1;
+    long long vstride = 0;
+    int pstride = 0;
+    int sstride = 0;
+};
+
 struct Tile {
     int tx, ty, tz;     // brick extent in nodes (tx * ty * tz <= 256; the other threads only stage data)
     int ntx, nty, ntz;  // bricks per axis
@@ -173,7 +183,7 @@ namespace mycel {
 void op_upload_model(mycel_op *op);
 void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStream_t s);
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s,
-                    double *d_dot_partial = nullptr, const int *d_stop = nullptr);
+                    double *d_dot_partial = nullptr, const int *d_stop = nullptr, const Batch &bt = Batch{});
 int apply_k_blocks(const mycel_op *op);
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s,
                         double *d_dot_partial = nullptr, const int *d_stop = nullptr);
@@ -195,6 +205,10 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
                          const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
                          mycel_pcg_result *res);
 void pcg_free(PcgWork *w);
+// R independent systems A x_k = b_k (b, x: R box-DOF vectors back to back) solved in lockstep, each with its
+// own state and stopping test; MYCEL_PRECISION_MIXED_GMG only. res: R results.
+int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, double tol, int max_iter, int precision,
+                        mycel_pcg_result *res);
 
 // ---- gmg.hip ---------------------------------------------------------------------------------
 // Builds the hierarchy on first use and refreshes its moduli, constraints and coarsest inverse when
@@ -202,8 +216,13 @@ void pcg_free(PcgWork *w);
 void gmg_prepare(mycel_op *op, hipStream_t s);
 // z = M(r) (one FP32 V-cycle) plus per-block partials of z.r and z.ap, gmg_partials(op) of each.
 // fine_ready: the finest level's b, x = d = D^-1 b / theta were written by the caller (gmg_fine_view).
+// n_rhs: right-hand sides in one pass (grid dimension y; vectors of RHS k at k * n_dofs of each level;
+// partials of RHS k at k * gmg_partials(op)); at most what gmg_reserve provided.
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
-                        double *part_zap, hipStream_t s, bool fine_ready = false);
+                        double *part_zap, hipStream_t s, bool fine_ready = false, int n_rhs = 1);
+// Sizes the level vectors for n_rhs right-hand sides (after gmg_prepare); bumps gmg_generation when they move.
+void gmg_reserve(mycel_op *op, int n_rhs);
+uint64_t gmg_generation(const mycel_op *op);
 // The finest level's V-cycle inputs, for a caller that fuses their initialisation into its own kernel.
 struct GmgFineView {
     float *b, *x, *d;

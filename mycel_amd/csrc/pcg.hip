@@ -68,6 +68,18 @@ struct PcgWork {
     hipGraphExec_t graph_gmg = nullptr;
     const double *graph_gmg_x = nullptr;
     const Gmg *graph_gmg_h = nullptr;        // hierarchy the multigrid graph was captured with
+    uint64_t graph_gmg_gen = 0;
+    // Several right-hand sides in lockstep (multigrid only): R slices of every vector and partial array,
+    // one state per RHS.
+    int b_cap = 0;
+    DevBuf<double> br, bz, bp, bap, bpart_pap, bpart_rr, bpart1, bpart_zr, bpart_zap;
+    DevBuf<PcgState> bstate;
+    PcgState *h_bstate = nullptr;            // pinned, b_cap entries
+    hipGraphExec_t graph_b = nullptr;
+    const double *graph_b_x = nullptr;
+    const Gmg *graph_b_h = nullptr;
+    uint64_t graph_b_gen = 0;
+    int graph_b_R = 0;
 };
 
 namespace {
@@ -218,6 +230,19 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, Pcg
                                                             const double *__restrict__ ap,
                                                             double *__restrict__ part_rr, GmgFineView fv) {
     __shared__ double red[4];
+    {  // grid dimension y: the right-hand side (vectors at y * n, partials at y * blocks, one state each)
+        const long long o = static_cast<long long>(blockIdx.y) * n;
+        st += blockIdx.y;
+        part_pap += blockIdx.y * nb_mv;
+        x += o;
+        r += o;
+        p += o;
+        ap += o;
+        part_rr += blockIdx.y * gridDim.x;
+        fv.b += o;
+        fv.x += o;
+        fv.d += o;
+    }
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
     double xv[4], pv[4], rv[4], av[4];
     float iv[4];
@@ -270,6 +295,15 @@ __global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState
                                                        const double *__restrict__ part_zr,
                                                        const double *__restrict__ part_zap, int nb_z,
                                                        double *__restrict__ p, const double *__restrict__ z) {
+    {  // grid dimension y: the right-hand side
+        const long long o = static_cast<long long>(blockIdx.y) * n;
+        st += blockIdx.y;
+        part_rr += blockIdx.y * nb_vec;
+        part_zr += blockIdx.y * nb_z;
+        part_zap += blockIdx.y * nb_z;
+        p += o;
+        z += o;
+    }
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
     double pv[4], zv[4];
 #pragma unroll
@@ -320,7 +354,8 @@ __global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState
 // p = z (the first search direction of the preconditioned iteration).
 __global__ __launch_bounds__(256) void k_copy(int n, const double *__restrict__ src, double *__restrict__ dst) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    const long long o = static_cast<long long>(blockIdx.y) * n;
+    if (i < n) dst[o + i] = src[o + i];
 }
 
 // x[constrained] = 0; flag a nonzero projected x; partials of ||Pb||^2 (PcgSolver.cs:113-137).
@@ -328,6 +363,13 @@ __global__ __launch_bounds__(256) void k_pcg_init(int n, const uint8_t *__restri
                                                   const double *__restrict__ b, double *__restrict__ x,
                                                   PcgState *__restrict__ st, double *__restrict__ part1) {
     __shared__ double red[4];
+    {  // grid dimension y: the right-hand side
+        const long long o = static_cast<long long>(blockIdx.y) * n;
+        b += o;
+        x += o;
+        st += blockIdx.y;
+        part1 += blockIdx.y * gridDim.x;
+    }
     double v = 0.0;
     int nz = 0;
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
@@ -353,6 +395,13 @@ __global__ __launch_bounds__(256) void k_pcg_init_r(int n, const uint8_t *__rest
                                                     const double *__restrict__ b, const double *__restrict__ ap,
                                                     double *__restrict__ r, double *__restrict__ part1) {
     __shared__ double red[4];
+    {  // grid dimension y: the right-hand side
+        const long long o = static_cast<long long>(blockIdx.y) * n;
+        b += o;
+        if (ap) ap += o;
+        r += o;
+        part1 += blockIdx.y * gridDim.x;
+    }
     double v = 0.0;
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
 #pragma unroll
@@ -455,10 +504,21 @@ __global__ __launch_bounds__(256) void k_mixed_update_p(int n, const PcgState *_
 }
 
 // Sums the partials into one state field (single block).
-__global__ __launch_bounds__(256) void k_pcg_sum_to(const double *__restrict__ part, int nb, double *__restrict__ out) {
+// Grid dimension y: the right-hand side (partials at y * nb, result at out + y * out_stride doubles).
+__global__ __launch_bounds__(256) void k_pcg_sum_to(const double *__restrict__ part, int nb, double *__restrict__ out,
+                                                    int out_stride) {
     __shared__ double red[4];
-    const double s = sum_partials256(part, nb, red);
-    if (threadIdx.x == 0) *out = s;
+    const double s = sum_partials256(part + blockIdx.y * nb, nb, red);
+    if (threadIdx.x == 0) out[static_cast<long long>(blockIdx.y) * out_stride] = s;
+}
+
+// A right-hand side that needs no iteration (zero RHS, converged start, no iterations allowed).
+__global__ void k_pcg_halt_at_start(PcgState *__restrict__ st, int message, double rel) {
+    st->message = message;
+    st->iterations = 0;
+    st->final_rel = rel;
+    st->done = 1;
+    st->halt = 1;
 }
 
 __global__ void k_pcg_start(PcgState *__restrict__ st, double tol, double bnorm, double rel, int max_iter, int mixed,
@@ -547,6 +607,45 @@ void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipS
                        w->part_zr.get(), w->part_zap.get(), w->nb_z, w->p.get(), w->z.get());
 }
 
+void ensure_batch(mycel_op *op, PcgWork *w, int R) {
+    if (R <= w->b_cap) return;
+    const size_t n = static_cast<size_t>(w->n) * R;
+    const int nb_node = cdiv(op->d.n_nodes, 256);
+    w->br.resize(n);
+    w->bz.resize(n);
+    w->bp.resize(n);
+    w->bap.resize(n);
+    w->bpart_pap.resize(static_cast<size_t>(nb_node) * R);
+    w->bpart_rr.resize(static_cast<size_t>(w->nb_vec) * R);
+    w->bpart1.resize(static_cast<size_t>(w->nb_vec) * R);
+    w->bpart_zr.resize(static_cast<size_t>(gmg_partials(op)) * R);
+    w->bpart_zap.resize(static_cast<size_t>(gmg_partials(op)) * R);
+    w->bstate.resize(R);
+    if (w->h_bstate) (void)hipHostFree(w->h_bstate);
+    w->h_bstate = nullptr;
+    MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_bstate), sizeof(PcgState) * R, hipHostMallocDefault));
+    w->b_cap = R;
+    if (w->graph_b) (void)hipGraphExecDestroy(w->graph_b);
+    w->graph_b = nullptr;
+}
+
+// One multigrid-preconditioned iteration for R right-hand sides (the kernels of enqueue_iteration_gmg on
+// grid rows y = 0..R-1).
+void enqueue_iteration_gmg_batch(mycel_op *op, PcgWork *w, double *d_X, int R, int slot, hipStream_t s) {
+    PcgState *st = w->bstate.get();
+    const int nb_node = cdiv(op->d.n_nodes, 256);
+    const int words = static_cast<int>(sizeof(PcgState) / sizeof(int));
+    const Batch bt{R, static_cast<long long>(w->n), nb_node, words};
+    launch_apply_k(op, w->bp.get(), w->bap.get(), kApplyAProjected, s, w->bpart_pap.get(), &st->halt, bt);
+    hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec, R), dim3(256), 0, s, w->n, slot, st,
+                       w->bpart_pap.get(), nb_node, d_X, w->br.get(), w->bp.get(), w->bap.get(), w->bpart_rr.get(),
+                       gmg_fine_view(op));
+    gmg_enqueue_vcycle(op, w->br.get(), w->bap.get(), w->bz.get(), w->bpart_zr.get(), w->bpart_zap.get(), s, true,
+                       R);
+    hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec, R), dim3(256), 0, s, w->n, slot, st, w->bpart_rr.get(),
+                       w->nb_vec, w->bpart_zr.get(), w->bpart_zap.get(), w->nb_z, w->bp.get(), w->bz.get());
+}
+
 void read_state(PcgWork *w, hipStream_t s) {
     MYCEL_HIP(hipMemcpyAsync(w->h_state, w->state.get(), sizeof(PcgState), hipMemcpyDeviceToHost, s));
     MYCEL_HIP(hipStreamSynchronize(s));
@@ -571,6 +670,8 @@ void pcg_free(PcgWork *w) {
     if (w->graph64) (void)hipGraphExecDestroy(w->graph64);
     if (w->graph32) (void)hipGraphExecDestroy(w->graph32);
     if (w->graph_gmg) (void)hipGraphExecDestroy(w->graph_gmg);
+    if (w->graph_b) (void)hipGraphExecDestroy(w->graph_b);
+    if (w->h_bstate) (void)hipHostFree(w->h_bstate);
     if (w->h_state) (void)hipHostFree(w->h_state);
     delete w;
 }
@@ -607,7 +708,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
 
     MYCEL_HIP(hipMemsetAsync(st, 0, sizeof(PcgState), s));
     hipLaunchKernelGGL(k_pcg_init, gv, b256, 0, s, n, op->d_flags.get(), d_b, d_x, st, w->part1.get());
-    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->bnorm_sq);
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->bnorm_sq, 0);
     MYCEL_LAUNCH_CHECK();
     read_state(w, s);
     const double bnorm = std::sqrt(w->h_state->bnorm_sq);
@@ -623,11 +724,11 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     if (warm) launch_apply_k(op, d_x, w->ap.get(), kApplyAProjected, s);
     hipLaunchKernelGGL(k_pcg_init_r, gv, b256, 0, s, n, op->d_flags.get(), d_b, warm ? w->ap.get() : nullptr,
                        w->r.get(), w->part1.get());
-    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->r_norm_sq);
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->r_norm_sq, 0);
     if (gmg) {
         gmg_enqueue_vcycle(op, w->r.get(), w->r.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s);
         hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), b256, 0, s, n, w->z.get(), w->p.get());
-        hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part_zr.get(), w->nb_z, &st->rz0);
+        hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part_zr.get(), w->nb_z, &st->rz0, 0);
     } else if (mixed) {
         hipLaunchKernelGGL(k_pcg_init_z<float>, gv, b256, 0, s, n, w->r.get(), w->r32.get(), op->d_inv_diag32.get(),
                            w->z32.get(), w->p32.get(), w->part1.get());
@@ -636,7 +737,8 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         hipLaunchKernelGGL(k_pcg_init_z<double>, gv, b256, 0, s, n, w->r.get(), w->r.get(), op->d_inv_diag.get(),
                            w->z.get(), w->p.get(), w->part1.get());
     }
-    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->rz0);
+    // rz0 = r.z: the Jacobi forms left its partials in part1; the multigrid form summed them above.
+    if (!gmg) hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->rz0, 0);
     MYCEL_LAUNCH_CHECK();
     read_state(w, s);
     // A cold start uses ||Pb||^2 itself as ||r||^2 (PcgSolver.cs:185-192).
@@ -661,7 +763,8 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
 
     hipGraphExec_t graph;
     if (gmg) {
-        if (w->graph_gmg && (w->graph_gmg_x != d_x || w->graph_gmg_h != op->gmg)) {
+        if (w->graph_gmg && (w->graph_gmg_x != d_x || w->graph_gmg_h != op->gmg ||
+                             w->graph_gmg_gen != gmg_generation(op))) {
             MYCEL_HIP(hipGraphExecDestroy(w->graph_gmg));
             w->graph_gmg = nullptr;
         }
@@ -671,6 +774,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             });
             w->graph_gmg_x = d_x;
             w->graph_gmg_h = op->gmg;
+            w->graph_gmg_gen = gmg_generation(op);
         }
         graph = w->graph_gmg;
     } else if (mixed) {
@@ -766,6 +870,129 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         }
     }
     res->elapsed_ms = elapsed();
+    return MYCEL_OK;
+}
+
+int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, double tol, int max_iter, int precision,
+                        mycel_pcg_result *res) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    if (R < 1) throw Error(MYCEL_E_ARGUMENT, "at least one right-hand side is required");
+    if (precision != MYCEL_PRECISION_MIXED_GMG)
+        throw Error(MYCEL_E_ARGUMENT, "several right-hand sides in one solve need MYCEL_PRECISION_MIXED_GMG");
+    if (op->active_fixed < 6)  // PcgSolver.cs:95-100
+        throw Error(MYCEL_E_INVALID_OPERATION,
+                    "Model has fewer than 6 unique fixed DOFs on active nodes (found " +
+                        std::to_string(op->active_fixed) +
+                        "). At least 6 fixed DOFs are required to prevent rigid body modes in 3D.");
+    for (int k = 0; k < R; k++) res[k] = mycel_pcg_result{};
+    ensure_work(op);
+    PcgWork *w = op->pcg;
+    hipStream_t s = op->stream;
+    gmg_prepare(op, s);
+    ensure_gmg(op, w);
+    gmg_reserve(op, R);
+    ensure_batch(op, w, R);
+    const int n = w->n;
+    const long long nn = n;
+    PcgState *st = w->bstate.get();
+    const dim3 b256(256);
+    const int dstride = static_cast<int>(sizeof(PcgState) / sizeof(double));
+    const int nb_node = cdiv(op->d.n_nodes, 256);
+    const Batch bt_plain{R, nn, 0, 0};
+
+    // The single-RHS set-up of pcg_solve_device, for every RHS at once (PcgSolver.cs:113-209): x[c] = 0,
+    // ||Pb||^2, r = P(b - A x) (exactly Pb for a zero start), z = M r, p = z, rz.
+    MYCEL_HIP(hipMemsetAsync(st, 0, sizeof(PcgState) * R, s));
+    hipLaunchKernelGGL(k_pcg_init, dim3(w->nb_vec, R), b256, 0, s, n, op->d_flags.get(), d_B, d_X, st,
+                       w->bpart1.get());
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1, R), b256, 0, s, w->bpart1.get(), w->nb_vec, &st->bnorm_sq, dstride);
+    launch_apply_k(op, d_X, w->bap.get(), kApplyAProjected, s, nullptr, nullptr, bt_plain);
+    hipLaunchKernelGGL(k_pcg_init_r, dim3(w->nb_vec, R), b256, 0, s, n, op->d_flags.get(), d_B, w->bap.get(),
+                       w->br.get(), w->bpart1.get());
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1, R), b256, 0, s, w->bpart1.get(), w->nb_vec, &st->r_norm_sq, dstride);
+    gmg_enqueue_vcycle(op, w->br.get(), w->br.get(), w->bz.get(), w->bpart_zr.get(), w->bpart_zap.get(), s, false, R);
+    hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256), R), b256, 0, s, n, w->bz.get(), w->bp.get());
+    hipLaunchKernelGGL(k_pcg_sum_to, dim3(1, R), b256, 0, s, w->bpart_zr.get(), w->nb_z, &st->rz0, dstride);
+    MYCEL_LAUNCH_CHECK();
+    MYCEL_HIP(hipMemcpyAsync(w->h_bstate, st, sizeof(PcgState) * R, hipMemcpyDeviceToHost, s));
+    MYCEL_HIP(hipStreamSynchronize(s));
+    int active = 0;
+    for (int k = 0; k < R; k++) {
+        const PcgState &h = w->h_bstate[k];
+        const double bnorm = std::sqrt(h.bnorm_sq);
+        if (bnorm <= 1e-30) {  // :139-151
+            MYCEL_HIP(hipMemsetAsync(d_X + k * nn, 0, sizeof(double) * n, s));
+            hipLaunchKernelGGL(k_pcg_halt_at_start, dim3(1), dim3(1), 0, s, st + k, MYCEL_PCG_RHS_ZERO, 0.0);
+            continue;
+        }
+        const double rr0 = h.nonzero_x ? h.r_norm_sq : h.bnorm_sq;
+        const double rel0 = std::sqrt(rr0) / bnorm;
+        if (rel0 <= tol) {
+            hipLaunchKernelGGL(k_pcg_halt_at_start, dim3(1), dim3(1), 0, s, st + k, MYCEL_PCG_INITIAL_CONVERGED, rel0);
+        } else if (max_iter <= 0) {
+            hipLaunchKernelGGL(k_pcg_halt_at_start, dim3(1), dim3(1), 0, s, st + k, MYCEL_PCG_MAX_ITERATIONS, rel0);
+        } else {
+            hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1), 0, s, st + k, tol, bnorm, rel0, max_iter, 0,
+                               kReliableDelta);
+            active++;
+        }
+    }
+    MYCEL_LAUNCH_CHECK();
+    if (active) {
+        if (w->graph_b && (w->graph_b_x != d_X || w->graph_b_h != op->gmg || w->graph_b_gen != gmg_generation(op) ||
+                           w->graph_b_R != R)) {
+            MYCEL_HIP(hipGraphExecDestroy(w->graph_b));
+            w->graph_b = nullptr;
+        }
+        if (!w->graph_b) {
+            w->graph_b = capture(s, [&] {
+                for (int t = 0; t < kGmgGraphIters; t++) enqueue_iteration_gmg_batch(op, w, d_X, R, t & 1, s);
+            });
+            w->graph_b_x = d_X;
+            w->graph_b_h = op->gmg;
+            w->graph_b_gen = gmg_generation(op);
+            w->graph_b_R = R;
+        }
+        const long long max_batches = static_cast<long long>(max_iter) + 64;
+        for (long long batch = 0;; batch++) {
+            if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "PCG batch loop did not terminate");
+            MYCEL_HIP(hipGraphLaunch(w->graph_b, s));
+            MYCEL_HIP(hipMemcpyAsync(w->h_bstate, st, sizeof(PcgState) * R, hipMemcpyDeviceToHost, s));
+            MYCEL_HIP(hipStreamSynchronize(s));
+            bool all = true;
+            for (int k = 0; k < R; k++) all = all && w->h_bstate[k].done;
+            if (all) break;
+        }
+    } else {
+        MYCEL_HIP(hipMemcpyAsync(w->h_bstate, st, sizeof(PcgState) * R, hipMemcpyDeviceToHost, s));
+        MYCEL_HIP(hipStreamSynchronize(s));
+    }
+    for (int k = 0; k < R; k++) {
+        const PcgState h = w->h_bstate[k];
+        mycel_pcg_result &r = res[k];
+        r.iterations = h.iterations;
+        r.final_residual = h.final_rel;
+        r.message = h.message;
+        r.converged = h.message == MYCEL_PCG_CONVERGED || h.message == MYCEL_PCG_RHS_ZERO ||
+                      h.message == MYCEL_PCG_INITIAL_CONVERGED;
+        const bool broke = h.message == MYCEL_PCG_CURVATURE_BREAKDOWN || h.message == MYCEL_PCG_STEP_BREAKDOWN ||
+                           h.message == MYCEL_PCG_DIRECTION_BREAKDOWN || h.message == MYCEL_PCG_STAGNATION;
+        if (broke && h.iterations < max_iter) {  // as the single-RHS multigrid solve: finish in FP64 Jacobi
+            mycel_pcg_result r64{};
+            pcg_solve_device(op, d_B + k * nn, d_X + k * nn, tol, max_iter - h.iterations, MYCEL_PRECISION_FP64,
+                             nullptr, nullptr, nullptr, &r64);
+            r.iterations = h.iterations + r64.iterations;
+            r.final_residual = r64.final_residual;
+            r.message = r64.message;
+            r.converged = r64.converged;
+            r.fallback_iterations = r64.iterations;
+        }
+    }
+    const double ms = elapsed();
+    for (int k = 0; k < R; k++) res[k].elapsed_ms = ms;
     return MYCEL_OK;
 }
 

@@ -374,6 +374,31 @@ class PcgSolver:
                                          cptr, cb, None, C.byref(res)))
         return _pcg_result(res)
 
+    def SolveBatch(self, op: FemOperator, B, X, tol=1e-6, maxIterations=5000,
+                   precision=L.PRECISION_MIXED_GMG) -> List["PcgResult"]:
+        """R systems in lockstep (mycel_pcg_solve_batch): B, X are C-contiguous (R, TotalDofs) float64 arrays;
+        X holds the warm starts and receives the solutions."""
+        if not (isinstance(X, np.ndarray) and X.dtype == np.float64 and X.flags["C_CONTIGUOUS"] and X.ndim == 2
+                and X.shape[1] == op._n):
+            raise L.ArgumentError(L.E_ARGUMENT, "X must be a contiguous float64 array of shape (R, TotalDofs)")
+        B = np.ascontiguousarray(B, np.float64)
+        if B.shape != X.shape:
+            raise L.ArgumentError(L.E_ARGUMENT, "B and X must have the same shape")
+        R = int(X.shape[0])
+        res = (L.PcgResultC * R)()
+        L.check(L.load().mycel_pcg_solve_batch(op._h, L.ptr(B), L.ptr(X), R, float(tol), int(maxIterations),
+                                               int(precision), res))
+        return [_pcg_result(r) for r in res]
+
+    def solve_batch_device(self, op: FemOperator, B_t, X_t, tol=1e-6, maxIterations=5000,
+                           precision=L.PRECISION_MIXED_GMG) -> List["PcgResult"]:
+        """R systems in lockstep (mycel_pcg_solve_batch_d): B_t, X_t are (R, TotalDofs) float64 device tensors."""
+        R = int(B_t.shape[0])
+        res = (L.PcgResultC * R)()
+        L.check(L.load().mycel_pcg_solve_batch_d(op._h, L.device_ptr(B_t), L.device_ptr(X_t), R, float(tol),
+                                                 int(maxIterations), int(precision), res))
+        return [_pcg_result(r) for r in res]
+
     def solve_device(self, op: FemOperator, b_t, x_t, tol=1e-6, maxIterations=5000, precision=L.PRECISION_FP64):
         res = L.PcgResultC()
         L.check(L.load().mycel_pcg_solve_d(op._h, L.device_ptr(b_t), L.device_ptr(x_t), float(tol), int(maxIterations),

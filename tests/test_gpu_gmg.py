@@ -133,3 +133,65 @@ def test_gmg_simp_matches_fp64_history(gpu):
     print(f"[gmg] simp compliance rel diff {np.max(np.abs(cg - c64) / c64):.2e} it fp64={it64} gmg={itg}", flush=True)
     assert np.max(np.abs(cg - c64) / c64) <= 1e-7
     assert sum(itg) < sum(it64)
+
+
+def test_gmg_batch_equals_single_solves(gpu):
+    """Several right-hand sides in one lockstep solve give, per RHS, bit-for-bit the single-RHS multigrid
+    solve: different loads (different iteration counts), a zero RHS and a warm start that is already
+    converged each keep their own result."""
+    p = W.cantilever_simp(nx=30, ny=10, nz=10)
+    p.density = random_field(7, p.n_elem, 0.05, 1.0)
+    m = oracle_model(p)
+    f = oracle_forces(p)
+    fb = np.zeros(p.n_dofs)
+    for j in range(p.ny + 1):
+        fb[3 * p.node(p.nx // 2, j, p.nz) + 1] = 0.5
+    B = np.stack([np.where(m.constrained, 0.0, f), np.where(m.constrained, 0.0, fb), np.zeros(p.n_dofs),
+                  np.where(m.constrained, 0.0, 3.0 * f + fb)])
+    op = FemOperator(p.fe_model())
+    try:
+        single_x, single_r = [], []
+        for k in range(4):
+            x = np.zeros(p.n_dofs)
+            single_r.append(PcgSolver().Solve(op, B[k], x, MultigridPreconditioner(), tol=1e-9, maxIterations=10000))
+            single_x.append(x)
+        X = np.zeros_like(B)
+        X[3] = single_x[3]  # warm start at the converged solution
+        rb = PcgSolver().SolveBatch(op, B, X, tol=1e-9, maxIterations=10000)
+    finally:
+        op.close()
+    print(f"[gmg] batch it={[r.Iterations for r in rb]} msg={[r.MessageCode for r in rb]} "
+          f"single it={[r.Iterations for r in single_r]}", flush=True)
+    for k in range(3):
+        assert rb[k].Converged and rb[k].Iterations == single_r[k].Iterations
+        assert rb[k].MessageCode == single_r[k].MessageCode
+        assert np.array_equal(X[k], single_x[k]), k
+    assert rb[2].MessageCode == 1 and not X[2].any()  # zero RHS -> zero solution
+    assert rb[3].MessageCode == 2 and rb[3].Iterations == 0  # initial guess already converged
+    assert np.array_equal(X[3], single_x[3])
+
+
+def test_simp_cases_multigrid_batched(gpu):
+    """Multi-case SIMP with the multigrid solver (all cases in one lockstep solve per iteration) follows the
+    oracle's FP64 Jacobi trajectory: both solve to 1e-10."""
+    p, cases = W.multi_rhs(nx=24, ny=8, nz=8, n_rhs=3)
+    import oracle as O
+    f = np.stack([O.assemble_forces(p.n_nodes, c) for c in cases])
+    kw = dict(volume_fraction=0.25, filter_radius_mm=2.0, penal=3.0, move_limit=0.2, max_iterations=6,
+              solver_tol_initial=1e-10, solver_tol_final=1e-10, solver_max_iterations=50000,
+              protect_boundary_hinges=False)
+    ref = oracle_model(p).simp(f, **kw)
+    assert ref["status"] == 0
+    o = OptimizerOptions(VolumeFraction=0.25, FilterRadiusMm=2.0, Penal=3.0, MoveLimit=0.2, MaxIterations=6,
+                         SolverToleranceInitial=1e-10, SolverToleranceFinal=1e-10, SolverMaxIterations=50000,
+                         ProtectBoundaryHinges=False, Precision=M.PRECISION_MIXED_GMG)
+    res = TopologyOptimizer.RunCases(p.fe_model(), f, o)
+    c = np.array([s.Compliance for s in res.Snapshots])
+    c_ref = np.array([s["compliance"] for s in ref["snapshots"]])
+    it = [s.SolverIterations for s in res.Snapshots]
+    it_ref = [s["solver_iterations"] for s in ref["snapshots"]]
+    print(f"[gmg] simp cases batched compliance {np.max(np.abs(c - c_ref) / c_ref):.2e} it={it} oracle it={it_ref}",
+          flush=True)
+    assert np.max(np.abs(c - c_ref) / c_ref) < 1e-7
+    assert np.abs(res.FinalDesign - ref["rho"]).max() < 1e-6
+    assert sum(it) < sum(it_ref)
