@@ -83,23 +83,36 @@ __device__ __forceinline__ void corner_rows_slab(const KeSym<T> &ks, const T (&l
     acc2 = on ? fma_t(E, t2, acc2) : acc2;
 }
 
-// One 3x3 layer of node values at node-row offsets rows[dy] (already including z) and columns xs[dx].
+// Load of a T at a byte offset from a wave-uniform base: the base stays in scalar registers and the offset is
+// one 32-bit VGPR, so each load costs no 64-bit address arithmetic and constant offsets fold into the
+// instruction.
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *__restrict__ base, uint32_t byte_off, int imm = 0) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + static_cast<size_t>(byte_off) + imm);
+}
+
+// One 3x3 layer of node values: row byte offsets rows[dy] (including z) and column byte offsets xs[dx], both
+// relative to xb (and, for the flags, fb with the same node numbering at one byte per DOF).
 template <typename T, int MODE>
-__device__ __forceinline__ void load_layer(const T *__restrict__ x, const uint8_t *__restrict__ flags,
-                                           const int (&xs)[3], const int (&rows)[3], T (&L)[9][3]) {
+__device__ __forceinline__ void load_layer(const T *__restrict__ xb, const uint8_t *__restrict__ fb,
+                                           const uint32_t (&xs)[3], const uint32_t (&rows)[3], T (&L)[9][3]) {
 #pragma unroll
     for (int dy = 0; dy < 3; dy++)
 #pragma unroll
         for (int dx = 0; dx < 3; dx++) {
-            const int m = 3 * (rows[dy] + xs[dx]);
+            const uint32_t o = rows[dy] + xs[dx];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                T v = x[m + d];
-                if (MODE == kApplyAGeneral && (flags[m + d] & kFlagConstrained)) v = T(0);
+                T v = ld_off(xb, o, d * static_cast<int>(sizeof(T)));
+                if (MODE == kApplyAGeneral && (ld_off(fb, o / static_cast<uint32_t>(sizeof(T)), d) & kFlagConstrained))
+                    v = T(0);
                 L[dx + 3 * dy][d] = v;
             }
         }
 }
+
+// First node layer of a launch's node range [n0, ...): the k0 argument of ku_gather for every node of it.
+__device__ __forceinline__ int first_layer(const Dims &g, int n0) { return n0 / (g.nx1 * g.ny1); }
 
 // Raw (K x) at node n = (i, j, k) into kx[3], and the node's own x values into xc[3] (masked like the
 // neighbours in kApplyAGeneral mode).
@@ -109,33 +122,48 @@ __device__ __forceinline__ void load_layer(const T *__restrict__ x, const uint8_
 // whose upper layer reuses the registers of the first slab's lower one. The E values and both first layers
 // are requested together up front, so the memory latencies overlap instead of chaining (no branch on the
 // moduli before the gather).
+//
+// Addressing: k0 <= k is a node layer shared by the whole wave (the first layer of the caller's node range).
+// x, flags and ep are rebased at layers max(k0 - 1, 0) and k0, which keeps every offset of the gather within
+// four node layers, and each neighbour is read at a 32-bit byte offset from that uniform base.
 template <typename T, int MODE>
 __device__ __forceinline__ void ku_gather(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
                                           const T *__restrict__ x, const uint8_t *__restrict__ flags, int i, int j,
-                                          int k, T (&kx)[3], T (&xc)[3]) {
-    const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
+                                          int k, int k0, T (&kx)[3], T (&xc)[3]) {
+    const int zb = k0 > 0 ? k0 - 1 : 0;
+    const size_t layer = static_cast<size_t>(g.nx1) * g.ny1;
+    const T *xb = x + 3 * layer * zb;
+    const uint8_t *fb = MODE == kApplyAGeneral ? flags + 3 * layer * zb : nullptr;
+    const T *eb = ep + static_cast<size_t>(g.px) * g.py * k0;
+    constexpr uint32_t sT = sizeof(T);
+    const uint32_t ex = sT, ey = sT * g.px, ez = ey * g.py;  // E strides (bytes)
+    const uint32_t e0 = sT * ((i + 1) + g.px * ((j + 1) + g.py * (k + 1 - k0)));
     T E[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) E[a] = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
-    const int xs[3] = {i > 0 ? i - 1 : 0, i, i < g.nx ? i + 1 : g.nx};
-    const int ys[3] = {j > 0 ? j - 1 : 0, j, j < g.ny ? j + 1 : g.ny};
-    const int zs[3] = {k > 0 ? k - 1 : 0, k, k < g.nz ? k + 1 : g.nz};
-    int rows[3];
+    for (int a = 0; a < 8; a++)
+        E[a] = ld_off(eb, e0 - (corner_x(a) ? ex : 0u) - (corner_y(a) ? ey : 0u) - (corner_z(a) ? ez : 0u));
+    const uint32_t sx = 3 * sT, sy = sx * g.nx1, sz = sy * g.ny1;  // node strides (bytes)
+    const uint32_t oc = sx * i;
+    const uint32_t xs[3] = {i > 0 ? oc - sx : oc, oc, i < g.nx ? oc + sx : oc};
+    const uint32_t oy[3] = {j > 0 ? sy * (j - 1) : 0u, sy * j, j < g.ny ? sy * (j + 1) : sy * j};
+    const uint32_t zc = sz * (k - zb);
+    const uint32_t oz[3] = {k > 0 ? zc - sz : zc, zc, k < g.nz ? zc + sz : zc};
+    uint32_t rows[3];
     T lo[9][3], mid[9][3];
 #pragma unroll
-    for (int dy = 0; dy < 3; dy++) rows[dy] = g.nx1 * (ys[dy] + g.ny1 * zs[0]);
-    load_layer<T, MODE>(x, flags, xs, rows, lo);
+    for (int dy = 0; dy < 3; dy++) rows[dy] = oy[dy] + oz[0];
+    load_layer<T, MODE>(xb, fb, xs, rows, lo);
 #pragma unroll
-    for (int dy = 0; dy < 3; dy++) rows[dy] = g.nx1 * (ys[dy] + g.ny1 * zs[1]);
-    load_layer<T, MODE>(x, flags, xs, rows, mid);
+    for (int dy = 0; dy < 3; dy++) rows[dy] = oy[dy] + oz[1];
+    load_layer<T, MODE>(xb, fb, xs, rows, mid);
     T acc0 = 0, acc1 = 0, acc2 = 0;
     corner_rows_slab<4>(ks, lo, mid, E[4], acc0, acc1, acc2);
     corner_rows_slab<5>(ks, lo, mid, E[5], acc0, acc1, acc2);
     corner_rows_slab<6>(ks, lo, mid, E[6], acc0, acc1, acc2);
     corner_rows_slab<7>(ks, lo, mid, E[7], acc0, acc1, acc2);
 #pragma unroll
-    for (int dy = 0; dy < 3; dy++) rows[dy] = g.nx1 * (ys[dy] + g.ny1 * zs[2]);
-    load_layer<T, MODE>(x, flags, xs, rows, lo);  // layer k+1
+    for (int dy = 0; dy < 3; dy++) rows[dy] = oy[dy] + oz[2];
+    load_layer<T, MODE>(xb, fb, xs, rows, lo);  // layer k+1
     corner_rows_slab<0>(ks, mid, lo, E[0], acc0, acc1, acc2);
     corner_rows_slab<1>(ks, mid, lo, E[1], acc0, acc1, acc2);
     corner_rows_slab<2>(ks, mid, lo, E[2], acc0, acc1, acc2);
@@ -147,11 +175,12 @@ __device__ __forceinline__ void ku_gather(const Dims &g, const KeSym<T> &ks, con
     for (int d = 0; d < 3; d++) xc[d] = mid[4][d];
 }
 
-// Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op.
+// Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op. k0: as in
+// ku_gather.
 template <typename T, int MODE>
 __device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
                                                const T *__restrict__ x, const uint8_t *__restrict__ flags,
-                                               T *__restrict__ y, int n) {
+                                               T *__restrict__ y, int n, int k0) {
     double dotv = 0.0;
     if (n >= g.n_nodes) return dotv;
     const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
@@ -161,7 +190,7 @@ __device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks
         for (int d = 0; d < 3; d++) fl[d] = flags[3 * n + d];
     }
     T out[3], xc[3];
-    ku_gather<T, MODE>(g, ks, ep, x, flags, i, j, k, out, xc);
+    ku_gather<T, MODE>(g, ks, ep, x, flags, i, j, k, k0, out, xc);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int dof = 3 * n + d;

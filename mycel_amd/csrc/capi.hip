@@ -76,6 +76,9 @@ mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *f
     const long long total = static_cast<long long>(g.nx) * g.ny * g.nz;
     const long long nodes = static_cast<long long>(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
     require(3 * nodes < (1LL << 31) - 64, MYCEL_E_OUT_OF_RANGE, "grid too large for 32-bit DOF indices");
+    // The K*u gather addresses a few node layers with 32-bit byte offsets: at most 2^25 nodes per layer.
+    require(static_cast<long long>(g.nx + 1) * (g.ny + 1) <= (1LL << 25), MYCEL_E_OUT_OF_RANGE,
+            "grid cross-section (nx+1)(ny+1) beyond 2^25 nodes");
     (void)total;
     auto op = std::make_unique<mycel_op>();
     op->device = g_device;

@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const KeSym<T> ks, c
     if (DOT) part += rhs * bt.pstride;
     if (stop) stop += rhs * bt.sstride;
     if (DOT && stop && *stop) return;
-    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const double dotv = apply_k_node<T, MODE>(g, ks, ep, x, flags, y, n);
+    const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
+    const double dotv = apply_k_node<T, MODE>(g, ks, ep, x, flags, y, n, first_layer(g, n0));
     if (DOT) {
         const double s = block_sum256(dotv, red);
         if (threadIdx.x == 0) part[blockIdx.x] = s;

@@ -23,13 +23,15 @@
 // restriction b_c = P^T r_f (27 fine nodes per coarse node with weights 1, 1/2, 1/4, 1/8); the restriction
 // also starts the coarse pre-smoother (x = d = D^-1 b / theta), which saves a launch per level.
 //
-// Coarsest level (<= kCoarseMaxDofs DOFs): one workgroup assembles the dense FP64 matrix into LDS (one row
-// per thread, fixed element order), inverts it by Gauss-Jordan elimination (SPD, no pivoting) and writes
-// the inverse; each V-cycle then applies it as a dense product. Everything is deterministic: no atomics.
+// Coarsest level (<= kCoarseMaxDofs DOFs): one workgroup assembles the dense FP64 Galerkin matrix into LDS
+// (fixed element order), inverts it by Gauss-Jordan elimination (SPD, no pivoting) and writes the inverse
+// column-major; each V-cycle then applies it as a dense product spread over all waves of the tail
+// workgroup. Everything is deterministic: no atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 
 #include "apply_k.hpp"
@@ -61,11 +63,11 @@ __device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, i
 __device__ __forceinline__ void cheb_node(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
                                           const float *__restrict__ x, const float *__restrict__ b,
                                           float *__restrict__ d, const float *__restrict__ inv, float cd, float cr,
-                                          float *__restrict__ x_out, int n) {
+                                          float *__restrict__ x_out, int n, int k0) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
     float kx[3], xc[3];
-    ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, kx, xc);
+    ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, k0, kx, xc);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int dof = 3 * n + c;
@@ -79,11 +81,11 @@ __device__ __forceinline__ void cheb_node(const Dims &g, const KeSym<float> &ks,
 // r = b - K x on unconstrained DOFs, 0 on constrained ones.
 __device__ __forceinline__ void residual_node(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
                                               const float *__restrict__ x, const float *__restrict__ b,
-                                              const float *__restrict__ inv, float *__restrict__ r, int n) {
+                                              const float *__restrict__ inv, float *__restrict__ r, int n, int k0) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
     float kx[3], xc[3];
-    ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, kx, xc);
+    ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, k0, kx, xc);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int dof = 3 * n + c;
@@ -163,9 +165,9 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, co
                                                      const float *__restrict__ x, const float *__restrict__ b,
                                                      float *__restrict__ d, const float *__restrict__ inv, float cd,
                                                      float cr, float *__restrict__ x_out) {
-    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, n);
+    if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, n, first_layer(g, n0));
 }
 
 // The last post-smoothing step on the finest level: writes x' as the FP64 preconditioned residual z and
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
                                                            const double *__restrict__ ap64,
                                                            double *__restrict__ part_zr, double *__restrict__ part_zap) {
     __shared__ double red[8];
-    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     x += o;
     b += o;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
         int i, j, k;
         node_ijk(g, n, i, j, k);
         float kx[3], xc[3];
-        ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, kx, xc);
+        ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, first_layer(g, n0), kx, xc);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int dof = 3 * n + c;
@@ -215,9 +217,9 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
 __global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks, const float *__restrict__ ep,
                                                          const float *__restrict__ x, const float *__restrict__ b,
                                                          const float *__restrict__ inv, float *__restrict__ r) {
-    const int n = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    if (n < g.n_nodes) residual_node(g, ks, ep, x + o, b + o, inv, r + o, n);
+    if (n < g.n_nodes) residual_node(g, ks, ep, x + o, b + o, inv, r + o, n, first_layer(g, n0));
 }
 
 // b0 = float(r64) on unconstrained DOFs; the finest pre-smoother's first step x = d = inv b0 / theta.
@@ -291,7 +293,17 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
 // the level above; lv[count-1] is the coarsest level, solved with its dense inverse. On return the
 // corrected lv[0] iterate is in x2 (three buffer swaps: one pre- and two post-smoothing steps).
 constexpr int kMaxTail = 8;
-constexpr int kTailNodes = 128;  // levels up to this many nodes run in the tail (larger ones are FMA-bound on one CU)
+constexpr int kTailThreads = 512;  // 8 waves: up to 256 VGPRs each, so the three inlined gathers do not spill
+constexpr int kTailNodes = 128;  // default: levels up to this many nodes run in the tail (larger ones are FMA-bound
+                                 // on one CU); MYCEL_GMG_TAIL_NODES overrides it
+
+int tail_nodes() {
+    static const int v = [] {
+        const char *e = std::getenv("MYCEL_GMG_TAIL_NODES");
+        return e && *e ? std::min(std::max(0, std::atoi(e)), 1 << 16) : kTailNodes;
+    }();
+    return v;
+}
 struct TailLevel {
     Dims d;
     KeSym<float> ks;
@@ -306,7 +318,7 @@ struct TailArgs {
     float theta_inv, cd1, cr1;
 };
 
-__global__ __launch_bounds__(1024) void k_gmg_tail(TailArgs t) {
+__global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
     // One workgroup per right-hand side (blockIdx.x): level l's vectors of this RHS start at rhs * n_dofs(l).
     // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
     const long long rhs = blockIdx.x;
@@ -321,25 +333,43 @@ __global__ __launch_bounds__(1024) void k_gmg_tail(TailArgs t) {
     for (int l = 0; l < last; l++) {
         const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
         for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-            cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, t.cd1, t.cr1, nxt(l), n);
+            cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, t.cd1, t.cr1, nxt(l), n, 0);
         __syncthreads();
         flip ^= 1u << l;
         for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-            residual_node(L.d, L.ks, L.ep, cur(l), B(l), L.inv, Rv(l), n);
+            residual_node(L.d, L.ks, L.ep, cur(l), B(l), L.inv, Rv(l), n, 0);
         __syncthreads();
         for (int n = threadIdx.x; n < C.d.n_nodes; n += blockDim.x)
             restrict_node(L.d, Rv(l), C.d, C.inv, t.theta_inv, B(l + 1), cur(l + 1), D(l + 1), n);
         __syncthreads();
     }
     {
-        const TailLevel &T = t.lv[last];
-        const int n = T.d.n_dofs;
+        // Coarsest level: x = A^-1 b with all waves. Thread (part, row) sums columns [part * span, +span) of its
+        // row (the inverse is stored column-major, so a wave's loads are contiguous and all of a thread's loads
+        // are in flight together); the kParts partial sums of a row are added in a fixed order.
+        constexpr int kParts = kTailThreads / kCoarseMaxDofs, kSpan = (kCoarseMaxDofs + kParts - 1) / kParts;
+        __shared__ double bsh[kCoarseMaxDofs];
+        __shared__ double psum[kParts][kCoarseMaxDofs];
+        const int n = t.lv[last].d.n_dofs, span = (n + kParts - 1) / kParts;
         float *xo = cur(last);
         const float *bo = B(last);
-        for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        if (threadIdx.x < n) bsh[threadIdx.x] = static_cast<double>(bo[threadIdx.x]);
+        __syncthreads();
+        const int part = threadIdx.x / kCoarseMaxDofs, row = threadIdx.x - part * kCoarseMaxDofs;
+        if (part < kParts && row < n) {
             double s = 0.0;
-            for (int j = 0; j < n; j++) s = __fma_rn(t.ainv[row * n + j], static_cast<double>(bo[j]), s);
-            xo[row] = static_cast<float>(s);
+#pragma unroll
+            for (int q = 0; q < kSpan; q++) {
+                const int j = part * span + q;
+                if (q < span && j < n) s = __fma_rn(t.ainv[static_cast<size_t>(j) * n + row], bsh[j], s);
+            }
+            psum[part][row] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < n) {
+            double s = psum[0][threadIdx.x];
+            for (int p = 1; p < kParts; p++) s += psum[p][threadIdx.x];
+            xo[threadIdx.x] = static_cast<float>(s);
         }
         __syncthreads();
     }
@@ -350,7 +380,7 @@ __global__ __launch_bounds__(1024) void k_gmg_tail(TailArgs t) {
         for (int pass = 0; pass < 2; pass++) {
             const float cd = pass == 0 ? 0.0f : t.cd1, cr = pass == 0 ? t.theta_inv : t.cr1;
             for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-                cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, cd, cr, nxt(l), n);
+                cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, cd, cr, nxt(l), n, 0);
             __syncthreads();
             flip ^= 1u << l;
         }
@@ -456,7 +486,7 @@ __global__ __launch_bounds__(256) void k_gmg_galerkin_blocks(Dims gs, const floa
 
 // Coarsest level: assemble the dense matrix into LDS (one row per thread, fixed order; identity rows and zero
 // columns on constrained DOFs), invert it by Gauss-Jordan elimination (SPD, no pivoting) and store the
-// inverse. With `blocks` the matrix is the Galerkin product P^T A_s P summed from the second-coarsest
+// inverse column-major (ainv[j * n + i] = (A^-1)_ij). With `blocks` the matrix is the Galerkin product P^T A_s P summed from the second-coarsest
 // level's element blocks; without (a single-level hierarchy) it is assembled from Ke and the level's moduli.
 // The elimination keeps the matrix in registers: thread (ti, tj) = (t / 8, t % 8) owns rows ti + 128 s and
 // columns tj + 8 c; each step broadcasts the pivot column and the scaled pivot row through LDS.
@@ -579,7 +609,7 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
 #pragma unroll
         for (int c = 0; c < CS; c++) {
             const int i = ti + 128 * s, j = tj + 8 * c;
-            if (i < n && j < n) ainv[i * n + j] = a[s][c];
+            if (i < n && j < n) ainv[j * n + i] = a[s][c];  // column-major
         }
 }
 
@@ -755,7 +785,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     // Levels [t0, top] run in the single-workgroup tail kernel (the finest level never does: it has the FP64
     // entry and exit kernels).
     int t0 = top;
-    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= kTailNodes) t0--;
+    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= tail_nodes()) t0--;
     if (top == 0) t0 = 0;
     TailArgs ta{};
     ta.count = top - t0 + 1;
@@ -781,7 +811,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
                            C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
-    hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(1024), 0, s, ta);
+    hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(kTailThreads), 0, s, ta);
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
