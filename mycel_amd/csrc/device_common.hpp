@@ -49,31 +49,68 @@ __device__ __forceinline__ double sum_partials256(const double *__restrict__ par
 // partials l, l + 64, ... and the lanes are then combined by the butterfly. Every wave of every block
 // gets the bit-identical value.
 // Four independent accumulators (lane l, stride 256, offsets 0/64/128/192) keep the dependent-add chain
-// short; they are combined as (a0 + a1) + (a2 + a3) before the butterfly.
-__device__ __forceinline__ double wave_sum_partials(const double *__restrict__ part, int n) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int i = threadIdx.x & 63; i < n; i += 256) {
-        a0 += part[i];
-        if (i + 64 < n) a1 += part[i + 64];
-        if (i + 128 < n) a2 += part[i + 128];
-        if (i + 192 < n) a3 += part[i + 192];
+// short; they are combined as (a0 + a1) + (a2 + a3) before the butterfly. The loads of a round (16 per lane
+// and array, 1024 partials) are all issued before the first add, so up to 1024 partials cost one memory
+// round trip instead of one per 256; K arrays are summed together, each in exactly that order.
+template <int K>
+__device__ __forceinline__ void wave_sum_partials_n(const double *const (&part)[K], const int (&n)[K],
+                                                    double (&out)[K]) {
+    constexpr int U = 16;
+    double acc[K][4];
+    int nmax = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0;
+        nmax = n[k] > nmax ? n[k] : nmax;
     }
-    return wave_sum((a0 + a1) + (a2 + a3));
+    for (int b = threadIdx.x & 63; b < nmax; b += 64 * U) {
+        double v[K][U];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int i = b + 64 * q;
+                v[k][q] = i < n[k] ? part[k][i] : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int q = 0; q < U; q++) acc[k][q & 3] += v[k][q];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = wave_sum((acc[k][0] + acc[k][1]) + (acc[k][2] + acc[k][3]));
 }
 
-// Two interleaved partial arrays (part[2i], part[2i + 1]) summed per wave as above.
+__device__ __forceinline__ double wave_sum_partials(const double *__restrict__ part, int n) {
+    const double *const p[1] = {part};
+    const int m[1] = {n};
+    double s[1];
+    wave_sum_partials_n<1>(p, m, s);
+    return s[0];
+}
+
+// Two interleaved partial arrays (part[2i], part[2i + 1]) summed per wave: lane l adds entries l, l + 128, ...
+// into one accumulator and l + 64, l + 192, ... into another, then a0 + a1 and the butterfly. A round's
+// loads (8 per lane and array) are issued together.
 __device__ __forceinline__ void wave_sum_partials2(const double *__restrict__ part, int n, double &s0, double &s1) {
-    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-    for (int i = threadIdx.x & 63; i < n; i += 128) {
-        a0 += part[2 * i];
-        b0 += part[2 * i + 1];
-        if (i + 64 < n) {
-            a1 += part[2 * (i + 64)];
-            b1 += part[2 * (i + 64) + 1];
+    constexpr int U = 8;
+    double a[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+    for (int b = threadIdx.x & 63; b < n; b += 64 * U) {
+        double va[U], vb[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int i = b + 64 * q;
+            va[q] = i < n ? part[2 * i] : 0.0;
+            vb[q] = i < n ? part[2 * i + 1] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            a[q & 1] += va[q];
+            c[q & 1] += vb[q];
         }
     }
-    s0 = wave_sum(a0 + a1);
-    s1 = wave_sum(b0 + b1);
+    s0 = wave_sum(a[0] + a[1]);
+    s1 = wave_sum(c[0] + c[1]);
 }
 
 // Two fixed-order block sums with one LDS round trip.

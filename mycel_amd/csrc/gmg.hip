@@ -93,38 +93,53 @@ __device__ __forceinline__ void residual_node(const Dims &g, const KeSym<float> 
     }
 }
 
+// Restriction taps along one axis: fine index 2C + d for d = -1, 0, 1 with weights 1/2, 1, 1/2; a tap
+// outside [0, nf] gets weight 0 and a clamped (valid) index, so all 27 loads of a coarse node are
+// independent and issued together.
+__device__ __forceinline__ void restrict_taps(int C, int nf, int (&idx)[3], float (&w)[3]) {
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int f = 2 * C + t - 1;
+        const bool in = f >= 0 && f <= nf;
+        idx[t] = in ? f : (f < 0 ? 0 : nf);
+        w[t] = in ? (t == 1 ? 1.0f : 0.5f) : 0.0f;
+    }
+}
+
 // b_c = P^T r_f at coarse node n, then the coarse pre-smoother's first step x_c = d_c = inv_c b_c / theta.
+// The taps are summed z-major, then y, then x; taps outside the fine grid add nothing.
 __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
                                               const float *__restrict__ invc, float theta_inv, float *__restrict__ bc,
                                               float *__restrict__ xcv, float *__restrict__ dc, int n) {
     int I, J, K;
     node_ijk(gc, n, I, J, K);
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-    for (int dk = -1; dk <= 1; dk++) {
-        const int k = 2 * K + dk;
-        if (k < 0 || k > gf.nz) continue;
-        const float wz = dk == 0 ? 1.0f : 0.5f;
-        for (int dj = -1; dj <= 1; dj++) {
-            const int j = 2 * J + dj;
-            if (j < 0 || j > gf.ny) continue;
-            const float wyz = wz * (dj == 0 ? 1.0f : 0.5f);
-            for (int di = -1; di <= 1; di++) {
-                const int i = 2 * I + di;
-                if (i < 0 || i > gf.nx) continue;
-                const float w = wyz * (di == 0 ? 1.0f : 0.5f);
-                const int m = 3 * (i + gf.nx1 * (j + gf.ny1 * k));
-                s0 += w * rf[m];
-                s1 += w * rf[m + 1];
-                s2 += w * rf[m + 2];
-            }
-        }
+    int ix[3], iy[3], iz[3];
+    float wx[3], wy[3], wz[3];
+    restrict_taps(I, gf.nx, ix, wx);
+    restrict_taps(J, gf.ny, iy, wy);
+    restrict_taps(K, gf.nz, iz, wz);
+    float v[27][3];  // all taps loaded first, then summed: one memory round trip
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        const int m = 3 * (ix[t % 3] + gf.nx1 * (iy[(t / 3) % 3] + gf.ny1 * iz[t / 9]));
+        v[t][0] = rf[m];
+        v[t][1] = rf[m + 1];
+        v[t][2] = rf[m + 2];
     }
-    const float s[3] = {s0, s1, s2};
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        const float w = wz[t / 9] * wy[(t / 3) % 3] * wx[t % 3];
+        s0 += w != 0.0f ? w * v[t][0] : 0.0f;
+        s1 += w != 0.0f ? w * v[t][1] : 0.0f;
+        s2 += w != 0.0f ? w * v[t][2] : 0.0f;
+    }
+    const float sv[3] = {s0, s1, s2};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int dof = 3 * n + c;
         const float iv = invc[dof];
-        const float bi = iv > 0.0f ? s[c] : 0.0f;
+        const float bi = iv > 0.0f ? sv[c] : 0.0f;
         const float xi = (iv * bi) * theta_inv;
         bc[dof] = bi;
         xcv[dof] = xi;
@@ -133,29 +148,41 @@ __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__res
 }
 
 // x_f += P x_c at fine node n, unconstrained DOFs only (trilinear: weight 1 at an even fine index, 1/2 from
-// each neighbour at an odd one).
+// each neighbour at an odd one). All 8 candidate coarse nodes are read (indices always valid); those that
+// do not contribute have weight 0 and add nothing.
 __device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__ xf, const float *__restrict__ invf,
                                              const Dims &gc, const float *__restrict__ xcv, int n) {
     int i, j, k;
     node_ijk(gf, n, i, j, k);
     const int is[2] = {i >> 1, (i + 1) >> 1}, js[2] = {j >> 1, (j + 1) >> 1}, ks_[2] = {k >> 1, (k + 1) >> 1};
-    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-    const float w = ((i & 1) ? 0.5f : 1.0f) * ((j & 1) ? 0.5f : 1.0f) * ((k & 1) ? 0.5f : 1.0f);
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-    for (int c = 0; c < nk; c++)
-        for (int b = 0; b < nj; b++)
-            for (int a = 0; a < ni; a++) {
-                const int m = 3 * (is[a] + gc.nx1 * (js[b] + gc.ny1 * ks_[c]));
-                s0 += w * xcv[m];
-                s1 += w * xcv[m + 1];
-                s2 += w * xcv[m + 2];
-            }
-    const float s[3] = {s0, s1, s2};
+    const float wx[2] = {(i & 1) ? 0.5f : 1.0f, (i & 1) ? 0.5f : 0.0f};
+    const float wy[2] = {(j & 1) ? 0.5f : 1.0f, (j & 1) ? 0.5f : 0.0f};
+    const float wz[2] = {(k & 1) ? 0.5f : 1.0f, (k & 1) ? 0.5f : 0.0f};
+    float v[8][3], iv[3], xo[3];  // every load first, then the sums: one memory round trip
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int m = 3 * (is[t & 1] + gc.nx1 * (js[(t >> 1) & 1] + gc.ny1 * ks_[t >> 2]));
+        v[t][0] = xcv[m];
+        v[t][1] = xcv[m + 1];
+        v[t][2] = xcv[m + 2];
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const int dof = 3 * n + c;
-        if (invf[dof] > 0.0f) xf[dof] += s[c];
+        iv[c] = invf[3 * n + c];
+        xo[c] = xf[3 * n + c];
     }
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const float w = wx[t & 1] * wy[(t >> 1) & 1] * wz[t >> 2];
+        s0 += w != 0.0f ? w * v[t][0] : 0.0f;
+        s1 += w != 0.0f ? w * v[t][1] : 0.0f;
+        s2 += w != 0.0f ? w * v[t][2] : 0.0f;
+    }
+    const float sv[3] = {s0, s1, s2};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        if (iv[c] > 0.0f) xf[3 * n + c] = xo[c] + sv[c];
 }
 
 // One Chebyshev step (one node per thread).
@@ -255,23 +282,21 @@ __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *
     const int n = dof / 3, c = dof - 3 * n;
     int I, J, K;
     node_ijk(gc, n, I, J, K);
-    float sum = 0.0f;
-    for (int dk = -1; dk <= 1; dk++) {
-        const int k = 2 * K + dk;
-        if (k < 0 || k > gf.nz) continue;
-        const float wz = dk == 0 ? 1.0f : 0.5f;
-        for (int dj = -1; dj <= 1; dj++) {
-            const int j = 2 * J + dj;
-            if (j < 0 || j > gf.ny) continue;
-            const float wyz = wz * (dj == 0 ? 1.0f : 0.5f);
-            for (int di = -1; di <= 1; di++) {
-                const int i = 2 * I + di;
-                if (i < 0 || i > gf.nx) continue;
-                sum += (wyz * (di == 0 ? 1.0f : 0.5f)) * rf[3 * (i + gf.nx1 * (j + gf.ny1 * k)) + c];
-            }
-        }
-    }
+    int ix[3], iy[3], iz[3];
+    float wx[3], wy[3], wz[3];
+    restrict_taps(I, gf.nx, ix, wx);
+    restrict_taps(J, gf.ny, iy, wy);
+    restrict_taps(K, gf.nz, iz, wz);
+    float v[27];  // all taps loaded first, then summed: one memory round trip
+#pragma unroll
+    for (int t = 0; t < 27; t++) v[t] = rf[3 * (ix[t % 3] + gf.nx1 * (iy[(t / 3) % 3] + gf.ny1 * iz[t / 9])) + c];
     const float iv = invc[dof];
+    float sum = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        const float w = wz[t / 9] * wy[(t / 3) % 3] * wx[t % 3];
+        sum += w != 0.0f ? w * v[t] : 0.0f;
+    }
     const float bi = iv > 0.0f ? sum : 0.0f;
     const float xi = (iv * bi) * theta_inv;
     bc[dof] = bi;
@@ -287,15 +312,19 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
                      xcv + static_cast<long long>(blockIdx.y) * gc.n_dofs, n);
 }
 
-// The small levels of the V-cycle in one workgroup: every phase is a loop over the level's nodes, phases
-// are separated by workgroup barriers (the waves of one workgroup share the CU's L1, so a barrier orders
-// their global writes and reads). lv[0] arrives with b, x = d = inv b / theta set by the restriction from
-// the level above; lv[count-1] is the coarsest level, solved with its dense inverse. On return the
-// corrected lv[0] iterate is in x2 (three buffer swaps: one pre- and two post-smoothing steps).
+// The small levels of the V-cycle in one workgroup, resident in LDS. On entry every thread loads its share
+// of lv[0]'s b, x, d (written by the restriction from the level above) and of every tail level's moduli and
+// inverse diagonal, all loads in flight together (one memory round trip), and stores them into LDS; every
+// phase after that (smoothing, residual, restriction, coarsest solve, prolongation) reads and writes LDS
+// only, separated by workgroup barriers, and the corrected lv[0] iterate is written back at the end. A
+// chain of global accesses would cost a memory round trip (~1 us) per phase instead. lv[count-1] is the
+// coarsest level, solved with its dense inverse (read from global memory, column-major).
 constexpr int kMaxTail = 8;
-constexpr int kTailThreads = 512;  // 8 waves: up to 256 VGPRs each, so the three inlined gathers do not spill
-constexpr int kTailNodes = 128;  // default: levels up to this many nodes run in the tail (larger ones are FMA-bound
-                                 // on one CU); MYCEL_GMG_TAIL_NODES overrides it
+constexpr int kTailThreads = 512;    // 8 waves: up to 256 VGPRs each, so the inlined gathers do not spill
+constexpr int kTailPer = 3;          // a tail level's arrays hold <= kTailPer * kTailThreads floats each
+constexpr int kTailNodes = 512;      // default: levels up to this many nodes run in the tail (larger ones are
+                                     // FMA-bound on one CU); MYCEL_GMG_TAIL_NODES overrides it
+constexpr int kTailLdsMax = 144 << 10;  // dynamic LDS of the tail (gfx950: 160 KiB per workgroup)
 
 int tail_nodes() {
     static const int v = [] {
@@ -307,53 +336,84 @@ int tail_nodes() {
 struct TailLevel {
     Dims d;
     KeSym<float> ks;
-    const float *ep;
-    const float *inv;
-    float *b, *x, *x2, *dv, *r;
+    const float *ep;   // padded moduli (global)
+    const float *inv;  // inverse diagonal (global)
+    int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
 };
 struct TailArgs {
     TailLevel lv[kMaxTail];
     int count;
     const double *ainv;
+    const float *b0, *x0, *d0;  // lv[0]'s b, x, d (global, RHS 0; RHS k at k * n_dofs)
+    float *out0;                // where lv[0]'s corrected iterate goes (global, same RHS layout)
     float theta_inv, cd1, cr1;
 };
 
 __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
-    // One workgroup per right-hand side (blockIdx.x): level l's vectors of this RHS start at rhs * n_dofs(l).
+    extern __shared__ __align__(16) float sm[];
+    constexpr int kParts = kTailThreads / kCoarseMaxDofs, kSpan = (kCoarseMaxDofs + kParts - 1) / kParts;
+    __shared__ double bsh[kCoarseMaxDofs];
+    __shared__ double psum[kParts][kCoarseMaxDofs];
+    // One workgroup per right-hand side (blockIdx.x).
+    const long long o0 = static_cast<long long>(blockIdx.x) * t.lv[0].d.n_dofs;
+    {
+        // Segments: lv[0]'s b, x, d, then (moduli, inverse diagonal) of each level; every segment holds
+        // <= kTailPer * kTailThreads floats (checked on the host).
+        constexpr int kSeg = 3 + 2 * kMaxTail;
+        const int nseg = 3 + 2 * t.count;
+        float v[kSeg][kTailPer];
+#pragma unroll
+        for (int sg = 0; sg < kSeg; sg++) {
+            const TailLevel &L = t.lv[sg < 3 ? 0 : (sg - 3) >> 1];
+            const float *src = sg == 0 ? t.b0 + o0 : sg == 1 ? t.x0 + o0 : sg == 2 ? t.d0 + o0
+                               : ((sg - 3) & 1) ? L.inv : L.ep;
+            const int n = sg >= nseg ? 0 : (sg >= 3 && !((sg - 3) & 1)) ? L.d.n_pad : L.d.n_dofs;
+#pragma unroll
+            for (int u = 0; u < kTailPer; u++) {
+                const int i = threadIdx.x + u * kTailThreads;
+                v[sg][u] = i < n ? src[i] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int sg = 0; sg < kSeg; sg++) {
+            const TailLevel &L = t.lv[sg < 3 ? 0 : (sg - 3) >> 1];
+            const int dst = sg == 0 ? L.o_b : sg == 1 ? L.o_x : sg == 2 ? L.o_d : ((sg - 3) & 1) ? L.o_inv : L.o_e;
+            const int n = sg >= nseg ? 0 : (sg >= 3 && !((sg - 3) & 1)) ? L.d.n_pad : L.d.n_dofs;
+#pragma unroll
+            for (int u = 0; u < kTailPer; u++) {
+                const int i = threadIdx.x + u * kTailThreads;
+                if (i < n) sm[dst + i] = v[sg][u];
+            }
+        }
+    }
+    __syncthreads();
     // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
-    const long long rhs = blockIdx.x;
-    auto off = [&](int l) { return rhs * t.lv[l].d.n_dofs; };
-    auto B = [&](int l) { return t.lv[l].b + off(l); };
-    auto D = [&](int l) { return t.lv[l].dv + off(l); };
-    auto Rv = [&](int l) { return t.lv[l].r + off(l); };
     unsigned flip = 0;
-    auto cur = [&](int l) { return (((flip >> l) & 1u) ? t.lv[l].x2 : t.lv[l].x) + off(l); };
-    auto nxt = [&](int l) { return (((flip >> l) & 1u) ? t.lv[l].x : t.lv[l].x2) + off(l); };
+    auto S = [&](int off) { return sm + off; };
+    auto cur = [&](int l) { return sm + (((flip >> l) & 1u) ? t.lv[l].o_x2 : t.lv[l].o_x); };
+    auto nxt = [&](int l) { return sm + (((flip >> l) & 1u) ? t.lv[l].o_x : t.lv[l].o_x2); };
     const int last = t.count - 1;
     for (int l = 0; l < last; l++) {
         const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
-        for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-            cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, t.cd1, t.cr1, nxt(l), n, 0);
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
+            cheb_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_d), S(L.o_inv), t.cd1, t.cr1, nxt(l), n, 0);
         __syncthreads();
         flip ^= 1u << l;
-        for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-            residual_node(L.d, L.ks, L.ep, cur(l), B(l), L.inv, Rv(l), n, 0);
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
+            residual_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_inv), S(L.o_r), n, 0);
         __syncthreads();
-        for (int n = threadIdx.x; n < C.d.n_nodes; n += blockDim.x)
-            restrict_node(L.d, Rv(l), C.d, C.inv, t.theta_inv, B(l + 1), cur(l + 1), D(l + 1), n);
+        for (int n = threadIdx.x; n < C.d.n_nodes; n += kTailThreads)
+            restrict_node(L.d, S(L.o_r), C.d, S(C.o_inv), t.theta_inv, S(C.o_b), cur(l + 1), S(C.o_d), n);
         __syncthreads();
     }
     {
         // Coarsest level: x = A^-1 b with all waves. Thread (part, row) sums columns [part * span, +span) of its
-        // row (the inverse is stored column-major, so a wave's loads are contiguous and all of a thread's loads
-        // are in flight together); the kParts partial sums of a row are added in a fixed order.
-        constexpr int kParts = kTailThreads / kCoarseMaxDofs, kSpan = (kCoarseMaxDofs + kParts - 1) / kParts;
-        __shared__ double bsh[kCoarseMaxDofs];
-        __shared__ double psum[kParts][kCoarseMaxDofs];
-        const int n = t.lv[last].d.n_dofs, span = (n + kParts - 1) / kParts;
+        // row (the inverse is column-major, so a wave's loads are contiguous and all of a thread's loads are
+        // in flight together); the kParts partial sums of a row are added in a fixed order.
+        const TailLevel &T = t.lv[last];
+        const int n = T.d.n_dofs, span = (n + kParts - 1) / kParts;
         float *xo = cur(last);
-        const float *bo = B(last);
-        if (threadIdx.x < n) bsh[threadIdx.x] = static_cast<double>(bo[threadIdx.x]);
+        if (threadIdx.x < n) bsh[threadIdx.x] = static_cast<double>(sm[T.o_b + threadIdx.x]);
         __syncthreads();
         const int part = threadIdx.x / kCoarseMaxDofs, row = threadIdx.x - part * kCoarseMaxDofs;
         if (part < kParts && row < n) {
@@ -375,16 +435,19 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
     }
     for (int l = last - 1; l >= 0; l--) {
         const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
-        for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x) prolong_node(L.d, cur(l), L.inv, C.d, cur(l + 1), n);
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
+            prolong_node(L.d, cur(l), S(L.o_inv), C.d, cur(l + 1), n);
         __syncthreads();
         for (int pass = 0; pass < 2; pass++) {
             const float cd = pass == 0 ? 0.0f : t.cd1, cr = pass == 0 ? t.theta_inv : t.cr1;
-            for (int n = threadIdx.x; n < L.d.n_nodes; n += blockDim.x)
-                cheb_node(L.d, L.ks, L.ep, cur(l), B(l), D(l), L.inv, cd, cr, nxt(l), n, 0);
+            for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
+                cheb_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_d), S(L.o_inv), cd, cr, nxt(l), n, 0);
             __syncthreads();
             flip ^= 1u << l;
         }
     }
+    const float *res = cur(0);
+    for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) t.out0[o0 + q] = res[q];
 }
 
 // ---- set-up ----------------------------------------------------------------------------------------
@@ -743,6 +806,8 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
     const size_t lds = sizeof(double) * std::max(nc * nc, 4 * kCoarseMaxDofs);
     MYCEL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gmg_coarse_factor),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    MYCEL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gmg_tail),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsMax));
     if (g->levels >= 2) {  // Galerkin coarsest operator from the second-coarsest level's elements
         GmgLevel &S = g->lv[g->levels - 2];
         hipLaunchKernelGGL(k_gmg_galerkin_blocks, dim3(S.d.n_elem), b256, 0, s, S.d, S.ep, S.inv.get(),
@@ -783,17 +848,48 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         std::swap(cur[l], nxt[l]);
     };
     // Levels [t0, top] run in the single-workgroup tail kernel (the finest level never does: it has the FP64
-    // entry and exit kernels).
+    // entry and exit kernels), as many as fit its node limit, its per-array limit and its LDS.
+    auto tail_floats = [&](int l) { return g->lv[l].d.n_pad + 6 * g->lv[l].d.n_dofs; };
+    auto tail_fits = [&](int l) {
+        const Dims &d = g->lv[l].d;
+        return d.n_dofs <= kTailPer * kTailThreads && d.n_pad <= kTailPer * kTailThreads;
+    };
     int t0 = top;
-    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= tail_nodes()) t0--;
+    long long lds_floats = tail_floats(top);
+    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= tail_nodes() && tail_fits(t0 - 1) &&
+           4 * (lds_floats + tail_floats(t0 - 1)) <= kTailLdsMax) {
+        t0--;
+        lds_floats += tail_floats(t0);
+    }
     if (top == 0) t0 = 0;
     TailArgs ta{};
     ta.count = top - t0 + 1;
-    if (ta.count > kMaxTail) throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: tail too deep");
+    if (ta.count > kMaxTail || !tail_fits(top) || 4 * lds_floats > kTailLdsMax)
+        throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: coarsest level does not fit the tail kernel");
+    int off = 0;
     for (int l = t0; l <= top; l++) {
         GmgLevel &L = g->lv[l];
-        ta.lv[l - t0] = TailLevel{L.d, L.ks, L.ep, L.inv.get(), L.b.get(), L.x.get(), L.x2.get(), L.dv.get(), L.r.get()};
+        TailLevel &T = ta.lv[l - t0];
+        T.d = L.d;
+        T.ks = L.ks;
+        T.ep = L.ep;
+        T.inv = L.inv.get();
+        const int nd = L.d.n_dofs;
+        T.o_e = off;
+        off += L.d.n_pad;
+        T.o_inv = off;
+        T.o_b = off + nd;
+        T.o_x = off + 2 * nd;
+        T.o_x2 = off + 3 * nd;
+        T.o_d = off + 4 * nd;
+        T.o_r = off + 5 * nd;
+        off += 6 * nd;
     }
+    const size_t tail_lds = sizeof(float) * static_cast<size_t>(off);
+    ta.b0 = g->lv[t0].b.get();
+    ta.x0 = cur[t0];
+    ta.d0 = g->lv[t0].dv.get();
+    ta.out0 = ta.count > 1 ? nxt[t0] : cur[t0];  // the host-side swap below names it the current iterate
     ta.ainv = g->ainv.get();
     ta.theta_inv = g->theta_inv;
     ta.cd1 = g->cd[1];
@@ -811,7 +907,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
                            C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
-    hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(kTailThreads), 0, s, ta);
+    hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(kTailThreads), tail_lds, s, ta);
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];

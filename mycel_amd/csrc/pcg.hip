@@ -313,9 +313,11 @@ __global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState
         zv[q] = z[i];
     }
     if (st->halt) return;
-    const double rr = wave_sum_partials(part_rr, nb_vec);
-    const double zr = wave_sum_partials(part_zr, nb_z);
-    const double zap = wave_sum_partials(part_zap, nb_z);
+    const double *const parts[3] = {part_rr, part_zr, part_zap};
+    const int counts[3] = {nb_vec, nb_z, nb_z};
+    double sums[3];
+    wave_sum_partials_n<3>(parts, counts, sums);
+    const double rr = sums[0], zr = sums[1], zap = sums[2];
     const double rel = sqrt(rr) / st->bnorm;
     const int t = st->iter[slot];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
