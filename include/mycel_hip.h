@@ -229,6 +229,12 @@ typedef struct {
     int32_t precision;          /* MYCEL_PRECISION_* for the linear solves */
     int32_t enable_heaviside;   /* 0: smooth Heaviside projection of the filtered field, beta 1 -> 8
                                    (Optimization/HeavisideProjection.cs, TopologyOptimizer.cs:193-201,281-289,315-318) */
+    double initial_lambda;      /* 1.0: the OC multiplier's first guess (TopologyOptimizer.cs:154); a resumed run
+                                   passes the checkpointed one */
+    const double *initial_design; /* NULL: optional FP64 initial design density, box- or active-sized; when set it
+                                     replaces the float32 initial_density of mycel_simp_run (an exact resume from
+                                     a checkpoint's FP64 design; the reference's InitialDensity is float32) */
+    int32_t initial_design_len;
 } mycel_simp_options;
 
 /* Fills the reference defaults (OptimizerOptions.cs:5-26). */
@@ -248,9 +254,11 @@ typedef struct {
     double solve_ms, sensitivity_ms, filter_ms, update_ms, total_ms;
 } mycel_snapshot;
 
-/* progress.Report(snapshot) (TopologyOptimizer.cs:372). rho_bar: active-compact FP64, valid during
- * the call only. Return nonzero to stop after this iteration (cooperative cancel). */
-typedef int32_t (*mycel_snapshot_cb)(const mycel_snapshot *snap, const double *rho_bar, void *user);
+/* progress.Report(snapshot) (TopologyOptimizer.cs:341-372). rho_bar (the snapshot's PhysicalDensity) and rho
+ * (its DesignDensity, after this iteration's OC update): active-compact FP64, valid during the call only.
+ * Return nonzero to stop after this iteration (cooperative cancel). */
+typedef int32_t (*mycel_snapshot_cb)(const mycel_snapshot *snap, const double *rho_bar, const double *rho,
+                                     void *user);
 
 typedef struct {
     int32_t converged;
@@ -285,6 +293,14 @@ MYCEL_API int32_t mycel_simp_run_cases(mycel_op *op, const double *f, int32_t n_
                                        int32_t snapshots_capacity, double *rho_out, double *rho_bar_out,
                                        double *u_out);
 
+
+/* ---- Run persistence (Persistence/RunWriter.cs, Persistence/RunReader.cs) -------------------- */
+/* The 8-bit frame of an iteration file and its checksum (RunWriter.cs:215-229,376-388): q[i] =
+ * round-half-away(clamp(phys[i], 0, 1) * 255) in float32, checksum = FNV-1a (32-bit) over the bytes q.
+ * q may be NULL (checksum only). */
+MYCEL_API int32_t mycel_run_quantize(const float *phys, int64_t n, uint8_t *q, uint32_t *checksum);
+/* FNV-1a (32-bit) over n bytes, as RunReader.ReadIteration verifies a frame (RunReader.cs:213-223). */
+MYCEL_API int32_t mycel_run_checksum(const uint8_t *q, int64_t n, uint32_t *checksum);
 
 /* ---- Job server (continuous batching; no counterpart in the reference, whose only job machinery is
  * one run at a time behind a RAM guard: Optimization/OptimizationCoordinator.cs:214-353,

@@ -110,7 +110,8 @@ class SimpOptionsC(C.Structure):
                 ("solver_tol_final", C.c_double), ("solver_max_iterations", C.c_int32), ("rho_min", C.c_double),
                 ("protect_boundary_hinges", C.c_int32), ("start_iteration", C.c_int32),
                 ("enforce_min_filter_radius", C.c_int32), ("continuation", C.c_int32), ("precision", C.c_int32),
-                ("enable_heaviside", C.c_int32)]
+                ("enable_heaviside", C.c_int32), ("initial_lambda", C.c_double), ("initial_design", C.c_void_p),
+                ("initial_design_len", C.c_int32)]
 
 
 class SnapshotC(C.Structure):
@@ -142,7 +143,8 @@ class JobInfoC(C.Structure):
 
 
 PROGRESS_CB = C.CFUNCTYPE(None, C.c_int32, C.c_double, C.c_void_p)
-SNAPSHOT_CB = C.CFUNCTYPE(C.c_int32, C.POINTER(SnapshotC), C.POINTER(C.c_double), C.c_void_p)
+SNAPSHOT_CB = C.CFUNCTYPE(C.c_int32, C.POINTER(SnapshotC), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.c_void_p)
 
 _vp = C.c_void_p
 _i = C.c_int32
@@ -188,6 +190,8 @@ SIGNATURES = {
     "mycel_sensitivities": (_i, [_vp, _vp, _vp, _vp, _d]),
     "mycel_oc_update": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _d, C.POINTER(_d)]),
     "mycel_simp_options_default": (None, [C.POINTER(SimpOptionsC)]),
+    "mycel_run_quantize": (_i, [_vp, C.c_int64, _vp, C.POINTER(C.c_uint32)]),
+    "mycel_run_checksum": (_i, [_vp, C.c_int64, C.POINTER(C.c_uint32)]),
     "mycel_server_create": (_i, [_d, _i, C.POINTER(_vp)]),
     "mycel_server_destroy": (_i, [_vp]),
     "mycel_server_submit": (_i, [_vp, C.POINTER(JobDescC), C.POINTER(C.c_int64)]),

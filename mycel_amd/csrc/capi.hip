@@ -225,7 +225,16 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
 
     // Initial design (:91-122).
     std::vector<double> rho(std::max(1, na));
-    if (init_density) {
+    if (o.initial_design) {  // FP64 design (exact resume); same length rules as InitialDensity
+        if (o.initial_design_len == d.n_elem)
+            for (int i = 0; i < na; i++) rho[i] = o.initial_design[op->active[i]];
+        else if (o.initial_design_len == na)
+            for (int i = 0; i < na; i++) rho[i] = o.initial_design[i];
+        else
+            throw Error(MYCEL_E_ARGUMENT, "initial_design length (" + std::to_string(o.initial_design_len) +
+                                              ") must match either TotalElements (" + std::to_string(d.n_elem) +
+                                              ") or ActiveElementCount (" + std::to_string(na) + ").");
+    } else if (init_density) {
         if (init_len == d.n_elem)
             for (int i = 0; i < na; i++) rho[i] = init_density[op->active[i]];
         else if (init_len == na)
@@ -259,10 +268,14 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
     for (int k = 0; k < n_cases; k++)  // b = P F (:146-152)
         launch_project(d.n_dofs, op->d_flags.get(), d_f.get() + k * nd, d_b.get() + k * nd, s);
     OcWork oc;
-    std::vector<double> h_rho_bar;
-    if (cb) h_rho_bar.resize(nab);
+    std::vector<double> h_rho_bar, h_rho;
+    if (cb) {
+        h_rho_bar.resize(nab);
+        h_rho.resize(nab);
+    }
 
-    double lambda = 1.0, max_change = 1.0;
+    double lambda = o.initial_lambda > 0.0 && std::isfinite(o.initial_lambda) ? o.initial_lambda : 1.0;
+    double max_change = 1.0;
     double hist[6];
     int hist_n = 0;
     int n_snaps = 0;
@@ -372,8 +385,9 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
         bool stop = false;
         if (cb) {
             d_rho_bar.download(h_rho_bar.data(), nab, s);
+            d_rho.download(h_rho.data(), nab, s);
             MYCEL_HIP(hipStreamSynchronize(s));
-            stop = cb(&sn, h_rho_bar.data(), user) != 0;
+            stop = cb(&sn, h_rho_bar.data(), h_rho.data(), user) != 0;
         }
         // Convergence over the last six compliances (:374-390).
         if (hist_n == 6) {
@@ -868,7 +882,37 @@ void mycel_simp_options_default(mycel_simp_options *o) {
     o->start_iteration = 1;
     o->enforce_min_filter_radius = 1;
     o->continuation = 0;
+    o->initial_lambda = 1.0;
     o->precision = MYCEL_PRECISION_FP64;
+}
+
+// Persistence/RunWriter.cs:215-229 (8-bit frame) and :376-388 (FNV-1a over the frame bytes). MathF.Round
+// with MidpointRounding.AwayFromZero of v >= 0 is floor(v + 0.5), formed exactly in double.
+int32_t mycel_run_quantize(const float *phys, int64_t n, uint8_t *q, uint32_t *checksum) {
+    return guarded([&] {
+        require(n >= 0, MYCEL_E_ARGUMENT, "negative length");
+        require((phys || n == 0) && checksum, MYCEL_E_ARGUMENT_NULL, "null argument");
+        uint32_t h = 2166136261u;
+        for (int64_t i = 0; i < n; i++) {
+            const float c = std::min(std::max(phys[i], 0.0f), 1.0f);
+            const float v = c * 255.0f;
+            const uint8_t b = (v == v) ? static_cast<uint8_t>(std::floor(static_cast<double>(v) + 0.5)) : 0;
+            if (q) q[i] = b;
+            h = (h ^ b) * 16777619u;
+        }
+        *checksum = h;
+    });
+}
+
+// Persistence/RunReader.cs:213-223: the checksum a reader recomputes over a frame's bytes.
+int32_t mycel_run_checksum(const uint8_t *q, int64_t n, uint32_t *checksum) {
+    return guarded([&] {
+        require(n >= 0, MYCEL_E_ARGUMENT, "negative length");
+        require((q || n == 0) && checksum, MYCEL_E_ARGUMENT_NULL, "null argument");
+        uint32_t h = 2166136261u;
+        for (int64_t i = 0; i < n; i++) h = (h ^ q[i]) * 16777619u;
+        *checksum = h;
+    });
 }
 
 int32_t mycel_simp_run(mycel_op *op, const double *f, const mycel_simp_options *opt, const uint8_t *passive_mask_box,

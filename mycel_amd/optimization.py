@@ -144,11 +144,14 @@ class OptimizerOptions:
     WarmStartDisplacements: Optional[np.ndarray] = None
     EnforceMinimumFilterRadius: bool = True
     Precision: int = L.PRECISION_FP64
+    InitialLambda: float = 1.0  # the OC multiplier's first guess (TopologyOptimizer.cs:154); a resume passes the saved one
+    InitialDesign64: Optional[np.ndarray] = None  # FP64 initial design (box- or active-sized): an exact resume
 
 
 @dataclass
 class IterationSnapshot:
-    """Optimization/IterationSnapshot.cs:12-31 plus lambda; PhysicalDensity is float32 box-sized."""
+    """Optimization/IterationSnapshot.cs:12-31 plus lambda; PhysicalDensity (rho_bar) and DesignDensity (rho,
+    after this iteration's OC update) are float32 box-sized, as TopologyOptimizer.cs:340-347 builds them."""
 
     Iteration: int = 0
     Compliance: float = 0.0
@@ -165,6 +168,7 @@ class IterationSnapshot:
     UpdateMs: float = 0.0
     TotalMs: float = 0.0
     PhysicalDensity: Optional[np.ndarray] = None
+    DesignDensity: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -220,7 +224,12 @@ class TopologyOptimizer:
         co = L.SimpOptionsC(o.VolumeFraction, o.FilterRadiusMm, o.Penal, o.MoveLimit, o.MaxIterations,
                             o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
                             int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
-                            int(o.Continuation), int(o.Precision), int(o.EnableHeaviside))
+                            int(o.Continuation), int(o.Precision), int(o.EnableHeaviside), float(o.InitialLambda))
+        design64 = None
+        if o.InitialDesign64 is not None:
+            design64 = np.ascontiguousarray(o.InitialDesign64, np.float64).ravel()
+            co.initial_design = design64.ctypes.data
+            co.initial_design_len = int(design64.size)
         patch = np.concatenate([lc.Patch.Nodes for lc in model.Loads]).astype(np.int32) if model.Loads else np.zeros(0, np.int32)
         pm = L.u8(o.PassiveMask) if o.PassiveMask is not None else None
         idn = np.ascontiguousarray(o.InitialDensity, np.float32) if o.InitialDensity is not None else None
@@ -236,11 +245,14 @@ class TopologyOptimizer:
         collected: List[IterationSnapshot] = []
         act = model.ActiveElements
 
-        def _cb(sp, rbp, _u):
+        def _cb(sp, rbp, rp, _u):
             s = _snap(sp.contents)
             box = np.zeros(model.TotalElements, np.float32)
             box[act] = np.ctypeslib.as_array(rbp, shape=(na,)).astype(np.float32)
             s.PhysicalDensity = box
+            des = np.zeros(model.TotalElements, np.float32)
+            des[act] = np.ctypeslib.as_array(rp, shape=(na,)).astype(np.float32)
+            s.DesignDensity = des
             collected.append(s)
             if progress is not None:
                 progress(s)

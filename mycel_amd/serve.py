@@ -44,7 +44,7 @@ def _simp_c(o: OptimizerOptions) -> L.SimpOptionsC:
     return L.SimpOptionsC(o.VolumeFraction, o.FilterRadiusMm, o.Penal, o.MoveLimit, o.MaxIterations,
                           o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
                           int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
-                          int(o.Continuation), int(o.Precision), int(o.EnableHeaviside))
+                          int(o.Continuation), int(o.Precision), int(o.EnableHeaviside), float(o.InitialLambda))
 
 
 class JobServer:
