@@ -45,7 +45,16 @@ namespace {
 constexpr int kCoarseMaxDofs = 140;  // 140^2 doubles = 156,800 B of LDS (gfx950: 160 KiB per workgroup)
 constexpr int kMaxLevels = 16;
 constexpr double kSmoothRange = 30.0;  // Chebyshev interval [lmax / 30, lmax]
-constexpr int kFineDegree = 3;          // Chebyshev degree on the finest level (2 on the coarser ones)
+constexpr int kFineDegree = 3;          // Chebyshev degree on the finest level (2 on the coarser ones) ...
+constexpr int kMaxDegree = 8;           // ... unless MYCEL_GMG_FINE_DEGREE asks for another in [2, kMaxDegree]
+
+int fine_degree() {
+    static const int v = [] {
+        const char *e = std::getenv("MYCEL_GMG_FINE_DEGREE");
+        return e && *e ? std::min(std::max(2, std::atoi(e)), kMaxDegree) : kFineDegree;
+    }();
+    return v;
+}
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -340,13 +349,25 @@ struct TailLevel {
     const float *inv;  // inverse diagonal (global)
     int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
 };
+constexpr int kTailSegs = 3 + 2 * kMaxTail;
+constexpr int kAinvPer = (kCoarseMaxDofs * kCoarseMaxDofs + kTailThreads - 1) / kTailThreads;
+// One array staged into LDS on entry: n floats from src (+ the RHS offset for the first three) to LDS offset dst.
+struct TailSeg {
+    const float *src;
+    int n, dst;
+};
 struct TailArgs {
-    TailLevel lv[kMaxTail];
-    int count;
+    // The staging table comes first and is contiguous, so its scalar loads merge and are all issued before
+    // the first vector load (fields scattered over the levels cost one scalar-load latency each).
+    TailSeg seg[kTailSegs];  // lv[0]'s b, x, d (RHS 0; RHS k at k * n_dofs), then (moduli, D^-1) per level;
+                             // unused entries: a valid pointer and n = 0
+    int ainv_lds;            // float offset of the coarsest inverse in LDS, or -1 to read it from global memory
+    int ainv_nn;             // entries of the inverse staged into LDS (n^2, or 0)
     const double *ainv;
-    const float *b0, *x0, *d0;  // lv[0]'s b, x, d (global, RHS 0; RHS k at k * n_dofs)
-    float *out0;                // where lv[0]'s corrected iterate goes (global, same RHS layout)
+    float *out0;             // where lv[0]'s corrected iterate goes (global, RHS layout as above)
     float theta_inv, cd1, cr1;
+    int count;
+    TailLevel lv[kMaxTail];
 };
 
 __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
@@ -357,33 +378,46 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
     // One workgroup per right-hand side (blockIdx.x).
     const long long o0 = static_cast<long long>(blockIdx.x) * t.lv[0].d.n_dofs;
     {
-        // Segments: lv[0]'s b, x, d, then (moduli, inverse diagonal) of each level; every segment holds
-        // <= kTailPer * kTailThreads floats (checked on the host).
-        constexpr int kSeg = 3 + 2 * kMaxTail;
-        const int nseg = 3 + 2 * t.count;
-        float v[kSeg][kTailPer];
+        // Every segment holds <= kTailPer * kTailThreads floats (checked on the host); the coarsest inverse
+        // (n^2 doubles) joins the same round trip when it fits in LDS.
+        // Loads are unconditional at a clamped index (every table entry points at valid memory; unused entries
+        // have n = 0), so the staging is one basic block: the scheduler issues every scalar load of the
+        // table, then every vector load, and waits once.
+        float v[kTailSegs][kTailPer];
 #pragma unroll
-        for (int sg = 0; sg < kSeg; sg++) {
-            const TailLevel &L = t.lv[sg < 3 ? 0 : (sg - 3) >> 1];
-            const float *src = sg == 0 ? t.b0 + o0 : sg == 1 ? t.x0 + o0 : sg == 2 ? t.d0 + o0
-                               : ((sg - 3) & 1) ? L.inv : L.ep;
-            const int n = sg >= nseg ? 0 : (sg >= 3 && !((sg - 3) & 1)) ? L.d.n_pad : L.d.n_dofs;
+        for (int sg = 0; sg < kTailSegs; sg++) {
+            const TailSeg g = t.seg[sg];
+            const float *src = g.src + (sg < 3 ? o0 : 0);
+            const int last = g.n > 0 ? g.n - 1 : 0;
 #pragma unroll
             for (int u = 0; u < kTailPer; u++) {
                 const int i = threadIdx.x + u * kTailThreads;
-                v[sg][u] = i < n ? src[i] : 0.0f;
+                const float x = src[i < last ? i : last];
+                v[sg][u] = i < g.n ? x : 0.0f;
             }
         }
+        double va[kAinvPer];
+        const int nn = t.ainv_nn;
 #pragma unroll
-        for (int sg = 0; sg < kSeg; sg++) {
-            const TailLevel &L = t.lv[sg < 3 ? 0 : (sg - 3) >> 1];
-            const int dst = sg == 0 ? L.o_b : sg == 1 ? L.o_x : sg == 2 ? L.o_d : ((sg - 3) & 1) ? L.o_inv : L.o_e;
-            const int n = sg >= nseg ? 0 : (sg >= 3 && !((sg - 3) & 1)) ? L.d.n_pad : L.d.n_dofs;
+        for (int u = 0; u < kAinvPer; u++) {
+            const int i = threadIdx.x + u * kTailThreads;
+            va[u] = t.ainv[i < nn ? i : 0];
+        }
+#pragma unroll
+        for (int sg = 0; sg < kTailSegs; sg++) {
+            const TailSeg g = t.seg[sg];
+            const int n = g.n;
 #pragma unroll
             for (int u = 0; u < kTailPer; u++) {
                 const int i = threadIdx.x + u * kTailThreads;
-                if (i < n) sm[dst + i] = v[sg][u];
+                if (i < n) sm[g.dst + i] = v[sg][u];
             }
+        }
+        double *al = reinterpret_cast<double *>(sm + (t.ainv_lds >= 0 ? t.ainv_lds : 0));
+#pragma unroll
+        for (int u = 0; u < kAinvPer; u++) {
+            const int i = threadIdx.x + u * kTailThreads;
+            if (i < nn) al[i] = va[u];
         }
     }
     __syncthreads();
@@ -418,10 +452,19 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
         const int part = threadIdx.x / kCoarseMaxDofs, row = threadIdx.x - part * kCoarseMaxDofs;
         if (part < kParts && row < n) {
             double s = 0.0;
+            if (t.ainv_lds >= 0) {  // staged in LDS on entry
+                const double *ai = reinterpret_cast<const double *>(sm + t.ainv_lds);
 #pragma unroll
-            for (int q = 0; q < kSpan; q++) {
-                const int j = part * span + q;
-                if (q < span && j < n) s = __fma_rn(t.ainv[static_cast<size_t>(j) * n + row], bsh[j], s);
+                for (int q = 0; q < kSpan; q++) {
+                    const int j = part * span + q;
+                    if (q < span && j < n) s = __fma_rn(ai[j * n + row], bsh[j], s);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < kSpan; q++) {
+                    const int j = part * span + q;
+                    if (q < span && j < n) s = __fma_rn(t.ainv[static_cast<size_t>(j) * n + row], bsh[j], s);
+                }
             }
             psum[part][row] = s;
         }
@@ -695,7 +738,8 @@ struct Gmg {
     DevBuf<double> blocks;  // Galerkin element blocks of the second-coarsest level
     uint64_t epoch = ~0ull;
     float theta_inv = 0;
-    float cd[kFineDegree], cr[kFineDegree];  // Chebyshev step k >= 1: d' = cd[k] d + cr[k] D^-1 r
+    float cd[kMaxDegree], cr[kMaxDegree];  // Chebyshev step k >= 1: d' = cd[k] d + cr[k] D^-1 r
+    int fine_degree = kFineDegree;
     int rhs_cap = 1;    // right-hand sides the level vectors hold
     uint64_t gen = 0;   // bumped when the level vectors move (captured graphs must be rebuilt)
 };
@@ -742,7 +786,8 @@ static void gmg_build(mycel_op *op) {
     double rho = 1.0 / sigma;
     g->cd[0] = 0.0f;
     g->cr[0] = g->theta_inv;
-    for (int k = 1; k < kFineDegree; k++) {
+    g->fine_degree = fine_degree();
+    for (int k = 1; k < kMaxDegree; k++) {
         const double rn = 1.0 / (2.0 * sigma - rho);
         g->cd[k] = static_cast<float>(rn * rho);
         g->cr[k] = static_cast<float>(2.0 * rn / delta);
@@ -885,10 +930,25 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         T.o_r = off + 5 * nd;
         off += 6 * nd;
     }
+    // The coarsest inverse joins the LDS image when it fits (8-byte aligned after the level arrays).
+    const int nc = g->lv[top].d.n_dofs;
+    const int ainv_off = (off + 1) & ~1;
+    ta.ainv_lds = 4 * (static_cast<long long>(ainv_off) + 2LL * nc * nc) <= kTailLdsMax ? ainv_off : -1;
+    if (ta.ainv_lds >= 0) off = ainv_off + 2 * nc * nc;
     const size_t tail_lds = sizeof(float) * static_cast<size_t>(off);
-    ta.b0 = g->lv[t0].b.get();
-    ta.x0 = cur[t0];
-    ta.d0 = g->lv[t0].dv.get();
+    ta.ainv_nn = ta.ainv_lds >= 0 ? nc * nc : 0;
+    {
+        const TailLevel &T0 = ta.lv[0];
+        ta.seg[0] = TailSeg{g->lv[t0].b.get(), T0.d.n_dofs, T0.o_b};
+        ta.seg[1] = TailSeg{cur[t0], T0.d.n_dofs, T0.o_x};
+        ta.seg[2] = TailSeg{g->lv[t0].dv.get(), T0.d.n_dofs, T0.o_d};
+        for (int l = 0; l < ta.count; l++) {
+            const TailLevel &T = ta.lv[l];
+            ta.seg[3 + 2 * l] = TailSeg{T.ep, T.d.n_pad, T.o_e};
+            ta.seg[4 + 2 * l] = TailSeg{T.inv, T.d.n_dofs, T.o_inv};
+        }
+        for (int sg = 3 + 2 * ta.count; sg < kTailSegs; sg++) ta.seg[sg] = TailSeg{ta.seg[0].src, 0, 0};
+    }
     ta.out0 = ta.count > 1 ? nxt[t0] : cur[t0];  // the host-side swap below names it the current iterate
     ta.ainv = g->ainv.get();
     ta.theta_inv = g->theta_inv;
@@ -901,7 +961,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
                            F.inv.get(), g->theta_inv, F.b.get(), cur[0], F.dv.get());
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
-        for (int k = 1; k < (l == 0 ? kFineDegree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
+        for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
         hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                            L.inv.get(), L.r.get());
         hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
@@ -913,11 +973,12 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv.get(), C.d, cur[l + 1]);
         cheb(l, 0.0f, g->theta_inv);
-        for (int k = 1; k < (l == 0 ? kFineDegree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k]);
+        for (int k = 1; k < (l == 0 ? g->fine_degree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k]);
     }
     // Finest level: the last post-smoothing step writes z (FP64) and the partials. A single-level hierarchy
     // (a grid of <= kCoarseMaxDofs DOFs) was solved directly above: the step is then the identity (cd = cr = 0).
-    const float cd = top == 0 ? 0.0f : g->cd[kFineDegree - 1], cr = top == 0 ? 0.0f : g->cr[kFineDegree - 1];
+    const int fd = g->fine_degree;
+    const float cd = top == 0 ? 0.0f : g->cd[fd - 1], cr = top == 0 ? 0.0f : g->cr[fd - 1];
     hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(), F.dv.get(),
                        F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
     MYCEL_LAUNCH_CHECK();
