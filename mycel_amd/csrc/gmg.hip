@@ -370,11 +370,19 @@ struct TailArgs {
     TailLevel lv[kMaxTail];
 };
 
-__global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs t) {
+static_assert(sizeof(TailArgs) % 4 == 0 && sizeof(TailArgs) / 4 <= kTailThreads, "tail arguments: one word per thread");
+
+__global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
     extern __shared__ __align__(16) float sm[];
     constexpr int kParts = kTailThreads / kCoarseMaxDofs, kSpan = (kCoarseMaxDofs + kParts - 1) / kParts;
     __shared__ double bsh[kCoarseMaxDofs];
     __shared__ double psum[kParts][kCoarseMaxDofs];
+    // The arguments (~1.6 KB) are copied into LDS with one vector load per thread: read field by field they
+    // are scalar loads from the kernel-argument segment, each wait a memory latency, in every phase.
+    __shared__ TailArgs t;
+    if (threadIdx.x < sizeof(TailArgs) / 4)
+        reinterpret_cast<uint32_t *>(&t)[threadIdx.x] = reinterpret_cast<const uint32_t *>(&targ)[threadIdx.x];
+    __syncthreads();
     // One workgroup per right-hand side (blockIdx.x).
     const long long o0 = static_cast<long long>(blockIdx.x) * t.lv[0].d.n_dofs;
     {
