@@ -18,6 +18,10 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   independent load cases (1-4 seeded 4x4-node patches each), 20 SIMP iterations on the summed compliance
   (vf 0.25, rmin 2.0, p = 3, solves to 1e-8 warm-started per case) with the multigrid solver; one SIMP
   iteration with the plain FP64 Jacobi solver is timed beside it. Reports SIMP iterations/s.
+--mode sweep: K*u (FP64 and FP32) on the holed box and on solid boxes up to 256x128x128 (12.7M DOFs), 100
+  launches each timed with HIP events on the operator's stream; reports time per launch and the fraction of
+  the HBM and FP64 rooflines at each size (the fixed launch and latency cost of one K*u is amortised as the
+  grid grows). Prints one JSON line.
 --mode summarize: no GPU; reads rocprofv3 CSV output (--stats-dir, --fetch-dir, --write-dir) of a
   `--mode kernels` run and prints per-kernel durations and HBM bytes per launch. FETCH_SIZE and WRITE_SIZE
   (KiB per dispatch) are each scaled by the factor that makes the largest device copy of the run read and
@@ -46,7 +50,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "summarize"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "sweep", "summarize"), default="solve")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
@@ -306,6 +310,59 @@ def mode_kernels(args):
     op.close()
 
 
+SWEEP_GRIDS = ((80, 40, 40, True), (100, 50, 50, False), (160, 80, 80, False), (200, 100, 100, False),
+               (256, 128, 128, False))
+
+
+def mode_sweep(args):
+    """K*u roofline fractions against grid size. Algorithmic bytes per launch: 2 * 3 * s per active node
+    (x read, y written) + s per active element (the modulus), s = 8 (FP64) or 4 (FP32); flops 1,152 per
+    active element."""
+    import numpy as np
+
+    from mycel_amd import FemOperator
+    from mycel_amd import workloads as W
+
+    rows = []
+    for nx, ny, nz, holed in SWEEP_GRIDS:
+        if holed:
+            p = W.matvec_pcg_workstation(seed=args.seed)
+        else:
+            p = W.Problem(nx, ny, nz, name=f"box {nx}x{ny}x{nz}")
+            W._clamped_x0(p)
+            rng = np.random.default_rng(args.seed)
+            p.density = 0.01 + 0.99 * rng.random(p.n_elem)
+        act = p.active
+        nodes = np.zeros((nz + 1, ny + 1, nx + 1), bool)
+        ex, ey, ez = act % nx, (act // nx) % ny, act // (nx * ny)
+        for oz in (0, 1):
+            for oy in (0, 1):
+                for ox in (0, 1):
+                    nodes[ez + oz, ey + oy, ex + ox] = True
+        na, nn = act.size, int(nodes.sum())
+        op = FemOperator(p.fe_model())
+        row = {"grid": [nx, ny, nz], "holed": holed, "active_elements": int(na), "active_nodes": nn,
+               "box_dofs": 3 * (nx + 1) * (ny + 1) * (nz + 1)}
+        for name, fp32, sz in (("fp64", False, 8), ("fp32", True, 4)):
+            ms = time_matvecs(op, args.matvecs, fp32=fp32)
+            byt = 2 * 3 * sz * nn + sz * na
+            gbs = byt / (ms * 1e-3) / 1e9
+            row[name] = {"us_per_launch": ms * 1e3, "algorithmic_bytes": byt, "achieved_gbs": gbs,
+                         "hbm_frac": gbs / HBM_PEAK_GBS,
+                         "tflops": 1152.0 * na / (ms * 1e-3) / 1e12}
+            if not fp32:
+                row[name]["fp64_frac"] = row[name]["tflops"] / FP64_PEAK_TFLOPS
+        op.close()
+        print(f"[sweep] {nx}x{ny}x{nz}{' holed' if holed else ''}: fp64 {row['fp64']['us_per_launch']:.2f} us "
+              f"(hbm {row['fp64']['hbm_frac']:.3f}, fp64 {row['fp64']['fp64_frac']:.3f}), fp32 "
+              f"{row['fp32']['us_per_launch']:.2f} us (hbm {row['fp32']['hbm_frac']:.3f})", file=sys.stderr,
+              flush=True)
+        rows.append(row)
+    print(json.dumps({"metric": "K*u roofline fraction vs grid size", "unit": "fraction", "dtype": "f64/f32",
+                      "data": "synthetic", "launches": args.matvecs, "hbm_peak_gbs": HBM_PEAK_GBS,
+                      "fp64_peak_tflops": FP64_PEAK_TFLOPS, "sizes": rows}), flush=True)
+
+
 def mode_serve(args):
     import numpy as np
 
@@ -534,7 +591,8 @@ def main():
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
-    {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve, "cases": mode_cases}[args.mode](args)
+    {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve, "cases": mode_cases,
+     "sweep": mode_sweep}[args.mode](args)
 
 
 if __name__ == "__main__":
