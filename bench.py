@@ -343,6 +343,13 @@ def mode_sweep(args):
         op = FemOperator(p.fe_model())
         row = {"grid": [nx, ny, nz], "holed": holed, "active_elements": int(na), "active_nodes": nn,
                "box_dofs": 3 * (nx + 1) * (ny + 1) * (nz + 1)}
+        cols = {}
+        for kz in (0, 2, 4, 8):  # K*u form: one node per thread, or kz-node z-marching columns
+            op.SetMatvecColumns(kz)
+            cols[str(kz)] = {"fp64_us": time_matvecs(op, args.matvecs) * 1e3,
+                             "fp32_us": time_matvecs(op, args.matvecs, fp32=True) * 1e3}
+        row["columns"] = cols
+        op.SetMatvecColumns(-1)
         for name, fp32, sz in (("fp64", False, 8), ("fp32", True, 4)):
             ms = time_matvecs(op, args.matvecs, fp32=fp32)
             byt = 2 * 3 * sz * nn + sz * na
@@ -355,7 +362,8 @@ def mode_sweep(args):
         op.close()
         print(f"[sweep] {nx}x{ny}x{nz}{' holed' if holed else ''}: fp64 {row['fp64']['us_per_launch']:.2f} us "
               f"(hbm {row['fp64']['hbm_frac']:.3f}, fp64 {row['fp64']['fp64_frac']:.3f}), fp32 "
-              f"{row['fp32']['us_per_launch']:.2f} us (hbm {row['fp32']['hbm_frac']:.3f})", file=sys.stderr,
+              f"{row['fp32']['us_per_launch']:.2f} us (hbm {row['fp32']['hbm_frac']:.3f}); by columns "
+              + " ".join(f"{k}:{v['fp64_us']:.1f}/{v['fp32_us']:.1f}" for k, v in cols.items()), file=sys.stderr,
               flush=True)
         rows.append(row)
     print(json.dumps({"metric": "K*u roofline fraction vs grid size", "unit": "fraction", "dtype": "f64/f32",

@@ -143,6 +143,10 @@ MYCEL_API int32_t mycel_op_apply_k_d(mycel_op *op, const double *d_x, double *d_
 MYCEL_API int32_t mycel_op_apply_a_d(mycel_op *op, const double *d_x, double *d_y, int32_t x_is_projected, void *stream);
 /* The FP32 operator of the mixed-precision solver (FP32 Ke and E_e), device pointers, FP32 DOF vectors. */
 MYCEL_API int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *d_y, void *stream);
+/* K*u kernel form (a tuning option with no counterpart in the reference): columns = 0 runs one node per
+ * thread, 2 / 4 / 8 let each thread march that many nodes along z (less L1 traffic on large grids), -1 (the
+ * default) chooses by grid size. All forms give bit-identical results. */
+MYCEL_API int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns);
 /* FemOperator.DiagA / Ke / ConstrainedDofs / ActiveElements properties. */
 MYCEL_API int32_t mycel_op_get_diag_a(mycel_op *op, double *diag);
 MYCEL_API int32_t mycel_op_get_ke(mycel_op *op, double *ke576);

@@ -175,6 +175,95 @@ __device__ __forceinline__ void ku_gather(const Dims &g, const KeSym<T> &ks, con
     for (int d = 0; d < 3; d++) xc[d] = mid[4][d];
 }
 
+// Mode handling and store of one node's three rows (out = raw K x at the node, xc = its own x values as the
+// gather read them); returns the node's x.y contribution.
+template <typename T, int MODE>
+__device__ __forceinline__ double finish_node(const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                              T *__restrict__ y, int n, T (&out)[3], const T (&xc)[3]) {
+    double dotv = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int dof = 3 * n + d;
+        if (MODE == kApplyAProjected) {
+            if (flags[dof] & kFlagFixed) out[d] = xc[d];
+        } else if (MODE == kApplyAGeneral) {
+            if (flags[dof] & kFlagConstrained) out[d] = x[dof];
+        }
+        y[dof] = out[d];
+        dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
+    }
+    return dotv;
+}
+
+// z-marching form for large grids: one thread owns a column of KZ nodes (i, j, k0 .. k0 + KZ - 1) and keeps
+// the three node layers of the current node in registers, so each further node costs one new 3x3 layer and the
+// four moduli above it instead of three layers and eight moduli: about a third of the L1 traffic of the
+// one-node-per-thread gather, which is what bounds that gather on large grids. kb: a node layer <= k0 shared
+// by the wave (the addressing base, as in ku_gather). Returns the column's x.y contribution.
+template <typename T, int MODE, int KZ>
+__device__ __forceinline__ double apply_k_column(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
+                                                 const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                                 T *__restrict__ y, int i, int j, int k0, int kb) {
+    const int zb = kb > 0 ? kb - 1 : 0;
+    const size_t layer = static_cast<size_t>(g.nx1) * g.ny1;
+    const T *xb = x + 3 * layer * zb;
+    const uint8_t *fb = MODE == kApplyAGeneral ? flags + 3 * layer * zb : nullptr;
+    const T *eb = ep + static_cast<size_t>(g.px) * g.py * kb;
+    constexpr uint32_t sT = sizeof(T);
+    const uint32_t ex = sT, ey = sT * g.px, ez = ey * g.py;
+    const uint32_t sx = 3 * sT, sy = sx * g.nx1, sz = sy * g.ny1;
+    const uint32_t oc = sx * i;
+    const uint32_t xs[3] = {i > 0 ? oc - sx : oc, oc, i < g.nx ? oc + sx : oc};
+    const uint32_t oy[3] = {j > 0 ? sy * (j - 1) : 0u, sy * j, j < g.ny ? sy * (j + 1) : sy * j};
+    // Byte offset of node layer k (clamped to [0, nz]) and of padded modulus layer kp, from the bases.
+    auto zoff = [&](int k) { return sz * static_cast<uint32_t>((k < 0 ? 0 : k > g.nz ? g.nz : k) - zb); };
+    const uint32_t e_ij = sT * ((i + 1) + g.px * (j + 1));
+    auto eoff = [&](int kp) { return e_ij + ez * static_cast<uint32_t>(kp - kb); };
+    auto load = [&](int k, T (&L)[9][3]) {
+        const uint32_t z = zoff(k);
+        const uint32_t rows[3] = {oy[0] + z, oy[1] + z, oy[2] + z};
+        load_layer<T, MODE>(xb, fb, xs, rows, L);
+    };
+    // E of the four elements in padded modulus layer kp around (i, j): corners 0-3 (node at their bottom)
+    // for kp = k + 1, corners 4-7 (node at their top) for kp = k.
+    auto load_e = [&](int kp, T (&E4)[4]) {
+        const uint32_t o = eoff(kp);
+#pragma unroll
+        for (int a = 0; a < 4; a++) E4[a] = ld_off(eb, o - (corner_x(a) ? ex : 0u) - (corner_y(a) ? ey : 0u));
+    };
+    T la[9][3], lb[9][3], lc[9][3], e_lo[4], e_hi[4];
+    load(k0 - 1, la);
+    load(k0, lb);
+    load_e(k0, e_lo);
+    double dotv = 0.0;
+#pragma unroll
+    for (int st = 0; st < KZ; st++) {
+        const int k = k0 + st;
+        if (k > g.nz) break;
+        // Roles rotate with the step: lo / mid / hi are la / lb / lc for st % 3 == 0, then lb / lc / la, ...
+        T(&lo)[9][3] = st % 3 == 0 ? la : st % 3 == 1 ? lb : lc;
+        T(&mid)[9][3] = st % 3 == 0 ? lb : st % 3 == 1 ? lc : la;
+        T(&hi)[9][3] = st % 3 == 0 ? lc : st % 3 == 1 ? la : lb;
+        load(k + 1, hi);
+        load_e(k + 1, e_hi);
+        T acc0 = 0, acc1 = 0, acc2 = 0;
+        corner_rows_slab<4>(ks, lo, mid, e_lo[0], acc0, acc1, acc2);
+        corner_rows_slab<5>(ks, lo, mid, e_lo[1], acc0, acc1, acc2);
+        corner_rows_slab<6>(ks, lo, mid, e_lo[2], acc0, acc1, acc2);
+        corner_rows_slab<7>(ks, lo, mid, e_lo[3], acc0, acc1, acc2);
+        corner_rows_slab<0>(ks, mid, hi, e_hi[0], acc0, acc1, acc2);
+        corner_rows_slab<1>(ks, mid, hi, e_hi[1], acc0, acc1, acc2);
+        corner_rows_slab<2>(ks, mid, hi, e_hi[2], acc0, acc1, acc2);
+        corner_rows_slab<3>(ks, mid, hi, e_hi[3], acc0, acc1, acc2);
+        T out[3] = {acc0, acc1, acc2};
+        const T xc[3] = {mid[4][0], mid[4][1], mid[4][2]};
+        dotv += finish_node<T, MODE>(x, flags, y, i + g.nx1 * (j + g.ny1 * k), out, xc);
+#pragma unroll
+        for (int a = 0; a < 4; a++) e_lo[a] = e_hi[a];
+    }
+    return dotv;
+}
+
 // Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op. k0: as in
 // ku_gather.
 template <typename T, int MODE>

@@ -668,6 +668,15 @@ int32_t mycel_op_get_diag_a(mycel_op *op, double *diag) {
     });
 }
 
+int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns) {
+    return guarded([&] {
+        require(op, MYCEL_E_ARGUMENT_NULL, "op");
+        require(columns == -1 || columns == 0 || columns == 2 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
+                "matvec columns must be -1, 0, 2, 4 or 8");
+        op->ku_columns = columns;
+    });
+}
+
 int32_t mycel_op_get_ke(mycel_op *op, double *ke) {
     return guarded([&] {
         require(op && ke, MYCEL_E_ARGUMENT_NULL, "op/ke");

@@ -172,6 +172,34 @@ __global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const KeSym<T> ks, c
     }
 }
 
+// z-marching K*u (apply_k_column): thread t owns column (i, j) = t mod (nx1 ny1) of z-block t / (nx1 ny1),
+// nodes k0 = KZ * z-block onwards; the lanes of a wave are consecutive columns of one z-block (or two, where
+// a wave straddles), so their loads stay coalesced. Per-block partials of x.y as k_apply_k.
+template <typename T, int MODE, bool DOT, int KZ>
+__global__ __launch_bounds__(256) void k_apply_k_zm(Dims g, const KeSym<T> ks, const T *__restrict__ ep,
+                                                    const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                                    T *__restrict__ y, double *__restrict__ part,
+                                                    const int *__restrict__ stop, Batch bt) {
+    __shared__ double red[4];
+    const int rhs = blockIdx.y;
+    x += rhs * bt.vstride;
+    y += rhs * bt.vstride;
+    if (DOT) part += rhs * bt.pstride;
+    if (stop) stop += rhs * bt.sstride;
+    if (DOT && stop && *stop) return;
+    const int cols = g.nx1 * g.ny1;
+    const int t0 = xcd_block(blockIdx.x, gridDim.x) * 256, t = t0 + threadIdx.x;
+    const int zb = t / cols, c = t - zb * cols;
+    const int k0 = zb * KZ;
+    double dotv = 0.0;
+    if (k0 <= g.nz)
+        dotv = apply_k_column<T, MODE, KZ>(g, ks, ep, x, flags, y, c % g.nx1, c / g.nx1, k0, (t0 / cols) * KZ);
+    if (DOT) {
+        const double s = block_sum256(dotv, red);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
 // LDS-tiled K*u: one block per brick of nodes (apply_k_tile); dynamic LDS holds the halo and E.
 template <typename T, int MODE, bool DOT>
 __global__ __launch_bounds__(256, 2) void k_apply_k_tiled(Dims g, Tile tl, const KeSym<T> ks, const T *__restrict__ ep,
@@ -358,14 +386,60 @@ static bool use_tiled() {
     return tiled;
 }
 
-int apply_k_blocks(const mycel_op *op) {
-    if (use_tiled()) return op->tile.ntx * op->tile.nty * op->tile.ntz;
-    return cdiv(op->d.n_nodes, 256);
-}
 
 template <typename T>
 static size_t tile_lds_bytes(const Tile &t) {
     return sizeof(T) * (3 * tile_halo_nodes(t.tx, t.ty, t.tz) + tile_elems(t.tx, t.ty, t.tz));
+}
+
+// Column length of the z-marching K*u for this operator, 0 for the one-node-per-thread gather: the
+// operator's setting (mycel_op_set_matvec_columns), else MYCEL_KU_ZM (0, 2, 4 or 8), else by grid size. The
+// column form also needs its offsets (2 KZ + 2 node layers from a block's base) to fit in 32 bits.
+template <typename T>
+static bool zm_fits(const mycel_op *op, int kz) {
+    return (2.0 * kz + 2.0) * 3.0 * sizeof(T) * op->d.nx1 * op->d.ny1 < 4294967296.0;
+}
+
+// By grid size (measured on MI355X: 4-node columns take FP64 K*u from 46 to 35 us at 1.06M nodes and from 169
+// to 114 us at 4.3M nodes; below ~500k nodes the one-node-per-thread gather has more waves to hide latency
+// with). The K*u launches that write dot partials always follow this rule, so an operator's partial count
+// never changes.
+static int dot_columns(const mycel_op *op) {
+    const int kz = op->d.n_nodes >= (1 << 19) ? 4 : 0;
+    return kz && zm_fits<double>(op, kz) ? kz : 0;
+}
+
+template <typename T>
+static int zm_kz(const mycel_op *op) {
+    static const int forced = [] {
+        const char *v = std::getenv("MYCEL_KU_ZM");
+        return v && *v ? std::atoi(v) : -1;
+    }();
+    const int kz = op->ku_columns >= 0 ? op->ku_columns : forced >= 0 ? forced : dot_columns(op);
+    if (kz != 2 && kz != 4 && kz != 8) return 0;
+    return zm_fits<T>(op, kz) ? kz : 0;
+}
+
+static int zm_blocks(const mycel_op *op, int kz) { return cdiv(op->d.nx1 * op->d.ny1 * cdiv(op->d.nz1, kz), 256); }
+
+int apply_k_blocks(const mycel_op *op, bool batched) {
+    if (use_tiled() && !batched) return op->tile.ntx * op->tile.nty * op->tile.ntz;
+    if (const int kz = dot_columns(op)) return zm_blocks(op, kz);
+    return cdiv(op->d.n_nodes, 256);
+}
+
+template <typename T, int MODE, bool DOT>
+static void launch_zm(mycel_op *op, int kz, const KeSym<T> &ke, const T *ep, const T *d_x, T *d_y, hipStream_t s,
+                      double *part, const int *stop, const Batch &bt) {
+    const dim3 grid(zm_blocks(op, kz), bt.count), block(256);
+    const uint8_t *fl = op->d_flags.get();
+    if (kz == 2)
+        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 2>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
+    else if (kz == 4)
+        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 4>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
+    else
+        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 8>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
+    MYCEL_LAUNCH_CHECK();
 }
 
 template <typename T>
@@ -373,6 +447,18 @@ static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const T *ep, cons
                              hipStream_t s, double *d_dot_partial, const int *d_stop, const Batch &bt) {
     const dim3 block(256);
     const uint8_t *fl = op->d_flags.get();
+    if (const int kz = d_dot_partial ? dot_columns(op) : zm_kz<T>(op); kz && !(use_tiled() && bt.count == 1)) {
+        if (d_dot_partial) {
+            if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
+            launch_zm<T, kApplyAProjected, true>(op, kz, ke, ep, d_x, d_y, s, d_dot_partial, d_stop, bt);
+        } else if (mode == kApplyK)
+            launch_zm<T, kApplyK, false>(op, kz, ke, ep, d_x, d_y, s, nullptr, nullptr, bt);
+        else if (mode == kApplyAProjected)
+            launch_zm<T, kApplyAProjected, false>(op, kz, ke, ep, d_x, d_y, s, nullptr, nullptr, bt);
+        else
+            launch_zm<T, kApplyAGeneral, false>(op, kz, ke, ep, d_x, d_y, s, nullptr, nullptr, bt);
+        return;
+    }
     if (use_tiled() && bt.count == 1) {
         const dim3 grid(apply_k_blocks(op));
         const Tile tl = op->tile;

@@ -141,6 +141,7 @@ Tile choose_tile(const Dims &d);
 struct mycel_op {
     int device = 0;
     hipStream_t stream = nullptr;
+    int ku_columns = -1;  // K*u form: 0 one node per thread, 2/4/8 z-marching columns, -1 by grid size
     mycel::Dims d{};
     double h = 1, e0 = 1, nu = 0.3, emin = 0, penal = 3;
 
@@ -184,7 +185,8 @@ void op_upload_model(mycel_op *op);
 void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStream_t s);
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s,
                     double *d_dot_partial = nullptr, const int *d_stop = nullptr, const Batch &bt = Batch{});
-int apply_k_blocks(const mycel_op *op);
+// Blocks (= dot partials) of the K*u launch that writes x.y partials; batched: a multi-RHS launch.
+int apply_k_blocks(const mycel_op *op, bool batched = false);
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s,
                         double *d_dot_partial = nullptr, const int *d_stop = nullptr);
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s);

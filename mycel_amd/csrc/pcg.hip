@@ -612,7 +612,7 @@ void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipS
 void ensure_batch(mycel_op *op, PcgWork *w, int R) {
     if (R <= w->b_cap) return;
     const size_t n = static_cast<size_t>(w->n) * R;
-    const int nb_node = cdiv(op->d.n_nodes, 256);
+    const int nb_node = apply_k_blocks(op, true);
     w->br.resize(n);
     w->bz.resize(n);
     w->bp.resize(n);
@@ -635,7 +635,7 @@ void ensure_batch(mycel_op *op, PcgWork *w, int R) {
 // grid rows y = 0..R-1).
 void enqueue_iteration_gmg_batch(mycel_op *op, PcgWork *w, double *d_X, int R, int slot, hipStream_t s) {
     PcgState *st = w->bstate.get();
-    const int nb_node = cdiv(op->d.n_nodes, 256);
+    const int nb_node = apply_k_blocks(op, true);  // partials of the batched K*u launch
     const int words = static_cast<int>(sizeof(PcgState) / sizeof(int));
     const Batch bt{R, static_cast<long long>(w->n), nb_node, words};
     launch_apply_k(op, w->bp.get(), w->bap.get(), kApplyAProjected, s, w->bpart_pap.get(), &st->halt, bt);
@@ -902,7 +902,6 @@ int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, dou
     PcgState *st = w->bstate.get();
     const dim3 b256(256);
     const int dstride = static_cast<int>(sizeof(PcgState) / sizeof(double));
-    const int nb_node = cdiv(op->d.n_nodes, 256);
     const Batch bt_plain{R, nn, 0, 0};
 
     // The single-RHS set-up of pcg_solve_device, for every RHS at once (PcgSolver.cs:113-209): x[c] = 0,

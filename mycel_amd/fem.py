@@ -288,6 +288,11 @@ class FemOperator:
         L.check(L.load().mycel_op_apply_k(self._h, L.ptr(x), L.ptr(out)))
         return out
 
+    def SetMatvecColumns(self, columns: int) -> None:
+        """K*u kernel form (no counterpart in the reference): 0 = one node per thread, 2/4/8 = each thread
+        marches that many nodes along z, -1 = by grid size. Every form gives bit-identical results."""
+        L.check(L.load().mycel_op_set_matvec_columns(self._h, int(columns)))
+
     def ApplyA(self, x, y=None, xIsAlreadyProjected=False):
         """y = [PKP + (I-P)] x (Fem/FemOperator.cs:250-294). Returns y."""
         x = self._vec(x)
