@@ -264,6 +264,60 @@ __device__ __forceinline__ double apply_k_column(const Dims &g, const KeSym<T> &
     return dotv;
 }
 
+// Half of K x at node (i, j, k) for the slab-split gathers (two lanes per node): half 0 sums the four
+// elements below the node (corners 4-7, node layers k-1 and k), half 1 the four above it. Half 1 runs the
+// very same instructions as half 0 on z-mirrored data: the cube's stiffness satisfies
+// Ke[A][B] = S Ke[A^4][B^4] S with S = diag(1, 1, -1), so the elements above are the lower-corner rows applied
+// to layers k+1 (as the lower layer) and k with their z components negated, and the result's z row negated.
+// Both halves therefore share a wave without diverging, and each loads two layers, in one round trip.
+// kx receives this half's sum, xc the node's own x values (k0: as in ku_gather).
+template <typename T>
+__device__ __forceinline__ void ku_gather_half(const Dims &g, const KeSym<T> &ks, const T *__restrict__ ep,
+                                               const T *__restrict__ x, int i, int j, int k, int k0, int half,
+                                               T (&kx)[3], T (&xc)[3]) {
+    const int zb = k0 > 0 ? k0 - 1 : 0;
+    const size_t layer = static_cast<size_t>(g.nx1) * g.ny1;
+    const T *xb = x + 3 * layer * zb;
+    const T *eb = ep + static_cast<size_t>(g.px) * g.py * k0;
+    constexpr uint32_t sT = sizeof(T);
+    const uint32_t ex = sT, ey = sT * g.px, ez = ey * g.py;
+    // Padded modulus index of the element at quadrant (0, 0) of this half: layer k (below) or k + 1 (above).
+    const uint32_t e0 = sT * ((i + 1) + g.px * ((j + 1) + g.py * (k + half - k0)));
+    T E[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) E[q] = ld_off(eb, e0 - (corner_x(q) ? ex : 0u) - (corner_y(q) ? ey : 0u));
+    const uint32_t sx = 3 * sT, sy = sx * g.nx1, sz = sy * g.ny1;
+    const uint32_t oc = sx * i;
+    const uint32_t xs[3] = {i > 0 ? oc - sx : oc, oc, i < g.nx ? oc + sx : oc};
+    const uint32_t oy[3] = {j > 0 ? sy * (j - 1) : 0u, sy * j, j < g.ny ? sy * (j + 1) : sy * j};
+    const int kl = half ? (k < g.nz ? k + 1 : k) : (k > 0 ? k - 1 : k);  // the (mirrored) lower layer
+    const uint32_t zl = sz * (kl - zb), zm = sz * (k - zb);
+    uint32_t rows[3];
+    T lo[9][3], up[9][3];
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) rows[dy] = oy[dy] + zl;
+    load_layer<T, kApplyK>(xb, nullptr, xs, rows, lo);
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) rows[dy] = oy[dy] + zm;
+    load_layer<T, kApplyK>(xb, nullptr, xs, rows, up);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xc[d] = up[4][d];
+    const T sgn = half ? T(-1) : T(1);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        lo[q][2] *= sgn;
+        up[q][2] *= sgn;
+    }
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    corner_rows_slab<4>(ks, lo, up, E[0], acc0, acc1, acc2);
+    corner_rows_slab<5>(ks, lo, up, E[1], acc0, acc1, acc2);
+    corner_rows_slab<6>(ks, lo, up, E[2], acc0, acc1, acc2);
+    corner_rows_slab<7>(ks, lo, up, E[3], acc0, acc1, acc2);
+    kx[0] = acc0;
+    kx[1] = acc1;
+    kx[2] = acc2 * sgn;
+}
+
 // Returns this node's contribution x.y (FP64) for fused dot products; n >= n_nodes is a no-op. k0: as in
 // ku_gather.
 template <typename T, int MODE>

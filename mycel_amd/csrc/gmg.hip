@@ -258,6 +258,84 @@ __global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks
     if (n < g.n_nodes) residual_node(g, ks, ep, x + o, b + o, inv, r + o, n, first_layer(g, n0));
 }
 
+// Slab-split forms of the three gather kernels (two lanes per node, ku_gather_half): a 256-thread block covers
+// 128 nodes, wave w of it nodes t0 + 32 w .. + 31, lanes 0-31 the elements below their node and lanes 32-63
+// the elements above; the two halves meet with one cross-lane add. Against one lane per node this halves
+// the serial instruction stream of a wave and needs one memory round trip instead of two, which is what
+// bounds these launches on the coarse levels (few waves per SIMD).
+constexpr int kSplitNodes = 128;  // nodes per 256-thread block
+
+// Combined K x (kx) and own x values (xc) for the node of this lane pair; true on the lane that finishes it.
+__device__ __forceinline__ bool split_gather(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
+                                             const float *__restrict__ x, int t0, float (&kx)[3], float (&xc)[3],
+                                             int &n) {
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    n = t0 + (threadIdx.x >> 6) * 32 + (lane & 31);
+#pragma unroll
+    for (int c = 0; c < 3; c++) kx[c] = xc[c] = 0.0f;
+    if (n < g.n_nodes) {
+        int i, j, k;
+        node_ijk(g, n, i, j, k);
+        ku_gather_half<float>(g, ks, ep, x, i, j, k, first_layer(g, t0), half, kx, xc);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) kx[c] += __shfl_down(kx[c], 32, 64);  // every lane takes part
+    return half == 0 && n < g.n_nodes;
+}
+
+__global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                    const float *__restrict__ x, const float *__restrict__ b,
+                                                    float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                    float cr, float *__restrict__ x_out) {
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    d += o;
+    x_out += o;
+    float kx[3], xc[3];
+    int n;
+    if (split_gather(g, ks, ep, x, xcd_block(blockIdx.x, gridDim.x) * kSplitNodes, kx, xc, n)) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            const float r = b[dof] - kx[c];
+            const float dn = cd * d[dof] + cr * (inv[dof] * r);
+            x_out[dof] = xc[c] + dn;
+            d[dof] = dn;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gmg_residual_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                        const float *__restrict__ x, const float *__restrict__ b,
+                                                        const float *__restrict__ inv, float *__restrict__ r) {
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    r += o;
+    float kx[3], xc[3];
+    int n;
+    if (split_gather(g, ks, ep, x, xcd_block(blockIdx.x, gridDim.x) * kSplitNodes, kx, xc, n)) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            r[dof] = inv[dof] > 0.0f ? b[dof] - kx[c] : 0.0f;
+        }
+    }
+}
+
+// Whether the residual on every multi-block level and the Chebyshev steps below the finest level use the
+// slab-split gathers (MYCEL_GMG_SPLIT=0 turns them off). Measured per kernel on the holed box: residual 7.8 ->
+// 7.4 us on the finest level and ~4.4 -> ~4.1 us below it, coarse Chebyshev steps ~4.5 -> ~4.4 us; on the
+// finest level the Chebyshev steps, which run ~2 waves per SIMD, were no faster split.
+bool gmg_split() {
+    static const bool v = [] {
+        const char *e = std::getenv("MYCEL_GMG_SPLIT");
+        return !(e && *e == '0');
+    }();
+    return v;
+}
+
 // b0 = float(r64) on unconstrained DOFs; the finest pre-smoother's first step x = d = inv b0 / theta.
 __global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__restrict__ r64,
                                                        const float *__restrict__ inv, float theta_inv,
@@ -736,7 +814,8 @@ struct GmgLevel {
     float kdiag = 0.0f;  // diagonal entry of the level's Ke
     const float *ep = nullptr;
     DevBuf<float> ep_own, inv, b, x, x2, dv, r;
-    int blocks = 0;
+    int blocks = 0;    // 256 nodes per block (one lane per node)
+    int blocks_s = 0;  // kSplitNodes per block (the slab-split gathers)
 };
 
 struct Gmg {
@@ -824,6 +903,7 @@ static void gmg_build(mycel_op *op) {
         L.dv.resize(d.n_dofs);
         L.r.resize(d.n_dofs);
         L.blocks = cdiv(d.n_nodes, 256);
+        L.blocks_s = cdiv(d.n_nodes, kSplitNodes);
         g->levels = l + 1;
         if (d.n_dofs <= kCoarseMaxDofs) break;
         if (d.nx < 2 || d.ny < 2 || d.nz < 2)
@@ -889,6 +969,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     if (n_rhs < 1 || n_rhs > g->rhs_cap) throw Error(MYCEL_E_ARGUMENT, "multigrid: RHS count beyond the reserved");
     const int R = n_rhs;
     const int top = g->levels - 1;
+    const bool split = gmg_split();
     float *cur[kMaxLevels], *nxt[kMaxLevels];
     for (int l = 0; l <= top; l++) {
         cur[l] = g->lv[l].x.get();
@@ -896,8 +977,12 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     }
     auto cheb = [&](int l, float cd, float cr) {
         GmgLevel &L = g->lv[l];
-        hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(), L.dv.get(),
-                           L.inv.get(), cd, cr, nxt[l]);
+        if (split && l > 0)
+            hipLaunchKernelGGL(k_gmg_cheb_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
+                               L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
+        else
+            hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
+                               L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
         std::swap(cur[l], nxt[l]);
     };
     // Levels [t0, top] run in the single-workgroup tail kernel (the finest level never does: it has the FP64
@@ -970,8 +1055,12 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
-        hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                           L.inv.get(), L.r.get());
+        if (split)
+            hipLaunchKernelGGL(k_gmg_residual_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
+                               L.inv.get(), L.r.get());
+        else
+            hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
+                               L.inv.get(), L.r.get());
         hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
                            C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
@@ -987,8 +1076,8 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     // (a grid of <= kCoarseMaxDofs DOFs) was solved directly above: the step is then the identity (cd = cr = 0).
     const int fd = g->fine_degree;
     const float cd = top == 0 ? 0.0f : g->cd[fd - 1], cr = top == 0 ? 0.0f : g->cr[fd - 1];
-    hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(), F.dv.get(),
-                       F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
+    hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
+                       F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
     MYCEL_LAUNCH_CHECK();
 }
 
