@@ -261,7 +261,14 @@ def mode_solve(args):
             "compliance_rel_err_vs_fp64_1e-12": abs(cmx - c_ref) / abs(c_ref), "u_rel_err": u_err},
         "fp64": {"solves_per_s": solves64, "pcg_iterations": r64.Iterations, "ms_per_solve": el64 / args.steps * 1e3,
                  "ms_per_pcg_iteration": el64 / args.steps * 1e3 / r64.Iterations,
-                 "compliance_rel_err_vs_fp64_1e-12": abs(c64 - c_ref) / abs(c_ref)},
+                 "compliance_rel_err_vs_fp64_1e-12": abs(c64 - c_ref) / abs(c_ref),
+                 # One Jacobi-PCG iteration: the matvec plus 64 B per active DOF (read x, r, p, ap, 1/diag;
+                 # write x, r, p).
+                 "iteration_roofline": {
+                     "algorithmic_bytes": bytes_mv + 64 * 3 * nn,
+                     "achieved_gbs": (bytes_mv + 64 * 3 * nn) / (el64 / args.steps / r64.Iterations) / 1e9,
+                     "hbm_frac": (bytes_mv + 64 * 3 * nn) / (el64 / args.steps / r64.Iterations) / 1e9
+                     / HBM_PEAK_GBS}},
         "mixed_speedup_vs_fp64": solvesmg / solves64,
         "mixed_host_buffers_ms_per_solve": host_ms,
         "ms_per_matvec": {"fp64": mv_ms, "fp32": mv32_ms},
