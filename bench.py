@@ -274,13 +274,13 @@ def mode_solve(args):
         "ms_per_matvec": {"fp64": mv_ms, "fp32": mv32_ms},
         "gdof_per_s_matvec": {"active": 3 * nn / (mv_ms * 1e-3) / 1e9, "box": p.n_dofs / (mv_ms * 1e-3) / 1e9},
         "roofline": {
-            "kernel": "k_apply_k<double,kApplyK>",
+            "kernel": KU_KERNEL,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("k_apply_k<double>", WORKLOAD),
+            "traffic": traffic_from_profiles("k_apply_k_ec<double>", WORKLOAD),
             "algorithmic_bytes_per_launch": bytes_mv,
             "avg_launch_ms": mv_ms,
             "fp64": {"achieved_tflops": flops_mv / (mv_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
@@ -327,6 +327,7 @@ def mode_sweep(args):
     active element."""
     import numpy as np
 
+    import mycel_amd as M
     from mycel_amd import FemOperator
     from mycel_amd import workloads as W
 
@@ -351,12 +352,15 @@ def mode_sweep(args):
         row = {"grid": [nx, ny, nz], "holed": holed, "active_elements": int(na), "active_nodes": nn,
                "box_dofs": 3 * (nx + 1) * (ny + 1) * (nz + 1)}
         cols = {}
-        for kz in (0, 2, 4, 8):  # K*u form: one node per thread, or kz-node z-marching columns
-            op.SetMatvecColumns(kz)
-            cols[str(kz)] = {"fp64_us": time_matvecs(op, args.matvecs) * 1e3,
-                             "fp32_us": time_matvecs(op, args.matvecs, fp32=True) * 1e3}
-        row["columns"] = cols
-        op.SetMatvecColumns(-1)
+        # K*u forms: the node gather (one node per thread, or kz-node z-marching columns) and the
+        # element-centric product with kz node layers per tile.
+        for form, tag, kzs in ((M.MATVEC_NODE_GATHER, "gather", (0, 2, 4, 8)), (M.MATVEC_ELEMENT, "element", (2, 4, 8))):
+            for kz in kzs:
+                op.SetMatvecForm(form, kz)
+                cols[f"{tag}{kz}"] = {"fp64_us": time_matvecs(op, args.matvecs) * 1e3,
+                                      "fp32_us": time_matvecs(op, args.matvecs, fp32=True) * 1e3}
+        row["forms"] = cols
+        op.SetMatvecForm(M.MATVEC_AUTO)
         for name, fp32, sz in (("fp64", False, 8), ("fp32", True, 4)):
             ms = time_matvecs(op, args.matvecs, fp32=fp32)
             byt = 2 * 3 * sz * nn + sz * na
@@ -369,7 +373,7 @@ def mode_sweep(args):
         op.close()
         print(f"[sweep] {nx}x{ny}x{nz}{' holed' if holed else ''}: fp64 {row['fp64']['us_per_launch']:.2f} us "
               f"(hbm {row['fp64']['hbm_frac']:.3f}, fp64 {row['fp64']['fp64_frac']:.3f}), fp32 "
-              f"{row['fp32']['us_per_launch']:.2f} us (hbm {row['fp32']['hbm_frac']:.3f}); by columns "
+              f"{row['fp32']['us_per_launch']:.2f} us (hbm {row['fp32']['hbm_frac']:.3f}); by form "
               + " ".join(f"{k}:{v['fp64_us']:.1f}/{v['fp32_us']:.1f}" for k, v in cols.items()), file=sys.stderr,
               flush=True)
         rows.append(row)
@@ -518,6 +522,8 @@ def mode_cases(args):
 
 
 COPY_BYTES = 256 << 20  # the device copy in --mode kernels
+# The K*u kernel ApplyK runs on the holed box (the automatic form: element-centric, 2 node layers per tile).
+KU_KERNEL = "k_apply_k_ec<double,kApplyK,2>"
 
 
 def _csv_rows(directory, suffix):
@@ -588,8 +594,13 @@ def mode_summarize(args):
                 traffic[k] = b
     print("\n".join(lines))
     if args.traffic_json and traffic:
-        ku = {f"k_apply_k<{t}>@{WORKLOAD}": traffic[f"k_apply_k<{t}, 0, false>"]
-              for t in ("double", "float") if f"k_apply_k<{t}, 0, false>" in traffic}
+        # Plain K*u launches (mode 0, no dot partials) of either form, keyed without the template tail.
+        ku = {}
+        for k, v in traffic.items():
+            name, targs = k.split("<", 1)
+            parts = [a.strip() for a in targs.rstrip(">").split(",")]
+            if len(parts) >= 3 and parts[1] == "0" and parts[2] == "false":
+                ku[f"{name}<{parts[0]}>@{WORKLOAD}"] = v
         with open(args.traffic_json, "w") as fh:
             json.dump(ku, fh, indent=1, sort_keys=True)
 

@@ -70,6 +70,11 @@ extern "C" {
 #define MYCEL_PRECISION_MIXED_BF16 2
 #define MYCEL_PRECISION_MIXED_GMG 3
 
+/* K*u forms (mycel_op_set_matvec_form). */
+#define MYCEL_MATVEC_AUTO 0        /* by grid size (the default) */
+#define MYCEL_MATVEC_NODE_GATHER 1 /* one thread per node (columns 0) or per 2/4/8-node z-column */
+#define MYCEL_MATVEC_ELEMENT 2     /* element-centric: 8x8 element-column tiles, columns = 2/4/8 node layers */
+
 typedef struct mycel_op mycel_op;
 typedef struct mycel_filter mycel_filter;
 
@@ -143,9 +148,14 @@ MYCEL_API int32_t mycel_op_apply_k_d(mycel_op *op, const double *d_x, double *d_
 MYCEL_API int32_t mycel_op_apply_a_d(mycel_op *op, const double *d_x, double *d_y, int32_t x_is_projected, void *stream);
 /* The FP32 operator of the mixed-precision solver (FP32 Ke and E_e), device pointers, FP32 DOF vectors. */
 MYCEL_API int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *d_y, void *stream);
-/* K*u kernel form (a tuning option with no counterpart in the reference): columns = 0 runs one node per
- * thread, 2 / 4 / 8 let each thread march that many nodes along z (less L1 traffic on large grids), -1 (the
- * default) chooses by grid size. All forms give bit-identical results. */
+/* K*u kernel form (a tuning option with no counterpart in the reference) for ApplyK / ApplyA and the FP32
+ * operator; the solvers' fused-dot launches always use the automatic form. MYCEL_MATVEC_NODE_GATHER: columns
+ * = 0 runs one node per thread, 2 / 4 / 8 let each thread march that many nodes along z (all bit-identical to
+ * each other). MYCEL_MATVEC_ELEMENT: each element's product once, in the monomial basis, columns = 2 / 4 / 8
+ * node layers per tile (all bit-identical to each other). MYCEL_MATVEC_AUTO (columns ignored): by grid size.
+ * The two families agree to rounding (both within 1e-12 of the FP64 reference). */
+MYCEL_API int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns);
+/* The gather form with columns 0 / 2 / 4 / 8, or -1 for MYCEL_MATVEC_AUTO. */
 MYCEL_API int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns);
 /* FemOperator.DiagA / Ke / ConstrainedDofs / ActiveElements properties. */
 MYCEL_API int32_t mycel_op_get_diag_a(mycel_op *op, double *diag);

@@ -4,7 +4,8 @@ The compute path is the HIP library libmycel_hip.so (C ABI in include/mycel_hip.
 the host-side mirror of the reference's managed API (Mycel.Core.Fem / Mycel.Core.Optimization)
 over that ABI. There is no CPU fallback: importing the API without the built library raises.
 """
-from ._lib import (PRECISION_FP64, PRECISION_MIXED_BF16, PRECISION_MIXED_FP32, PRECISION_MIXED_GMG, ArgumentError,
+from ._lib import (MATVEC_AUTO, MATVEC_ELEMENT, MATVEC_NODE_GATHER, PRECISION_FP64, PRECISION_MIXED_BF16,
+                   PRECISION_MIXED_FP32, PRECISION_MIXED_GMG, ArgumentError,
                    ArgumentNullError, ArgumentOutOfRangeError, DeviceError, InvalidOperationError, MycelError,
                    OperationCanceledError, OutOfMemoryError)
 from .fem import (FeMaterial, FemOperator, FeModel, GridSpec, Hex8Element, JacobiPreconditioner, LoadCase,
@@ -14,6 +15,7 @@ from .optimization import (DensityFilter, IterationSnapshot, OcOptimizer, Optimi
 
 __all__ = [
     "PRECISION_FP64", "PRECISION_MIXED_FP32", "PRECISION_MIXED_BF16", "PRECISION_MIXED_GMG",
+    "MATVEC_AUTO", "MATVEC_NODE_GATHER", "MATVEC_ELEMENT",
     "ArgumentError", "ArgumentNullError", "ArgumentOutOfRangeError", "DeviceError", "InvalidOperationError",
     "MycelError", "OperationCanceledError", "OutOfMemoryError",
     "FeMaterial", "FemOperator", "FeModel", "GridSpec", "Hex8Element", "JacobiPreconditioner", "LoadCase",

@@ -27,6 +27,9 @@ PRECISION_FP64 = 0
 PRECISION_MIXED_FP32 = 1
 PRECISION_MIXED_BF16 = 2
 PRECISION_MIXED_GMG = 3
+MATVEC_AUTO = 0
+MATVEC_NODE_GATHER = 1
+MATVEC_ELEMENT = 2
 
 PCG_MESSAGES = {
     0: None,
@@ -191,6 +194,7 @@ SIGNATURES = {
     "mycel_oc_update": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _d, C.POINTER(_d)]),
     "mycel_simp_options_default": (None, [C.POINTER(SimpOptionsC)]),
     "mycel_op_set_matvec_columns": (_i, [_vp, _i]),
+    "mycel_op_set_matvec_form": (_i, [_vp, _i, _i]),
     "mycel_run_quantize": (_i, [_vp, C.c_int64, _vp, C.POINTER(C.c_uint32)]),
     "mycel_run_checksum": (_i, [_vp, C.c_int64, C.POINTER(C.c_uint32)]),
     "mycel_server_create": (_i, [_d, _i, C.POINTER(_vp)]),

@@ -280,7 +280,7 @@ __device__ __forceinline__ void ku_gather_half(const Dims &g, const KeSym<T> &ks
     const T *xb = x + 3 * layer * zb;
     const T *eb = ep + static_cast<size_t>(g.px) * g.py * k0;
     constexpr uint32_t sT = sizeof(T);
-    const uint32_t ex = sT, ey = sT * g.px, ez = ey * g.py;
+    const uint32_t ex = sT, ey = sT * g.px;
     // Padded modulus index of the element at quadrant (0, 0) of this half: layer k (below) or k + 1 (above).
     const uint32_t e0 = sT * ((i + 1) + g.px * ((j + 1) + g.py * (k + half - k0)));
     T E[4];
@@ -344,101 +344,6 @@ __device__ __forceinline__ double apply_k_node(const Dims &g, const KeSym<T> &ks
         }
         y[dof] = out[d];
         dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
-    }
-    return dotv;
-}
-
-}  // namespace mycel
-
-namespace mycel {
-
-// ---------------------------------------------------------------------------------------------------
-// LDS-tiled form. A block owns a TX x TY x TZ brick of nodes (256 threads, one node each). It first
-// stages the brick plus a one-node halo of x (as three SoA planes, so neighbouring lanes read
-// neighbouring 8-byte words: no bank conflicts) and the (TX+1)(TY+1)(TZ+1) incident element moduli into
-// LDS with contiguous loads, then every thread gathers its 27 neighbours from LDS. Each x value is read
-// from L2 about (H / 256) times instead of 27 times through L1 at a 24-byte stride.
-// ---------------------------------------------------------------------------------------------------
-__host__ __device__ constexpr int tile_halo_nodes(int tx, int ty, int tz) { return (tx + 2) * (ty + 2) * (tz + 2); }
-__host__ __device__ constexpr int tile_elems(int tx, int ty, int tz) { return (tx + 1) * (ty + 1) * (tz + 1); }
-
-template <typename T, int MODE, bool DOT>
-__device__ __forceinline__ double apply_k_tile(const Dims &g, const Tile &tl, const KeSym<T> &ks,
-                                               const T *__restrict__ ep, const T *__restrict__ x,
-                                               const uint8_t *__restrict__ flags, T *__restrict__ y, int tile,
-                                               T *__restrict__ lds) {
-    const int HX = tl.tx + 2, HY = tl.ty + 2, HZ = tl.tz + 2, H = HX * HY * HZ;
-    const int EX = tl.tx + 1, EY = tl.ty + 1, EH = EX * EY * (tl.tz + 1);
-    T *su0 = lds, *su1 = lds + H, *su2 = lds + 2 * H, *se = lds + 3 * H;
-    const int bx = tile % tl.ntx, rr = tile / tl.ntx, by = rr % tl.nty, bz = rr / tl.nty;
-    const int x0 = bx * tl.tx, y0 = by * tl.ty, z0 = bz * tl.tz;
-    for (int h = threadIdx.x; h < H; h += 256) {
-        const int hx = h % HX, hr = h / HX, hy = hr % HY, hz = hr / HY;
-        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-        T v0 = 0, v1 = 0, v2 = 0;
-        if (gx >= 0 && gx <= g.nx && gy >= 0 && gy <= g.ny && gz >= 0 && gz <= g.nz) {
-            const int m = 3 * (gx + g.nx1 * (gy + g.ny1 * gz));
-            v0 = x[m];
-            v1 = x[m + 1];
-            v2 = x[m + 2];
-            if (MODE == kApplyAGeneral) {
-                if (flags[m] & kFlagConstrained) v0 = T(0);
-                if (flags[m + 1] & kFlagConstrained) v1 = T(0);
-                if (flags[m + 2] & kFlagConstrained) v2 = T(0);
-            }
-        }
-        su0[h] = v0;
-        su1[h] = v1;
-        su2[h] = v2;
-    }
-    for (int q = threadIdx.x; q < EH; q += 256) {
-        const int ex = q % EX, er = q / EX, ey = er % EY, ez = er / EY;
-        const int px = x0 + ex, py = y0 + ey, pz = z0 + ez;  // padded coordinates of element (x0-1+ex, ...)
-        se[q] = (px < g.px && py < g.py && pz < g.pz) ? ep[px + g.px * (py + g.py * pz)] : T(0);
-    }
-    __syncthreads();
-    double dotv = 0.0;
-    const int tx = threadIdx.x % tl.tx, tr = threadIdx.x / tl.tx, ty = tr % tl.ty, tz = tr / tl.ty;
-    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
-    if (tz < tl.tz && gx <= g.nx && gy <= g.ny && gz <= g.nz) {
-        T E[8];
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-            E[a] = se[(tx + 1 - corner_x(a)) + EX * ((ty + 1 - corner_y(a)) + EY * (tz + 1 - corner_z(a)))];
-        T acc0 = 0, acc1 = 0, acc2 = 0;
-        const bool any = (E[0] != T(0)) | (E[1] != T(0)) | (E[2] != T(0)) | (E[3] != T(0)) | (E[4] != T(0)) |
-                         (E[5] != T(0)) | (E[6] != T(0)) | (E[7] != T(0));
-        const int c0 = (tx + 1) + HX * ((ty + 1) + HY * (tz + 1));
-        if (any) {
-            T u[27][3];
-#pragma unroll
-            for (int dz = 0; dz < 3; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 3; dy++)
-#pragma unroll
-                    for (int dx = 0; dx < 3; dx++) {
-                        const int l = c0 + (dx - 1) + HX * ((dy - 1) + HY * (dz - 1));
-                        const int s = dx + 3 * dy + 9 * dz;
-                        u[s][0] = su0[l];
-                        u[s][1] = su1[l];
-                        u[s][2] = su2[l];
-                    }
-            all_corners<0>(ks, u, E, acc0, acc1, acc2);
-        }
-        const T xc[3] = {su0[c0], su1[c0], su2[c0]};
-        T out[3] = {acc0, acc1, acc2};
-        const int n = gx + g.nx1 * (gy + g.ny1 * gz);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const int dof = 3 * n + d;
-            if (MODE == kApplyAProjected) {
-                if (flags[dof] & kFlagFixed) out[d] = x[dof];
-            } else if (MODE == kApplyAGeneral) {
-                if (flags[dof] & kFlagConstrained) out[d] = x[dof];
-            }
-            y[dof] = out[d];
-            if (DOT) dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
-        }
     }
     return dotv;
 }

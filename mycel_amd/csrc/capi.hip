@@ -83,7 +83,6 @@ mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *f
     auto op = std::make_unique<mycel_op>();
     op->device = g_device;
     op->d = make_dims(g.nx, g.ny, g.nz);
-    op->tile = choose_tile(op->d);
     op->h = g.h;
     op->e0 = e0;
     op->nu = nu;
@@ -134,6 +133,11 @@ mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *f
         for (int c = 0; c < 24; c++)
             if (std::fabs(op->ke[r * 24 + c] - kKeSign[r][c] * op->ks64.v[kKeClass[r][c]]) > 1e-14 * kmax)
                 throw Error(MYCEL_E_ARGUMENT, "element stiffness does not have the isotropic cube symmetry");
+    if (ke_mono_values(op->ke, op->km64) > 1e-12)
+        throw Error(MYCEL_E_ARGUMENT, "element stiffness does not have the monomial-basis pattern of the cube");
+    op->km32 = {static_cast<float>(op->km64.a), static_cast<float>(op->km64.b), static_cast<float>(op->km64.c),
+                static_cast<float>(op->km64.d), static_cast<float>(op->km64.e), static_cast<float>(op->km64.f),
+                static_cast<float>(op->km64.h), static_cast<float>(op->km64.q)};
     double cst[780];
     std::copy(op->ke, op->ke + 576, cst);
     std::copy(op->diag_ke, op->diag_ke + 24, cst + 576);
@@ -668,13 +672,25 @@ int32_t mycel_op_get_diag_a(mycel_op *op, double *diag) {
     });
 }
 
-int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns) {
+int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns) {
     return guarded([&] {
         require(op, MYCEL_E_ARGUMENT_NULL, "op");
-        require(columns == -1 || columns == 0 || columns == 2 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
-                "matvec columns must be -1, 0, 2, 4 or 8");
-        op->ku_columns = columns;
+        if (form == MYCEL_MATVEC_NODE_GATHER)
+            require(columns == 0 || columns == 2 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
+                    "gather columns must be 0, 2, 4 or 8");
+        else if (form == MYCEL_MATVEC_ELEMENT)
+            require(columns == 2 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
+                    "element-centric columns must be 2, 4 or 8");
+        else
+            require(form == MYCEL_MATVEC_AUTO, MYCEL_E_ARGUMENT, "unknown matvec form");
+        op->ku_form = form;
+        op->ku_kz = form == MYCEL_MATVEC_AUTO ? 0 : columns;
     });
+}
+
+int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns) {
+    if (columns == -1) return mycel_op_set_matvec_form(op, MYCEL_MATVEC_AUTO, 0);
+    return mycel_op_set_matvec_form(op, MYCEL_MATVEC_NODE_GATHER, columns);
 }
 
 int32_t mycel_op_get_ke(mycel_op *op, double *ke) {

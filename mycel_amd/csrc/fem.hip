@@ -20,6 +20,7 @@
 
 #include "apply_k.hpp"
 #include "device_common.hpp"
+#include "ku_element.hpp"
 #include "mycel_internal.hpp"
 
 namespace mycel {
@@ -172,6 +173,30 @@ __global__ __launch_bounds__(256, 2) void k_apply_k(Dims g, const KeSym<T> ks, c
     }
 }
 
+// Element-centric K*u (ec_tile): one 8 x 8 tile of element columns per wavefront, four tiles per block, KZ node
+// layers per tile. Per-block partials of x.y as k_apply_k.
+template <typename T, int MODE, bool DOT, int KZ>
+__global__ __launch_bounds__(256) void k_apply_k_ec(Dims g, const KeMono<T> km, const T *__restrict__ ep,
+                                                    const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                                    T *__restrict__ y, double *__restrict__ part,
+                                                    const int *__restrict__ stop, Batch bt, int n_tiles) {
+    __shared__ double red[4];
+    const int rhs = blockIdx.y;
+    x += rhs * bt.vstride;
+    y += rhs * bt.vstride;
+    if (DOT) part += rhs * bt.pstride;
+    if (stop) stop += rhs * bt.sstride;
+    if (DOT && stop && *stop) return;
+    // Wave-uniform by construction; readfirstlane lets the tile arithmetic and addressing live in SGPRs.
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    double dotv = 0.0;
+    if (tile < n_tiles) dotv = ec_tile<T, MODE, KZ>(g, km, ep, x, flags, y, tile);
+    if (DOT) {
+        const double s = block_sum256(dotv, red);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
 // z-marching K*u (apply_k_column): thread t owns column (i, j) = t mod (nx1 ny1) of z-block t / (nx1 ny1),
 // nodes k0 = KZ * z-block onwards; the lanes of a wave are consecutive columns of one z-block (or two, where
 // a wave straddles), so their loads stay coalesced. Per-block partials of x.y as k_apply_k.
@@ -194,23 +219,6 @@ __global__ __launch_bounds__(256) void k_apply_k_zm(Dims g, const KeSym<T> ks, c
     double dotv = 0.0;
     if (k0 <= g.nz)
         dotv = apply_k_column<T, MODE, KZ>(g, ks, ep, x, flags, y, c % g.nx1, c / g.nx1, k0, (t0 / cols) * KZ);
-    if (DOT) {
-        const double s = block_sum256(dotv, red);
-        if (threadIdx.x == 0) part[blockIdx.x] = s;
-    }
-}
-
-// LDS-tiled K*u: one block per brick of nodes (apply_k_tile); dynamic LDS holds the halo and E.
-template <typename T, int MODE, bool DOT>
-__global__ __launch_bounds__(256, 2) void k_apply_k_tiled(Dims g, Tile tl, const KeSym<T> ks, const T *__restrict__ ep,
-                                                          const T *__restrict__ x, const uint8_t *__restrict__ flags,
-                                                          T *__restrict__ y, double *__restrict__ part,
-                                                          const int *__restrict__ stop) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ double red[4];
-    if (DOT && stop && *stop) return;
-    const int tile = xcd_block(blockIdx.x, gridDim.x);
-    const double dotv = apply_k_tile<T, MODE, DOT>(g, tl, ks, ep, x, flags, y, tile, reinterpret_cast<T *>(lds_raw));
     if (DOT) {
         const double s = block_sum256(dotv, red);
         if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -354,158 +362,106 @@ void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStr
     MYCEL_LAUNCH_CHECK();
 }
 
-// Brick shape for the tiled K*u: minimise thread slots (idle lanes cost like busy ones) plus the halo
-// staging, over bricks of 192..256 nodes.
-Tile choose_tile(const Dims &d) {
-    Tile best{};
-    double best_cost = 1e300;
-    for (int tx = 8; tx <= 64; tx++)
-        for (int ty = 1; ty <= 16; ty++)
-            for (int tz = 1; tz <= 16; tz++) {
-                const int nt = tx * ty * tz;
-                if (nt > 256 || nt < 192) continue;
-                const int ntx = cdiv(d.nx1, tx), nty = cdiv(d.ny1, ty), ntz = cdiv(d.nz1, tz);
-                const double tiles = static_cast<double>(ntx) * nty * ntz;
-                const double cost = tiles * (256.0 + 0.35 * tile_halo_nodes(tx, ty, tz));
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best = Tile{tx, ty, tz, ntx, nty, ntz};
-                }
-            }
-    return best;
-}
-
-// The per-node kernel is the default: on gfx950 the 27-node gather hits L1/L2 well enough that staging the
-// brick through LDS (one extra barrier, halo re-reads) measured slower (FP64 12.2 us vs 11.1 us per K*u on
-// the 80x40x40 holed box). MYCEL_KU_KERNEL=tiled selects the LDS form.
-static bool use_tiled() {
-    static const bool tiled = [] {
-        const char *v = std::getenv("MYCEL_KU_KERNEL");
-        return v && std::string(v) == "tiled";
-    }();
-    return tiled;
-}
-
-
-template <typename T>
-static size_t tile_lds_bytes(const Tile &t) {
-    return sizeof(T) * (3 * tile_halo_nodes(t.tx, t.ty, t.tz) + tile_elems(t.tx, t.ty, t.tz));
-}
-
-// Column length of the z-marching K*u for this operator, 0 for the one-node-per-thread gather: the
-// operator's setting (mycel_op_set_matvec_columns), else MYCEL_KU_ZM (0, 2, 4 or 8), else by grid size. The
-// column form also needs its offsets (2 KZ + 2 node layers from a block's base) to fit in 32 bits.
+// Column length of the z-marching gather: its offsets (2 KZ + 2 node layers from a block's base) must fit in
+// 32 bits.
 template <typename T>
 static bool zm_fits(const mycel_op *op, int kz) {
     return (2.0 * kz + 2.0) * 3.0 * sizeof(T) * op->d.nx1 * op->d.ny1 < 4294967296.0;
 }
 
-// By grid size (measured on MI355X: 4-node columns take FP64 K*u from 46 to 35 us at 1.06M nodes and from 169
-// to 114 us at 4.3M nodes; below ~500k nodes the one-node-per-thread gather has more waves to hide latency
-// with). The K*u launches that write dot partials always follow this rule, so an operator's partial count
-// never changes.
-static int dot_columns(const mycel_op *op) {
-    const int kz = op->d.n_nodes >= (1 << 19) ? 4 : 0;
-    return kz && zm_fits<double>(op, kz) ? kz : 0;
-}
-
-template <typename T>
-static int zm_kz(const mycel_op *op) {
-    static const int forced = [] {
-        const char *v = std::getenv("MYCEL_KU_ZM");
-        return v && *v ? std::atoi(v) : -1;
-    }();
-    const int kz = op->ku_columns >= 0 ? op->ku_columns : forced >= 0 ? forced : dot_columns(op);
-    if (kz != 2 && kz != 4 && kz != 8) return 0;
-    return zm_fits<T>(op, kz) ? kz : 0;
-}
-
 static int zm_blocks(const mycel_op *op, int kz) { return cdiv(op->d.nx1 * op->d.ny1 * cdiv(op->d.nz1, kz), 256); }
+static int ec_blocks(const mycel_op *op, int kz) { return cdiv(ec_tiles(op->d, kz), 4); }
 
-int apply_k_blocks(const mycel_op *op, bool batched) {
-    if (use_tiled() && !batched) return op->tile.ntx * op->tile.nty * op->tile.ntz;
-    if (const int kz = dot_columns(op)) return zm_blocks(op, kz);
-    return cdiv(op->d.n_nodes, 256);
+struct KuForm {
+    int form;  // MYCEL_MATVEC_NODE_GATHER or MYCEL_MATVEC_ELEMENT
+    int kz;    // node layers per thread (gather: 0 = one node per thread) or per tile (element: 2, 4, 8)
+};
+
+// The default form, by grid size. The element-centric product does about a third of the gather's arithmetic;
+// measured on MI355X (FP64 K*u, best gather form -> element form): 9.1 -> 6.0 us on the 80x40x40 holed box
+// (136k nodes, 2 layers per tile), 13.6 -> 8.6 us at 263k nodes (4 layers), 35.6 -> 19.5 us at 1.06M nodes and
+// 115 -> 62 us at 4.3M nodes (8 layers). Longer columns amortise the extra element layer each tile forms below
+// its first node layer; shorter ones keep enough waves on small grids. The K*u launches that write dot
+// partials always use this rule, so an operator's partial count never changes.
+static KuForm ku_auto(const mycel_op *op) {
+    const int n = op->d.n_nodes;
+    return {MYCEL_MATVEC_ELEMENT, n < (1 << 18) ? 2 : n < (1 << 20) ? 4 : 8};
 }
+
+// The form of a launch: the automatic one for launches with dot partials, else the operator's setting
+// (mycel_op_set_matvec_form); a gather column length whose offsets would not fit falls back to one node per
+// thread.
+template <typename T>
+static KuForm ku_form(const mycel_op *op, bool dot) {
+    if (dot || op->ku_form == MYCEL_MATVEC_AUTO) return ku_auto(op);
+    if (op->ku_form == MYCEL_MATVEC_NODE_GATHER && op->ku_kz && !zm_fits<T>(op, op->ku_kz))
+        return {MYCEL_MATVEC_NODE_GATHER, 0};
+    return {op->ku_form, op->ku_kz};
+}
+
+static int ku_blocks(const mycel_op *op, const KuForm &f) {
+    if (f.form == MYCEL_MATVEC_ELEMENT) return ec_blocks(op, f.kz);
+    return f.kz ? zm_blocks(op, f.kz) : cdiv(op->d.n_nodes, 256);
+}
+
+int apply_k_blocks(const mycel_op *op) { return ku_blocks(op, ku_auto(op)); }
 
 template <typename T, int MODE, bool DOT>
-static void launch_zm(mycel_op *op, int kz, const KeSym<T> &ke, const T *ep, const T *d_x, T *d_y, hipStream_t s,
-                      double *part, const int *stop, const Batch &bt) {
-    const dim3 grid(zm_blocks(op, kz), bt.count), block(256);
+static void launch_ku(mycel_op *op, const KuForm &f, const KeSym<T> &ke, const KeMono<T> &km, const T *ep,
+                      const T *d_x, T *d_y, hipStream_t s, double *part, const int *stop, const Batch &bt) {
+    const dim3 grid(ku_blocks(op, f), bt.count), block(256);
     const uint8_t *fl = op->d_flags.get();
-    if (kz == 2)
-        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 2>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
-    else if (kz == 4)
-        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 4>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
-    else
-        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 8>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
+    if (f.form == MYCEL_MATVEC_ELEMENT) {
+        const int nt = ec_tiles(op->d, f.kz);
+        if (f.kz == 2)
+            hipLaunchKernelGGL((k_apply_k_ec<T, MODE, DOT, 2>), grid, block, 0, s, op->d, km, ep, d_x, fl, d_y, part,
+                               stop, bt, nt);
+        else if (f.kz == 4)
+            hipLaunchKernelGGL((k_apply_k_ec<T, MODE, DOT, 4>), grid, block, 0, s, op->d, km, ep, d_x, fl, d_y, part,
+                               stop, bt, nt);
+        else
+            hipLaunchKernelGGL((k_apply_k_ec<T, MODE, DOT, 8>), grid, block, 0, s, op->d, km, ep, d_x, fl, d_y, part,
+                               stop, bt, nt);
+    } else if (f.kz == 2) {
+        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 2>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop,
+                           bt);
+    } else if (f.kz == 4) {
+        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 4>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop,
+                           bt);
+    } else if (f.kz == 8) {
+        hipLaunchKernelGGL((k_apply_k_zm<T, MODE, DOT, 8>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop,
+                           bt);
+    } else {
+        hipLaunchKernelGGL((k_apply_k<T, MODE, DOT>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, part, stop, bt);
+    }
     MYCEL_LAUNCH_CHECK();
 }
 
 template <typename T>
-static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const T *ep, const T *d_x, T *d_y, int mode,
-                             hipStream_t s, double *d_dot_partial, const int *d_stop, const Batch &bt) {
-    const dim3 block(256);
-    const uint8_t *fl = op->d_flags.get();
-    if (const int kz = d_dot_partial ? dot_columns(op) : zm_kz<T>(op); kz && !(use_tiled() && bt.count == 1)) {
-        if (d_dot_partial) {
-            if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
-            launch_zm<T, kApplyAProjected, true>(op, kz, ke, ep, d_x, d_y, s, d_dot_partial, d_stop, bt);
-        } else if (mode == kApplyK)
-            launch_zm<T, kApplyK, false>(op, kz, ke, ep, d_x, d_y, s, nullptr, nullptr, bt);
-        else if (mode == kApplyAProjected)
-            launch_zm<T, kApplyAProjected, false>(op, kz, ke, ep, d_x, d_y, s, nullptr, nullptr, bt);
-        else
-            launch_zm<T, kApplyAGeneral, false>(op, kz, ke, ep, d_x, d_y, s, nullptr, nullptr, bt);
-        return;
-    }
-    if (use_tiled() && bt.count == 1) {
-        const dim3 grid(apply_k_blocks(op));
-        const Tile tl = op->tile;
-        const size_t lds = tile_lds_bytes<T>(tl);
-        if (d_dot_partial) {
-            if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
-            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyAProjected, true>), grid, block, lds, s, op->d, tl, ke, ep,
-                               d_x, fl, d_y, d_dot_partial, d_stop);
-        } else if (mode == kApplyK) {
-            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyK, false>), grid, block, lds, s, op->d, tl, ke, ep, d_x, fl,
-                               d_y, nullptr, nullptr);
-        } else if (mode == kApplyAProjected) {
-            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyAProjected, false>), grid, block, lds, s, op->d, tl, ke, ep,
-                               d_x, fl, d_y, nullptr, nullptr);
-        } else {
-            hipLaunchKernelGGL((k_apply_k_tiled<T, kApplyAGeneral, false>), grid, block, lds, s, op->d, tl, ke, ep,
-                               d_x, fl, d_y, nullptr, nullptr);
-        }
-        MYCEL_LAUNCH_CHECK();
-        return;
-    }
-    const dim3 grid(cdiv(op->d.n_nodes, 256), bt.count);
+static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const KeMono<T> &km, const T *ep, const T *d_x,
+                             T *d_y, int mode, hipStream_t s, double *d_dot_partial, const int *d_stop,
+                             const Batch &bt) {
+    const KuForm f = ku_form<T>(op, d_dot_partial != nullptr);
     if (d_dot_partial) {
         if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
-        hipLaunchKernelGGL((k_apply_k<T, kApplyAProjected, true>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
-                           d_dot_partial, d_stop, bt);
+        launch_ku<T, kApplyAProjected, true>(op, f, ke, km, ep, d_x, d_y, s, d_dot_partial, d_stop, bt);
     } else if (mode == kApplyK) {
-        hipLaunchKernelGGL((k_apply_k<T, kApplyK, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y, nullptr,
-                           nullptr, bt);
+        launch_ku<T, kApplyK, false>(op, f, ke, km, ep, d_x, d_y, s, nullptr, nullptr, bt);
     } else if (mode == kApplyAProjected) {
-        hipLaunchKernelGGL((k_apply_k<T, kApplyAProjected, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
-                           nullptr, nullptr, bt);
+        launch_ku<T, kApplyAProjected, false>(op, f, ke, km, ep, d_x, d_y, s, nullptr, nullptr, bt);
     } else {
-        hipLaunchKernelGGL((k_apply_k<T, kApplyAGeneral, false>), grid, block, 0, s, op->d, ke, ep, d_x, fl, d_y,
-                           nullptr, nullptr, bt);
+        launch_ku<T, kApplyAGeneral, false>(op, f, ke, km, ep, d_x, d_y, s, nullptr, nullptr, bt);
     }
-    MYCEL_LAUNCH_CHECK();
 }
 
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s, double *d_dot_partial,
                     const int *d_stop, const Batch &bt) {
-    launch_apply_k_t<double>(op, op->ks64, op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, bt);
+    launch_apply_k_t<double>(op, op->ks64, op->km64, op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, bt);
 }
 
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s, double *d_dot_partial,
                         const int *d_stop) {
-    launch_apply_k_t<float>(op, op->ks32, op->d_ep32.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, Batch{});
+    launch_apply_k_t<float>(op, op->ks32, op->km32, op->d_ep32.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, Batch{});
 }
 
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s) {

@@ -1,0 +1,211 @@
+// Element-centric K*u: every element's product Ke u_e is formed once, in the monomial basis of the trilinear
+// element (KeMono in ke_sym.hpp), and each node sums the corner values of its elements in a fixed order.
+//
+// One wavefront owns a tile of 8 x 8 element columns, elements (i0 - 1 + a, j0 - 1 + b) for lane a + 8 b, and
+// marches them along z. Per element layer a lane loads only the element's upper face (4 nodes) and its modulus:
+// the lower face is the face the lane loaded one step before. The Walsh-Hadamard transform factors by face
+// (corner values -> face sums in x and y, then faces -> element coefficients in z), so the face sums carry from
+// step to step as well; the inverse transform is linear, so the upper-face part of one element and the
+// lower-face part of the element above are added before the x/y stage, once per node layer. The node owned by
+// lane (a, b), (i0 - 1 + a, j0 - 1 + b), collects the corner values of the four element columns around it from
+// lanes a - 1 (a DPP row shift) and b - 1 (a lane permute), in the fixed order
+//     (own x0y0 + left x1y0) + below(own x0y1 + left x1y1).
+// Lanes with a = 0 or b = 0 own the tile's halo columns: they form elements for their neighbours and write no
+// node, so a tile writes 7 x 7 node columns. No atomics and no barriers; the result is deterministic.
+//
+// Work per element: one face transform (24 add/sub), the element coefficients (21), the sparse product with
+// the modulus (43) and the select on E != 0 (18), the inverse (54): about 160 operations instead of the 576
+// multiply-adds of the dense product, plus 9 exchanged values per node.
+#pragma once
+
+#include "apply_k.hpp"
+#include "ke_sym.hpp"
+
+namespace mycel {
+
+constexpr int kEcTile = 8;              // element columns per tile edge
+constexpr int kEcOut = kEcTile - 1;     // node columns per tile edge that a tile writes
+
+template <typename T>
+struct FaceSums {
+    T s, x, y, xy;  // sums of a face's 4 corner values weighted by 1, xi, eta, xi eta
+};
+
+// Corner values c0..c3 at (x0,y0), (x1,y0), (x1,y1), (x0,y1) of one face.
+template <typename T>
+__device__ __forceinline__ FaceSums<T> face_sums(T c0, T c1, T c2, T c3) {
+    const T r0 = c1 + c0, r0x = c1 - c0, r1 = c2 + c3, r1x = c2 - c3;
+    return {r1 + r0, r1x + r0x, r1 - r0, r1x - r0x};
+}
+
+// Value of lane - 1 within a row of 16 lanes (lanes with a = 0 read a value they do not use).
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_left(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// Value of lane - 8 (lanes with b = 0 read a value they do not use).
+template <typename T>
+__device__ __forceinline__ T from_below(T v) {
+    return __shfl_up(v, kEcTile, 64);
+}
+
+// Tiles of a grid: cdiv(nx1, 7) x cdiv(ny1, 7) x cdiv(nz1, KZ), x fastest.
+__host__ __device__ inline int ec_tiles(const Dims &g, int kz) {
+    return ((g.nx1 + kEcOut - 1) / kEcOut) * ((g.ny1 + kEcOut - 1) / kEcOut) * ((g.nz1 + kz - 1) / kz);
+}
+
+// One wavefront's tile: writes (K x) (MODE: as ku_gather) for node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7
+// node columns and returns the lane's x.y contribution. Called by all 64 lanes of the wave (tile is uniform).
+template <typename T, int MODE, int KZ>
+__device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
+                                          const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                          T *__restrict__ y, int tile) {
+    const int lane = threadIdx.x & 63, a = lane & (kEcTile - 1), b = lane >> 3;
+    const int ntx = (g.nx1 + kEcOut - 1) / kEcOut, nty = (g.ny1 + kEcOut - 1) / kEcOut;
+    const int tx = tile % ntx, tr = tile / ntx, ty = tr % nty, tz = tr / nty;
+    const int ie = kEcOut * tx - 1 + a, je = kEcOut * ty - 1 + b, k0 = KZ * tz;
+    const bool col_ok = ie <= g.nx && je <= g.ny;  // the padded moduli hold this element column
+    const bool out_ok = a > 0 && b > 0 && col_ok;  // (ie, je) is a node column of this tile's output
+    constexpr uint32_t sT = sizeof(T), sx = 3 * sT;
+    const int i0 = min(max(ie, 0), g.nx), i1 = min(ie + 1, g.nx), j0 = min(max(je, 0), g.ny), j1 = min(je + 1, g.ny);
+    const uint32_t o[4] = {sx * (i0 + g.nx1 * j0), sx * (i1 + g.nx1 * j0), sx * (i1 + g.nx1 * j1),
+                           sx * (i0 + g.nx1 * j1)};
+    const uint32_t eo = col_ok ? sT * ((ie + 1) + g.px * (je + 1)) : 0u;
+    const size_t layer = static_cast<size_t>(g.nx1) * g.ny1, elayer = static_cast<size_t>(g.px) * g.py;
+
+    // Node layer kn (clamped to the grid: out-of-grid layers belong to elements of modulus 0).
+    auto load_face = [&](int kn, T (&v)[4][3]) {
+        const size_t kc = static_cast<size_t>(min(max(kn, 0), g.nz));
+        const T *xl = x + 3 * layer * kc;
+        const uint8_t *fl = MODE == kApplyAGeneral ? flags + 3 * layer * kc : nullptr;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                T vv = ld_off(xl, o[q], d * static_cast<int>(sT));
+                if (MODE == kApplyAGeneral && (ld_off(fl, o[q] / sT, d) & kFlagConstrained)) vv = T(0);
+                v[q][d] = vv;
+            }
+    };
+
+    T cur[4][3];
+    load_face(k0 - 1, cur);
+    FaceSums<T> lo[3], carry[3];
+    T xc[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lo[d] = face_sums(cur[0][d], cur[1][d], cur[2][d], cur[3][d]);
+        carry[d] = {T(0), T(0), T(0), T(0)};
+        xc[d] = cur[0][d];
+    }
+    const T amb = km.a - km.b, fmh = km.f - km.h;
+    const int kend = min(k0 + KZ - 1, g.nz);  // last node layer written
+    double dotv = 0.0;
+#pragma unroll
+    for (int s = 0; s <= KZ; s++) {
+        const int ke = k0 - 1 + s;  // element layer of this step; node layer ke is completed by it
+        if (ke > kend) break;
+        T up[4][3];
+        load_face(ke + 1, up);
+        const T E = col_ok ? ld_off(ep + elayer * (ke + 1), eo) : T(0);
+        FaceSums<T> hi[3];
+        // Element coefficients per component: monomials xi, eta, zeta, xi eta, xi zeta, eta zeta, xi eta zeta.
+        T c[3][7];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            hi[d] = face_sums(up[0][d], up[1][d], up[2][d], up[3][d]);
+            c[d][0] = hi[d].x + lo[d].x;
+            c[d][1] = hi[d].y + lo[d].y;
+            c[d][2] = hi[d].s - lo[d].s;
+            c[d][3] = hi[d].xy + lo[d].xy;
+            c[d][4] = hi[d].x - lo[d].x;
+            c[d][5] = hi[d].y - lo[d].y;
+            c[d][6] = hi[d].xy - lo[d].xy;
+        }
+        // gm = E M c (same layout as c). Modulus-0 elements contribute exactly 0 (a select, so non-finite values
+        // at their nodes stay out).
+        const bool on = E != T(0);
+        T gm[3][7];
+        {
+            const T tr = (c[0][0] + c[1][1]) + c[2][2];
+            const T bt = (E * km.b) * tr, ab = E * amb;
+            gm[0][0] = on ? fma_t(ab, c[0][0], bt) : T(0);
+            gm[1][1] = on ? fma_t(ab, c[1][1], bt) : T(0);
+            gm[2][2] = on ? fma_t(ab, c[2][2], bt) : T(0);
+            const T ce = E * km.c;
+            const T s_uv = on ? ce * (c[0][1] + c[1][0]) : T(0);  // (u_eta, v_xi)
+            const T s_uw = on ? ce * (c[0][2] + c[2][0]) : T(0);  // (u_zeta, w_xi)
+            const T s_vw = on ? ce * (c[1][2] + c[2][1]) : T(0);  // (v_zeta, w_eta)
+            gm[0][1] = s_uv;
+            gm[1][0] = s_uv;
+            gm[0][2] = s_uw;
+            gm[2][0] = s_uw;
+            gm[1][2] = s_vw;
+            gm[2][1] = s_vw;
+            const T de = E * km.d, ee = E * km.e;
+            gm[0][3] = on ? fma_t(de, c[0][3], ee * c[2][5]) : T(0);  // (u_xi.eta, w_eta.zeta)
+            gm[2][5] = on ? fma_t(ee, c[0][3], de * c[2][5]) : T(0);
+            gm[0][4] = on ? fma_t(de, c[0][4], ee * c[1][5]) : T(0);  // (u_xi.zeta, v_eta.zeta)
+            gm[1][5] = on ? fma_t(ee, c[0][4], de * c[1][5]) : T(0);
+            gm[1][3] = on ? fma_t(de, c[1][3], ee * c[2][4]) : T(0);  // (v_xi.eta, w_xi.zeta)
+            gm[2][4] = on ? fma_t(ee, c[1][3], de * c[2][4]) : T(0);
+            const T tw = (c[0][5] + c[1][4]) + c[2][3];           // (u_eta.zeta, v_xi.zeta, w_xi.eta)
+            const T ht = (E * km.h) * tw, fh = E * fmh;
+            gm[0][5] = on ? fma_t(fh, c[0][5], ht) : T(0);
+            gm[1][4] = on ? fma_t(fh, c[1][4], ht) : T(0);
+            gm[2][3] = on ? fma_t(fh, c[2][3], ht) : T(0);
+            const T qe = E * km.q;
+            gm[0][6] = on ? qe * c[0][6] : T(0);
+            gm[1][6] = on ? qe * c[1][6] : T(0);
+            gm[2][6] = on ? qe * c[2][6] : T(0);
+        }
+        // Inverse transform, z stage: the element's lower-face part joins the carried upper-face part of the
+        // element below (node layer ke); its own upper-face part is carried to node layer ke + 1.
+        FaceSums<T> F[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const T gx = gm[d][0], gy = gm[d][1], gz = gm[d][2], gxy = gm[d][3], gxz = gm[d][4], gyz = gm[d][5],
+                    gxyz = gm[d][6];
+            F[d].s = carry[d].s - gz;
+            F[d].x = (carry[d].x + gx) - gxz;
+            F[d].y = (carry[d].y + gy) - gyz;
+            F[d].xy = (carry[d].xy + gxy) - gxyz;
+            carry[d] = {gz, gx + gxz, gy + gyz, gxy + gxyz};
+            lo[d] = hi[d];
+        }
+        if (s > 0) {
+            T out[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // x/y stage: the face's corner values, then the node's four element columns.
+                const T r0 = F[d].s - F[d].y, r1 = F[d].s + F[d].y, r0x = F[d].x - F[d].xy, r1x = F[d].x + F[d].xy;
+                const T f0 = r0 - r0x, f1 = r0 + r0x, f3 = r1 - r1x, f2 = r1 + r1x;
+                const T q0 = f0 + from_left(f1), q3 = f3 + from_left(f2);
+                out[d] = q0 + from_below(q3);
+            }
+            if (out_ok) {
+                const int n = ie + g.nx1 * (je + g.ny1 * ke);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const int dof = 3 * n + d;
+                    if (MODE == kApplyAProjected) {
+                        if (flags[dof] & kFlagFixed) out[d] = xc[d];
+                    } else if (MODE == kApplyAGeneral) {
+                        if (flags[dof] & kFlagConstrained) out[d] = x[dof];
+                    }
+                    y[dof] = out[d];
+                    dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) xc[d] = up[0][d];
+    }
+    return dotv;
+}
+
+}  // namespace mycel

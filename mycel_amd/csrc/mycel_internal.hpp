@@ -127,12 +127,6 @@ struct Batch {
     int sstride = 0;
 };
 
-struct Tile {
-    int tx, ty, tz;     // brick extent in nodes (tx * ty * tz <= 256; the other threads only stage data)
-    int ntx, nty, ntz;  // bricks per axis
-};
-Tile choose_tile(const Dims &d);
-
 }  // namespace mycel
 
 // ------------------------------------------------------------------------------------------------
@@ -141,7 +135,8 @@ Tile choose_tile(const Dims &d);
 struct mycel_op {
     int device = 0;
     hipStream_t stream = nullptr;
-    int ku_columns = -1;  // K*u form: 0 one node per thread, 2/4/8 z-marching columns, -1 by grid size
+    int ku_form = MYCEL_MATVEC_AUTO;  // K*u form of the launches without dot partials (mycel_op_set_matvec_form)
+    int ku_kz = 0;                    // its node layers per thread or tile
     mycel::Dims d{};
     double h = 1, e0 = 1, nu = 0.3, emin = 0, penal = 3;
 
@@ -156,8 +151,9 @@ struct mycel_op {
     double ke[576];
     double diag_ke[24];
     mycel::KeSym<double> ks64;  // Ke as 14 symmetry-class values (kernel arguments)
-    mycel::Tile tile{};         // brick shape of the tiled K*u
     mycel::KeSym<float> ks32;
+    mycel::KeMono<double> km64;  // Ke in the monomial basis (element-centric K*u)
+    mycel::KeMono<float> km32;
 
     // Device state.
     mycel::DevBuf<double> d_ke;        // [0,576) Ke, [576,600) DiagKe, [600,744) Bc(1), [744,780) D(1,nu)
@@ -186,7 +182,7 @@ void op_update_e(mycel_op *op, const double *d_density_box, double penal, hipStr
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s,
                     double *d_dot_partial = nullptr, const int *d_stop = nullptr, const Batch &bt = Batch{});
 // Blocks (= dot partials) of the K*u launch that writes x.y partials; batched: a multi-RHS launch.
-int apply_k_blocks(const mycel_op *op, bool batched = false);
+int apply_k_blocks(const mycel_op *op);
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s,
                         double *d_dot_partial = nullptr, const int *d_stop = nullptr);
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s);

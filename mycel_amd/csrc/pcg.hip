@@ -51,6 +51,8 @@ struct PcgState {
     int message, iterations, nonzero_x, outer;
     double final_rel, last_rel;
     int last_iter, pad;
+    double best_true;    // mixed: smallest true (FP64) relres seen at a reliable update or at the start
+    int no_gain, pad2;   // mixed: consecutive reliable updates that did not improve on best_true
 };
 
 struct PcgWork {
@@ -88,6 +90,12 @@ constexpr int kVecPerBlock = 1024;  // DOFs per block in the vector kernels (4 p
 constexpr int kGraphIters = 32;     // iterations per captured batch (even: the slot parity repeats)
 constexpr int kGmgGraphIters = 4;   // multigrid iterations are ~30 launches each and few in number
 constexpr double kReliableDelta = 0.1;
+// A reliable update is triggered once the FP32 recursive residual has dropped by kReliableDelta. When the FP32
+// recurrences drift from the true residual (extreme stiffness contrast), they keep "converging" while the true
+// residual does not: kReliableStall consecutive updates whose true relres is not below kReliableGain times the
+// best one so far end the FP32 phase as stagnation.
+constexpr double kReliableGain = 0.5;
+constexpr int kReliableStall = 3;
 
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __fma_rn(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __fmaf_rn(a, b, c); }
@@ -476,6 +484,12 @@ __global__ __launch_bounds__(256) void k_mixed_resume(PcgState *__restrict__ st,
         halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);
         return;
     }
+    st->no_gain = rel < kReliableGain * st->best_true ? 0 : st->no_gain + 1;
+    st->best_true = fmin(st->best_true, rel);
+    if (st->no_gain >= kReliableStall) {
+        halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
+        return;
+    }
     const double beta = rzn / st->rz[st->pause_slot];
     if (!isfinite(beta) || beta < 0.0) {
         halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
@@ -541,6 +555,8 @@ __global__ void k_pcg_start(PcgState *__restrict__ st, double tol, double bnorm,
     st->outer = 0;
     st->last_rel = rel;
     st->last_iter = 0;
+    st->best_true = rel;
+    st->no_gain = 0;
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -612,7 +628,7 @@ void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipS
 void ensure_batch(mycel_op *op, PcgWork *w, int R) {
     if (R <= w->b_cap) return;
     const size_t n = static_cast<size_t>(w->n) * R;
-    const int nb_node = apply_k_blocks(op, true);
+    const int nb_node = apply_k_blocks(op);
     w->br.resize(n);
     w->bz.resize(n);
     w->bp.resize(n);
@@ -635,7 +651,7 @@ void ensure_batch(mycel_op *op, PcgWork *w, int R) {
 // grid rows y = 0..R-1).
 void enqueue_iteration_gmg_batch(mycel_op *op, PcgWork *w, double *d_X, int R, int slot, hipStream_t s) {
     PcgState *st = w->bstate.get();
-    const int nb_node = apply_k_blocks(op, true);  // partials of the batched K*u launch
+    const int nb_node = apply_k_blocks(op);  // partials of the batched K*u launch
     const int words = static_cast<int>(sizeof(PcgState) / sizeof(int));
     const Batch bt{R, static_cast<long long>(w->n), nb_node, words};
     launch_apply_k(op, w->bp.get(), w->bap.get(), kApplyAProjected, s, w->bpart_pap.get(), &st->halt, bt);

@@ -288,9 +288,16 @@ class FemOperator:
         L.check(L.load().mycel_op_apply_k(self._h, L.ptr(x), L.ptr(out)))
         return out
 
+    def SetMatvecForm(self, form: int, columns: int = 0) -> None:
+        """K*u kernel form of ApplyK / ApplyA and the FP32 operator (no counterpart in the reference; the
+        solvers' fused-dot launches always use the automatic form). MATVEC_NODE_GATHER: columns 0 = one node
+        per thread, 2/4/8 = each thread marches that many nodes along z. MATVEC_ELEMENT: each element's product
+        once in the monomial basis, columns 2/4/8 node layers per tile. MATVEC_AUTO: by grid size. Forms of one
+        family give bit-identical results; the two families agree to rounding."""
+        L.check(L.load().mycel_op_set_matvec_form(self._h, int(form), int(columns)))
+
     def SetMatvecColumns(self, columns: int) -> None:
-        """K*u kernel form (no counterpart in the reference): 0 = one node per thread, 2/4/8 = each thread
-        marches that many nodes along z, -1 = by grid size. Every form gives bit-identical results."""
+        """The gather form with columns 0/2/4/8, or -1 for the automatic form."""
         L.check(L.load().mycel_op_set_matvec_columns(self._h, int(columns)))
 
     def ApplyA(self, x, y=None, xIsAlreadyProjected=False):
