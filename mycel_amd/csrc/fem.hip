@@ -406,6 +406,8 @@ static int ku_blocks(const mycel_op *op, const KuForm &f) {
 
 int apply_k_blocks(const mycel_op *op) { return ku_blocks(op, ku_auto(op)); }
 
+int ku_element_layers(const mycel_op *op) { return ku_auto(op).kz; }
+
 template <typename T, int MODE, bool DOT>
 static void launch_ku(mycel_op *op, const KuForm &f, const KeSym<T> &ke, const KeMono<T> &km, const T *ep,
                       const T *d_x, T *d_y, hipStream_t s, double *part, const int *stop, const Batch &bt) {

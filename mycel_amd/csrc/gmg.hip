@@ -35,6 +35,7 @@
 #include <memory>
 
 #include "apply_k.hpp"
+#include "ku_element.hpp"
 #include "device_common.hpp"
 #include "mycel_internal.hpp"
 
@@ -256,6 +257,97 @@ __global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks
     const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     if (n < g.n_nodes) residual_node(g, ks, ep, x + o, b + o, inv, r + o, n, first_layer(g, n0));
+}
+
+// Element-centric forms of the finest level's three gather kernels (ec_tile_run: each element's product once,
+// 8x8 element-column tiles, KZ node layers per tile, four tiles per 256-thread block); the same epilogues.
+__device__ __forceinline__ int ec_wave_tile() {
+    return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256) void k_gmg_cheb_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
+                                                     const float *__restrict__ x, const float *__restrict__ b,
+                                                     float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                     float cr, float *__restrict__ x_out, int n_tiles) {
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    d += o;
+    x_out += o;
+    const int tile = ec_wave_tile();
+    if (tile >= n_tiles) return;  // wave-uniform; no barrier follows
+    ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, [&](int n, float(&kx)[3], const float(&xc)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            const float r = b[dof] - kx[c];
+            const float dn = cd * d[dof] + cr * (inv[dof] * r);
+            x_out[dof] = xc[c] + dn;
+            d[dof] = dn;
+        }
+    });
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256) void k_gmg_residual_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
+                                                         const float *__restrict__ x, const float *__restrict__ b,
+                                                         const float *__restrict__ inv, float *__restrict__ r,
+                                                         int n_tiles) {
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    r += o;
+    const int tile = ec_wave_tile();
+    if (tile >= n_tiles) return;
+    ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, [&](int n, float(&kx)[3], const float(&)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            r[dof] = inv[dof] > 0.0f ? b[dof] - kx[c] : 0.0f;
+        }
+    });
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
+                                                           const float *__restrict__ x, const float *__restrict__ b,
+                                                           const float *__restrict__ d, const float *__restrict__ inv,
+                                                           float cd, float cr, double *__restrict__ z64,
+                                                           const double *__restrict__ r64,
+                                                           const double *__restrict__ ap64,
+                                                           double *__restrict__ part_zr, double *__restrict__ part_zap,
+                                                           int n_tiles) {
+    __shared__ double red[8];
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    d += o;
+    z64 += o;
+    r64 += o;
+    ap64 += o;
+    part_zr += blockIdx.y * gridDim.x;
+    part_zap += blockIdx.y * gridDim.x;
+    double zr = 0.0, zap = 0.0;
+    const int tile = ec_wave_tile();
+    if (tile < n_tiles)
+        ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, [&](int n, float(&kx)[3], const float(&xc)[3]) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int dof = 3 * n + c;
+                const float r = b[dof] - kx[c];
+                const float dn = cd * d[dof] + cr * (inv[dof] * r);
+                const double z = static_cast<double>(xc[c] + dn);
+                z64[dof] = z;
+                zr = __fma_rn(z, r64[dof], zr);
+                zap = __fma_rn(z, ap64[dof], zap);
+            }
+        });
+    block_sum256x2(zr, zap, red);
+    if (threadIdx.x == 0) {
+        part_zr[blockIdx.x] = zr;
+        part_zap[blockIdx.x] = zap;
+    }
 }
 
 // Slab-split forms of the three gather kernels (two lanes per node, ku_gather_half): a 256-thread block covers
@@ -810,6 +902,7 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
 struct GmgLevel {
     Dims d{};
     KeSym<float> ks{};
+    KeMono<float> km{};  // the same Ke in the monomial basis (element-centric kernels)
     double scale = 1.0;  // h_l / h
     float kdiag = 0.0f;  // diagonal entry of the level's Ke
     const float *ep = nullptr;
@@ -851,7 +944,20 @@ uint64_t gmg_generation(const mycel_op *op) { return op->gmg ? op->gmg->gen : 0;
 
 void gmg_free(Gmg *g) { delete g; }
 
-int gmg_partials(const mycel_op *op) { return cdiv(op->d.n_nodes, 256); }
+// Node layers per tile of the finest level's element-centric kernels (the operator's automatic K*u rule), or 0
+// for the node gathers (MYCEL_GMG_FINE_EC=0).
+static int fine_ec_kz(const mycel_op *op) {
+    static const bool on = [] {
+        const char *e = std::getenv("MYCEL_GMG_FINE_EC");
+        return !(e && *e == '0');
+    }();
+    return on ? ku_element_layers(op) : 0;
+}
+
+int gmg_partials(const mycel_op *op) {
+    const int kz = fine_ec_kz(op);
+    return kz ? cdiv(ec_tiles(op->d, kz), 4) : cdiv(op->d.n_nodes, 256);
+}
 
 // Builds the level structure; op->gmg is set only once the whole hierarchy exists.
 static void gmg_build(mycel_op *op) {
@@ -889,6 +995,12 @@ static void gmg_build(mycel_op *op) {
         L.d = d;
         L.scale = scale;
         for (int q = 0; q < kKeClasses; q++) L.ks.v[q] = static_cast<float>(op->ks32.v[q] * scale);
+        {
+            const KeMono<double> &m = op->km64;
+            L.km = {static_cast<float>(m.a * scale), static_cast<float>(m.b * scale), static_cast<float>(m.c * scale),
+                    static_cast<float>(m.d * scale), static_cast<float>(m.e * scale), static_cast<float>(m.f * scale),
+                    static_cast<float>(m.h * scale), static_cast<float>(m.q * scale)};
+        }
         L.kdiag = static_cast<float>(op->ke[0] * scale);
         if (l == 0) {
             L.ep = op->d_ep32.get();
@@ -975,9 +1087,22 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         cur[l] = g->lv[l].x.get();
         nxt[l] = g->lv[l].x2.get();
     }
+    const int fkz = fine_ec_kz(op);  // finest level: element-centric kernels with fkz layers per tile (0: gathers)
+    const int f_tiles = fkz ? ec_tiles(g->lv[0].d, fkz) : 0;
+    const dim3 f_grid(cdiv(f_tiles, 4), R);
     auto cheb = [&](int l, float cd, float cr) {
         GmgLevel &L = g->lv[l];
-        if (split && l > 0)
+        if (l == 0 && fkz) {
+            if (fkz == 2)
+                hipLaunchKernelGGL(k_gmg_cheb_ec<2>, f_grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], f_tiles);
+            else if (fkz == 4)
+                hipLaunchKernelGGL(k_gmg_cheb_ec<4>, f_grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], f_tiles);
+            else
+                hipLaunchKernelGGL(k_gmg_cheb_ec<8>, f_grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], f_tiles);
+        } else if (split && l > 0)
             hipLaunchKernelGGL(k_gmg_cheb_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                                L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
         else
@@ -1055,7 +1180,17 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
-        if (split)
+        if (l == 0 && fkz) {
+            if (fkz == 2)
+                hipLaunchKernelGGL(k_gmg_residual_ec<2>, f_grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), f_tiles);
+            else if (fkz == 4)
+                hipLaunchKernelGGL(k_gmg_residual_ec<4>, f_grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), f_tiles);
+            else
+                hipLaunchKernelGGL(k_gmg_residual_ec<8>, f_grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), f_tiles);
+        } else if (split)
             hipLaunchKernelGGL(k_gmg_residual_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                                L.inv.get(), L.r.get());
         else
@@ -1076,8 +1211,18 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     // (a grid of <= kCoarseMaxDofs DOFs) was solved directly above: the step is then the identity (cd = cr = 0).
     const int fd = g->fine_degree;
     const float cd = top == 0 ? 0.0f : g->cd[fd - 1], cr = top == 0 ? 0.0f : g->cr[fd - 1];
-    hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
-                       F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
+    if (fkz == 2)
+        hipLaunchKernelGGL(k_gmg_cheb_final_ec<2>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+    else if (fkz == 4)
+        hipLaunchKernelGGL(k_gmg_cheb_final_ec<4>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+    else if (fkz == 8)
+        hipLaunchKernelGGL(k_gmg_cheb_final_ec<8>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+    else
+        hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
+                           F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
     MYCEL_LAUNCH_CHECK();
 }
 

@@ -58,12 +58,13 @@ __host__ __device__ inline int ec_tiles(const Dims &g, int kz) {
     return ((g.nx1 + kEcOut - 1) / kEcOut) * ((g.ny1 + kEcOut - 1) / kEcOut) * ((g.nz1 + kz - 1) / kz);
 }
 
-// One wavefront's tile: writes (K x) (MODE: as ku_gather) for node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7
-// node columns and returns the lane's x.y contribution. Called by all 64 lanes of the wave (tile is uniform).
-template <typename T, int MODE, int KZ>
-__device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
-                                          const T *__restrict__ x, const uint8_t *__restrict__ flags,
-                                          T *__restrict__ y, int tile) {
+// One wavefront's tile: node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7 node columns. For every node it writes,
+// fin(n, kx, xc) receives the raw (K x) at node n and the node's own x values as loaded (with the constrained
+// DOFs zeroed when MASK). Called by all 64 lanes of the wave (tile is wave-uniform).
+template <typename T, bool MASK, int KZ, typename Fin>
+__device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
+                                            const T *__restrict__ x, const uint8_t *__restrict__ flags, int tile,
+                                            Fin &&fin) {
     const int lane = threadIdx.x & 63, a = lane & (kEcTile - 1), b = lane >> 3;
     const int ntx = (g.nx1 + kEcOut - 1) / kEcOut, nty = (g.ny1 + kEcOut - 1) / kEcOut;
     const int tx = tile % ntx, tr = tile / ntx, ty = tr % nty, tz = tr / nty;
@@ -81,13 +82,13 @@ __device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, co
     auto load_face = [&](int kn, T (&v)[4][3]) {
         const size_t kc = static_cast<size_t>(min(max(kn, 0), g.nz));
         const T *xl = x + 3 * layer * kc;
-        const uint8_t *fl = MODE == kApplyAGeneral ? flags + 3 * layer * kc : nullptr;
+        const uint8_t *fl = MASK ? flags + 3 * layer * kc : nullptr;
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 T vv = ld_off(xl, o[q], d * static_cast<int>(sT));
-                if (MODE == kApplyAGeneral && (ld_off(fl, o[q] / sT, d) & kFlagConstrained)) vv = T(0);
+                if (MASK && (ld_off(fl, o[q] / sT, d) & kFlagConstrained)) vv = T(0);
                 v[q][d] = vv;
             }
     };
@@ -104,7 +105,6 @@ __device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, co
     }
     const T amb = km.a - km.b, fmh = km.f - km.h;
     const int kend = min(k0 + KZ - 1, g.nz);  // last node layer written
-    double dotv = 0.0;
 #pragma unroll
     for (int s = 0; s <= KZ; s++) {
         const int ke = k0 - 1 + s;  // element layer of this step; node layer ke is completed by it
@@ -187,24 +187,32 @@ __device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, co
                 const T q0 = f0 + from_left(f1), q3 = f3 + from_left(f2);
                 out[d] = q0 + from_below(q3);
             }
-            if (out_ok) {
-                const int n = ie + g.nx1 * (je + g.ny1 * ke);
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const int dof = 3 * n + d;
-                    if (MODE == kApplyAProjected) {
-                        if (flags[dof] & kFlagFixed) out[d] = xc[d];
-                    } else if (MODE == kApplyAGeneral) {
-                        if (flags[dof] & kFlagConstrained) out[d] = x[dof];
-                    }
-                    y[dof] = out[d];
-                    dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
-                }
-            }
+            if (out_ok) fin(ie + g.nx1 * (je + g.ny1 * ke), out, xc);
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) xc[d] = up[0][d];
     }
+}
+
+// K x (MODE: as ku_gather) for one tile; returns the lane's x.y contribution.
+template <typename T, int MODE, int KZ>
+__device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
+                                          const T *__restrict__ x, const uint8_t *__restrict__ flags,
+                                          T *__restrict__ y, int tile) {
+    double dotv = 0.0;
+    ec_tile_run<T, MODE == kApplyAGeneral, KZ>(g, km, ep, x, flags, tile, [&](int n, T(&out)[3], const T(&xc)[3]) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int dof = 3 * n + d;
+            if (MODE == kApplyAProjected) {
+                if (flags[dof] & kFlagFixed) out[d] = xc[d];
+            } else if (MODE == kApplyAGeneral) {
+                if (flags[dof] & kFlagConstrained) out[d] = x[dof];
+            }
+            y[dof] = out[d];
+            dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
+        }
+    });
     return dotv;
 }
 

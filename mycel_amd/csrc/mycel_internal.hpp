@@ -183,6 +183,8 @@ void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipS
                     double *d_dot_partial = nullptr, const int *d_stop = nullptr, const Batch &bt = Batch{});
 // Blocks (= dot partials) of the K*u launch that writes x.y partials; batched: a multi-RHS launch.
 int apply_k_blocks(const mycel_op *op);
+// Node layers per tile of the automatic element-centric K*u form on this operator's grid.
+int ku_element_layers(const mycel_op *op);
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s,
                         double *d_dot_partial = nullptr, const int *d_stop = nullptr);
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s);
