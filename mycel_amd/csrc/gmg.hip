@@ -260,10 +260,95 @@ __global__ __launch_bounds__(256, 2) void k_gmg_residual(Dims g, KeSym<float> ks
 }
 
 // Element-centric forms of the finest level's three gather kernels (ec_tile_run: each element's product once,
-// 8x8 element-column tiles, KZ node layers per tile, four tiles per 256-thread block); the same epilogues.
+// 8x8 element-column tiles, KZ node layers per tile, four tiles per 256-thread block); the same epilogues, with
+// their loads issued at the start of the step that completes the node.
 __device__ __forceinline__ int ec_wave_tile() {
     return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
 }
+
+struct ChebFin {  // x' = x + d', d' = cd d + cr D^-1 (b - K x)
+    const float *b, *inv;
+    float *d, *x_out;
+    float cd, cr;
+    struct Pre {
+        float b[3], d[3], inv[3];
+    };
+    __device__ Pre pre(int n) const {
+        Pre p;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            p.b[c] = b[3 * n + c];
+            p.d[c] = d[3 * n + c];
+            p.inv[c] = inv[3 * n + c];
+        }
+        return p;
+    }
+    __device__ void operator()(int n, float (&kx)[3], const float (&xc)[3], const Pre &p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float r = p.b[c] - kx[c];
+            const float dn = cd * p.d[c] + cr * (p.inv[c] * r);
+            x_out[3 * n + c] = xc[c] + dn;
+            d[3 * n + c] = dn;
+        }
+    }
+};
+
+struct ResidualFin {  // r = b - K x on unconstrained DOFs, 0 on constrained ones
+    const float *b, *inv;
+    float *r;
+    struct Pre {
+        float b[3], inv[3];
+    };
+    __device__ Pre pre(int n) const {
+        Pre p;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            p.b[c] = b[3 * n + c];
+            p.inv[c] = inv[3 * n + c];
+        }
+        return p;
+    }
+    __device__ void operator()(int n, float (&kx)[3], const float (&)[3], const Pre &p) const {
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[3 * n + c] = p.inv[c] > 0.0f ? p.b[c] - kx[c] : 0.0f;
+    }
+};
+
+struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.r64 and z.ap64 contributions
+    const float *b, *d, *inv;
+    double *z64;
+    const double *r64, *ap64;
+    float cd, cr;
+    double zr = 0.0, zap = 0.0;
+    struct Pre {
+        float b[3], d[3], inv[3];
+        double r[3], ap[3];
+    };
+    __device__ Pre pre(int n) const {
+        Pre p;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            p.b[c] = b[3 * n + c];
+            p.d[c] = d[3 * n + c];
+            p.inv[c] = inv[3 * n + c];
+            p.r[c] = r64[3 * n + c];
+            p.ap[c] = ap64[3 * n + c];
+        }
+        return p;
+    }
+    __device__ void operator()(int n, float (&kx)[3], const float (&xc)[3], const Pre &p) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float r = p.b[c] - kx[c];
+            const float dn = cd * p.d[c] + cr * (p.inv[c] * r);
+            const double z = static_cast<double>(xc[c] + dn);
+            z64[3 * n + c] = z;
+            zr = __fma_rn(z, p.r[c], zr);
+            zap = __fma_rn(z, p.ap[c], zap);
+        }
+    }
+};
 
 template <int KZ>
 __global__ __launch_bounds__(256) void k_gmg_cheb_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
@@ -271,22 +356,10 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_ec(Dims g, KeMono<float> km, c
                                                      float *__restrict__ d, const float *__restrict__ inv, float cd,
                                                      float cr, float *__restrict__ x_out, int n_tiles) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    x += o;
-    b += o;
-    d += o;
-    x_out += o;
     const int tile = ec_wave_tile();
     if (tile >= n_tiles) return;  // wave-uniform; no barrier follows
-    ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, [&](int n, float(&kx)[3], const float(&xc)[3]) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int dof = 3 * n + c;
-            const float r = b[dof] - kx[c];
-            const float dn = cd * d[dof] + cr * (inv[dof] * r);
-            x_out[dof] = xc[c] + dn;
-            d[dof] = dn;
-        }
-    });
+    ChebFin fin{b + o, inv, d + o, x_out + o, cd, cr};
+    ec_tile_run<float, false, KZ>(g, km, ep, x + o, nullptr, tile, fin);
 }
 
 template <int KZ>
@@ -295,18 +368,10 @@ __global__ __launch_bounds__(256) void k_gmg_residual_ec(Dims g, KeMono<float> k
                                                          const float *__restrict__ inv, float *__restrict__ r,
                                                          int n_tiles) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    x += o;
-    b += o;
-    r += o;
     const int tile = ec_wave_tile();
     if (tile >= n_tiles) return;
-    ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, [&](int n, float(&kx)[3], const float(&)[3]) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int dof = 3 * n + c;
-            r[dof] = inv[dof] > 0.0f ? b[dof] - kx[c] : 0.0f;
-        }
-    });
+    ResidualFin fin{b + o, inv, r + o};
+    ec_tile_run<float, false, KZ>(g, km, ep, x + o, nullptr, tile, fin);
 }
 
 template <int KZ>
@@ -320,33 +385,14 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float>
                                                            int n_tiles) {
     __shared__ double red[8];
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    x += o;
-    b += o;
-    d += o;
-    z64 += o;
-    r64 += o;
-    ap64 += o;
-    part_zr += blockIdx.y * gridDim.x;
-    part_zap += blockIdx.y * gridDim.x;
-    double zr = 0.0, zap = 0.0;
+    FinalFin fin{b + o, d + o, inv, z64 + o, r64 + o, ap64 + o, cd, cr};
     const int tile = ec_wave_tile();
-    if (tile < n_tiles)
-        ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, [&](int n, float(&kx)[3], const float(&xc)[3]) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const int dof = 3 * n + c;
-                const float r = b[dof] - kx[c];
-                const float dn = cd * d[dof] + cr * (inv[dof] * r);
-                const double z = static_cast<double>(xc[c] + dn);
-                z64[dof] = z;
-                zr = __fma_rn(z, r64[dof], zr);
-                zap = __fma_rn(z, ap64[dof], zap);
-            }
-        });
+    if (tile < n_tiles) ec_tile_run<float, false, KZ>(g, km, ep, x + o, nullptr, tile, fin);
+    double zr = fin.zr, zap = fin.zap;
     block_sum256x2(zr, zap, red);
     if (threadIdx.x == 0) {
-        part_zr[blockIdx.x] = zr;
-        part_zap[blockIdx.x] = zap;
+        part_zr[blockIdx.y * gridDim.x + blockIdx.x] = zr;
+        part_zap[blockIdx.y * gridDim.x + blockIdx.x] = zap;
     }
 }
 

@@ -58,9 +58,11 @@ __host__ __device__ inline int ec_tiles(const Dims &g, int kz) {
     return ((g.nx1 + kEcOut - 1) / kEcOut) * ((g.ny1 + kEcOut - 1) / kEcOut) * ((g.nz1 + kz - 1) / kz);
 }
 
-// One wavefront's tile: node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7 node columns. For every node it writes,
-// fin(n, kx, xc) receives the raw (K x) at node n and the node's own x values as loaded (with the constrained
-// DOFs zeroed when MASK). Called by all 64 lanes of the wave (tile is wave-uniform).
+// One wavefront's tile: node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7 node columns. The epilogue fin has two
+// stages: fin.pre(n) at the start of the step that completes node n (its loads then overlap the step's face
+// loads and arithmetic; halo lanes pass a valid clamped node), and fin(n, kx, xc, pre) for every node the tile
+// writes, with the raw (K x) at node n and the node's own x values as loaded (constrained DOFs zeroed when
+// MASK). Called by all 64 lanes of the wave (tile is wave-uniform).
 template <typename T, bool MASK, int KZ, typename Fin>
 __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
                                             const T *__restrict__ x, const uint8_t *__restrict__ flags, int tile,
@@ -109,6 +111,8 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
     for (int s = 0; s <= KZ; s++) {
         const int ke = k0 - 1 + s;  // element layer of this step; node layer ke is completed by it
         if (ke > kend) break;
+        decltype(fin.pre(0)) pre{};
+        if (s > 0) pre = fin.pre(i0 + g.nx1 * (j0 + g.ny1 * ke));
         T up[4][3];
         load_face(ke + 1, up);
         const T E = col_ok ? ld_off(ep + elayer * (ke + 1), eo) : T(0);
@@ -187,33 +191,55 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
                 const T q0 = f0 + from_left(f1), q3 = f3 + from_left(f2);
                 out[d] = q0 + from_below(q3);
             }
-            if (out_ok) fin(ie + g.nx1 * (je + g.ny1 * ke), out, xc);
+            if (out_ok) fin(ie + g.nx1 * (je + g.ny1 * ke), out, xc, pre);
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) xc[d] = up[0][d];
     }
 }
 
+// Epilogue of K x (MODE: as ku_gather): mode handling, store, and the x.y contribution.
+template <typename T, int MODE>
+struct KuFin {
+    const T *x;
+    const uint8_t *flags;
+    T *y;
+    double dot = 0.0;
+    struct Pre {
+        uint8_t fl[3];
+        T xd[3];
+    };
+    __device__ Pre pre(int n) const {
+        Pre p{};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (MODE != kApplyK) p.fl[d] = flags[3 * n + d];
+            if (MODE == kApplyAGeneral) p.xd[d] = x[3 * n + d];
+        }
+        return p;
+    }
+    __device__ void operator()(int n, T (&out)[3], const T (&xc)[3], const Pre &p) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (MODE == kApplyAProjected) {
+                if (p.fl[d] & kFlagFixed) out[d] = xc[d];
+            } else if (MODE == kApplyAGeneral) {
+                if (p.fl[d] & kFlagConstrained) out[d] = p.xd[d];
+            }
+            y[3 * n + d] = out[d];
+            dot += static_cast<double>(MODE == kApplyAGeneral ? p.xd[d] : xc[d]) * static_cast<double>(out[d]);
+        }
+    }
+};
+
 // K x (MODE: as ku_gather) for one tile; returns the lane's x.y contribution.
 template <typename T, int MODE, int KZ>
 __device__ __forceinline__ double ec_tile(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
                                           const T *__restrict__ x, const uint8_t *__restrict__ flags,
                                           T *__restrict__ y, int tile) {
-    double dotv = 0.0;
-    ec_tile_run<T, MODE == kApplyAGeneral, KZ>(g, km, ep, x, flags, tile, [&](int n, T(&out)[3], const T(&xc)[3]) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const int dof = 3 * n + d;
-            if (MODE == kApplyAProjected) {
-                if (flags[dof] & kFlagFixed) out[d] = xc[d];
-            } else if (MODE == kApplyAGeneral) {
-                if (flags[dof] & kFlagConstrained) out[d] = x[dof];
-            }
-            y[dof] = out[d];
-            dotv += static_cast<double>(MODE == kApplyAGeneral ? x[dof] : xc[d]) * static_cast<double>(out[d]);
-        }
-    });
-    return dotv;
+    KuFin<T, MODE> fin{x, flags, y};
+    ec_tile_run<T, MODE == kApplyAGeneral, KZ>(g, km, ep, x, flags, tile, fin);
+    return fin.dot;
 }
 
 }  // namespace mycel
