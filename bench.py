@@ -590,7 +590,7 @@ def mode_summarize(args):
         for k in sorted(set(fetch) & set(write)):
             b = fetch[k][0] * 1024 * fs + write[k][0] * 1024 * ws
             lines.append(f"{k} | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | {b:.0f}")
-            if k.startswith("k_apply_k<"):
+            if k.startswith("k_apply_k"):
                 traffic[k] = b
     print("\n".join(lines))
     if args.traffic_json and traffic:
