@@ -579,7 +579,7 @@ struct TailArgs {
                              // unused entries: a valid pointer and n = 0
     int ainv_lds;            // float offset of the coarsest inverse in LDS, or -1 to read it from global memory
     int ainv_nn;             // entries of the inverse staged into LDS (n^2, or 0)
-    const double *ainv;
+    const float *ainv;
     float *out0;             // where lv[0]'s corrected iterate goes (global, RHS layout as above)
     float theta_inv, cd1, cr1;
     int count;
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
                 v[sg][u] = i < g.n ? x : 0.0f;
             }
         }
-        double va[kAinvPer];
+        float va[kAinvPer];
         const int nn = t.ainv_nn;
 #pragma unroll
         for (int u = 0; u < kAinvPer; u++) {
@@ -637,7 +637,7 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
                 if (i < n) sm[g.dst + i] = v[sg][u];
             }
         }
-        double *al = reinterpret_cast<double *>(sm + (t.ainv_lds >= 0 ? t.ainv_lds : 0));
+        float *al = sm + (t.ainv_lds >= 0 ? t.ainv_lds : 0);
 #pragma unroll
         for (int u = 0; u < kAinvPer; u++) {
             const int i = threadIdx.x + u * kTailThreads;
@@ -651,15 +651,23 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
     auto cur = [&](int l) { return sm + (((flip >> l) & 1u) ? t.lv[l].o_x2 : t.lv[l].o_x); };
     auto nxt = [&](int l) { return sm + (((flip >> l) & 1u) ? t.lv[l].o_x : t.lv[l].o_x2); };
     const int last = t.count - 1;
-    for (int l = 0; l < last; l++) {
-        const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
+    // One Chebyshev step and the residual of a tail level, one node gather per node. (Forming each element's
+    // product once, in an LDS scratch with a second barrier per phase, measured slower: 122.5 vs 126.8 solves/s.)
+    auto cheb_phase = [&](const TailLevel &L, int l, float cd, float cr) {
         for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
-            cheb_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_d), S(L.o_inv), t.cd1, t.cr1, nxt(l), n, 0);
+            cheb_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_d), S(L.o_inv), cd, cr, nxt(l), n, 0);
         __syncthreads();
         flip ^= 1u << l;
+    };
+    auto residual_phase = [&](const TailLevel &L, int l) {
         for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
             residual_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_inv), S(L.o_r), n, 0);
         __syncthreads();
+    };
+    for (int l = 0; l < last; l++) {
+        const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
+        cheb_phase(L, l, t.cd1, t.cr1);
+        residual_phase(L, l);
         for (int n = threadIdx.x; n < C.d.n_nodes; n += kTailThreads)
             restrict_node(L.d, S(L.o_r), C.d, S(C.o_inv), t.theta_inv, S(C.o_b), cur(l + 1), S(C.o_d), n);
         __syncthreads();
@@ -677,17 +685,18 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
         if (part < kParts && row < n) {
             double s = 0.0;
             if (t.ainv_lds >= 0) {  // staged in LDS on entry
-                const double *ai = reinterpret_cast<const double *>(sm + t.ainv_lds);
+                const float *ai = sm + t.ainv_lds;
 #pragma unroll
                 for (int q = 0; q < kSpan; q++) {
                     const int j = part * span + q;
-                    if (q < span && j < n) s = __fma_rn(ai[j * n + row], bsh[j], s);
+                    if (q < span && j < n) s = __fma_rn(static_cast<double>(ai[j * n + row]), bsh[j], s);
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < kSpan; q++) {
                     const int j = part * span + q;
-                    if (q < span && j < n) s = __fma_rn(t.ainv[static_cast<size_t>(j) * n + row], bsh[j], s);
+                    if (q < span && j < n)
+                        s = __fma_rn(static_cast<double>(t.ainv[static_cast<size_t>(j) * n + row]), bsh[j], s);
                 }
             }
             psum[part][row] = s;
@@ -705,13 +714,8 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
         for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
             prolong_node(L.d, cur(l), S(L.o_inv), C.d, cur(l + 1), n);
         __syncthreads();
-        for (int pass = 0; pass < 2; pass++) {
-            const float cd = pass == 0 ? 0.0f : t.cd1, cr = pass == 0 ? t.theta_inv : t.cr1;
-            for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
-                cheb_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_d), S(L.o_inv), cd, cr, nxt(l), n, 0);
-            __syncthreads();
-            flip ^= 1u << l;
-        }
+        cheb_phase(L, l, 0.0f, t.theta_inv);
+        cheb_phase(L, l, t.cd1, t.cr1);
     }
     const float *res = cur(0);
     for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) t.out0[o0 + q] = res[q];
@@ -816,7 +820,8 @@ __global__ __launch_bounds__(256) void k_gmg_galerkin_blocks(Dims gs, const floa
 
 // Coarsest level: assemble the dense matrix into LDS (one row per thread, fixed order; identity rows and zero
 // columns on constrained DOFs), invert it by Gauss-Jordan elimination (SPD, no pivoting) and store the
-// inverse column-major (ainv[j * n + i] = (A^-1)_ij). With `blocks` the matrix is the Galerkin product P^T A_s P summed from the second-coarsest
+// inverse column-major in FP32 (ainv[j * n + i] = (A^-1)_ij; the V-cycle is FP32: the tail stages half the
+// bytes and still accumulates the product in FP64; outer iteration counts were unchanged on every test case). With `blocks` the matrix is the Galerkin product P^T A_s P summed from the second-coarsest
 // level's element blocks; without (a single-level hierarchy) it is assembled from Ke and the level's moduli.
 // The elimination keeps the matrix in registers: thread (ti, tj) = (t / 8, t % 8) owns rows ti + 128 s and
 // columns tj + 8 c; each step broadcasts the pivot column and the scaled pivot row through LDS.
@@ -824,7 +829,7 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
                                                             const double *__restrict__ ke, double scale,
                                                             const float *__restrict__ inv, Dims gs,
                                                             const double *__restrict__ blocks,
-                                                            double *__restrict__ ainv) {
+                                                            float *__restrict__ ainv) {
     extern __shared__ __align__(16) double A[];
     const int n = g.n_dofs;
     for (int q = threadIdx.x; q < n * n; q += blockDim.x) A[q] = 0.0;
@@ -939,7 +944,7 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
 #pragma unroll
         for (int c = 0; c < CS; c++) {
             const int i = ti + 128 * s, j = tj + 8 * c;
-            if (i < n && j < n) ainv[j * n + i] = a[s][c];  // column-major
+            if (i < n && j < n) ainv[j * n + i] = static_cast<float>(a[s][c]);  // column-major
         }
 }
 
@@ -960,7 +965,7 @@ struct GmgLevel {
 struct Gmg {
     GmgLevel lv[kMaxLevels];
     int levels = 0;
-    DevBuf<double> ainv;
+    DevBuf<float> ainv;  // coarsest inverse, column-major
     DevBuf<double> blocks;  // Galerkin element blocks of the second-coarsest level
     uint64_t epoch = ~0ull;
     float theta_inv = 0;
@@ -1200,11 +1205,10 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         T.o_r = off + 5 * nd;
         off += 6 * nd;
     }
-    // The coarsest inverse joins the LDS image when it fits (8-byte aligned after the level arrays).
+    // The coarsest inverse (FP32) joins the LDS image when it fits.
     const int nc = g->lv[top].d.n_dofs;
-    const int ainv_off = (off + 1) & ~1;
-    ta.ainv_lds = 4 * (static_cast<long long>(ainv_off) + 2LL * nc * nc) <= kTailLdsMax ? ainv_off : -1;
-    if (ta.ainv_lds >= 0) off = ainv_off + 2 * nc * nc;
+    ta.ainv_lds = 4 * (static_cast<long long>(off) + static_cast<long long>(nc) * nc) <= kTailLdsMax ? off : -1;
+    if (ta.ainv_lds >= 0) off += nc * nc;
     const size_t tail_lds = sizeof(float) * static_cast<size_t>(off);
     ta.ainv_nn = ta.ainv_lds >= 0 ? nc * nc : 0;
     {

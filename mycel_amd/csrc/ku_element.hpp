@@ -38,6 +38,72 @@ __device__ __forceinline__ FaceSums<T> face_sums(T c0, T c1, T c2, T c3) {
     return {r1 + r0, r1x + r0x, r1 - r0, r1x - r0x};
 }
 
+// Element coefficients per component from the lower (z0) and upper (z1) face sums: monomials xi, eta, zeta,
+// xi eta, xi zeta, eta zeta, xi eta zeta (the constant one has no stiffness and is not formed).
+template <typename T>
+__device__ __forceinline__ void mono_coefficients(const FaceSums<T> (&lo)[3], const FaceSums<T> (&hi)[3],
+                                                  T (&c)[3][7]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        c[d][0] = hi[d].x + lo[d].x;
+        c[d][1] = hi[d].y + lo[d].y;
+        c[d][2] = hi[d].s - lo[d].s;
+        c[d][3] = hi[d].xy + lo[d].xy;
+        c[d][4] = hi[d].x - lo[d].x;
+        c[d][5] = hi[d].y - lo[d].y;
+        c[d][6] = hi[d].xy - lo[d].xy;
+    }
+}
+
+// gm = E M c (same layout as c). Modulus-0 elements give exactly 0 (a select, so non-finite values at their
+// nodes stay out).
+template <typename T>
+__device__ __forceinline__ void mono_product(const KeMono<T> &km, T E, const T (&c)[3][7], T (&gm)[3][7]) {
+    const bool on = E != T(0);
+    const T amb = km.a - km.b, fmh = km.f - km.h;
+    const T tr = (c[0][0] + c[1][1]) + c[2][2];
+    const T bt = (E * km.b) * tr, ab = E * amb;
+    gm[0][0] = on ? fma_t(ab, c[0][0], bt) : T(0);
+    gm[1][1] = on ? fma_t(ab, c[1][1], bt) : T(0);
+    gm[2][2] = on ? fma_t(ab, c[2][2], bt) : T(0);
+    const T ce = E * km.c;
+    const T s_uv = on ? ce * (c[0][1] + c[1][0]) : T(0);  // (u_eta, v_xi)
+    const T s_uw = on ? ce * (c[0][2] + c[2][0]) : T(0);  // (u_zeta, w_xi)
+    const T s_vw = on ? ce * (c[1][2] + c[2][1]) : T(0);  // (v_zeta, w_eta)
+    gm[0][1] = s_uv;
+    gm[1][0] = s_uv;
+    gm[0][2] = s_uw;
+    gm[2][0] = s_uw;
+    gm[1][2] = s_vw;
+    gm[2][1] = s_vw;
+    const T de = E * km.d, ee = E * km.e;
+    gm[0][3] = on ? fma_t(de, c[0][3], ee * c[2][5]) : T(0);  // (u_xi.eta, w_eta.zeta)
+    gm[2][5] = on ? fma_t(ee, c[0][3], de * c[2][5]) : T(0);
+    gm[0][4] = on ? fma_t(de, c[0][4], ee * c[1][5]) : T(0);  // (u_xi.zeta, v_eta.zeta)
+    gm[1][5] = on ? fma_t(ee, c[0][4], de * c[1][5]) : T(0);
+    gm[1][3] = on ? fma_t(de, c[1][3], ee * c[2][4]) : T(0);  // (v_xi.eta, w_xi.zeta)
+    gm[2][4] = on ? fma_t(ee, c[1][3], de * c[2][4]) : T(0);
+    const T tw = (c[0][5] + c[1][4]) + c[2][3];  // (u_eta.zeta, v_xi.zeta, w_xi.eta)
+    const T ht = (E * km.h) * tw, fh = E * fmh;
+    gm[0][5] = on ? fma_t(fh, c[0][5], ht) : T(0);
+    gm[1][4] = on ? fma_t(fh, c[1][4], ht) : T(0);
+    gm[2][3] = on ? fma_t(fh, c[2][3], ht) : T(0);
+    const T qe = E * km.q;
+    gm[0][6] = on ? qe * c[0][6] : T(0);
+    gm[1][6] = on ? qe * c[1][6] : T(0);
+    gm[2][6] = on ? qe * c[2][6] : T(0);
+}
+
+// Inverse transform of one face from its sums: the corner values f0..f3 at (x0,y0), (x1,y0), (x1,y1), (x0,y1).
+template <typename T>
+__device__ __forceinline__ void face_corners(const FaceSums<T> &F, T &f0, T &f1, T &f2, T &f3) {
+    const T r0 = F.s - F.y, r1 = F.s + F.y, r0x = F.x - F.xy, r1x = F.x + F.xy;
+    f0 = r0 - r0x;
+    f1 = r0 + r0x;
+    f3 = r1 - r1x;
+    f2 = r1 + r1x;
+}
+
 // Value of lane - 1 within a row of 16 lanes (lanes with a = 0 read a value they do not use).
 __device__ __forceinline__ float from_left(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
@@ -105,7 +171,6 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
         carry[d] = {T(0), T(0), T(0), T(0)};
         xc[d] = cur[0][d];
     }
-    const T amb = km.a - km.b, fmh = km.f - km.h;
     const int kend = min(k0 + KZ - 1, g.nz);  // last node layer written
 #pragma unroll
     for (int s = 0; s <= KZ; s++) {
@@ -117,56 +182,11 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
         load_face(ke + 1, up);
         const T E = col_ok ? ld_off(ep + elayer * (ke + 1), eo) : T(0);
         FaceSums<T> hi[3];
-        // Element coefficients per component: monomials xi, eta, zeta, xi eta, xi zeta, eta zeta, xi eta zeta.
-        T c[3][7];
+        T c[3][7], gm[3][7];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            hi[d] = face_sums(up[0][d], up[1][d], up[2][d], up[3][d]);
-            c[d][0] = hi[d].x + lo[d].x;
-            c[d][1] = hi[d].y + lo[d].y;
-            c[d][2] = hi[d].s - lo[d].s;
-            c[d][3] = hi[d].xy + lo[d].xy;
-            c[d][4] = hi[d].x - lo[d].x;
-            c[d][5] = hi[d].y - lo[d].y;
-            c[d][6] = hi[d].xy - lo[d].xy;
-        }
-        // gm = E M c (same layout as c). Modulus-0 elements contribute exactly 0 (a select, so non-finite values
-        // at their nodes stay out).
-        const bool on = E != T(0);
-        T gm[3][7];
-        {
-            const T tr = (c[0][0] + c[1][1]) + c[2][2];
-            const T bt = (E * km.b) * tr, ab = E * amb;
-            gm[0][0] = on ? fma_t(ab, c[0][0], bt) : T(0);
-            gm[1][1] = on ? fma_t(ab, c[1][1], bt) : T(0);
-            gm[2][2] = on ? fma_t(ab, c[2][2], bt) : T(0);
-            const T ce = E * km.c;
-            const T s_uv = on ? ce * (c[0][1] + c[1][0]) : T(0);  // (u_eta, v_xi)
-            const T s_uw = on ? ce * (c[0][2] + c[2][0]) : T(0);  // (u_zeta, w_xi)
-            const T s_vw = on ? ce * (c[1][2] + c[2][1]) : T(0);  // (v_zeta, w_eta)
-            gm[0][1] = s_uv;
-            gm[1][0] = s_uv;
-            gm[0][2] = s_uw;
-            gm[2][0] = s_uw;
-            gm[1][2] = s_vw;
-            gm[2][1] = s_vw;
-            const T de = E * km.d, ee = E * km.e;
-            gm[0][3] = on ? fma_t(de, c[0][3], ee * c[2][5]) : T(0);  // (u_xi.eta, w_eta.zeta)
-            gm[2][5] = on ? fma_t(ee, c[0][3], de * c[2][5]) : T(0);
-            gm[0][4] = on ? fma_t(de, c[0][4], ee * c[1][5]) : T(0);  // (u_xi.zeta, v_eta.zeta)
-            gm[1][5] = on ? fma_t(ee, c[0][4], de * c[1][5]) : T(0);
-            gm[1][3] = on ? fma_t(de, c[1][3], ee * c[2][4]) : T(0);  // (v_xi.eta, w_xi.zeta)
-            gm[2][4] = on ? fma_t(ee, c[1][3], de * c[2][4]) : T(0);
-            const T tw = (c[0][5] + c[1][4]) + c[2][3];           // (u_eta.zeta, v_xi.zeta, w_xi.eta)
-            const T ht = (E * km.h) * tw, fh = E * fmh;
-            gm[0][5] = on ? fma_t(fh, c[0][5], ht) : T(0);
-            gm[1][4] = on ? fma_t(fh, c[1][4], ht) : T(0);
-            gm[2][3] = on ? fma_t(fh, c[2][3], ht) : T(0);
-            const T qe = E * km.q;
-            gm[0][6] = on ? qe * c[0][6] : T(0);
-            gm[1][6] = on ? qe * c[1][6] : T(0);
-            gm[2][6] = on ? qe * c[2][6] : T(0);
-        }
+        for (int d = 0; d < 3; d++) hi[d] = face_sums(up[0][d], up[1][d], up[2][d], up[3][d]);
+        mono_coefficients(lo, hi, c);
+        mono_product(km, E, c, gm);
         // Inverse transform, z stage: the element's lower-face part joins the carried upper-face part of the
         // element below (node layer ke); its own upper-face part is carried to node layer ke + 1.
         FaceSums<T> F[3];
@@ -186,8 +206,8 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 // x/y stage: the face's corner values, then the node's four element columns.
-                const T r0 = F[d].s - F[d].y, r1 = F[d].s + F[d].y, r0x = F[d].x - F[d].xy, r1x = F[d].x + F[d].xy;
-                const T f0 = r0 - r0x, f1 = r0 + r0x, f3 = r1 - r1x, f2 = r1 + r1x;
+                T f0, f1, f2, f3;
+                face_corners(F[d], f0, f1, f2, f3);
                 const T q0 = f0 + from_left(f1), q3 = f3 + from_left(f2);
                 out[d] = q0 + from_below(q3);
             }
