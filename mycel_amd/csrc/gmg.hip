@@ -529,6 +529,68 @@ __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *
     dc[dof] = xi;
 }
 
+// The residual of a coarse level fused with the restriction to the next one (the kernels of those levels are
+// bound by their fixed cost, not by arithmetic). Half-wave h of a block owns coarse node 8 blockIdx.x + h; its
+// lane t < 27 forms the residual at restriction tap t (the fine node's own gather, so taps shared by neighbouring
+// coarse nodes are formed again), weighted; the half-wave adds its 32 lanes in a fixed butterfly, and the
+// restriction's coarse pre-smoothing first step follows (x = d = theta^-1 D^-1 b).
+__global__ __launch_bounds__(256) void k_gmg_residual_restrict(Dims gf, KeSym<float> ks, const float *__restrict__ ep,
+                                                               const float *__restrict__ xf,
+                                                               const float *__restrict__ bf,
+                                                               const float *__restrict__ invf, Dims gc,
+                                                               const float *__restrict__ invc, float theta_inv,
+                                                               float *__restrict__ bc, float *__restrict__ xcv,
+                                                               float *__restrict__ dc) {
+    const long long of = static_cast<long long>(blockIdx.y) * gf.n_dofs, oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
+    xf += of;
+    bf += of;
+    bc += oc;
+    xcv += oc;
+    dc += oc;
+    const int t = threadIdx.x & 31;
+    const int n = (blockIdx.x * 256 + threadIdx.x) >> 5;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    if (n < gc.n_nodes && t < 27) {
+        int I, J, K;
+        node_ijk(gc, n, I, J, K);
+        const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+        const int fi = 2 * I - 1 + dx, fj = 2 * J - 1 + dy, fk = 2 * K - 1 + dz;
+        if (fi >= 0 && fi <= gf.nx && fj >= 0 && fj <= gf.ny && fk >= 0 && fk <= gf.nz) {
+            const float w = (dz == 1 ? 1.0f : 0.5f) * (dy == 1 ? 1.0f : 0.5f) * (dx == 1 ? 1.0f : 0.5f);
+            float kx[3], xn[3];
+            ku_gather<float, kApplyK>(gf, ks, ep, xf, nullptr, fi, fj, fk, 0, kx, xn);
+            const int m = 3 * (fi + gf.nx1 * (fj + gf.ny1 * fk));
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[c] = w * (invf[m + c] > 0.0f ? bf[m + c] - kx[c] : 0.0f);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off, 32);
+    if (t == 0 && n < gc.n_nodes) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            const float iv = invc[dof];
+            const float bi = iv > 0.0f ? v[c] : 0.0f;
+            const float xi = (iv * bi) * theta_inv;
+            bc[dof] = bi;
+            xcv[dof] = xi;
+            dc[dof] = xi;
+        }
+    }
+}
+
+// Whether the coarse levels use k_gmg_residual_restrict (MYCEL_GMG_FUSE_RR=0: separate residual and restriction).
+bool fuse_rr() {
+    static const bool v = [] {
+        const char *e = std::getenv("MYCEL_GMG_FUSE_RR");
+        return !(e && *e == '0');
+    }();
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict__ xf, const float *__restrict__ invf,
                                                      Dims gc, const float *__restrict__ xcv) {
     const int n = blockIdx.x * 256 + threadIdx.x;
@@ -1236,6 +1298,12 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
+        if (l > 0 && fuse_rr()) {
+            hipLaunchKernelGGL(k_gmg_residual_restrict, dim3(cdiv(C.d.n_nodes, 8), R), b256, 0, s, L.d, L.ks, L.ep,
+                               cur[l], L.b.get(), L.inv.get(), C.d, C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1],
+                               C.dv.get());
+            continue;
+        }
         if (const int kz = lkz(l)) {
             const int nt = ec_tiles(L.d, kz);
             const dim3 grid(cdiv(nt, 4), R);
