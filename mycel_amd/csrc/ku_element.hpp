@@ -172,15 +172,28 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
         xc[d] = cur[0][d];
     }
     const int kend = min(k0 + KZ - 1, g.nz);  // last node layer written
+    // Software pipeline: step s issues the loads of step s + 1 (its upper face, its modulus and the epilogue's
+    // loads) before its own arithmetic and stores, so a tile waits for memory about once, not once per step.
+    // Every step runs (straight-line code); on the last z-block the steps past the grid read clamped layers,
+    // take modulus 0 and write nothing. eo = 0 (a valid word) outside the padded columns.
+    T up[4][3];
+    load_face(k0, up);
+    T Ev = ld_off(ep + elayer * (min(k0 - 1, g.nz) + 1), eo);
+    decltype(fin.pre(0)) pre{};
 #pragma unroll
     for (int s = 0; s <= KZ; s++) {
         const int ke = k0 - 1 + s;  // element layer of this step; node layer ke is completed by it
-        if (ke > kend) break;
-        decltype(fin.pre(0)) pre{};
-        if (s > 0) pre = fin.pre(i0 + g.nx1 * (j0 + g.ny1 * ke));
-        T up[4][3];
-        load_face(ke + 1, up);
-        const T E = col_ok ? ld_off(ep + elayer * (ke + 1), eo) : T(0);
+        const bool live = ke <= kend;
+        T upn[4][3];
+        T Evn = T(0);
+        decltype(fin.pre(0)) pren{};
+        if (s < KZ) {
+            const int kn = min(ke + 1, g.nz);
+            load_face(ke + 2, upn);
+            Evn = ld_off(ep + elayer * (kn + 1), eo);
+            pren = fin.pre(i0 + g.nx1 * (j0 + g.ny1 * kn));
+        }
+        const T E = col_ok && live ? Ev : T(0);
         FaceSums<T> hi[3];
         T c[3][7], gm[3][7];
 #pragma unroll
@@ -211,10 +224,18 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
                 const T q0 = f0 + from_left(f1), q3 = f3 + from_left(f2);
                 out[d] = q0 + from_below(q3);
             }
-            if (out_ok) fin(ie + g.nx1 * (je + g.ny1 * ke), out, xc, pre);
+            if (out_ok && live) fin(ie + g.nx1 * (je + g.ny1 * ke), out, xc, pre);
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) xc[d] = up[0][d];
+        if (s < KZ) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) up[q][d] = upn[q][d];
+            Ev = Evn;
+            pre = pren;
+        }
     }
 }
 
