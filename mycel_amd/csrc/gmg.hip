@@ -607,8 +607,8 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
 // chain of global accesses would cost a memory round trip (~1 us) per phase instead. lv[count-1] is the
 // coarsest level, solved with its dense inverse (read from global memory, column-major).
 constexpr int kMaxTail = 8;
-constexpr int kTailThreads = 512;    // 8 waves: up to 256 VGPRs each, so the inlined gathers do not spill
-constexpr int kTailPer = 3;          // a tail level's arrays hold <= kTailPer * kTailThreads floats each
+constexpr int kTailThreads = 1024;   // 16 waves (up to 128 VGPRs each): two lanes per node on the tail levels
+constexpr int kTailPer = 2;          // a tail level's arrays hold <= kTailPer * kTailThreads floats each
 constexpr int kTailNodes = 512;      // default: levels up to this many nodes run in the tail (larger ones are
                                      // FMA-bound on one CU); MYCEL_GMG_TAIL_NODES overrides it
 constexpr int kTailLdsMax = 144 << 10;  // dynamic LDS of the tail (gfx950: 160 KiB per workgroup)
@@ -616,7 +616,7 @@ constexpr int kTailLdsMax = 144 << 10;  // dynamic LDS of the tail (gfx950: 160 
 int tail_nodes() {
     static const int v = [] {
         const char *e = std::getenv("MYCEL_GMG_TAIL_NODES");
-        return e && *e ? std::min(std::max(0, std::atoi(e)), 1 << 16) : kTailNodes;
+        return e && *e ? std::min(std::max(0, std::atoi(e)), kTailThreads / 2) : kTailNodes;
     }();
     return v;
 }
@@ -715,15 +715,57 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
     const int last = t.count - 1;
     // One Chebyshev step and the residual of a tail level, one node gather per node. (Forming each element's
     // product once, in an LDS scratch with a second barrier per phase, measured slower: 122.5 vs 126.8 solves/s.)
+    // Every tail level has at most kTailThreads / 2 nodes (tail_nodes()), so each node gets two lanes (lanes w*64 + l and w*64 + l + 32,
+    // the elements below and above the node: ku_gather_half), which halves each lane's dependent FMA chain;
+    // the phases are bound by that chain, not by the CU's FMA rate. Half 0 finishes the node.
+    auto split_kx = [&](const TailLevel &L, int l, int &n, float (&kx)[3], float (&xc)[3]) {
+        const int lane = threadIdx.x & 63, half = lane >> 5;
+        n = (threadIdx.x >> 6) * 32 + (lane & 31);
+#pragma unroll
+        for (int c = 0; c < 3; c++) kx[c] = xc[c] = 0.0f;
+        if (n < L.d.n_nodes) {
+            int i, j, k;
+            node_ijk(L.d, n, i, j, k);
+            ku_gather_half<float>(L.d, L.ks, S(L.o_e), cur(l), i, j, k, 0, half, kx, xc);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) kx[c] += __shfl_down(kx[c], 32, 64);  // every lane takes part
+        return half == 0 && n < L.d.n_nodes;
+    };
     auto cheb_phase = [&](const TailLevel &L, int l, float cd, float cr) {
-        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
-            cheb_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_d), S(L.o_inv), cd, cr, nxt(l), n, 0);
+        {
+            int n;
+            float kx[3], xc[3];
+            if (split_kx(L, l, n, kx, xc)) {
+                const float *b = S(L.o_b), *inv = S(L.o_inv);
+                float *d = S(L.o_d), *xo = nxt(l);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int dof = 3 * n + c;
+                    const float r = b[dof] - kx[c];
+                    const float dn = cd * d[dof] + cr * (inv[dof] * r);
+                    xo[dof] = xc[c] + dn;
+                    d[dof] = dn;
+                }
+            }
+        }
         __syncthreads();
         flip ^= 1u << l;
     };
     auto residual_phase = [&](const TailLevel &L, int l) {
-        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
-            residual_node(L.d, L.ks, S(L.o_e), cur(l), S(L.o_b), S(L.o_inv), S(L.o_r), n, 0);
+        {
+            int n;
+            float kx[3], xc[3];
+            if (split_kx(L, l, n, kx, xc)) {
+                const float *b = S(L.o_b), *inv = S(L.o_inv);
+                float *r = S(L.o_r);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const int dof = 3 * n + c;
+                    r[dof] = inv[dof] > 0.0f ? b[dof] - kx[c] : 0.0f;
+                }
+            }
+        }
         __syncthreads();
     };
     for (int l = 0; l < last; l++) {
