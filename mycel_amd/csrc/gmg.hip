@@ -642,7 +642,10 @@ struct TailArgs {
     int ainv_lds;            // float offset of the coarsest inverse in LDS, or -1 to read it from global memory
     int ainv_nn;             // entries of the inverse staged into LDS (n^2, or 0)
     const float *ainv;
-    float *out0;             // where lv[0]'s corrected iterate goes (global, RHS layout as above)
+    float *out0;             // where lv[0]'s corrected iterate goes (global, RHS layout as above), unless:
+    float *pro_x;            // non-null: the level above lv[0] (its iterate, RHS layout as above), into which the
+    const float *pro_inv;    //   tail prolongs lv[0]'s correction itself (k_gmg_prolong's arithmetic) from LDS
+    Dims pro_d;
     float theta_inv, cd1, cr1;
     int count;
     TailLevel lv[kMaxTail];
@@ -822,7 +825,13 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
         cheb_phase(L, l, t.cd1, t.cr1);
     }
     const float *res = cur(0);
-    for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) t.out0[o0 + q] = res[q];
+    if (t.pro_x) {
+        float *xf = t.pro_x + static_cast<long long>(blockIdx.x) * t.pro_d.n_dofs;
+        for (int n = threadIdx.x; n < t.pro_d.n_nodes; n += kTailThreads)
+            prolong_node(t.pro_d, xf, t.pro_inv, t.lv[0].d, res, n);
+    } else {
+        for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) t.out0[o0 + q] = res[q];
+    }
 }
 
 // ---- set-up ----------------------------------------------------------------------------------------
@@ -1367,11 +1376,20 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
                            C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
+    if (t0 > 0) {
+        // The tail prolongs into level t0 - 1 itself (one launch fewer): into that level's current iterate as
+        // the pre-smoothing above has left it (the smoothing steps swap the level's iterate buffers).
+        ta.pro_x = cur[t0 - 1];
+        ta.pro_inv = g->lv[t0 - 1].inv.get();
+        ta.pro_d = g->lv[t0 - 1].d;
+    }
     hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(kTailThreads), tail_lds, s, ta);
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
-        hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv.get(), C.d, cur[l + 1]);
+        if (l != t0 - 1)  // level t0 - 1 was prolonged into by the tail
+            hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv.get(), C.d,
+                               cur[l + 1]);
         cheb(l, 0.0f, g->theta_inv);
         for (int k = 1; k < (l == 0 ? g->fine_degree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k]);
     }
