@@ -354,7 +354,7 @@ def mode_sweep(args):
         cols = {}
         # K*u forms: the node gather (one node per thread, or kz-node z-marching columns) and the
         # element-centric product with kz node layers per tile.
-        for form, tag, kzs in ((M.MATVEC_NODE_GATHER, "gather", (0, 2, 4, 8)), (M.MATVEC_ELEMENT, "element", (2, 4, 8))):
+        for form, tag, kzs in ((M.MATVEC_NODE_GATHER, "gather", (0, 2, 4, 8)), (M.MATVEC_ELEMENT, "element", (2, 3, 4, 8))):
             for kz in kzs:
                 op.SetMatvecForm(form, kz)
                 cols[f"{tag}{kz}"] = {"fp64_us": time_matvecs(op, args.matvecs) * 1e3,
@@ -522,8 +522,8 @@ def mode_cases(args):
 
 
 COPY_BYTES = 256 << 20  # the device copy in --mode kernels
-# The K*u kernel ApplyK runs on the holed box (the automatic form: element-centric, 2 node layers per tile).
-KU_KERNEL = "k_apply_k_ec<double,kApplyK,2>"
+# The K*u kernel ApplyK runs on the holed box (the automatic form: element-centric, 3 node layers per tile).
+KU_KERNEL = "k_apply_k_ec<double,kApplyK,3>"
 
 
 def _csv_rows(directory, suffix):

@@ -73,7 +73,7 @@ extern "C" {
 /* K*u forms (mycel_op_set_matvec_form). */
 #define MYCEL_MATVEC_AUTO 0        /* by grid size (the default) */
 #define MYCEL_MATVEC_NODE_GATHER 1 /* one thread per node (columns 0) or per 2/4/8-node z-column */
-#define MYCEL_MATVEC_ELEMENT 2     /* element-centric: 8x8 element-column tiles, columns = 2/4/8 node layers */
+#define MYCEL_MATVEC_ELEMENT 2     /* element-centric: 8x8 element-column tiles, columns = 2/3/4/8 node layers */
 
 typedef struct mycel_op mycel_op;
 typedef struct mycel_filter mycel_filter;
@@ -151,8 +151,8 @@ MYCEL_API int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *
 /* K*u kernel form (a tuning option with no counterpart in the reference) for ApplyK / ApplyA and the FP32
  * operator; the solvers' fused-dot launches always use the automatic form. MYCEL_MATVEC_NODE_GATHER: columns
  * = 0 runs one node per thread, 2 / 4 / 8 let each thread march that many nodes along z (all bit-identical to
- * each other). MYCEL_MATVEC_ELEMENT: each element's product once, in the monomial basis, columns = 2 / 4 / 8
- * node layers per tile (all bit-identical to each other). MYCEL_MATVEC_AUTO (columns ignored): by grid size.
+ * each other). MYCEL_MATVEC_ELEMENT: each element's product once, in the monomial basis, columns = 2 / 3 / 4 /
+ * 8 node layers per tile (all bit-identical to each other). MYCEL_MATVEC_AUTO (columns ignored): by grid size.
  * The two families agree to rounding (both within 1e-12 of the FP64 reference). */
 MYCEL_API int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns);
 /* The gather form with columns 0 / 2 / 4 / 8, or -1 for MYCEL_MATVEC_AUTO. */

@@ -679,8 +679,8 @@ int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns) {
             require(columns == 0 || columns == 2 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
                     "gather columns must be 0, 2, 4 or 8");
         else if (form == MYCEL_MATVEC_ELEMENT)
-            require(columns == 2 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
-                    "element-centric columns must be 2, 4 or 8");
+            require(columns == 2 || columns == 3 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
+                    "element-centric columns must be 2, 3, 4 or 8");
         else
             require(form == MYCEL_MATVEC_AUTO, MYCEL_E_ARGUMENT, "unknown matvec form");
         op->ku_form = form;

@@ -378,14 +378,14 @@ struct KuForm {
 };
 
 // The default form, by grid size. The element-centric product does about a third of the gather's arithmetic;
-// measured on MI355X (FP64 K*u, best gather form -> element form): 9.1 -> 6.0 us on the 80x40x40 holed box
-// (136k nodes, 2 layers per tile), 13.6 -> 8.6 us at 263k nodes (4 layers), 35.6 -> 19.5 us at 1.06M nodes and
-// 115 -> 62 us at 4.3M nodes (8 layers). Longer columns amortise the extra element layer each tile forms below
+// measured on MI355X (FP64 K*u, best gather form -> element form): 9.1 -> 5.4 us on the 80x40x40 holed box
+// (136k nodes, 3 layers per tile; 6.0 us with 2), 13.7 -> 7.5 us at 263k nodes (3 layers; 8.4 with 4),
+// 35.0 -> 18.3 us at 1.06M nodes and 115 -> 63 us at 4.3M nodes (8 layers). Longer columns amortise the extra element layer each tile forms below
 // its first node layer; shorter ones keep enough waves on small grids. The K*u launches that write dot
 // partials always use this rule, so an operator's partial count never changes.
 static KuForm ku_auto(const mycel_op *op) {
     const int n = op->d.n_nodes;
-    return {MYCEL_MATVEC_ELEMENT, n < (1 << 18) ? 2 : n < (1 << 20) ? 4 : 8};
+    return {MYCEL_MATVEC_ELEMENT, n < (1 << 19) ? 3 : n < (1 << 20) ? 4 : 8};
 }
 
 // The form of a launch: the automatic one for launches with dot partials, else the operator's setting
@@ -417,6 +417,9 @@ static void launch_ku(mycel_op *op, const KuForm &f, const KeSym<T> &ke, const K
         const int nt = ec_tiles(op->d, f.kz);
         if (f.kz == 2)
             hipLaunchKernelGGL((k_apply_k_ec<T, MODE, DOT, 2>), grid, block, 0, s, op->d, km, ep, d_x, fl, d_y, part,
+                               stop, bt, nt);
+        else if (f.kz == 3)
+            hipLaunchKernelGGL((k_apply_k_ec<T, MODE, DOT, 3>), grid, block, 0, s, op->d, km, ep, d_x, fl, d_y, part,
                                stop, bt, nt);
         else if (f.kz == 4)
             hipLaunchKernelGGL((k_apply_k_ec<T, MODE, DOT, 4>), grid, block, 0, s, op->d, km, ep, d_x, fl, d_y, part,

@@ -1266,6 +1266,9 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             if (kz == 2)
                 hipLaunchKernelGGL(k_gmg_cheb_ec<2>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
                                    L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
+            else if (kz == 3)
+                hipLaunchKernelGGL(k_gmg_cheb_ec<3>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
             else if (kz == 4)
                 hipLaunchKernelGGL(k_gmg_cheb_ec<4>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
                                    L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
@@ -1361,6 +1364,9 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             if (kz == 2)
                 hipLaunchKernelGGL(k_gmg_residual_ec<2>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
                                    L.inv.get(), L.r.get(), nt);
+            else if (kz == 3)
+                hipLaunchKernelGGL(k_gmg_residual_ec<3>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), nt);
             else if (kz == 4)
                 hipLaunchKernelGGL(k_gmg_residual_ec<4>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
                                    L.inv.get(), L.r.get(), nt);
@@ -1399,6 +1405,9 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     const float cd = top == 0 ? 0.0f : g->cd[fd - 1], cr = top == 0 ? 0.0f : g->cr[fd - 1];
     if (fkz == 2)
         hipLaunchKernelGGL(k_gmg_cheb_final_ec<2>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+    else if (fkz == 3)
+        hipLaunchKernelGGL(k_gmg_cheb_final_ec<3>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
                            F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
     else if (fkz == 4)
         hipLaunchKernelGGL(k_gmg_cheb_final_ec<4>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),

@@ -292,7 +292,7 @@ class FemOperator:
         """K*u kernel form of ApplyK / ApplyA and the FP32 operator (no counterpart in the reference; the
         solvers' fused-dot launches always use the automatic form). MATVEC_NODE_GATHER: columns 0 = one node
         per thread, 2/4/8 = each thread marches that many nodes along z. MATVEC_ELEMENT: each element's product
-        once in the monomial basis, columns 2/4/8 node layers per tile. MATVEC_AUTO: by grid size. Forms of one
+        once in the monomial basis, columns 2/3/4/8 node layers per tile. MATVEC_AUTO: by grid size. Forms of one
         family give bit-identical results; the two families agree to rounding."""
         L.check(L.load().mycel_op_set_matvec_form(self._h, int(form), int(columns)))
 
