@@ -603,14 +603,15 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
 // of lv[0]'s b, x, d (written by the restriction from the level above) and of every tail level's moduli and
 // inverse diagonal, all loads in flight together (one memory round trip), and stores them into LDS; every
 // phase after that (smoothing, residual, restriction, coarsest solve, prolongation) reads and writes LDS
-// only, separated by workgroup barriers, and the corrected lv[0] iterate is written back at the end. A
-// chain of global accesses would cost a memory round trip (~1 us) per phase instead. lv[count-1] is the
-// coarsest level, solved with its dense inverse (read from global memory, column-major).
+// only, separated by workgroup barriers. At the end the tail prolongs lv[0]'s correction into the level above
+// (or, when lv[0] is the finest level, writes lv[0]'s iterate back). A chain of global accesses would cost a
+// memory round trip (~1 us) per phase instead. lv[count-1] is the coarsest level, solved with its dense FP32
+// inverse (column-major; staged into LDS with the rest when it fits).
 constexpr int kMaxTail = 8;
 constexpr int kTailThreads = 1024;   // 16 waves (up to 128 VGPRs each): two lanes per node on the tail levels
 constexpr int kTailPer = 2;          // a tail level's arrays hold <= kTailPer * kTailThreads floats each
-constexpr int kTailNodes = 512;      // default: levels up to this many nodes run in the tail (larger ones are
-                                     // FMA-bound on one CU); MYCEL_GMG_TAIL_NODES overrides it
+constexpr int kTailNodes = 512;      // levels up to this many nodes run in the tail (larger ones are FMA-bound
+                                     // on one CU); MYCEL_GMG_TAIL_NODES lowers it (at most kTailThreads / 2)
 constexpr int kTailLdsMax = 144 << 10;  // dynamic LDS of the tail (gfx950: 160 KiB per workgroup)
 
 int tail_nodes() {
