@@ -66,6 +66,20 @@ def test_gmg_bimodal_contrast(gpu):
     _check_against_oracle(W.precision_variant(), max_gmg_iters=60000)
 
 
+@pytest.mark.parametrize("shape", [(37, 19, 11), (33, 21, 9)], ids=["box-37x19x11", "lshape-33x21x9"])
+def test_gmg_odd_grids(gpu, shape):
+    """Odd element counts on every axis: coarse grids of ceil(n/2) elements, restriction and prolongation taps
+    at odd boundaries, a tail whose top level (264 / 280 nodes) lies two levels below the finest and prolongs
+    into a launched level; on a box and on the L-shaped domain (whose tip load partly falls on inactive
+    nodes)."""
+    nx, ny, nz = shape
+    p = W.cantilever_simp(nx, ny, nz)
+    if shape == (33, 21, 9):
+        p.domain = W.l_shape_domain(nx, ny, nz)
+    p.density = random_field(nx + ny, p.n_elem, 0.01, 1.0)
+    _check_against_oracle(p, max_gmg_iters=200)
+
+
 def test_gmg_refresh_after_density_update_and_warm_start(gpu):
     """The hierarchy follows UpdateDensities (the same answer as a fresh operator), and a warm start from the
     converged solution stops at once."""
