@@ -211,13 +211,15 @@ def test_simp_cases_multigrid_batched(gpu):
     assert sum(it) < sum(it_ref)
 
 
-def test_large_grid_column_matvec_in_solvers(gpu):
-    """From 512k nodes on, the K*u launches inside the solvers (with their p.Ap partials, also batched) use
-    the z-marching column form. Checked without the oracle at this size: each solution's true FP64 residual,
-    recomputed with the one-node-per-thread K*u, meets the tolerance; the Jacobi and multigrid solutions agree;
-    a two-RHS lockstep solve is bit-identical to the single solves."""
-    p = W.cantilever_simp(nx=112, ny=80, nz=64)
-    assert (p.nx + 1) * (p.ny + 1) * (p.nz + 1) >= 1 << 19
+@pytest.mark.parametrize("shape", [(112, 80, 64), (160, 80, 80)], ids=["595k-nodes", "1.06M-nodes"])
+def test_large_grid_column_matvec_in_solvers(gpu, shape):
+    """On large grids the K*u launches inside the solvers (with their p.Ap partials, also batched) and the
+    finest multigrid level use the element-centric form with longer columns: 4 node layers per tile from 2^19
+    nodes, 8 from 2^20. Checked without the oracle at these sizes: each solution's true FP64 residual,
+    recomputed with the one-node-per-thread gather, meets the tolerance; the Jacobi and multigrid solutions
+    agree; a two-RHS lockstep solve is bit-identical to the single solves."""
+    p = W.cantilever_simp(nx=shape[0], ny=shape[1], nz=shape[2])
+    assert (p.nx + 1) * (p.ny + 1) * (p.nz + 1) >= (1 << 19 if shape[0] == 112 else 1 << 20)
     p.density = random_field(9, p.n_elem, 0.05, 1.0)
     fm = p.fe_model()
     f = fm.AssembleForces()
