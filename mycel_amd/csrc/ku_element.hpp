@@ -124,15 +124,35 @@ __host__ __device__ inline int ec_tiles(const Dims &g, int kz) {
     return ((g.nx1 + kEcOut - 1) / kEcOut) * ((g.ny1 + kEcOut - 1) / kEcOut) * ((g.nz1 + kz - 1) / kz);
 }
 
-// One wavefront's tile: node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7 node columns. The epilogue fin has two
-// stages: fin.pre(n) at the start of the step that completes node n (its loads then overlap the step's face
-// loads and arithmetic; halo lanes pass a valid clamped node), and fin(n, kx, xc, pre) for every node the tile
-// writes, with the raw (K x) at node n and the node's own x values as loaded (constrained DOFs zeroed when
-// MASK). Called by all 64 lanes of the wave (tile is wave-uniform).
-template <typename T, bool MASK, int KZ, typename Fin>
-__device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
-                                            const T *__restrict__ x, const uint8_t *__restrict__ flags, int tile,
-                                            Fin &&fin) {
+// Where a tile reads x: the vector itself, with constrained DOFs read as 0 when MASK. A source has load (the raw
+// words of one DOF at vector offset at + byte offset o, component d), value (raw words -> x) and ready, which
+// the march calls once, after its first loads are issued and before any value is used (false: the march ends
+// there and writes nothing).
+template <typename T, bool MASK>
+struct EcVector {
+    const T *x;
+    const uint8_t *flags;
+    using Raw = T;
+    __device__ Raw load(size_t at, uint32_t o, int d) const {
+        T v = ld_off(x + at, o, d * static_cast<int>(sizeof(T)));
+        if (MASK && (ld_off(flags + at, o / static_cast<uint32_t>(sizeof(T)), d) & kFlagConstrained)) v = T(0);
+        return v;
+    }
+    __device__ T value(Raw r) const { return r; }
+    template <typename Fin>
+    __device__ bool ready(Fin &) {
+        return true;
+    }
+};
+
+// One wavefront's tile: node layers KZ*tz .. KZ*tz + KZ - 1 of its 7 x 7 node columns, x read through src. The
+// epilogue fin has two stages: fin.pre(n) at the start of the step that completes node n (its loads then
+// overlap the step's face loads and arithmetic; halo lanes pass a valid clamped node), and fin(n, kx, xc, pre)
+// for every node the tile writes, with the raw (K x) at node n and the node's own x values as read. Called by
+// all 64 lanes of the wave (tile is wave-uniform).
+template <typename T, int KZ, typename Src, typename Fin>
+__device__ __forceinline__ void ec_tile_march(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
+                                              Src &src, int tile, Fin &&fin) {
     const int lane = threadIdx.x & 63, a = lane & (kEcTile - 1), b = lane >> 3;
     const int ntx = (g.nx1 + kEcOut - 1) / kEcOut, nty = (g.ny1 + kEcOut - 1) / kEcOut;
     const int tx = tile % ntx, tr = tile / ntx, ty = tr % nty, tz = tr / nty;
@@ -146,45 +166,41 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
     const uint32_t eo = col_ok ? sT * ((ie + 1) + g.px * (je + 1)) : 0u;
     const size_t layer = static_cast<size_t>(g.nx1) * g.ny1, elayer = static_cast<size_t>(g.px) * g.py;
 
+    using Raw = typename Src::Raw;
     // Node layer kn (clamped to the grid: out-of-grid layers belong to elements of modulus 0).
-    auto load_face = [&](int kn, T (&v)[4][3]) {
-        const size_t kc = static_cast<size_t>(min(max(kn, 0), g.nz));
-        const T *xl = x + 3 * layer * kc;
-        const uint8_t *fl = MASK ? flags + 3 * layer * kc : nullptr;
+    auto load_face = [&](int kn, Raw (&v)[4][3]) {
+        const size_t at = 3 * layer * static_cast<size_t>(min(max(kn, 0), g.nz));
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                T vv = ld_off(xl, o[q], d * static_cast<int>(sT));
-                if (MASK && (ld_off(fl, o[q] / sT, d) & kFlagConstrained)) vv = T(0);
-                v[q][d] = vv;
-            }
+            for (int d = 0; d < 3; d++) v[q][d] = src.load(at, o[q], d);
     };
 
-    T cur[4][3];
+    Raw cur[4][3];
     load_face(k0 - 1, cur);
-    FaceSums<T> lo[3], carry[3];
-    T xc[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        lo[d] = face_sums(cur[0][d], cur[1][d], cur[2][d], cur[3][d]);
-        carry[d] = {T(0), T(0), T(0), T(0)};
-        xc[d] = cur[0][d];
-    }
     const int kend = min(k0 + KZ - 1, g.nz);  // last node layer written
     // Software pipeline: step s issues the loads of step s + 1 (its upper face, its modulus and the epilogue's
     // loads) before its own arithmetic and stores, so a tile waits for memory about once, not once per step.
     // Every step runs (straight-line code); on the last z-block the steps past the grid read clamped layers,
     // take modulus 0 and write nothing. eo = 0 (a valid word) outside the padded columns.
-    T up[4][3];
+    Raw up[4][3];
     load_face(k0, up);
     T Ev = ld_off(ep + elayer * (min(k0 - 1, g.nz) + 1), eo);
+    if (!src.ready(fin)) return;
+    FaceSums<T> lo[3], carry[3];
+    T xc[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lo[d] = face_sums(src.value(cur[0][d]), src.value(cur[1][d]), src.value(cur[2][d]), src.value(cur[3][d]));
+        carry[d] = {T(0), T(0), T(0), T(0)};
+        xc[d] = src.value(cur[0][d]);
+    }
     decltype(fin.pre(0)) pre{};
 #pragma unroll
     for (int s = 0; s <= KZ; s++) {
         const int ke = k0 - 1 + s;  // element layer of this step; node layer ke is completed by it
         const bool live = ke <= kend;
-        T upn[4][3];
+        Raw upn[4][3];
         T Evn = T(0);
         decltype(fin.pre(0)) pren{};
         if (s < KZ) {
@@ -197,7 +213,8 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
         FaceSums<T> hi[3];
         T c[3][7], gm[3][7];
 #pragma unroll
-        for (int d = 0; d < 3; d++) hi[d] = face_sums(up[0][d], up[1][d], up[2][d], up[3][d]);
+        for (int d = 0; d < 3; d++)
+            hi[d] = face_sums(src.value(up[0][d]), src.value(up[1][d]), src.value(up[2][d]), src.value(up[3][d]));
         mono_coefficients(lo, hi, c);
         mono_product(km, E, c, gm);
         // Inverse transform, z stage: the element's lower-face part joins the carried upper-face part of the
@@ -227,7 +244,7 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
             if (out_ok && live) fin(ie + g.nx1 * (je + g.ny1 * ke), out, xc, pre);
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) xc[d] = up[0][d];
+        for (int d = 0; d < 3; d++) xc[d] = src.value(up[0][d]);
         if (s < KZ) {
 #pragma unroll
             for (int q = 0; q < 4; q++)
@@ -237,6 +254,15 @@ __device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, 
             pre = pren;
         }
     }
+}
+
+// The march on a plain vector x (constrained DOFs read as 0 when MASK).
+template <typename T, bool MASK, int KZ, typename Fin>
+__device__ __forceinline__ void ec_tile_run(const Dims &g, const KeMono<T> &km, const T *__restrict__ ep,
+                                            const T *__restrict__ x, const uint8_t *__restrict__ flags, int tile,
+                                            Fin &&fin) {
+    EcVector<T, MASK> src{x, flags};
+    ec_tile_march<T, KZ>(g, km, ep, src, tile, fin);
 }
 
 // Epilogue of K x (MODE: as ku_gather): mode handling, store, and the x.y contribution.

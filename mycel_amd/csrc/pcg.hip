@@ -32,8 +32,10 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "device_common.hpp"
+#include "ku_element.hpp"
 #include "mycel_internal.hpp"
 
 namespace mycel {
@@ -59,6 +61,7 @@ struct PcgWork {
     int n = 0;
     int nb_mv = 0, nb_vec = 0;
     DevBuf<double> r, z, p, ap;              // FP64 iteration vectors (r / ap double as r64 / A x64 when mixed)
+    DevBuf<double> p2;                       // FP64 iterations with the fused step 4: p of the odd ones
     DevBuf<float> r32, z32, p32, ap32, x32;  // FP32 iteration vectors of the mixed path
     DevBuf<double> part_pap, part2, part1;
     DevBuf<PcgState> state;
@@ -128,9 +131,11 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
         av[q] = ap[i];
         iv[q] = inv[i];
     }
-    if (st->halt) return;
-    const double pap = wave_sum_partials(part_pap, nb_mv);
+    // The halt flag and rho0 are read ahead of the partial sum, so their loads overlap its loads.
+    const int halt = st->halt;
     const double rz0 = st->rz[slot];
+    const double pap = wave_sum_partials(part_pap, nb_mv);
+    if (halt) return;
     const bool curv = !(pap > 1e-30) || !isfinite(pap);  // PcgSolver.cs:245
     const double alpha = rz0 / pap;
     const bool step = !curv && (!isfinite(alpha) || alpha <= 0.0);  // :260
@@ -162,6 +167,84 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
     }
 }
 
+// The state a step 4 reads, loaded at kernel entry in one go: its loads are then in flight together with the
+// vector and partial loads instead of one dependent round trip each after the partial sums. Within a launch
+// only the lead thread writes the state, and only fields of the other slot, the halt fields and maxrel (whose
+// new value max(maxrel, rel) gives every reader the same max(maxrel, rel)).
+struct Step4In {
+    double bnorm, tol, delta, maxrel, prev_rel, rz, alpha;
+    int iter, stag, mixed, max_iter, halt;
+};
+__device__ __forceinline__ Step4In step4_in(const PcgState *__restrict__ st, int slot) {
+    Step4In in;
+    in.bnorm = st->bnorm;
+    in.tol = st->tol;
+    in.delta = st->delta;
+    in.maxrel = st->maxrel;
+    in.prev_rel = st->prev_rel[slot];
+    in.rz = st->rz[slot];
+    in.alpha = st->alpha[slot];
+    in.iter = st->iter[slot];
+    in.stag = st->stag[slot];
+    in.mixed = st->mixed;
+    in.max_iter = st->max_iter;
+    in.halt = st->halt;
+    return in;
+}
+
+// Jacobi PCG, step 4 decisions from the per-wave sums of the interleaved (r.r, r.z) partials: relres,
+// convergence (mixed: the reliable-update pause) and stagnation as PcgSolver.cs:279-313, then beta = rz' / rz
+// (:334). Returns false when the solve halts or pauses here (the lead thread records why); otherwise the lead
+// thread advances the other slot.
+__device__ __forceinline__ bool pcg_step4(PcgState *__restrict__ st, const Step4In &in, int slot,
+                                          const double *__restrict__ part2, int nb_vec, bool lead, double &beta) {
+    double rr, rzn;
+    wave_sum_partials2(part2, nb_vec, rr, rzn);
+    if (in.halt) return false;
+    const double rel = sqrt(rr) / in.bnorm;
+    const int t = in.iter;
+    if (in.mixed) {
+        // Reliable update: the true residual decides convergence; refresh it once the recursive residual
+        // has fallen by delta below its running maximum since the last refresh.
+        const double mx = fmax(in.maxrel, rel);
+        if (rel <= in.tol || rel < in.delta * mx) {
+            if (lead) {
+                st->pause = 1;
+                st->pause_slot = slot;
+                st->iterations = t;
+                st->last_rel = rel;
+                st->halt = 1;
+            }
+            return false;
+        }
+    } else if (rel <= in.tol) {  // PcgSolver.cs:279
+        if (lead) halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
+        return false;
+    }
+    const int stag = (fabs(rel - in.prev_rel) < 1e-15 * rel) ? in.stag + 1 : 0;  // :293-313
+    if (stag > 50) {
+        if (lead) halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
+        return false;
+    }
+    beta = rzn / in.rz;
+    if (!isfinite(beta) || beta < 0.0) {  // :334
+        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        return false;
+    }
+    if (lead) {
+        const int o = slot ^ 1;
+        st->rz[o] = rzn;
+        st->prev_rel[o] = rel;
+        st->stag[o] = stag;
+        st->iter[o] = t + 1;
+        st->last_rel = rel;
+        st->last_iter = t;
+        if (in.mixed) st->maxrel = fmax(in.maxrel, rel);
+        if (t >= in.max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);  // :359-368
+    }
+    return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState *__restrict__ st,
                                                       const double *__restrict__ part2, int nb_vec,
@@ -174,56 +257,14 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
         pv[q] = p[i];
         zv[q] = z[i];
     }
-    if (st->halt) return;
-    double rr, rzn;
-    wave_sum_partials2(part2, nb_vec, rr, rzn);
-    const double rel = sqrt(rr) / st->bnorm;
-    const int t = st->iter[slot];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (st->mixed) {
-        // Reliable update: the true residual decides convergence; refresh it once the recursive residual
-        // has fallen by delta below its running maximum since the last refresh.
-        const double mx = fmax(st->maxrel, rel);
-        if (rel <= st->tol || rel < st->delta * mx) {
-            if (lead) {
-                st->pause = 1;
-                st->pause_slot = slot;
-                st->iterations = t;
-                st->last_rel = rel;
-                st->halt = 1;
-            }
-            return;
-        }
-    } else if (rel <= st->tol) {  // PcgSolver.cs:279
-        if (lead) halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
-        return;
-    }
-    const int stag = (fabs(rel - st->prev_rel[slot]) < 1e-15 * rel) ? st->stag[slot] + 1 : 0;  // :293-313
-    if (stag > 50) {
-        if (lead) halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
-        return;
-    }
-    const double beta = rzn / st->rz[slot];
-    if (!isfinite(beta) || beta < 0.0) {  // :334
-        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
-        return;
-    }
+    const Step4In in = step4_in(st, slot);
+    double beta;
+    if (!pcg_step4(st, in, slot, part2, nb_vec, blockIdx.x == 0 && threadIdx.x == 0, beta)) return;
     const T b = static_cast<T>(beta);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
         if (i < n) p[i] = fmad(b, pv[q], zv[q]);
-    }
-    if (lead) {
-        const int o = slot ^ 1;
-        st->rz[o] = rzn;
-        st->prev_rel[o] = rel;
-        st->stag[o] = stag;
-        st->iter[o] = t + 1;
-        st->last_rel = rel;
-        st->last_iter = t;
-        if (st->mixed) st->maxrel = fmax(st->maxrel, rel);
-        if (t >= st->max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);  // :359-368
     }
 }
 
@@ -263,9 +304,10 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, Pcg
         av[q] = ap[i];
         iv[q] = fv.inv[i];
     }
-    if (st->halt) return;
-    const double pap = wave_sum_partials(part_pap, nb_mv);
+    const int halt = st->halt;  // read ahead of the partial sum, as in k_pcg_update_xr
     const double rz0 = st->rz[slot];
+    const double pap = wave_sum_partials(part_pap, nb_mv);
+    if (halt) return;
     const bool curv = !(pap > 1e-30) || !isfinite(pap);
     const double alpha = rz0 / pap;
     const bool step = !curv && (!isfinite(alpha) || alpha <= 0.0);
@@ -296,8 +338,50 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, Pcg
     if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
 }
 
-// Flexible PCG, step 4: relres, convergence and stagnation as PcgSolver.cs:279-313; beta = -alpha z.Ap / rz
-// (clamped at 0); p = z + beta p.
+// Flexible PCG, step 4 decisions from the per-wave sums of the r.r, z.r and z.Ap partials: relres, convergence
+// and stagnation as PcgSolver.cs:279-313, then beta = -alpha z.Ap / rz (clamped at 0). Returns false when the
+// solve halts here (the lead thread records why); otherwise the lead thread advances the other slot.
+__device__ __forceinline__ bool fpcg_step4(PcgState *__restrict__ st, const Step4In &in, int slot,
+                                           const double *__restrict__ part_rr, int nb_vec,
+                                           const double *__restrict__ part_zr, const double *__restrict__ part_zap,
+                                           int nb_z, bool lead, double &beta) {
+    const double *const parts[3] = {part_rr, part_zr, part_zap};
+    const int counts[3] = {nb_vec, nb_z, nb_z};
+    double sums[3];
+    wave_sum_partials_n<3>(parts, counts, sums);
+    if (in.halt) return false;
+    const double rr = sums[0], zr = sums[1], zap = sums[2];
+    const double rel = sqrt(rr) / in.bnorm;
+    const int t = in.iter;
+    if (rel <= in.tol) {
+        if (lead) halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
+        return false;
+    }
+    const int stag = (fabs(rel - in.prev_rel) < 1e-15 * rel) ? in.stag + 1 : 0;
+    if (stag > 50) {
+        if (lead) halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
+        return false;
+    }
+    beta = -in.alpha * zap / in.rz;
+    if (!isfinite(beta)) {
+        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        return false;
+    }
+    beta = fmax(beta, 0.0);
+    if (lead) {
+        const int o = slot ^ 1;
+        st->rz[o] = zr;
+        st->prev_rel[o] = rel;
+        st->stag[o] = stag;
+        st->iter[o] = t + 1;
+        st->last_rel = rel;
+        st->last_iter = t;
+        if (t >= in.max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);
+    }
+    return true;
+}
+
+// Flexible PCG, step 4: p = z + beta p.
 __global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState *__restrict__ st,
                                                        const double *__restrict__ part_rr, int nb_vec,
                                                        const double *__restrict__ part_zr,
@@ -320,45 +404,91 @@ __global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState
         pv[q] = p[i];
         zv[q] = z[i];
     }
-    if (st->halt) return;
-    const double *const parts[3] = {part_rr, part_zr, part_zap};
-    const int counts[3] = {nb_vec, nb_z, nb_z};
-    double sums[3];
-    wave_sum_partials_n<3>(parts, counts, sums);
-    const double rr = sums[0], zr = sums[1], zap = sums[2];
-    const double rel = sqrt(rr) / st->bnorm;
-    const int t = st->iter[slot];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (rel <= st->tol) {
-        if (lead) halt_with(st, MYCEL_PCG_CONVERGED, t, rel);
+    const Step4In in = step4_in(st, slot);
+    double beta;
+    if (!fpcg_step4(st, in, slot, part_rr, nb_vec, part_zr, part_zap, nb_z, blockIdx.x == 0 && threadIdx.x == 0,
+                    beta))
         return;
-    }
-    const int stag = (fabs(rel - st->prev_rel[slot]) < 1e-15 * rel) ? st->stag[slot] + 1 : 0;
-    if (stag > 50) {
-        if (lead) halt_with(st, MYCEL_PCG_STAGNATION, t, rel);
-        return;
-    }
-    double beta = -st->alpha[slot] * zap / st->rz[slot];
-    if (!isfinite(beta)) {
-        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
-        return;
-    }
-    beta = fmax(beta, 0.0);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
         if (i < n) p[i] = fmad(beta, pv[q], zv[q]);
     }
-    if (lead) {
-        const int o = slot ^ 1;
-        st->rz[o] = zr;
-        st->prev_rel[o] = rel;
-        st->stag[o] = stag;
-        st->iter[o] = t + 1;
-        st->last_rel = rel;
-        st->last_iter = t;
-        if (t >= st->max_iter) halt_with(st, MYCEL_PCG_MAX_ITERATIONS, t, rel);
+}
+
+// Step 4 (FP64; FLEX: the flexible form, else the Jacobi one) fused with the next iteration's A p: p' = z + beta p
+// is formed where the element-centric tile march reads x, and each tile writes A p', p' and the p'.A p' partials
+// of the nodes it owns (p' goes to a second buffer: other tiles still read p at their halo nodes). One launch and
+// one pass over p fewer than the p update followed by the K*u launch, with bit-identical values.
+template <bool FLEX>
+struct PcgNextP {
+    const double *p, *z;
+    PcgState *st;
+    const double *part_rr, *part_zr, *part_zap;  // Jacobi: part_rr holds the interleaved (r.r, r.z) partials
+    int nb_vec, nb_z, slot;
+    double beta;
+    Step4In in;  // loaded at kernel entry (step4_in)
+    struct Raw {
+        double p, z;
+    };
+    __device__ Raw load(size_t at, uint32_t o, int d) const {
+        return {ld_off(p + at, o, 8 * d), ld_off(z + at, o, 8 * d)};
     }
+    __device__ double value(Raw r) const { return fmad(beta, r.p, r.z); }
+    // Called once per wave after its first loads are issued, so the partials' round trip overlaps them; false
+    // (the solve halted before or halts here) ends the wave's march.
+    template <typename Fin>
+    __device__ bool ready(Fin &fin) {
+        const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+        fin.go = FLEX ? fpcg_step4(st, in, slot, part_rr, nb_vec, part_zr, part_zap, nb_z, lead, beta)
+                      : pcg_step4(st, in, slot, part_rr, nb_vec, lead, beta);
+        return fin.go;
+    }
+};
+
+// Epilogue: the projected operator's (fixed DOFs: out = p'), stores of A p' and p' unless the solve halted.
+struct PcgNextFin {
+    const uint8_t *flags;
+    double *ap, *p_next;
+    double dot = 0.0;
+    bool go = false;
+    struct Pre {
+        uint8_t fl[3];
+    };
+    __device__ Pre pre(int n) const {
+        Pre q;
+#pragma unroll
+        for (int d = 0; d < 3; d++) q.fl[d] = flags[3 * n + d];
+        return q;
+    }
+    __device__ void operator()(int n, double (&out)[3], const double (&xc)[3], const Pre &q) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (q.fl[d] & kFlagFixed) out[d] = xc[d];
+            if (go) {
+                ap[3 * n + d] = out[d];
+                p_next[3 * n + d] = xc[d];
+            }
+            dot += xc[d] * out[d];
+        }
+    }
+};
+
+template <bool FLEX, int KZ>
+__global__ __launch_bounds__(256) void k_pcg_step4_ap(Dims g, const KeMono<double> km, const double *__restrict__ ep,
+                                                      const uint8_t *__restrict__ flags, PcgNextP<FLEX> src,
+                                                      double *__restrict__ ap, double *__restrict__ p_next,
+                                                      double *__restrict__ part_pap, int n_tiles) {
+    __shared__ double red[4];
+    src.in = step4_in(src.st, src.slot);
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    PcgNextFin fin{flags, ap, p_next};
+    // A halted solve ends each wave's march at its ready() (every wave still joins the partial sum's barriers).
+    // Waves may see the halt flag differently when this launch's lead thread sets it: the solve is then over and
+    // nothing reads this launch's outputs.
+    if (tile < n_tiles) ec_tile_march<double, KZ>(g, km, ep, src, tile, fin);
+    const double sum = block_sum256(fin.dot, red);
+    if (threadIdx.x == 0 && fin.go) part_pap[blockIdx.x] = sum;  // wave 0 of every block has a tile
 }
 
 // p = z (the first search direction of the preconditioned iteration).
@@ -571,6 +701,7 @@ void ensure_work(mycel_op *op) {
     w->r.resize(w->n);
     w->z.resize(w->n);
     w->p.resize(w->n);
+    w->p2.resize(w->n);
     w->ap.resize(w->n);
     w->part_pap.resize(w->nb_mv);
     w->part2.resize(2 * w->nb_vec);
@@ -588,8 +719,60 @@ void ensure_mixed(PcgWork *w) {
     w->x32.resize(w->n);
 }
 
+// Step 4 fused with the next A p (k_pcg_step4_ap) in the FP64 iterations: on by default; MYCEL_PCG_FUSE_P=0
+// separates them (the p update, then the K*u launch). Values are identical either way.
+bool fuse_step4() {
+    static const bool on = [] {
+        const char *e = std::getenv("MYCEL_PCG_FUSE_P");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// p of FP64 iteration t (fused step 4): p for t even, p2 for t odd; step 4 of iteration t writes the other.
+static_assert(kGraphIters % 2 == 0 && kGmgGraphIters % 2 == 0, "fused step 4: p buffers alternate per iteration");
+
+template <bool FLEX>
+void launch_step4_ap(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
+    const int kz = ku_element_layers(op);
+    const int nt = ec_tiles(op->d, kz);
+    // update_xr reads the partials of this launch as those of the automatic K*u form
+    if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "fused step 4: partial count mismatch");
+    const double *p = slot ? w->p2.get() : w->p.get();
+    double *p_next = slot ? w->p.get() : w->p2.get();
+    const PcgNextP<FLEX> src{p, w->z.get(), w->state.get(), w->part2.get(), w->part_zr.get(), w->part_zap.get(),
+                             w->nb_vec, w->nb_z, slot, 0.0};
+    const dim3 grid(cdiv(nt, 4)), b256(256);
+    const double *ep = op->d_ep.get();
+    const uint8_t *fl = op->d_flags.get();
+    double *ap = w->ap.get(), *part = w->part_pap.get();
+    if (kz == 2)
+        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 2>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
+                           part, nt);
+    else if (kz == 3)
+        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 3>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
+                           part, nt);
+    else if (kz == 4)
+        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 4>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
+                           part, nt);
+    else if (kz == 8)
+        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 8>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
+                           part, nt);
+    else
+        throw Error(MYCEL_E_INVALID_OPERATION, "fused step 4: unsupported tile depth");
+}
+
+// One FP64 Jacobi iteration. Fused: its A p was formed by the step 4 before it (or, for the first, before the
+// loop).
 void enqueue_iteration64(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
     PcgState *st = w->state.get();
+    if (fuse_step4()) {
+        hipLaunchKernelGGL(k_pcg_update_xr<double>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st,
+                           w->part_pap.get(), w->nb_mv, d_x, w->r.get(), slot ? w->p2.get() : w->p.get(), w->ap.get(),
+                           w->z.get(), op->d_inv_diag.get(), w->part2.get());
+        launch_step4_ap<false>(op, w, slot, s);
+        return;
+    }
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
     hipLaunchKernelGGL(k_pcg_update_xr<double>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                        w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->z.get(), op->d_inv_diag.get(),
@@ -615,8 +798,19 @@ void ensure_gmg(mycel_op *op, PcgWork *w) {
     w->part_zap.resize(w->nb_z);
 }
 
+
+// One multigrid-preconditioned iteration. Fused: its A p was formed by the step 4 before it (or, for the first,
+// before the loop).
 void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
     PcgState *st = w->state.get();
+    if (fuse_step4()) {
+        hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+                           w->nb_mv, d_x, w->r.get(), slot ? w->p2.get() : w->p.get(), w->ap.get(), w->part2.get(),
+                           gmg_fine_view(op));
+        gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true);
+        launch_step4_ap<true>(op, w, slot, s);
+        return;
+    }
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
     hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                        w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->part2.get(), gmg_fine_view(op));
@@ -777,6 +971,9 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     }
     hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1), 0, s, st, tol, bnorm, rel0, max_iter, mixed ? 1 : 0,
                        kReliableDelta);
+    // The fused FP64 iterations start from A p of their first direction.
+    if (!mixed && fuse_step4())
+        launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
     MYCEL_LAUNCH_CHECK();
 
     hipGraphExec_t graph;
