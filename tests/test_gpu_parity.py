@@ -18,9 +18,9 @@ import oracle as _O
 
 def O_forces(p, case):
     return _O.assemble_forces(p.n_nodes, case)
-from mycel_amd import (DensityFilter, FemOperator, InvalidOperationError, JacobiPreconditioner, OcOptimizer,
-                       OperationCanceledError, OptimizerOptions, PcgSolver, SensitivityAnalysis, StaticAnalysis,
-                       StaticAnalysisOptions, TopologyOptimizer)
+from mycel_amd import (DensityFilter, FemOperator, InvalidOperationError, JacobiPreconditioner,
+                       MultigridPreconditioner, OcOptimizer, OperationCanceledError, OptimizerOptions, PcgSolver,
+                       SensitivityAnalysis, StaticAnalysis, StaticAnalysisOptions, TopologyOptimizer)
 from mycel_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -284,6 +284,54 @@ def test_solver_special_cases(gpu):
             PcgSolver().Solve(op, np.ones(few.n_dofs), np.zeros(few.n_dofs))
     finally:
         op.close()
+
+
+def _step4_solves():
+    """FP64 Jacobi and multigrid solves on one operator: converged, and stopped by the iteration limit at an odd
+    and an even count (the fused step 4 alternates two p buffers)."""
+    p = W.cantilever_simp(30, 10, 10)
+    p.density = random_field(23, p.n_elem, 0.01, 1.0)
+    fm = p.fe_model()
+    b = np.where(fm.ConstrainedDofs.astype(bool), 0.0, fm.AssembleForces())
+    out = {}
+    op = FemOperator(fm)
+    try:
+        cases = (("jacobi", JacobiPreconditioner(), 1e-10, 20000), ("jacobi7", JacobiPreconditioner(), 1e-14, 7),
+                 ("gmg", MultigridPreconditioner(), 1e-10, 1000), ("gmg6", MultigridPreconditioner(), 1e-14, 6))
+        for name, pre, tol, it in cases:
+            u = np.zeros(p.n_dofs)
+            r = PcgSolver().Solve(op, b, u, pre, tol=tol, maxIterations=it)
+            out[name] = u
+            out[name + "_meta"] = np.array([r.Iterations, r.MessageCode, r.FinalResidual])
+    finally:
+        op.close()
+    return out
+
+
+def test_fused_step4_matches_separate_kernels(gpu, tmp_path):
+    """The FP64 iterations form p' = z + beta p inside the next A p launch (MYCEL_PCG_FUSE_P, on by default).
+    A child process with MYCEL_PCG_FUSE_P=0 runs the separate p update and K*u launches: the solutions,
+    iteration counts, messages and residuals are bit-identical."""
+    import os
+    import subprocess
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    dst = tmp_path / "separate.npz"
+    code = ("import sys; sys.path[:0] = [%r, %r]; import numpy as np, test_gpu_parity as T; "
+            "np.savez(%r, **T._step4_solves())" % (root, here, str(dst)))
+    env = dict(os.environ, MYCEL_PCG_FUSE_P="0")
+    print("[parity] fused step 4: child process with MYCEL_PCG_FUSE_P=0", flush=True)
+    subprocess.run([sys.executable, "-c", code], env=env, cwd=root, check=True, timeout=240)
+    fused = _step4_solves()
+    with np.load(dst) as sep:
+        for k in fused:
+            assert np.array_equal(fused[k], sep[k]), k
+    print(f"[parity] fused step 4: jacobi it={fused['jacobi_meta'][0]:.0f} gmg it={fused['gmg_meta'][0]:.0f}",
+          flush=True)
+    assert fused["jacobi_meta"][1] == 0 and fused["gmg_meta"][1] == 0
+    assert fused["jacobi7_meta"][:2].tolist() == [7, 7] and fused["gmg6_meta"][:2].tolist() == [6, 7]
 
 
 @pytest.mark.parametrize("case", ["lshape-r2.5", "cantilever-r1.5", "box-r2.0"])
