@@ -1003,53 +1003,59 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
         }
     }
     __syncthreads();
-    constexpr int RS = 2, CS = (kCoarseMaxDofs + 7) / 8;
-    const int ti = threadIdx.x >> 3, tj = threadIdx.x & 7;
+    // Gauss-Jordan inversion in registers: thread (ti, tj) holds rows ti + 64 s and columns tj + 16 c of the
+    // matrix padded with the identity to 64 RS x 16 CS (the padding stays the identity: no pivot reaches it, and
+    // the row and column slices that hold only padding for this n are skipped). Pivot k = 16 C + tk runs with C
+    // unrolled, so column k is register a[s][C] of lane tk of each 16-lane group (a lane shuffle fetches it) and
+    // row k is in slice s = C / 4. Per pivot, one barrier: the lanes holding row k write it to an LDS buffer
+    // (two, alternating with k). Same products and differences as the textbook in-place form: row k <- row k / p
+    // (its pivot entry 1/p); row i <- row i - a_ik (row k / p) (its column-k entry -a_ik / p).
+    constexpr int RS = (kCoarseMaxDofs + 63) / 64, CS = (kCoarseMaxDofs + 15) / 16;
+    static_assert(2 * 16 * CS <= 4 * kCoarseMaxDofs, "row buffers within the launch's LDS");
+    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15, lane = threadIdx.x & 63;
+    const int rs_n = (n + 63) >> 6, cs_n = (n + 15) >> 4;  // slices holding rows / columns < n
     double a[RS][CS];
 #pragma unroll
     for (int s = 0; s < RS; s++)
 #pragma unroll
         for (int c = 0; c < CS; c++) {
-            const int i = ti + 128 * s, j = tj + 8 * c;
-            a[s][c] = (i < n && j < n) ? A[i * n + j] : 0.0;
+            const int i = ti + 64 * s, j = tj + 16 * c;
+            a[s][c] = (i < n && j < n) ? A[i * n + j] : (i == j ? 1.0 : 0.0);
         }
-    __syncthreads();  // A's LDS now holds the broadcast buffers
-    double *colk = A, *rowk = A + 2 * kCoarseMaxDofs;
-    for (int k = 0; k < n; k++) {
-        const int p = k & 1, ck = k >> 3;
-        double *cb = colk + p * kCoarseMaxDofs, *rb = rowk + p * kCoarseMaxDofs;
-        if (tj == (k & 7)) {
+    __syncthreads();  // A's LDS now holds the row buffers
 #pragma unroll
-            for (int s = 0; s < RS; s++)
+    for (int C = 0; C < CS; C++) {
+        const int SK = C >> 2;  // slice of rows 16 C .. 16 C + 15
+        for (int tk = 0; tk < 16 && 16 * C + tk < n; tk++) {
+            const int k = 16 * C + tk;
+            double *rb = A + (k & 1) * (16 * CS);
+            const bool own = ti == (k & 63);  // this lane holds part of row k, in a[SK]
+            double col[RS];
+#pragma unroll
+            for (int s = 0; s < RS; s++) col[s] = __shfl(a[s][C], (lane & ~15) | tk, 64);
+            if (own) {
 #pragma unroll
                 for (int c = 0; c < CS; c++)
-                    if (c == ck && ti + 128 * s < n) cb[ti + 128 * s] = a[s][c];
-        }
-        __syncthreads();
-        const double piv = 1.0 / cb[k];
-#pragma unroll
-        for (int s = 0; s < RS; s++) {
-            if (ti + 128 * s != k) continue;
-#pragma unroll
-            for (int c = 0; c < CS; c++) {
-                const int j = tj + 8 * c;
-                if (j >= n) continue;
-                const double v = (j == k) ? piv : a[s][c] * piv;
-                a[s][c] = v;
-                rb[j] = v;
+                    if (c < cs_n) rb[tj + 16 * c] = a[SK][c];
             }
-        }
-        __syncthreads();
+            __syncthreads();
+            const double piv = 1.0 / rb[k];
+            const bool kc = tj == tk;  // this lane holds column k (register C)
+            double r[CS];
 #pragma unroll
-        for (int s = 0; s < RS; s++) {
-            const int i = ti + 128 * s;
-            if (i >= n || i == k) continue;
-            const double f = cb[i];
+            for (int c = 0; c < CS; c++) r[c] = c < cs_n ? rb[tj + 16 * c] * piv : 0.0;  // row k / p
+            if (kc) r[C] = piv;
 #pragma unroll
-            for (int c = 0; c < CS; c++) {
-                const int j = tj + 8 * c;
-                if (j >= n) continue;
-                a[s][c] = (j == k) ? -f * piv : a[s][c] - f * rb[j];
+            for (int s = 0; s < RS; s++) {
+                if (s >= rs_n) continue;
+#pragma unroll
+                for (int c = 0; c < CS; c++)
+                    if (c < cs_n) a[s][c] = a[s][c] - col[s] * r[c];
+                if (kc) a[s][C] = 0.0 - col[s] * piv;  // column k: -a_ik / p
+            }
+            if (own) {
+#pragma unroll
+                for (int c = 0; c < CS; c++) a[SK][c] = r[c];  // row k / p, its pivot entry 1/p
             }
         }
     }
@@ -1057,7 +1063,7 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
     for (int s = 0; s < RS; s++)
 #pragma unroll
         for (int c = 0; c < CS; c++) {
-            const int i = ti + 128 * s, j = tj + 8 * c;
+            const int i = ti + 64 * s, j = tj + 16 * c;
             if (i < n && j < n) ainv[j * n + i] = static_cast<float>(a[s][c]);  // column-major
         }
 }
