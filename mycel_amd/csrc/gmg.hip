@@ -1004,8 +1004,9 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
     }
     __syncthreads();
     // Gauss-Jordan inversion in registers: thread (ti, tj) holds rows ti + 64 s and columns tj + 16 c of the
-    // matrix padded with the identity to 64 RS x 16 CS (the padding stays the identity: no pivot reaches it, and
-    // the row and column slices that hold only padding for this n are skipped). Pivot k = 16 C + tk runs with C
+    // matrix padded with the identity to 64 RS x 16 CS (the padding stays the identity: no pivot reaches it; row
+    // slices that hold only padding for this n are skipped; padded columns of a pivot row are 0, so their
+    // updates subtract 0 and need no test). Pivot k = 16 C + tk runs with C
     // unrolled, so column k is register a[s][C] of lane tk of each 16-lane group (a lane shuffle fetches it) and
     // row k is in slice s = C / 4. Per pivot, one barrier: the lanes holding row k write it to an LDS buffer
     // (two, alternating with k). Same products and differences as the textbook in-place form: row k <- row k / p
@@ -1013,7 +1014,7 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
     constexpr int RS = (kCoarseMaxDofs + 63) / 64, CS = (kCoarseMaxDofs + 15) / 16;
     static_assert(2 * 16 * CS <= 4 * kCoarseMaxDofs, "row buffers within the launch's LDS");
     const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15, lane = threadIdx.x & 63;
-    const int rs_n = (n + 63) >> 6, cs_n = (n + 15) >> 4;  // slices holding rows / columns < n
+    const int rs_n = (n + 63) >> 6;  // row slices holding rows < n
     double a[RS][CS];
 #pragma unroll
     for (int s = 0; s < RS; s++)
@@ -1035,22 +1036,20 @@ __global__ __launch_bounds__(1024) void k_gmg_coarse_factor(Dims g, const float 
             for (int s = 0; s < RS; s++) col[s] = __shfl(a[s][C], (lane & ~15) | tk, 64);
             if (own) {
 #pragma unroll
-                for (int c = 0; c < CS; c++)
-                    if (c < cs_n) rb[tj + 16 * c] = a[SK][c];
+                for (int c = 0; c < CS; c++) rb[tj + 16 * c] = a[SK][c];
             }
             __syncthreads();
             const double piv = 1.0 / rb[k];
             const bool kc = tj == tk;  // this lane holds column k (register C)
             double r[CS];
 #pragma unroll
-            for (int c = 0; c < CS; c++) r[c] = c < cs_n ? rb[tj + 16 * c] * piv : 0.0;  // row k / p
+            for (int c = 0; c < CS; c++) r[c] = rb[tj + 16 * c] * piv;  // row k / p
             if (kc) r[C] = piv;
 #pragma unroll
             for (int s = 0; s < RS; s++) {
                 if (s >= rs_n) continue;
 #pragma unroll
-                for (int c = 0; c < CS; c++)
-                    if (c < cs_n) a[s][c] = a[s][c] - col[s] * r[c];
+                for (int c = 0; c < CS; c++) a[s][c] = a[s][c] - col[s] * r[c];
                 if (kc) a[s][C] = 0.0 - col[s] * piv;  // column k: -a_ik / p
             }
             if (own) {
