@@ -30,6 +30,7 @@
 // alpha and the x, r updates (+ r.r partials), the V-cycle (+ z.r and z.Ap partials), beta and p.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -85,6 +86,9 @@ struct PcgWork {
     const Gmg *graph_b_h = nullptr;
     uint64_t graph_b_gen = 0;
     int graph_b_R = 0;
+    // Iterations of the last converged solve in each FP64 mode (the Jacobi one, the multigrid one): the next
+    // solve in that mode launches batches covering 3/4 of them before its first state read.
+    int last_iters_64 = 0, last_iters_gmg = 0;
 };
 
 namespace {
@@ -1016,6 +1020,14 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     // Every batch advances the iteration count by at least one or halts the solve, so max_iter bounds
     // the number of batches (a guard against a logic error, not a limit).
     const long long max_batches = static_cast<long long>(max_iter) + 64;
+    // Batches launched back to back before the first state read (FP64 modes), none on the first solve: each
+    // state read is a host round trip that leaves the GPU idle. A batch that runs past the halt only recomputes
+    // values nothing reads (every kernel that writes x or r tests the halt flag); in the Jacobi mode such a
+    // batch's kernels exit at entry, in the multigrid mode its V-cycles still run, so that mode covers only half
+    // of the previous converged solve's iterations and the Jacobi mode three quarters.
+    int &last_iters = gmg ? w->last_iters_gmg : w->last_iters_64;
+    const long long prior = std::min(last_iters, max_iter);
+    const long long blind = mixed ? 0 : gmg ? prior / 2 / kGmgGraphIters : prior * 3 / 4 / kGraphIters;
     for (long long batch = 0;; batch++) {
         if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "PCG batch loop did not terminate");
         if (cancel && *cancel) {
@@ -1023,6 +1035,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             throw Error(MYCEL_E_CANCELED, "The operation was canceled.");
         }
         MYCEL_HIP(hipGraphLaunch(graph, s));
+        if (batch < blind) continue;
         read_state(w, s);
         const PcgState &h = *w->h_state;
         if (cb) cb(h.done ? h.iterations : h.last_iter, h.done ? h.final_rel : h.last_rel, user);
@@ -1044,6 +1057,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         }
     }
     const PcgState h = *w->h_state;
+    if (!mixed) last_iters = h.message == MYCEL_PCG_CONVERGED ? h.iterations : 0;
     res->iterations = h.iterations;
     res->final_residual = h.final_rel;
     res->message = h.message;
