@@ -101,7 +101,10 @@ typedef struct {
     int32_t fallback_iterations; /* FP64 iterations after an FP32 breakdown (mixed precision only) */
 } mycel_pcg_result;
 
-/* IProgress<PcgProgress> (Fem/PcgResult.cs:5). Called on the calling thread between iterations. */
+/* IProgress<PcgProgress> (Fem/PcgResult.cs:5). Called on the calling thread between iterations, once per batch
+ * of iterations the device ran without a host round trip (32 Jacobi or 4 multigrid iterations), with the
+ * latest iteration and relative residual; after a solve in the same FP64 mode converged, the first batches
+ * of the next solve (covering up to 3/4 of that solve's count) run without a report. */
 typedef void (*mycel_pcg_progress_cb)(int32_t iteration, double relative_residual, void *user);
 
 /* ---- library ------------------------------------------------------------------------------ */
