@@ -64,6 +64,7 @@ struct PcgWork {
     DevBuf<double> r, z, p, ap;              // FP64 iteration vectors (r / ap double as r64 / A x64 when mixed)
     DevBuf<double> p2;                       // FP64 iterations with the fused step 4: p of the odd ones
     DevBuf<float> r32, z32, p32, ap32, x32;  // FP32 iteration vectors of the mixed path
+    DevBuf<float> p32b;                      // its p of the odd iterations (fused step 4)
     DevBuf<double> part_pap, part2, part1;
     DevBuf<PcgState> state;
     PcgState *h_state = nullptr;             // pinned
@@ -420,25 +421,28 @@ __global__ __launch_bounds__(256) void k_fpcg_update_p(int n, int slot, PcgState
     }
 }
 
-// Step 4 (FP64; FLEX: the flexible form, else the Jacobi one) fused with the next iteration's A p: p' = z + beta p
-// is formed where the element-centric tile march reads x, and each tile writes A p', p' and the p'.A p' partials
-// of the nodes it owns (p' goes to a second buffer: other tiles still read p at their halo nodes). One launch and
-// one pass over p fewer than the p update followed by the K*u launch, with bit-identical values.
-template <bool FLEX>
+// Step 4 (FLEX: the flexible form, FP64; else the Jacobi one, FP64 or the FP32 iteration of the reliable-update
+// solver) fused with the next iteration's A p: p' = z + beta p is formed where the element-centric tile march
+// reads x, and each tile writes A p', p' and the p'.A p' partials of the nodes it owns (p' goes to a second
+// buffer: other tiles still read p at their halo nodes). One launch and one pass over p fewer than the p update
+// followed by the K*u launch, with bit-identical values.
+template <typename T, bool FLEX>
 struct PcgNextP {
-    const double *p, *z;
+    const T *p, *z;
     PcgState *st;
     const double *part_rr, *part_zr, *part_zap;  // Jacobi: part_rr holds the interleaved (r.r, r.z) partials
     int nb_vec, nb_z, slot;
     double beta;
+    T bt;        // beta in the vector type, as the separate p update rounds it
     Step4In in;  // loaded at kernel entry (step4_in)
     struct Raw {
-        double p, z;
+        T p, z;
     };
     __device__ Raw load(size_t at, uint32_t o, int d) const {
-        return {ld_off(p + at, o, 8 * d), ld_off(z + at, o, 8 * d)};
+        constexpr int sT = static_cast<int>(sizeof(T));
+        return {ld_off(p + at, o, sT * d), ld_off(z + at, o, sT * d)};
     }
-    __device__ double value(Raw r) const { return fmad(beta, r.p, r.z); }
+    __device__ T value(Raw r) const { return fmad(bt, r.p, r.z); }
     // Called once per wave after its first loads are issued, so the partials' round trip overlaps them; false
     // (the solve halted before or halts here) ends the wave's march.
     template <typename Fin>
@@ -446,14 +450,16 @@ struct PcgNextP {
         const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
         fin.go = FLEX ? fpcg_step4(st, in, slot, part_rr, nb_vec, part_zr, part_zap, nb_z, lead, beta)
                       : pcg_step4(st, in, slot, part_rr, nb_vec, lead, beta);
+        bt = static_cast<T>(beta);
         return fin.go;
     }
 };
 
 // Epilogue: the projected operator's (fixed DOFs: out = p'), stores of A p' and p' unless the solve halted.
+template <typename T>
 struct PcgNextFin {
     const uint8_t *flags;
-    double *ap, *p_next;
+    T *ap, *p_next;
     double dot = 0.0;
     bool go = false;
     struct Pre {
@@ -465,7 +471,7 @@ struct PcgNextFin {
         for (int d = 0; d < 3; d++) q.fl[d] = flags[3 * n + d];
         return q;
     }
-    __device__ void operator()(int n, double (&out)[3], const double (&xc)[3], const Pre &q) {
+    __device__ void operator()(int n, T (&out)[3], const T (&xc)[3], const Pre &q) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             if (q.fl[d] & kFlagFixed) out[d] = xc[d];
@@ -473,24 +479,24 @@ struct PcgNextFin {
                 ap[3 * n + d] = out[d];
                 p_next[3 * n + d] = xc[d];
             }
-            dot += xc[d] * out[d];
+            dot += static_cast<double>(xc[d]) * static_cast<double>(out[d]);
         }
     }
 };
 
-template <bool FLEX, int KZ>
-__global__ __launch_bounds__(256) void k_pcg_step4_ap(Dims g, const KeMono<double> km, const double *__restrict__ ep,
-                                                      const uint8_t *__restrict__ flags, PcgNextP<FLEX> src,
-                                                      double *__restrict__ ap, double *__restrict__ p_next,
+template <typename T, bool FLEX, int KZ>
+__global__ __launch_bounds__(256) void k_pcg_step4_ap(Dims g, const KeMono<T> km, const T *__restrict__ ep,
+                                                      const uint8_t *__restrict__ flags, PcgNextP<T, FLEX> src,
+                                                      T *__restrict__ ap, T *__restrict__ p_next,
                                                       double *__restrict__ part_pap, int n_tiles) {
     __shared__ double red[4];
     src.in = step4_in(src.st, src.slot);
     const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
-    PcgNextFin fin{flags, ap, p_next};
+    PcgNextFin<T> fin{flags, ap, p_next};
     // A halted solve ends each wave's march at its ready() (every wave still joins the partial sum's barriers).
     // Waves may see the halt flag differently when this launch's lead thread sets it: the solve is then over and
     // nothing reads this launch's outputs.
-    if (tile < n_tiles) ec_tile_march<double, KZ>(g, km, ep, src, tile, fin);
+    if (tile < n_tiles) ec_tile_march<T, KZ>(g, km, ep, src, tile, fin);
     const double sum = block_sum256(fin.dot, red);
     if (threadIdx.x == 0 && fin.go) part_pap[blockIdx.x] = sum;  // wave 0 of every block has a tile
 }
@@ -641,7 +647,9 @@ __global__ __launch_bounds__(256) void k_mixed_resume(PcgState *__restrict__ st,
 }
 
 // Reliable update, part 4: p = z + beta p with the beta of k_mixed_resume.
-__global__ __launch_bounds__(256) void k_mixed_update_p(int n, const PcgState *__restrict__ st, float *__restrict__ p,
+// p_out = z + beta p_in (p_out may be p_in).
+__global__ __launch_bounds__(256) void k_mixed_update_p(int n, const PcgState *__restrict__ st,
+                                                        const float *p_in, float *p_out,
                                                         const float *__restrict__ z) {
     if (st->done) return;
     const float b = static_cast<float>(st->beta);
@@ -649,7 +657,7 @@ __global__ __launch_bounds__(256) void k_mixed_update_p(int n, const PcgState *_
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
-        if (i < n) p[i] = __fmaf_rn(b, p[i], z[i]);
+        if (i < n) p_out[i] = __fmaf_rn(b, p_in[i], z[i]);
     }
 }
 
@@ -719,11 +727,12 @@ void ensure_mixed(PcgWork *w) {
     w->r32.resize(w->n);
     w->z32.resize(w->n);
     w->p32.resize(w->n);
+    w->p32b.resize(w->n);
     w->ap32.resize(w->n);
     w->x32.resize(w->n);
 }
 
-// Step 4 fused with the next A p (k_pcg_step4_ap) in the FP64 iterations: on by default; MYCEL_PCG_FUSE_P=0
+// Step 4 fused with the next A p (k_pcg_step4_ap) in every solver mode: on by default; MYCEL_PCG_FUSE_P=0
 // separates them (the p update, then the K*u launch). Values are identical either way.
 bool fuse_step4() {
     static const bool on = [] {
@@ -733,37 +742,42 @@ bool fuse_step4() {
     return on;
 }
 
-// p of FP64 iteration t (fused step 4): p for t even, p2 for t odd; step 4 of iteration t writes the other.
+// p of iteration t (fused step 4): p (p32) for t even, p2 (p32b) for t odd; step 4 of iteration t writes the
+// other. A batch starts at t = 0, and so does the FP32 iteration after a reliable update.
 static_assert(kGraphIters % 2 == 0 && kGmgGraphIters % 2 == 0, "fused step 4: p buffers alternate per iteration");
 
-template <bool FLEX>
-void launch_step4_ap(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
+template <typename T, bool FLEX>
+void launch_step4_ap(mycel_op *op, const KeMono<T> &km, const T *ep, const T *p, const T *z, T *p_next, T *ap,
+                     PcgWork *w, int slot, hipStream_t s) {
     const int kz = ku_element_layers(op);
     const int nt = ec_tiles(op->d, kz);
     // update_xr reads the partials of this launch as those of the automatic K*u form
     if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "fused step 4: partial count mismatch");
-    const double *p = slot ? w->p2.get() : w->p.get();
-    double *p_next = slot ? w->p.get() : w->p2.get();
-    const PcgNextP<FLEX> src{p, w->z.get(), w->state.get(), w->part2.get(), w->part_zr.get(), w->part_zap.get(),
-                             w->nb_vec, w->nb_z, slot, 0.0};
+    const PcgNextP<T, FLEX> src{p, z, w->state.get(), w->part2.get(), w->part_zr.get(), w->part_zap.get(),
+                                w->nb_vec, w->nb_z, slot, 0.0, T(0)};
     const dim3 grid(cdiv(nt, 4)), b256(256);
-    const double *ep = op->d_ep.get();
     const uint8_t *fl = op->d_flags.get();
-    double *ap = w->ap.get(), *part = w->part_pap.get();
+    double *part = w->part_pap.get();
     if (kz == 2)
-        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 2>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
-                           part, nt);
+        hipLaunchKernelGGL((k_pcg_step4_ap<T, FLEX, 2>), grid, b256, 0, s, op->d, km, ep, fl, src, ap, p_next, part,
+                           nt);
     else if (kz == 3)
-        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 3>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
-                           part, nt);
+        hipLaunchKernelGGL((k_pcg_step4_ap<T, FLEX, 3>), grid, b256, 0, s, op->d, km, ep, fl, src, ap, p_next, part,
+                           nt);
     else if (kz == 4)
-        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 4>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
-                           part, nt);
+        hipLaunchKernelGGL((k_pcg_step4_ap<T, FLEX, 4>), grid, b256, 0, s, op->d, km, ep, fl, src, ap, p_next, part,
+                           nt);
     else if (kz == 8)
-        hipLaunchKernelGGL((k_pcg_step4_ap<FLEX, 8>), grid, b256, 0, s, op->d, op->km64, ep, fl, src, ap, p_next,
-                           part, nt);
+        hipLaunchKernelGGL((k_pcg_step4_ap<T, FLEX, 8>), grid, b256, 0, s, op->d, km, ep, fl, src, ap, p_next, part,
+                           nt);
     else
         throw Error(MYCEL_E_INVALID_OPERATION, "fused step 4: unsupported tile depth");
+}
+
+template <bool FLEX>
+void launch_step4_ap64(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
+    launch_step4_ap<double, FLEX>(op, op->km64, op->d_ep.get(), slot ? w->p2.get() : w->p.get(), w->z.get(),
+                                  slot ? w->p.get() : w->p2.get(), w->ap.get(), w, slot, s);
 }
 
 // One FP64 Jacobi iteration. Fused: its A p was formed by the step 4 before it (or, for the first, before the
@@ -774,7 +788,7 @@ void enqueue_iteration64(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStr
         hipLaunchKernelGGL(k_pcg_update_xr<double>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st,
                            w->part_pap.get(), w->nb_mv, d_x, w->r.get(), slot ? w->p2.get() : w->p.get(), w->ap.get(),
                            w->z.get(), op->d_inv_diag.get(), w->part2.get());
-        launch_step4_ap<false>(op, w, slot, s);
+        launch_step4_ap64<false>(op, w, slot, s);
         return;
     }
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
@@ -785,8 +799,19 @@ void enqueue_iteration64(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStr
                        w->nb_vec, w->p.get(), w->z.get());
 }
 
+// One FP32 iteration of the reliable-update solver. Fused: its A p was formed by the step 4 before it (or, for
+// the first after the start or a reliable update, by a plain K*u launch).
 void enqueue_iteration32(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
     PcgState *st = w->state.get();
+    if (fuse_step4()) {
+        float *p = slot ? w->p32b.get() : w->p32.get(), *p_next = slot ? w->p32.get() : w->p32b.get();
+        hipLaunchKernelGGL(k_pcg_update_xr<float>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st,
+                           w->part_pap.get(), w->nb_mv, w->x32.get(), w->r32.get(), p, w->ap32.get(), w->z32.get(),
+                           op->d_inv_diag32.get(), w->part2.get());
+        launch_step4_ap<float, false>(op, op->km32, op->d_ep32.get(), p, w->z32.get(), p_next, w->ap32.get(), w, slot,
+                                      s);
+        return;
+    }
     launch_apply_k_f32(op, w->p32.get(), w->ap32.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
     hipLaunchKernelGGL(k_pcg_update_xr<float>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                        w->nb_mv, w->x32.get(), w->r32.get(), w->p32.get(), w->ap32.get(), w->z32.get(),
@@ -812,7 +837,7 @@ void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipS
                            w->nb_mv, d_x, w->r.get(), slot ? w->p2.get() : w->p.get(), w->ap.get(), w->part2.get(),
                            gmg_fine_view(op));
         gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true);
-        launch_step4_ap<true>(op, w, slot, s);
+        launch_step4_ap64<true>(op, w, slot, s);
         return;
     }
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
@@ -975,9 +1000,13 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     }
     hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1), 0, s, st, tol, bnorm, rel0, max_iter, mixed ? 1 : 0,
                        kReliableDelta);
-    // The fused FP64 iterations start from A p of their first direction.
-    if (!mixed && fuse_step4())
-        launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
+    // The fused iterations start from A p of their first direction.
+    if (fuse_step4()) {
+        if (mixed)
+            launch_apply_k_f32(op, w->p32.get(), w->ap32.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
+        else
+            launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
+    }
     MYCEL_LAUNCH_CHECK();
 
     hipGraphExec_t graph;
@@ -1050,7 +1079,11 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
                                w->part1.get() + w->nb_vec);
             hipLaunchKernelGGL(k_mixed_resume, dim3(1), b256, 0, s, st, w->part1.get(), w->part1.get() + w->nb_vec,
                                w->nb_vec);
-            hipLaunchKernelGGL(k_mixed_update_p, gv, b256, 0, s, n, st, w->p32.get(), w->z32.get());
+            const float *p_in = fuse_step4() && w->h_state->pause_slot ? w->p32b.get() : w->p32.get();
+            hipLaunchKernelGGL(k_mixed_update_p, gv, b256, 0, s, n, st, p_in, w->p32.get(), w->z32.get());
+            if (fuse_step4())
+                launch_apply_k_f32(op, w->p32.get(), w->ap32.get(), kApplyAProjected, s, w->part_pap.get(),
+                                   &st->halt);
             MYCEL_LAUNCH_CHECK();
             read_state(w, s);
             if (w->h_state->done) break;

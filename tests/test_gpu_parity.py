@@ -287,8 +287,11 @@ def test_solver_special_cases(gpu):
 
 
 def _step4_solves():
-    """FP64 Jacobi and multigrid solves on one operator: converged, and stopped by the iteration limit at an odd
-    and an even count (the fused step 4 alternates two p buffers)."""
+    """FP64 Jacobi, multigrid and mixed FP32 (reliable updates) solves on one operator: converged, and stopped by
+    the iteration limit at an odd and an even count (the fused step 4 alternates two p buffers; the mixed solver
+    re-enters it after each reliable update)."""
+    import mycel_amd as M
+
     p = W.cantilever_simp(30, 10, 10)
     p.density = random_field(23, p.n_elem, 0.01, 1.0)
     fm = p.fe_model()
@@ -296,13 +299,18 @@ def _step4_solves():
     out = {}
     op = FemOperator(fm)
     try:
-        cases = (("jacobi", JacobiPreconditioner(), 1e-10, 20000), ("jacobi7", JacobiPreconditioner(), 1e-14, 7),
-                 ("gmg", MultigridPreconditioner(), 1e-10, 1000), ("gmg6", MultigridPreconditioner(), 1e-14, 6))
-        for name, pre, tol, it in cases:
+        fp64, mixed = M.PRECISION_FP64, M.PRECISION_MIXED_FP32
+        cases = (("jacobi", JacobiPreconditioner(), 1e-10, 20000, fp64),
+                 ("jacobi7", JacobiPreconditioner(), 1e-14, 7, fp64),
+                 ("gmg", MultigridPreconditioner(), 1e-10, 1000, None),
+                 ("gmg6", MultigridPreconditioner(), 1e-14, 6, None),
+                 ("mixed", JacobiPreconditioner(), 1e-9, 20000, mixed))
+        for name, pre, tol, it, prec in cases:
             u = np.zeros(p.n_dofs)
-            r = PcgSolver().Solve(op, b, u, pre, tol=tol, maxIterations=it)
+            kw = {} if prec is None else {"precision": prec}
+            r = PcgSolver().Solve(op, b, u, pre, tol=tol, maxIterations=it, **kw)
             out[name] = u
-            out[name + "_meta"] = np.array([r.Iterations, r.MessageCode, r.FinalResidual])
+            out[name + "_meta"] = np.array([r.Iterations, r.MessageCode, r.FinalResidual, r.OuterCorrections])
     finally:
         op.close()
     return out
@@ -328,9 +336,10 @@ def test_fused_step4_matches_separate_kernels(gpu, tmp_path):
     with np.load(dst) as sep:
         for k in fused:
             assert np.array_equal(fused[k], sep[k]), k
-    print(f"[parity] fused step 4: jacobi it={fused['jacobi_meta'][0]:.0f} gmg it={fused['gmg_meta'][0]:.0f}",
-          flush=True)
-    assert fused["jacobi_meta"][1] == 0 and fused["gmg_meta"][1] == 0
+    print(f"[parity] fused step 4: jacobi it={fused['jacobi_meta'][0]:.0f} gmg it={fused['gmg_meta'][0]:.0f} "
+          f"mixed it={fused['mixed_meta'][0]:.0f} reliable updates={fused['mixed_meta'][3]:.0f}", flush=True)
+    assert fused["jacobi_meta"][1] == 0 and fused["gmg_meta"][1] == 0 and fused["mixed_meta"][1] == 0
+    assert fused["mixed_meta"][3] >= 2  # the mixed solve went through reliable updates
     assert fused["jacobi7_meta"][:2].tolist() == [7, 7] and fused["gmg6_meta"][:2].tolist() == [6, 7]
 
 
