@@ -649,6 +649,11 @@ struct TailArgs {
     Dims pro_d;
     float theta_inv, cd1, cr1;
     int count;
+    // The outer iteration's halt flag of right-hand side k at halt[k * halt_stride], or null. A halted solve's
+    // tail returns after staging: the V-cycles a batch runs past the halt compute values nothing reads, and the
+    // flag cannot change while a V-cycle runs (only the outer-iteration kernels around it write it).
+    const int *halt;
+    int halt_stride;
     TailLevel lv[kMaxTail];
 };
 
@@ -693,6 +698,7 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
             const int i = threadIdx.x + u * kTailThreads;
             va[u] = t.ainv[i < nn ? i : 0];
         }
+        const int halted = t.halt ? t.halt[blockIdx.x * t.halt_stride] : 0;
 #pragma unroll
         for (int sg = 0; sg < kTailSegs; sg++) {
             const TailSeg g = t.seg[sg];
@@ -709,6 +715,7 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
             const int i = threadIdx.x + u * kTailThreads;
             if (i < nn) al[i] = va[u];
         }
+        if (halted) return;  // the same value in every thread of the workgroup
     }
     __syncthreads();
     // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
@@ -1245,7 +1252,8 @@ GmgFineView gmg_fine_view(mycel_op *op) {
 // z = M(r): one V-cycle. Also writes per-block partials of z.r and z.ap (gmg_partials(op) of each).
 // fine_ready: the caller has already written the finest level's b, x = d = D^-1 b / theta (gmg_fine_view).
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
-                        double *part_zap, hipStream_t s, bool fine_ready, int n_rhs) {
+                        double *part_zap, hipStream_t s, bool fine_ready, int n_rhs, const int *halt,
+                        int halt_stride) {
     Gmg *g = op->gmg;
     const dim3 b256(256);
     if (n_rhs < 1 || n_rhs > g->rhs_cap) throw Error(MYCEL_E_ARGUMENT, "multigrid: RHS count beyond the reserved");
@@ -1350,6 +1358,8 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     ta.theta_inv = g->theta_inv;
     ta.cd1 = g->cd[1];
     ta.cr1 = g->cr[1];
+    ta.halt = halt;
+    ta.halt_stride = halt_stride;
 
     GmgLevel &F = g->lv[0];
     if (!fine_ready)

@@ -217,9 +217,11 @@ void gmg_prepare(mycel_op *op, hipStream_t s);
 // z = M(r) (one FP32 V-cycle) plus per-block partials of z.r and z.ap, gmg_partials(op) of each.
 // fine_ready: the finest level's b, x = d = D^-1 b / theta were written by the caller (gmg_fine_view).
 // n_rhs: right-hand sides in one pass (grid dimension y; vectors of RHS k at k * n_dofs of each level;
-// partials of RHS k at k * gmg_partials(op)); at most what gmg_reserve provided.
+// partials of RHS k at k * gmg_partials(op)); at most what gmg_reserve provided. halt: the outer iteration's
+// halt flag of RHS k at halt[k * halt_stride], or null; a halted RHS skips the coarse-level tail.
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
-                        double *part_zap, hipStream_t s, bool fine_ready = false, int n_rhs = 1);
+                        double *part_zap, hipStream_t s, bool fine_ready = false, int n_rhs = 1,
+                        const int *halt = nullptr, int halt_stride = 0);
 // Sizes the level vectors for n_rhs right-hand sides (after gmg_prepare); bumps gmg_generation when they move.
 void gmg_reserve(mycel_op *op, int n_rhs);
 uint64_t gmg_generation(const mycel_op *op);

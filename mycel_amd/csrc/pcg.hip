@@ -836,14 +836,16 @@ void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipS
         hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                            w->nb_mv, d_x, w->r.get(), slot ? w->p2.get() : w->p.get(), w->ap.get(), w->part2.get(),
                            gmg_fine_view(op));
-        gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true);
+        gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true, 1,
+                           &st->halt);
         launch_step4_ap64<true>(op, w, slot, s);
         return;
     }
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
     hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                        w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->part2.get(), gmg_fine_view(op));
-    gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true);
+    gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true, 1,
+                       &st->halt);
     hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(), w->nb_vec,
                        w->part_zr.get(), w->part_zap.get(), w->nb_z, w->p.get(), w->z.get());
 }
@@ -882,7 +884,7 @@ void enqueue_iteration_gmg_batch(mycel_op *op, PcgWork *w, double *d_X, int R, i
                        w->bpart_pap.get(), nb_node, d_X, w->br.get(), w->bp.get(), w->bap.get(), w->bpart_rr.get(),
                        gmg_fine_view(op));
     gmg_enqueue_vcycle(op, w->br.get(), w->bap.get(), w->bz.get(), w->bpart_zr.get(), w->bpart_zap.get(), s, true,
-                       R);
+                       R, &st->halt, words);
     hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec, R), dim3(256), 0, s, w->n, slot, st, w->bpart_rr.get(),
                        w->nb_vec, w->bpart_zr.get(), w->bpart_zap.get(), w->nb_z, w->bp.get(), w->bz.get());
 }
