@@ -350,31 +350,61 @@ struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.
     }
 };
 
-template <int KZ>
+// The finest level's iterate as the tile march's source, with the outer iteration's halt flag of this launch's
+// right-hand side (HALT instantiations, launched for several right-hand sides): a converged case's V-cycles end
+// each wave's march before its arithmetic while the other cases run; nothing reads their outputs, and the flag
+// cannot change while a V-cycle runs. A single solve launches the instantiations without the test: there it
+// cost more than the iterations past convergence it skips.
+struct FineX {
+    const float *x;
+    int halted;
+    using Raw = float;
+    __device__ Raw load(size_t at, uint32_t o, int d) const { return ld_off(x + at, o, d * 4); }
+    __device__ float value(Raw r) const { return r; }
+    template <typename Fin>
+    __device__ bool ready(Fin &) const {
+        return !halted;
+    }
+};
+
+template <bool HALT, int KZ, typename Fin>
+__device__ __forceinline__ void fine_march(const Dims &g, const KeMono<float> &km, const float *ep, const float *x,
+                                           const int *halt, int halt_stride, int tile, Fin &fin) {
+    if constexpr (HALT) {
+        FineX src{x, halt[blockIdx.y * halt_stride]};
+        ec_tile_march<float, KZ>(g, km, ep, src, tile, fin);
+    } else {
+        ec_tile_run<float, false, KZ>(g, km, ep, x, nullptr, tile, fin);
+    }
+}
+
+template <int KZ, bool HALT>
 __global__ __launch_bounds__(256) void k_gmg_cheb_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
                                                      const float *__restrict__ x, const float *__restrict__ b,
                                                      float *__restrict__ d, const float *__restrict__ inv, float cd,
-                                                     float cr, float *__restrict__ x_out, int n_tiles) {
+                                                     float cr, float *__restrict__ x_out, int n_tiles,
+                                                     const int *__restrict__ halt, int halt_stride) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     const int tile = ec_wave_tile();
     if (tile >= n_tiles) return;  // wave-uniform; no barrier follows
     ChebFin fin{b + o, inv, d + o, x_out + o, cd, cr};
-    ec_tile_run<float, false, KZ>(g, km, ep, x + o, nullptr, tile, fin);
+    fine_march<HALT, KZ>(g, km, ep, x + o, halt, halt_stride, tile, fin);
 }
 
-template <int KZ>
+template <int KZ, bool HALT>
 __global__ __launch_bounds__(256) void k_gmg_residual_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
                                                          const float *__restrict__ x, const float *__restrict__ b,
                                                          const float *__restrict__ inv, float *__restrict__ r,
-                                                         int n_tiles) {
+                                                         int n_tiles, const int *__restrict__ halt,
+                                                         int halt_stride) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     const int tile = ec_wave_tile();
     if (tile >= n_tiles) return;
     ResidualFin fin{b + o, inv, r + o};
-    ec_tile_run<float, false, KZ>(g, km, ep, x + o, nullptr, tile, fin);
+    fine_march<HALT, KZ>(g, km, ep, x + o, halt, halt_stride, tile, fin);
 }
 
-template <int KZ>
+template <int KZ, bool HALT>
 __global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
                                                            const float *__restrict__ x, const float *__restrict__ b,
                                                            const float *__restrict__ d, const float *__restrict__ inv,
@@ -382,12 +412,13 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float>
                                                            const double *__restrict__ r64,
                                                            const double *__restrict__ ap64,
                                                            double *__restrict__ part_zr, double *__restrict__ part_zap,
-                                                           int n_tiles) {
+                                                           int n_tiles, const int *__restrict__ halt,
+                                                           int halt_stride) {
     __shared__ double red[8];
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     FinalFin fin{b + o, d + o, inv, z64 + o, r64 + o, ap64 + o, cd, cr};
     const int tile = ec_wave_tile();
-    if (tile < n_tiles) ec_tile_run<float, false, KZ>(g, km, ep, x + o, nullptr, tile, fin);
+    if (tile < n_tiles) fine_march<HALT, KZ>(g, km, ep, x + o, halt, halt_stride, tile, fin);
     double zr = fin.zr, zap = fin.zap;
     block_sum256x2(zr, zap, red);
     if (threadIdx.x == 0) {
@@ -1266,6 +1297,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         nxt[l] = g->lv[l].x2.get();
     }
     const int fkz = fine_ec_kz(op);  // finest level: element-centric kernels with fkz layers per tile (0: gathers)
+    const bool fh = n_rhs > 1 && halt;  // the finest level's kernels test the halt flag (several right-hand sides)
     const int f_tiles = fkz ? ec_tiles(g->lv[0].d, fkz) : 0;
     const dim3 f_grid(cdiv(f_tiles, 4), R);
     // Element-centric kernels on the finest level only: on the coarse levels, whose few waves each march a
@@ -1278,17 +1310,21 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             const int nt = ec_tiles(L.d, kz);
             const dim3 grid(cdiv(nt, 4), R);
             if (kz == 2)
-                hipLaunchKernelGGL(k_gmg_cheb_ec<2>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
+                hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<2, true> : k_gmg_cheb_ec<2, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
             else if (kz == 3)
-                hipLaunchKernelGGL(k_gmg_cheb_ec<3>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
+                hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<3, true> : k_gmg_cheb_ec<3, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
             else if (kz == 4)
-                hipLaunchKernelGGL(k_gmg_cheb_ec<4>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
+                hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<4, true> : k_gmg_cheb_ec<4, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
             else
-                hipLaunchKernelGGL(k_gmg_cheb_ec<8>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt);
+                hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<8, true> : k_gmg_cheb_ec<8, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
         } else if (split && l > 0)
             hipLaunchKernelGGL(k_gmg_cheb_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                                L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
@@ -1378,17 +1414,21 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             const int nt = ec_tiles(L.d, kz);
             const dim3 grid(cdiv(nt, 4), R);
             if (kz == 2)
-                hipLaunchKernelGGL(k_gmg_residual_ec<2>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt);
+                hipLaunchKernelGGL((fh ? k_gmg_residual_ec<2, true> : k_gmg_residual_ec<2, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
             else if (kz == 3)
-                hipLaunchKernelGGL(k_gmg_residual_ec<3>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt);
+                hipLaunchKernelGGL((fh ? k_gmg_residual_ec<3, true> : k_gmg_residual_ec<3, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
             else if (kz == 4)
-                hipLaunchKernelGGL(k_gmg_residual_ec<4>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt);
+                hipLaunchKernelGGL((fh ? k_gmg_residual_ec<4, true> : k_gmg_residual_ec<4, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
             else
-                hipLaunchKernelGGL(k_gmg_residual_ec<8>, grid, b256, 0, s, L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt);
+                hipLaunchKernelGGL((fh ? k_gmg_residual_ec<8, true> : k_gmg_residual_ec<8, false>), grid, b256, 0, s,
+                                   L.d, L.km, L.ep, cur[l], L.b.get(),
+                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
         } else if (split)
             hipLaunchKernelGGL(k_gmg_residual_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                                L.inv.get(), L.r.get());
@@ -1420,17 +1460,25 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     const int fd = g->fine_degree;
     const float cd = top == 0 ? 0.0f : g->cd[fd - 1], cr = top == 0 ? 0.0f : g->cr[fd - 1];
     if (fkz == 2)
-        hipLaunchKernelGGL(k_gmg_cheb_final_ec<2>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+        hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<2, true> : k_gmg_cheb_final_ec<2, false>), f_grid, b256, 0, s,
+                           F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           halt_stride);
     else if (fkz == 3)
-        hipLaunchKernelGGL(k_gmg_cheb_final_ec<3>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+        hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<3, true> : k_gmg_cheb_final_ec<3, false>), f_grid, b256, 0, s,
+                           F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           halt_stride);
     else if (fkz == 4)
-        hipLaunchKernelGGL(k_gmg_cheb_final_ec<4>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+        hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<4, true> : k_gmg_cheb_final_ec<4, false>), f_grid, b256, 0, s,
+                           F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           halt_stride);
     else if (fkz == 8)
-        hipLaunchKernelGGL(k_gmg_cheb_final_ec<8>, f_grid, b256, 0, s, F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles);
+        hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<8, true> : k_gmg_cheb_final_ec<8, false>), f_grid, b256, 0, s,
+                           F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
+                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           halt_stride);
     else
         hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
                            F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
