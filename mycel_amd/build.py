@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmycel_hip.so")
 
 
-def build(force: bool = True, jobs: int = 4) -> str:
+def build(force: bool = True, jobs: int = 6) -> str:
     cmd = ["make", "-C", CSRC, f"-j{jobs}"]
     if force:
         cmd.insert(1, "-B")

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_oc_step(OcArgs a, int slot, OcState *__
     const int pending_was = S.pending;
     S.pending = kOcPendNone;
     double eval_l = 0.0;
-    // OcOptimizer.cs:198-247, advanced until the next volume evaluation is needed.
+    // OcOptimizer.cs:51-100, advanced until the next volume evaluation is needed.
     if (S.phase == kOcPhaseLower) {
         if (S.v1 < S.target && S.l1 > 1e-50 && S.steps++ < 50) {
             S.l2 = S.l1;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_oc_step(OcArgs a, int slot, OcState *__
         }
     }
     if (S.phase == kOcPhaseDone) {
-        // Final densities with lMid (OcOptimizer.cs:250-276); rho_new may alias rho.
+        // Final densities with lMid (OcOptimizer.cs:102-129); rho_new may alias rho.
         const int base = blockIdx.x * 1024 + threadIdx.x;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_oc_step(OcArgs a, int slot, OcState *__
     }
 }
 
-// First launch: target volume, initial bracket (OcOptimizer.cs:176-196), volumes at l1 and l2.
+// First launch: target volume, initial bracket (OcOptimizer.cs:29-49), volumes at l1 and l2.
 __global__ __launch_bounds__(256) void k_oc_begin(OcArgs a, const double *__restrict__ spart, int nbs, double vf,
                                                   double lambda_guess, OcState *__restrict__ states,
                                                   double *__restrict__ parts, OcResult *__restrict__ res) {
