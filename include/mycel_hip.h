@@ -285,7 +285,8 @@ typedef struct {
     int32_t canceled;
 } mycel_simp_result;
 
-/* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407), Heaviside off.
+/* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407); Heaviside projection and p-continuation
+ * as the options select them.
  * f: assembled forces; passive_mask_box: optional OptimizerOptions.PassiveMask; patch_nodes: the
  * load-patch nodes (for ProtectBoundaryHinges); initial_density: optional float32, box- or
  * active-sized; warm_u: optional box DOFs. rho_out / rho_bar_out (active) and u_out (box DOFs)
