@@ -226,6 +226,20 @@ MYCEL_API int32_t mycel_oc_update(const double *rho, const double *dc, const dou
                         double rho_min, double lambda_guess, int32_t max_bisect, double volume_tol,
                         double *lambda_out);
 
+/* No counterpart in the reference: OcOptimizer.Update on device pointers for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It allocates
+ * nothing: its scratch (the bisection state and the partial volume sums) is one fixed device array of
+ * 640 KiB + 256 B on each device the library runs on, held until the process ends, which bounds n to
+ * MYCEL_OC_NOALLOC_MAX_ELEMENTS (larger: MYCEL_E_OUT_OF_RANGE). Every call on a device shares that array, so
+ * calls that overlap on one device race. d_passive may be NULL; d_rho_new may alias d_rho. Synchronous on
+ * `stream` (a hipStream_t as void*, NULL = the null stream). Results are identical to mycel_oc_update's. */
+#define MYCEL_OC_NOALLOC_MAX_ELEMENTS 16777216
+MYCEL_API int32_t mycel_oc_update_noalloc_d(const double *d_rho, const double *d_dc, const double *d_s,
+                                            const uint8_t *d_passive, double *d_rho_new, int32_t n,
+                                            double volume_fraction, double move_limit, double rho_min,
+                                            double lambda_guess, int32_t max_bisect, double volume_tol, void *stream,
+                                            double *lambda_out);
+
 /* ---- TopologyOptimizer (Optimization/TopologyOptimizer.cs) -------------------------------- */
 /* OptimizerOptions (Optimization/OptimizerOptions.cs:5-26): the fields the SIMP loop reads.
  * The material's Emin comes from the operator (FeMaterial.EminRatio), as in the reference. */

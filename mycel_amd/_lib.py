@@ -23,6 +23,8 @@ E_CANCELED = 6
 E_DEVICE = 7
 E_OUT_OF_MEMORY = 8
 
+OC_NOALLOC_MAX_ELEMENTS = 16777216
+
 PRECISION_FP64 = 0
 PRECISION_MIXED_FP32 = 1
 PRECISION_MIXED_BF16 = 2
@@ -192,6 +194,7 @@ SIGNATURES = {
     "mycel_filter_apply_adjoint": (_i, [_vp, _vp, _vp]),
     "mycel_sensitivities": (_i, [_vp, _vp, _vp, _vp, _d]),
     "mycel_oc_update": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _d, C.POINTER(_d)]),
+    "mycel_oc_update_noalloc_d": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i, _d, _vp, C.POINTER(_d)]),
     "mycel_simp_options_default": (None, [C.POINTER(SimpOptionsC)]),
     "mycel_op_set_matvec_columns": (_i, [_vp, _i]),
     "mycel_op_set_matvec_form": (_i, [_vp, _i, _i]),

@@ -891,6 +891,21 @@ int32_t mycel_oc_update(const double *rho, const double *dc, const double *sv, c
     });
 }
 
+int32_t mycel_oc_update_noalloc_d(const double *d_rho, const double *d_dc, const double *d_s, const uint8_t *d_passive,
+                                  double *d_rho_new, int32_t n, double vf, double move, double rho_min,
+                                  double lambda_guess, int32_t max_bisect, double vol_tol, void *stream,
+                                  double *lambda_out) {
+    return guarded([&] {
+        require(d_rho && d_dc && d_s && d_rho_new && lambda_out, MYCEL_E_ARGUMENT_NULL, "null argument");
+        require(n >= 0, MYCEL_E_ARGUMENT, "n");
+        require(n <= MYCEL_OC_NOALLOC_MAX_ELEMENTS, MYCEL_E_OUT_OF_RANGE,
+                "n exceeds MYCEL_OC_NOALLOC_MAX_ELEMENTS, the no-allocation OC update's limit");
+        MYCEL_HIP(hipSetDevice(g_device));
+        *lambda_out = oc_update_noalloc(d_rho, d_dc, d_s, d_passive, d_rho_new, n, vf, move, rho_min, lambda_guess,
+                                        max_bisect, vol_tol, static_cast<hipStream_t>(stream));
+    });
+}
+
 void mycel_simp_options_default(mycel_simp_options *o) {
     if (!o) return;
     *o = mycel_simp_options{};

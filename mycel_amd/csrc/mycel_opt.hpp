@@ -46,9 +46,19 @@ struct OcWork {
     OcWork &operator=(const OcWork &) = delete;
 };
 
+// Largest design vector of oc_update_noalloc: its partial sums (5 doubles per 1,024 elements) fill a fixed
+// device array of 640 KiB + 256 B.
+constexpr int kOcNoAllocMaxElements = 1 << 24;
+constexpr int kOcNoAllocMaxBlocks = kOcNoAllocMaxElements / 1024;
+
 double oc_update_device(OcWork &w, const double *d_rho, const double *d_dc, const double *d_s,
                         const uint8_t *d_passive, double *d_rho_new, int n, double vf, double move, double rho_min,
                         double lambda_guess, int max_bisect, double vol_tol, hipStream_t s, int *evals);
+
+// OcOptimizer.Update on device pointers with its scratch in a fixed device array: no allocation.
+double oc_update_noalloc(const double *d_rho, const double *d_dc, const double *d_s, const uint8_t *d_passive,
+                         double *d_rho_new, int n, double vf, double move, double rho_min, double lambda_guess,
+                         int max_bisect, double vol_tol, hipStream_t s);
 
 void launch_scatter_density(int na, const int32_t *d_active, const double *d_rho_bar, double *d_density,
                             hipStream_t s);

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "device_common.hpp"
@@ -479,6 +481,78 @@ OcWork::~OcWork() {
     if (h_res) (void)hipHostFree(h_res);
 }
 
+namespace {
+// Device memory of one OC update: the two-slot state, the volume partials (two buffers of 2 * nb), the
+// partial sums of s (nb) and the result the host polls.
+struct OcScratch {
+    OcState *states;
+    double *parts, *spart;
+    OcResult *res;
+};
+
+// The OC state machine on the stream: sum of s, the initial bracket, then batches of 16 steps with one
+// poll of the result after each batch. h_res receives the polled result (pinned or pageable host memory).
+double oc_run(const OcScratch &w, OcResult *h_res, const double *d_rho, const double *d_dc, const double *d_s,
+              const uint8_t *d_passive, double *d_rho_new, int n, double vf, double move, double rho_min,
+              double lambda_guess, int max_bisect, double vol_tol, hipStream_t s, int *evals) {
+    const int nb = cdiv(n, 1024);
+    OcArgs a{d_rho, d_dc, d_s, d_passive, d_rho_new, n, move, rho_min, vol_tol, max_bisect};
+    hipLaunchKernelGGL(k_oc_sum_s, dim3(nb), dim3(256), 0, s, d_s, n, w.spart);
+    hipLaunchKernelGGL(k_oc_begin, dim3(nb), dim3(256), 0, s, a, w.spart, nb, vf, lambda_guess, w.states, w.parts,
+                       w.res);
+    MYCEL_LAUNCH_CHECK();
+    // Worst case: 2 + 50 + 50 + max_bisect evaluations plus the final write.
+    const int max_launches = 2 + 50 + 50 + max_bisect + 2;
+    int launched = 0, slot = 0;
+    const int batch = 16;
+    while (true) {
+        for (int k = 0; k < batch; k++, launched++, slot ^= 1)
+            hipLaunchKernelGGL(k_oc_step, dim3(nb), dim3(256), 0, s, a, slot, w.states, w.parts, nb, w.res);
+        MYCEL_LAUNCH_CHECK();
+        MYCEL_HIP(hipMemcpyAsync(h_res, w.res, sizeof(OcResult), hipMemcpyDeviceToHost, s));
+        MYCEL_HIP(hipStreamSynchronize(s));
+        if (h_res->done) break;
+        if (launched > max_launches) throw Error(MYCEL_E_DEVICE, "OC bisection did not terminate");
+    }
+    if (evals) *evals = h_res->evals;
+    return h_res->lambda;
+}
+
+// ---- the OC update that allocates nothing ------------------------------------------------------
+// Its scratch lives in this file-scope device array: every device the library runs on holds one copy until
+// the process ends, and every call on a device shares it (calls that overlap on one device race). In host
+// code the array's name is not a device address: each device's copy is reached only through the address
+// hipGetSymbolAddress returns for it, kept in g_oc_scratch_ptr.
+constexpr int kOcStateBytes = 256;  // 2 OcState + OcResult, padded to the array's alignment
+constexpr size_t kOcScratchBytes = kOcStateBytes + sizeof(double) * 5 * static_cast<size_t>(kOcNoAllocMaxBlocks);
+static_assert(2 * sizeof(OcState) + sizeof(OcResult) <= kOcStateBytes, "OC state does not fit its slot");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kOcScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_oc_scratch_mutex;
+unsigned char *g_oc_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of the scratch array, looked up once per device (with that device current).
+unsigned char *oc_scratch_ptr() {
+    int dev = 0;
+    MYCEL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(MYCEL_E_DEVICE, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_oc_scratch_mutex);
+    if (!g_oc_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error(MYCEL_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_oc_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return g_oc_scratch_ptr[dev];
+}
+}  // namespace
+
 double oc_update_device(OcWork &w, const double *d_rho, const double *d_dc, const double *d_s,
                         const uint8_t *d_passive, double *d_rho_new, int n, double vf, double move, double rho_min,
                         double lambda_guess, int max_bisect, double vol_tol, hipStream_t s, int *evals) {
@@ -489,27 +563,30 @@ double oc_update_device(OcWork &w, const double *d_rho, const double *d_dc, cons
     const int nb = cdiv(n, 1024);
     if (w.parts.size() < static_cast<size_t>(4 * nb)) w.parts.resize(4 * nb);
     if (w.spart.size() < static_cast<size_t>(nb)) w.spart.resize(nb);
-    OcArgs a{d_rho, d_dc, d_s, d_passive, d_rho_new, n, move, rho_min, vol_tol, max_bisect};
-    hipLaunchKernelGGL(k_oc_sum_s, dim3(nb), dim3(256), 0, s, d_s, n, w.spart.get());
-    hipLaunchKernelGGL(k_oc_begin, dim3(nb), dim3(256), 0, s, a, w.spart.get(), nb, vf, lambda_guess, w.states.get(),
-                       w.parts.get(), w.res.get());
-    MYCEL_LAUNCH_CHECK();
-    // Worst case: 2 + 50 + 50 + max_bisect evaluations plus the final write.
-    const int max_launches = 2 + 50 + 50 + max_bisect + 2;
-    int launched = 0, slot = 0;
-    const int batch = 16;
-    while (true) {
-        for (int k = 0; k < batch; k++, launched++, slot ^= 1)
-            hipLaunchKernelGGL(k_oc_step, dim3(nb), dim3(256), 0, s, a, slot, w.states.get(), w.parts.get(), nb,
-                               w.res.get());
-        MYCEL_LAUNCH_CHECK();
-        MYCEL_HIP(hipMemcpyAsync(w.h_res, w.res.get(), sizeof(OcResult), hipMemcpyDeviceToHost, s));
-        MYCEL_HIP(hipStreamSynchronize(s));
-        if (w.h_res->done) break;
-        if (launched > max_launches) throw Error(MYCEL_E_DEVICE, "OC bisection did not terminate");
-    }
-    if (evals) *evals = w.h_res->evals;
-    return w.h_res->lambda;
+    const OcScratch sc{w.states.get(), w.parts.get(), w.spart.get(), w.res.get()};
+    return oc_run(sc, w.h_res, d_rho, d_dc, d_s, d_passive, d_rho_new, n, vf, move, rho_min, lambda_guess, max_bisect,
+                  vol_tol, s, evals);
+}
+
+double oc_update_noalloc(const double *d_rho, const double *d_dc, const double *d_s, const uint8_t *d_passive,
+                         double *d_rho_new, int n, double vf, double move, double rho_min, double lambda_guess,
+                         int max_bisect, double vol_tol, hipStream_t s) {
+    if (n > kOcNoAllocMaxElements)
+        throw Error(MYCEL_E_OUT_OF_RANGE, "n = " + std::to_string(n) + " exceeds the no-allocation OC update's limit of " +
+                                              std::to_string(kOcNoAllocMaxElements) + " elements");
+    if (n <= 0) return lambda_guess;
+    unsigned char *base = oc_scratch_ptr();
+    const int nb = cdiv(n, 1024);
+    OcScratch sc;
+    sc.states = reinterpret_cast<OcState *>(base);
+    sc.res = reinterpret_cast<OcResult *>(base + 2 * sizeof(OcState));
+    sc.spart = reinterpret_cast<double *>(base + kOcStateBytes);
+    sc.parts = sc.spart + nb;
+    // Every word the kernels read is written earlier in the same call (k_oc_sum_s writes spart, k_oc_begin the
+    // state, the result's done flag and the first partials), so nothing left by an earlier call is read.
+    OcResult h{};
+    return oc_run(sc, &h, d_rho, d_dc, d_s, d_passive, d_rho_new, n, vf, move, rho_min, lambda_guess, max_bisect,
+                  vol_tol, s, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------
