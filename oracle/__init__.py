@@ -67,7 +67,8 @@ class SimpOptions(C.Structure):
 class Snapshot(C.Structure):
     _fields_ = [("iteration", C.c_int), ("compliance", C.c_double), ("volume_fraction", C.c_double),
                 ("max_change", C.c_double), ("grey_level", C.c_double), ("solver_iterations", C.c_int),
-                ("solver_residual", C.c_double), ("max_displacement", C.c_double), ("lambda_", C.c_double)]
+                ("solver_residual", C.c_double), ("max_displacement", C.c_double), ("lambda_", C.c_double),
+                ("solver_tolerance", C.c_double)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -103,6 +104,8 @@ def _declare(L):
                                       C.POINTER(PcgResult)]
     L.orc_von_mises.argtypes = [_vp, _dp, _dp]
     L.orc_average_element_to_node.argtypes = [_vp, _dp, _dp]
+    L.orc_sample_nodal_scalar.restype = C.c_double
+    L.orc_sample_nodal_scalar.argtypes = [_vp, _dp] + [C.c_float] * 6
     L.orc_filter_create.argtypes = [C.c_int, C.c_int, C.c_int, _up, _ip, C.c_int, C.c_double,
                                     C.POINTER(C.c_void_p)]
     L.orc_filter_destroy.argtypes = [_vp]
@@ -283,6 +286,15 @@ class Model:
         lib().orc_average_element_to_node(self._h, np.ascontiguousarray(elem_box, np.float64), out)
         return out
 
+    def von_mises(self, u):
+        vm = np.zeros(self.n_elem)
+        lib().orc_von_mises(self._h, np.ascontiguousarray(u, np.float64), vm)
+        return vm
+
+    def sample_nodal_scalar(self, nodal, point, origin=(0.0, 0.0, 0.0)):
+        return lib().orc_sample_nodal_scalar(self._h, np.ascontiguousarray(nodal, np.float64), *[float(v) for v in point],
+                                             *[float(v) for v in origin])
+
     def simp(self, f, *, volume_fraction=0.4, filter_radius_mm=2.0, penal=3.0, move_limit=0.2,
              max_iterations=200, solver_tol_initial=1e-4, solver_tol_final=1e-6, solver_max_iterations=5000,
              rho_min=1e-3, protect_boundary_hinges=True, start_iteration=1, enforce_min_filter_radius=True,
@@ -306,7 +318,8 @@ class Model:
                                       int(threads), snaps, C.byref(nsn), C.byref(conv), rho, rb, u)
         hist = [dict(iteration=s.iteration, compliance=s.compliance, volume_fraction=s.volume_fraction,
                      max_change=s.max_change, grey_level=s.grey_level, solver_iterations=s.solver_iterations,
-                     solver_residual=s.solver_residual, max_displacement=s.max_displacement, lam=s.lambda_)
+                     solver_residual=s.solver_residual, max_displacement=s.max_displacement, lam=s.lambda_,
+                     solver_tolerance=s.solver_tolerance)
                 for s in snaps[:nsn.value]]
         return dict(status=st, snapshots=hist, converged=bool(conv.value), rho=rho, rho_bar=rb, u=u)
 

@@ -101,6 +101,8 @@ int orc_static_analysis(orc_model *m, const double *f, double *u_inout, double t
 void orc_von_mises(const orc_model *m, const double *u, double *vm_box);
 /* StaticAnalysis.AverageElementToNode (Fem/StaticAnalysis.cs:222-272). */
 void orc_average_element_to_node(const orc_model *m, const double *elem_box, double *nodal);
+double orc_sample_nodal_scalar(const orc_model *m, const double *nodal, float px, float py, float pz, float ox,
+                               float oy, float oz);
 
 /* DensityFilter (Optimization/DensityFilter.cs:34-303). */
 typedef struct orc_filter orc_filter;
@@ -158,6 +160,7 @@ typedef struct {
     double solver_residual;
     double max_displacement;
     double lambda;
+    double solver_tolerance; /* the relative tolerance this iteration's solves used (TopologyOptimizer.cs:212-224) */
 } orc_snapshot;
 
 /* TopologyOptimizer.Run (Optimization/TopologyOptimizer.cs:16-407), Heaviside off.

@@ -523,3 +523,150 @@ def test_simp_heaviside_first_iteration_from_primitives(oracle):
     r_off = oracle_model(p).simp(fa, **kw)
     c_on, c_off = r["snapshots"][0]["compliance"], r_off["snapshots"][0]["compliance"]
     assert abs(c_on - c_off) / c_off > 1e-3
+
+
+# ---------------------------------------------------------------------------------------------
+# PassiveRegion, AverageElementToNode, SampleNodalScalar, V15 (post-processing and set-up helpers)
+# ---------------------------------------------------------------------------------------------
+def test_passive_region_interior_node_and_radius(oracle):
+    """PassiveRegion.Build (Voxelization/PassiveRegion.cs:13-62): an interior node with radius 1 marks exactly the
+    8 elements around it; radius 2 marks the 4x4x4 block [n - 2, n + 1] per axis; blocks are clamped at the grid."""
+    nx, ny, nz = 7, 6, 5
+    node = 3 + (nx + 1) * (2 + (ny + 1) * 2)  # node (3, 2, 2)
+    m1 = oracle.passive_region(nx, ny, nz, np.array([node], np.int32), 1).reshape(nz, ny, nx)
+    expect = np.zeros((nz, ny, nx), np.uint8)
+    expect[1:3, 1:3, 2:4] = 1
+    assert np.array_equal(m1, expect)
+    m2 = oracle.passive_region(nx, ny, nz, np.array([node], np.int32), 2).reshape(nz, ny, nx)
+    expect = np.zeros((nz, ny, nx), np.uint8)
+    expect[0:4, 0:4, 1:5] = 1
+    assert np.array_equal(m2, expect)
+    corner = nx + (nx + 1) * (ny + (ny + 1) * nz)  # node (nx, ny, nz): clamped to the last element
+    m3 = oracle.passive_region(nx, ny, nz, np.array([corner], np.int32), 2).reshape(nz, ny, nx)
+    expect = np.zeros((nz, ny, nx), np.uint8)
+    expect[nz - 2:, ny - 2:, nx - 2:] = 1
+    assert np.array_equal(m3, expect)
+
+
+def test_average_element_to_node(oracle):
+    """StaticAnalysis.AverageElementToNode (Fem/StaticAnalysis.cs:222-272): the reference's 2x2x1 case
+    (StaticAnalysisTests.cs:12-35), a constant field stays constant on every active node, inactive nodes get 0,
+    and hand-computed averages at an edge node and at a node next to a hole."""
+    p = W.Problem(2, 2, 1, e0=1000.0, emin_ratio=1e-6)
+    p.fixed_dofs = np.arange(6, dtype=np.int32)
+    m = oracle_model(p)
+    nod = m.average_element_to_node(np.array([10.0, 20.0, 30.0, 40.0]))
+    assert nod[1 + 3 * 1] == 25.0 and nod[0] == 10.0
+    assert nod[1] == 15.0  # node (1, 0, 0): elements 0 and 1
+    q = W.Problem(4, 3, 2, domain=W.l_shape_domain(4, 3, 2))
+    q.fixed_dofs = W.fix_all(W.face_nodes_x(q, 0))
+    mq = oracle_model(q)
+    const = mq.average_element_to_node(np.full(q.n_elem, 3.5))
+    act = mq.active_nodes.astype(bool)
+    assert np.all(const[act] == 3.5) and np.all(const[~act] == 0.0)
+    vals = random_field(91, q.n_elem, -1, 1)
+    nod = mq.average_element_to_node(vals)
+    dom = q.domain.reshape(q.nz, q.ny, q.nx)
+    for n in range(q.n_nodes):
+        i, j, k = n % (q.nx + 1), (n // (q.nx + 1)) % (q.ny + 1), n // ((q.nx + 1) * (q.ny + 1))
+        es = [ex + q.nx * (ey + q.ny * ez) for ez in (k - 1, k) for ey in (j - 1, j) for ex in (i - 1, i)
+              if 0 <= ex < q.nx and 0 <= ey < q.ny and 0 <= ez < q.nz and dom[ez, ey, ex]]
+        if es:
+            s = 0.0
+            for e in sorted(es):
+                s += vals[e]
+            assert nod[n] == s / len(es), n
+        else:
+            assert nod[n] == 0.0
+
+
+def test_sample_nodal_scalar_reference_cases(oracle):
+    """StaticAnalysis.SampleNodalScalar (Fem/StaticAnalysis.cs:278-366) on the reference's three cases
+    (StaticAnalysisTests.cs:38-135), a linear field reproduced at random interior points, and the fallback's
+    first-index tie rule."""
+    one = W.Problem(1, 1, 1, e0=1000.0, emin_ratio=1e-6)
+    one.fixed_dofs = np.arange(6, dtype=np.int32)
+    m = oracle_model(one)
+    lin = np.array([2.0 * i + 3.0 * j + 5.0 * k for k in (0, 1) for j in (0, 1) for i in (0, 1)])
+    assert abs(m.sample_nodal_scalar(lin, (0.5, 0.5, 0.5)) - 5.0) < 1e-10
+    assert abs(m.sample_nodal_scalar(lin, (0.25, 0.75, 0.5)) - 5.25) < 1e-10
+    tens = np.arange(8) * 10.0
+    assert m.sample_nodal_scalar(tens, (5.0, 5.0, 5.0)) == tens[7]  # outside: nearest node (1, 1, 1)
+    two = W.Problem(2, 1, 1, domain=np.array([1, 0], np.uint8), e0=1000.0, emin_ratio=1e-6)
+    two.fixed_dofs = np.arange(6, dtype=np.int32)
+    m2 = oracle_model(two)
+    f = np.array([10.0 * i + 5.0 * j + 2.0 * k for k in (0, 1) for j in (0, 1) for i in (0, 1, 2)])
+    assert abs(m2.sample_nodal_scalar(f, (1.5, 0.5, 0.5)) - 13.5) < 1e-10  # void cell: its 4 active nodes
+    # Equidistant from nodes 0 and 1 (x = 0.5 on the y = -1 side): the first in index order wins.
+    assert m.sample_nodal_scalar(tens, (0.5, -1.0, 0.0)) == tens[0]
+    # A linear field on a box with an origin and a float pitch is reproduced at interior points.
+    q = W.Problem(5, 4, 3, h=0.75)
+    q.fixed_dofs = np.arange(6, dtype=np.int32)
+    mq = oracle_model(q)
+    org = (1.5, -2.0, 0.25)
+    idx = np.arange(q.n_nodes)
+    i, j, k = idx % 6, (idx // 6) % 5, idx // 30
+    field = 1.0 + 0.5 * i - 2.0 * j + 0.25 * k
+    rng = W.SplitMix64(17)
+    for _ in range(50):
+        u = rng.uniform(3)
+        pt = np.float32([org[0] + u[0] * 5 * 0.75, org[1] + u[1] * 4 * 0.75, org[2] + u[2] * 3 * 0.75])
+        loc = (pt.astype(np.float32) - np.float32(org)) / np.float32(0.75)
+        expect = 1.0 + 0.5 * float(loc[0]) - 2.0 * float(loc[1]) + 0.25 * float(loc[2])
+        assert abs(mq.sample_nodal_scalar(field, pt, org) - expect) < 1e-12
+
+
+def test_v15_von_mises_pure_shear_nu_025(oracle):
+    """V15 (FemValidationTests.cs:947-993): pure shear u_x = gamma y on a 2x2x2 grid, h = 2.5, E0 = 2000,
+    nu = 0.25, rho = 0.8, p = 3: every element's von Mises stress is sqrt(3) G gamma to 1e-6, G = E/(2(1+nu))."""
+    u = np.zeros(81)
+    for n in range(27):
+        j = (n // 3) % 3
+        u[3 * n] = 0.002 * j * 2.5
+    vm = oracle.Model(2, 2, 2, None, np.zeros(0, np.int32), 2.5, 2000.0, 0.25, 1e-6, 3.0, np.full(8, 0.8)).von_mises(u)
+    g = 0.8 ** 3 * 2000.0 / (2.0 * 1.25)
+    assert np.max(np.abs(vm - np.sqrt(3.0) * g * 0.002) / (np.sqrt(3.0) * g * 0.002)) < 1e-6
+
+
+# ---------------------------------------------------------------------------------------------
+# SIMP control: the tolerance schedule and the stop test (Optimization/TopologyOptimizer.cs)
+# ---------------------------------------------------------------------------------------------
+def _schedule(prev_max_change, tol_i, tol_f):
+    a = min(max((prev_max_change - 0.01) / 0.09, 0.0), 1.0)
+    lf, li = np.log10(max(tol_f, 1e-15)), np.log10(max(tol_i, 1e-15))
+    return 10.0 ** (lf + a * (li - lf))
+
+
+def test_simp_tolerance_schedule_closed_form(oracle):
+    """Each iteration's solver tolerance is 1e-4 on the first and, after that, the log interpolation between 1e-6
+    and 1e-4 by clamp((maxChange - 0.01) / 0.09) of the PREVIOUS iteration's max change (:212-224); every solve
+    stops at or below it."""
+    p = W.cantilever_simp(24, 8, 4)
+    r = oracle_model(p).simp(oracle_forces(p), volume_fraction=0.3, filter_radius_mm=1.5, max_iterations=40,
+                             protect_boundary_hinges=False)
+    sn = r["snapshots"]
+    assert sn[0]["solver_tolerance"] == 1e-4
+    tols = [s["solver_tolerance"] for s in sn]
+    for k in range(1, len(sn)):
+        expect = _schedule(sn[k - 1]["max_change"], 1e-4, 1e-6)
+        assert abs(tols[k] - expect) <= 1e-14 * expect, k
+        assert sn[k]["solver_residual"] <= tols[k]
+    assert min(tols) < 1e-5 < max(tols)  # the run spans the schedule
+
+
+def test_simp_stop_test_six_compliances(oracle):
+    """The run stops at the first iteration k with maxChange_k < 0.01 and |C_k - C_(k-5)| / C_k < 1e-3, once six
+    compliances are held (:374-390); no earlier iteration met both."""
+    p = W.cantilever_simp(16, 6, 4)
+    r = oracle_model(p).simp(oracle_forces(p), volume_fraction=0.4, filter_radius_mm=1.5, max_iterations=300,
+                             protect_boundary_hinges=False)
+    sn = r["snapshots"]
+    assert r["converged"] and len(sn) < 300
+    c = [s["compliance"] for s in sn]
+    mc = [s["max_change"] for s in sn]
+
+    def stops(k):  # 0-based k
+        return k >= 5 and mc[k] < 0.01 and abs(c[k] - c[k - 5]) / c[k] < 1e-3
+
+    assert stops(len(sn) - 1)
+    assert not any(stops(k) for k in range(len(sn) - 1))
