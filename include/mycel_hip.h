@@ -61,6 +61,10 @@ extern "C" {
  * MIXED_FP32: Jacobi-PCG iterating on FP32 vectors with the FP32 operator and FP64 dot products, with FP64
  *   reliable updates (the true residual r = P(b - K u) in FP64) and convergence on that true residual.
  * MIXED_BF16: reserved; rejected with MYCEL_E_ARGUMENT.
+ * FP64_ORDERED: the reference's Jacobi-PCG with every sum formed in the reference's order (colour-ordered K*u with
+ *   4-lane Ke rows, 8,192-DOF blocked dot products with 4 lane chains, sequential ||Pb||) and no contracted
+ *   multiply-adds: bit-identical to the reference restatement, for parity runs. In mycel_simp_run it also takes
+ *   the OC volumes, the compliance and the snapshot sums sequentially.
  * MIXED_GMG: FP64 flexible PCG (FP64 operator, residual, dot products, convergence test and update)
  *   preconditioned by one FP32 geometric-multigrid V-cycle per iteration in place of Jacobi: a second
  *   IPreconditioner (Fem/IPreconditioner.cs:9-20). Needs a grid that coarsens by 2 per axis down to
@@ -69,11 +73,13 @@ extern "C" {
 #define MYCEL_PRECISION_MIXED_FP32 1
 #define MYCEL_PRECISION_MIXED_BF16 2
 #define MYCEL_PRECISION_MIXED_GMG 3
+#define MYCEL_PRECISION_FP64_ORDERED 4
 
 /* K*u forms (mycel_op_set_matvec_form). */
 #define MYCEL_MATVEC_AUTO 0        /* by grid size (the default) */
 #define MYCEL_MATVEC_NODE_GATHER 1 /* one thread per node (columns 0) or per 2/4/8-node z-column */
 #define MYCEL_MATVEC_ELEMENT 2     /* element-centric: 8x8 element-column tiles, columns = 2/3/4/8 node layers */
+#define MYCEL_MATVEC_REFERENCE_ORDER 3 /* the reference's colour order and 4-lane row sums (bit-identical) */
 
 typedef struct mycel_op mycel_op;
 typedef struct mycel_filter mycel_filter;
@@ -99,6 +105,8 @@ typedef struct {
     double elapsed_ms;
     int32_t outer_corrections;   /* reliable updates with the FP64 operator (mixed precision only) */
     int32_t fallback_iterations; /* FP64 iterations after an FP32 breakdown (mixed precision only) */
+    double message_value;        /* the quantity a breakdown message quotes: pAp (CURVATURE), alpha (STEP) or
+                                    beta (DIRECTION) */
 } mycel_pcg_result;
 
 /* IProgress<PcgProgress> (Fem/PcgResult.cs:5). Called on the calling thread between iterations, once per batch
@@ -110,6 +118,9 @@ typedef void (*mycel_pcg_progress_cb)(int32_t iteration, double relative_residua
 /* ---- library ------------------------------------------------------------------------------ */
 MYCEL_API const char *mycel_last_error(void);
 MYCEL_API int32_t mycel_version(void);
+/* PcgResult.Message (Fem/PcgSolver.cs:149-367) of a result, as the reference words it (UTF-8, NUL-terminated,
+ * truncated to capacity - 1 bytes); an empty string for a converged solve, whose Message is null. */
+MYCEL_API int32_t mycel_pcg_message(const mycel_pcg_result *result, char *buf, int32_t capacity);
 /* Number of visible HIP devices; selects device `ordinal` for handles created afterwards. */
 MYCEL_API int32_t mycel_device_count(int32_t *count);
 MYCEL_API int32_t mycel_set_device(int32_t ordinal);
@@ -202,6 +213,11 @@ MYCEL_API int32_t mycel_static_analysis(mycel_op *op, const double *f, double *u
 MYCEL_API int32_t mycel_von_mises(mycel_op *op, const double *u, double *von_mises_box);
 /* StaticAnalysis.AverageElementToNode (Fem/StaticAnalysis.cs:222-272). */
 MYCEL_API int32_t mycel_average_element_to_node(mycel_op *op, const double *element_box, double *nodal);
+/* StaticAnalysis.SampleNodalScalar (Fem/StaticAnalysis.cs:278-366) at n_points points at once: points_xyz holds
+ * float32 (x, y, z) triples (System.Numerics.Vector3), origin_xyz the GridSpec.Origin; out[q] is the trilinear
+ * interpolation over the active nodes of the point's cell, or the value at the nearest active node. */
+MYCEL_API int32_t mycel_sample_nodal_scalar(mycel_op *op, const double *nodal, int32_t n_points,
+                                            const float *points_xyz, const float *origin_xyz, double *out);
 
 /* ---- DensityFilter (Optimization/DensityFilter.cs) ---------------------------------------- */
 /* new DensityFilter(grid, domain, activeElements, rmin) (Optimization/DensityFilter.cs:34-199). */
@@ -266,6 +282,17 @@ typedef struct {
                                      replaces the float32 initial_density of mycel_simp_run (an exact resume from
                                      a checkpoint's FP64 design; the reference's InitialDensity is float32) */
     int32_t initial_design_len;
+    /* The loop state a resumed run continues from (no counterpart in the reference, whose StartIteration restarts
+     * the schedule: TopologyOptimizer.cs:154-162,212-224). resume_state = 0 (the default, and what a zeroed struct
+     * holds): none, the start iteration solves to solver_tol_initial as the reference does. resume_state = 1: the
+     * start iteration's tolerance follows the schedule from resume_max_change (the previous iteration's max
+     * change), the Heaviside beta is resume_beta, and the stop test starts from resume_history, the last
+     * resume_history_len (<= 6) compliances, oldest first. */
+    int32_t resume_state;
+    double resume_max_change;
+    double resume_beta;
+    int32_t resume_history_len;
+    double resume_history[6];
 } mycel_simp_options;
 
 /* Fills the reference defaults (OptimizerOptions.cs:5-26). */
@@ -283,6 +310,7 @@ typedef struct {
     double max_displacement;
     double lambda;
     double solve_ms, sensitivity_ms, filter_ms, update_ms, total_ms;
+    double beta_next; /* the Heaviside beta of the next iteration (1 without projection): resume state */
 } mycel_snapshot;
 
 /* progress.Report(snapshot) (TopologyOptimizer.cs:341-372). rho_bar (the snapshot's PhysicalDensity) and rho

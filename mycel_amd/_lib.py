@@ -29,21 +29,12 @@ PRECISION_FP64 = 0
 PRECISION_MIXED_FP32 = 1
 PRECISION_MIXED_BF16 = 2
 PRECISION_MIXED_GMG = 3
+PRECISION_FP64_ORDERED = 4
 MATVEC_AUTO = 0
 MATVEC_NODE_GATHER = 1
 MATVEC_ELEMENT = 2
+MATVEC_REFERENCE_ORDER = 3
 
-PCG_MESSAGES = {
-    0: None,
-    1: "RHS is zero. Solution is zero.",
-    2: "Initial guess already converged.",
-    3: "PCG curvature breakdown (pᵀAp <= 0). Is the model not sufficiently constrained?",
-    4: "PCG step breakdown. Is the model not sufficiently constrained?",
-    5: "PCG stagnated: residual stopped decreasing. Is the model not sufficiently constrained?",
-    6: "PCG search direction breakdown. Is the model not sufficiently constrained?",
-    7: "PCG reached maximum iterations without converging. Is the model not sufficiently constrained?",
-    8: "The operation was canceled.",
-}
 
 
 class MycelError(RuntimeError):
@@ -106,7 +97,7 @@ class OpInfo(C.Structure):
 class PcgResultC(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("final_residual", C.c_double), ("converged", C.c_int32),
                 ("message", C.c_int32), ("elapsed_ms", C.c_double), ("outer_corrections", C.c_int32),
-                ("fallback_iterations", C.c_int32)]
+                ("fallback_iterations", C.c_int32), ("message_value", C.c_double)]
 
 
 class SimpOptionsC(C.Structure):
@@ -116,7 +107,8 @@ class SimpOptionsC(C.Structure):
                 ("protect_boundary_hinges", C.c_int32), ("start_iteration", C.c_int32),
                 ("enforce_min_filter_radius", C.c_int32), ("continuation", C.c_int32), ("precision", C.c_int32),
                 ("enable_heaviside", C.c_int32), ("initial_lambda", C.c_double), ("initial_design", C.c_void_p),
-                ("initial_design_len", C.c_int32)]
+                ("initial_design_len", C.c_int32), ("resume_state", C.c_int32), ("resume_max_change", C.c_double),
+                ("resume_beta", C.c_double), ("resume_history_len", C.c_int32), ("resume_history", C.c_double * 6)]
 
 
 class SnapshotC(C.Structure):
@@ -124,7 +116,7 @@ class SnapshotC(C.Structure):
                 ("max_change", C.c_double), ("grey_level", C.c_double), ("solver_iterations", C.c_int32),
                 ("solver_residual", C.c_double), ("max_displacement", C.c_double), ("lambda_", C.c_double),
                 ("solve_ms", C.c_double), ("sensitivity_ms", C.c_double), ("filter_ms", C.c_double),
-                ("update_ms", C.c_double), ("total_ms", C.c_double)]
+                ("update_ms", C.c_double), ("total_ms", C.c_double), ("beta_next", C.c_double)]
 
 
 class SimpResultC(C.Structure):
@@ -159,6 +151,7 @@ _d = C.c_double
 SIGNATURES = {
     "mycel_last_error": (C.c_char_p, []),
     "mycel_version": (_i, []),
+    "mycel_pcg_message": (_i, [C.POINTER(PcgResultC), C.c_char_p, _i]),
     "mycel_device_count": (_i, [C.POINTER(_i)]),
     "mycel_set_device": (_i, [_i]),
     "mycel_hex8_ke": (_i, [_d, _d, _vp]),
@@ -186,6 +179,7 @@ SIGNATURES = {
                                    C.POINTER(PcgResultC)]),
     "mycel_von_mises": (_i, [_vp, _vp, _vp]),
     "mycel_average_element_to_node": (_i, [_vp, _vp, _vp]),
+    "mycel_sample_nodal_scalar": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "mycel_filter_create": (_i, [C.POINTER(Grid), _vp, _vp, _i, _d, C.POINTER(_vp)]),
     "mycel_filter_destroy": (_i, [_vp]),
     "mycel_filter_get_s": (_i, [_vp, _vp]),

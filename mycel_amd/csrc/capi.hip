@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -62,7 +63,50 @@ struct DeviceGuard {
     ~DeviceGuard() { (void)hipSetDevice(prev); }
 };
 
+// A double in .NET's "E3" format with the invariant culture: d.dddE+ddd (at least three exponent digits).
+std::string net_e3(double v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "Infinity" : "-Infinity";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.3E", v);
+    std::string t(buf);
+    const size_t e = t.find('E');
+    if (e == std::string::npos || e + 2 > t.size()) return t;
+    std::string digits = t.substr(e + 2);
+    while (digits.size() < 3) digits.insert(digits.begin(), '0');
+    return t.substr(0, e + 2) + digits;
+}
+
 }  // namespace
+
+// PcgResult.Message (Fem/PcgSolver.cs:149,207,255,270,306,344,367); empty for a converged solve (Message null).
+std::string pcg_message_text(int code, double value) {
+    switch (code) {
+        case MYCEL_PCG_RHS_ZERO: return "RHS is zero. Solution is zero.";
+        case MYCEL_PCG_INITIAL_CONVERGED: return "Initial guess already converged.";
+        case MYCEL_PCG_CURVATURE_BREAKDOWN:
+            return "PCG curvature breakdown (p\u1D40Ap = " + net_e3(value) +
+                   " <= 0). Is the model not sufficiently constrained?";
+        case MYCEL_PCG_STEP_BREAKDOWN:
+            return "PCG step breakdown (alpha = " + net_e3(value) + "). Is the model not sufficiently constrained?";
+        case MYCEL_PCG_STAGNATION:
+            return "PCG stagnated: residual stopped decreasing. Is the model not sufficiently constrained?";
+        case MYCEL_PCG_DIRECTION_BREAKDOWN:
+            return "PCG search direction breakdown (beta = " + net_e3(value) +
+                   "). Is the model not sufficiently constrained?";
+        case MYCEL_PCG_MAX_ITERATIONS:
+            return "PCG reached maximum iterations without converging. Is the model not sufficiently constrained?";
+        case MYCEL_PCG_CANCELED: return "The operation was canceled.";
+        default: return "";
+    }
+}
+
+// InvalidOperationException text of a non-converged solve (TopologyOptimizer.cs:249-253,
+// StaticAnalysis.cs:88-92).
+static std::string not_converged_text(const mycel_pcg_result &r) {
+    return "FEM solver did not converge in " + std::to_string(r.iterations) + " iterations (" +
+           pcg_message_text(r.message, r.message_value) + "). Is the model not sufficiently constrained?";
+}
 
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
@@ -159,6 +203,8 @@ mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *f
     op->d_flags.upload(op->dof_flags.data(), d.n_dofs, s);
     op->d_domain.resize(d.n_elem);
     op->d_domain.upload(domain, d.n_elem, s);
+    op->d_active_nodes.resize(d.n_nodes);
+    op->d_active_nodes.upload(op->active_nodes.data(), d.n_nodes, s);
     op->d_active.resize(std::max<size_t>(1, op->active.size()));
     if (!op->active.empty()) op->d_active.upload(op->active.data(), op->active.size(), s);
     op->d_density.resize(d.n_elem);
@@ -184,6 +230,8 @@ void op_destroy(mycel_op *op) {
     if (op->stream) (void)hipStreamSynchronize(op->stream);
     pcg_free(op->pcg);
     op->pcg = nullptr;
+    ord_free(op->ord);
+    op->ord = nullptr;
     gmg_free(op->gmg);
     op->gmg = nullptr;
     if (op->stream) (void)hipStreamDestroy(op->stream);
@@ -259,8 +307,10 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
     DevBuf<double> d_dc_k(n_cases > 1 ? nab : 0);
     std::vector<mycel_pcg_result> case_res(n_cases);
     const bool heav = o.enable_heaviside != 0;
+    // The reference's evaluation order throughout (PCG, compliance, OC volumes, snapshot sums).
+    const bool ordered = o.precision == MYCEL_PRECISION_FP64_ORDERED;
     DevBuf<double> d_rho_tilde(heav ? nab : 0);
-    double beta = 1.0;  // :156
+    double beta = o.resume_state && o.resume_beta > 0.0 ? o.resume_beta : 1.0;  // :156, or the resumed run's
     DevBuf<uint8_t> d_passive(nab), d_active_nodes(d.n_nodes);
     DevBuf<double> d_u(ndc), d_f(ndc), d_b(ndc);
     d_rho.upload(rho.data(), nab, s);
@@ -279,9 +329,16 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
     }
 
     double lambda = o.initial_lambda > 0.0 && std::isfinite(o.initial_lambda) ? o.initial_lambda : 1.0;
-    double max_change = 1.0;
+    // A resumed run continues the tolerance schedule and the stop test from the saved state.
+    const bool resumed = o.resume_state != 0;
+    double max_change = resumed ? o.resume_max_change : 1.0;
     double hist[6];
     int hist_n = 0;
+    if (resumed) {
+        if (o.resume_history_len < 0 || o.resume_history_len > 6)
+            throw Error(MYCEL_E_ARGUMENT, "resume_history_len must be in [0, 6]");
+        for (int k = 0; k < o.resume_history_len; k++) hist[hist_n++] = o.resume_history[k];
+    }
     int n_snaps = 0;
     const int start = o.start_iteration, max_it = o.max_iterations;
     for (int iter = start; iter <= max_it; iter++) {
@@ -301,7 +358,7 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
         launch_scatter_density(na, op->d_active.get(), d_rho_bar.get(), op->d_density.get(), s);
         op_update_e(op, op->d_density.get(), penal, s);
         double tol;
-        if (iter == start) {
+        if (iter == start && !resumed) {
             tol = o.solver_tol_initial;
         } else {  // :219-223
             const double a = std::clamp((max_change - 0.01) / 0.09, 0.0, 1.0);
@@ -332,14 +389,12 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
                                  nullptr, nullptr, &pr);
                 solve_ms += ms_since(t_solve);
             }
-            if (!pr.converged)
-                throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " +
-                                                       std::to_string(pr.iterations) +
-                                                       " iterations. Is the model not sufficiently constrained?");
+            if (!pr.converged) throw Error(MYCEL_E_NOT_CONVERGED, not_converged_text(pr));
             solve_iters += pr.iterations;
             res_max = std::max(res_max, pr.final_residual);
             double cmk[2];
             compliance_maxdisp(d.n_nodes, d_active_nodes.get(), d_f.get() + k * nd, uk, d_part, cmk, s);
+            if (ordered) cmk[0] = seq_sum(op, d_f.get() + k * nd, uk, nullptr, d.n_dofs, s);  // :256-260
             cm[0] += cmk[0];
             cm[1] = std::max(cm[1], cmk[1]);
             const auto t_sens = clock::now();
@@ -358,13 +413,18 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
         sens_ms += ms_since(t_sens);
         const auto t_upd = clock::now();
         MYCEL_HIP(hipMemcpyAsync(d_rho_old.get(), d_rho.get(), sizeof(double) * nab, hipMemcpyDeviceToDevice, s));
-        lambda = oc_update_device(oc, d_rho.get(), d_dc.get(), flt.d_s.get(), d_passive.get(), d_rho.get(), na,
-                                  o.volume_fraction, o.move_limit, o.rho_min, lambda, 60, 1e-8, s, nullptr);
+        if (ordered)
+            lambda = oc_update_ordered(op, d_rho.get(), d_dc.get(), flt.d_s.get(), d_passive.get(), d_rho.get(), na,
+                                       o.volume_fraction, o.move_limit, o.rho_min, lambda, 60, 1e-8, s, nullptr);
+        else
+            lambda = oc_update_device(oc, d_rho.get(), d_dc.get(), flt.d_s.get(), d_passive.get(), d_rho.get(), na,
+                                      o.volume_fraction, o.move_limit, o.rho_min, lambda, 60, 1e-8, s, nullptr);
         const double upd_ms = ms_since(t_upd);
         if (heav && iter % 25 == 0 && beta < 8.0) beta = std::min(8.0, beta * 2.0);  // :315-318
         double met[3];
         simp_metrics(na, d_rho.get(), d_rho_old.get(), d_rho_bar.get(), heav ? d_rho_tilde.get() : nullptr, d_part,
                      met, s);
+        if (ordered) simp_sums_ordered(op, na, d_rho_bar.get(), heav ? d_rho_tilde.get() : d_rho_bar.get(), met, s);
         max_change = met[0];
         mycel_snapshot sn{};
         sn.iteration = iter;
@@ -381,6 +441,7 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
         sn.filter_ms = filter_ms;
         sn.update_ms = upd_ms;
         sn.total_ms = ms_since(t_total);
+        sn.beta_next = beta;
         if (snaps && n_snaps < snaps_cap) snaps[n_snaps] = sn;
         n_snaps++;
         result->total_iterations = n_snaps;
@@ -434,17 +495,20 @@ void static_analysis_run(mycel_op *op, const double *f, double *u, double tol, i
     d_u.upload(u, d.n_dofs, s);
     launch_project(d.n_dofs, op->d_flags.get(), d_f.get(), d_b.get(), s);
     pcg_solve_device(op, d_b.get(), d_u.get(), tol, max_iter, precision, nullptr, nullptr, nullptr, result);
-    if (throw_on_nonconv && !result->converged)
-        throw Error(MYCEL_E_NOT_CONVERGED, "FEM solver did not converge in " + std::to_string(result->iterations) +
-                                               " iterations. Is the model not sufficiently constrained?");
+    if (throw_on_nonconv && !result->converged) throw Error(MYCEL_E_NOT_CONVERGED, not_converged_text(*result));
     DevBuf<uint8_t> d_an(d.n_nodes);
     d_an.upload(op->active_nodes.data(), d.n_nodes, s);
     double cm[2];
     compliance_maxdisp(d.n_nodes, d_an.get(), d_f.get(), d_u.get(), d_part, cm, s);
+    // C = sum F_i u_i in index order under the reference's evaluation order (StaticAnalysis.cs:95-99).
+    if (precision == MYCEL_PRECISION_FP64_ORDERED) cm[0] = seq_sum(op, d_f.get(), d_u.get(), nullptr, d.n_dofs, s);
     *compliance = cm[0];
     *max_disp = std::sqrt(cm[1]);
     if (reactions) {  // R = K u sampled at FixedDofs (:103-110)
-        launch_apply_k(op, d_u.get(), op->d_y.get(), kApplyK, s);
+        if (precision == MYCEL_PRECISION_FP64_ORDERED)
+            launch_apply_k_ordered(op, d_u.get(), op->d_y.get(), kApplyK, s);
+        else
+            launch_apply_k(op, d_u.get(), op->d_y.get(), kApplyK, s);
         hipLaunchKernelGGL(k_select_fixed, dim3(cdiv(d.n_dofs, 256)), dim3(256), 0, s, d.n_dofs, op->d_flags.get(),
                            op->d_y.get(), d_b.get());
         MYCEL_LAUNCH_CHECK();
@@ -472,6 +536,17 @@ extern "C" {
 const char *mycel_last_error(void) { return g_last_error.c_str(); }
 
 int32_t mycel_version(void) { return 10000; }
+
+int32_t mycel_pcg_message(const mycel_pcg_result *result, char *buf, int32_t capacity) {
+    return guarded([&] {
+        require(result && buf, MYCEL_E_ARGUMENT_NULL, "result/buf");
+        require(capacity > 0, MYCEL_E_ARGUMENT, "capacity");
+        const std::string t = pcg_message_text(result->message, result->message_value);
+        const size_t n = std::min(t.size(), static_cast<size_t>(capacity - 1));
+        std::memcpy(buf, t.data(), n);
+        buf[n] = '\0';
+    });
+}
 
 int32_t mycel_device_count(int32_t *count) {
     return guarded([&] {
@@ -682,9 +757,10 @@ int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns) {
             require(columns == 2 || columns == 3 || columns == 4 || columns == 8, MYCEL_E_ARGUMENT,
                     "element-centric columns must be 2, 3, 4 or 8");
         else
-            require(form == MYCEL_MATVEC_AUTO, MYCEL_E_ARGUMENT, "unknown matvec form");
+            require(form == MYCEL_MATVEC_AUTO || form == MYCEL_MATVEC_REFERENCE_ORDER, MYCEL_E_ARGUMENT,
+                    "unknown matvec form");
         op->ku_form = form;
-        op->ku_kz = form == MYCEL_MATVEC_AUTO ? 0 : columns;
+        op->ku_kz = form == MYCEL_MATVEC_AUTO || form == MYCEL_MATVEC_REFERENCE_ORDER ? 0 : columns;
     });
 }
 
@@ -774,6 +850,26 @@ int32_t mycel_average_element_to_node(mycel_op *op, const double *elem, double *
         d_e.upload(elem, op->d.n_elem, s);
         launch_avg_e2n(op, d_e.get(), d_n.get(), s);
         d_n.download(nodal, op->d.n_nodes, s);
+        MYCEL_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int32_t mycel_sample_nodal_scalar(mycel_op *op, const double *nodal, int32_t n_points, const float *points_xyz,
+                                  const float *origin_xyz, double *out) {
+    return guarded([&] {
+        require(op && nodal && out && origin_xyz && (points_xyz || n_points == 0), MYCEL_E_ARGUMENT_NULL,
+                "op/nodal/points/origin/out");
+        require(n_points >= 0, MYCEL_E_ARGUMENT, "n_points");
+        if (n_points == 0) return;
+        DeviceGuard dg(op->device);
+        hipStream_t s = op->stream;
+        DevBuf<double> d_n(op->d.n_nodes), d_out(n_points);
+        DevBuf<float> d_p(3 * static_cast<size_t>(n_points));
+        DevBuf<int> d_fb(n_points);
+        d_n.upload(nodal, op->d.n_nodes, s);
+        d_p.upload(points_xyz, 3 * static_cast<size_t>(n_points), s);
+        launch_sample_nodal(op, d_n.get(), n_points, d_p.get(), origin_xyz, d_out.get(), d_fb.get(), s);
+        d_out.download(out, n_points, s);
         MYCEL_HIP(hipStreamSynchronize(s));
     });
 }

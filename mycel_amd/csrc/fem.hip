@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_sensitivities(Dims g, const int32_t *__
 
 // StaticAnalysis.AverageElementToNode (StaticAnalysis.cs:222-272): per node, active incident
 // elements summed in ascending element order.
-__global__ __launch_bounds__(256) void k_avg_e2n(Dims g, const double *__restrict__ ep,
+__global__ __launch_bounds__(256) void k_avg_e2n(Dims g, const uint8_t *__restrict__ domain,
                                                  const double *__restrict__ elem, double *__restrict__ nodal) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= g.n_nodes) return;
@@ -334,11 +334,103 @@ __global__ __launch_bounds__(256) void k_avg_e2n(Dims g, const double *__restric
             for (int ox = 1; ox >= 0; ox--) {
                 const int ex = i - ox, ey = j - oy, ez = k - oz;
                 if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
-                if (ep[(ex + 1) + g.px * ((ey + 1) + g.py * (ez + 1))] == 0.0) continue;
-                s += elem[ex + g.nx * (ey + g.ny * ez)];
+                const int e = ex + g.nx * (ey + g.ny * ez);
+                if (!domain[e]) continue;
+                s += elem[e];
                 cnt++;
             }
     nodal[n] = cnt > 0 ? s / cnt : 0.0;
+}
+
+// StaticAnalysis.SampleNodalScalar (StaticAnalysis.cs:278-366), one thread per point: the cell's local
+// coordinates in float32 as the reference forms them from its Vector3 point, origin and float pitch, the trilinear
+// weights in FP64 over the cell's active nodes (added in corner order 0..7). A point outside the grid, or whose
+// cell has no active node weight above 1e-12, is marked for the nearest-node fallback (fallback[q] = 1).
+__global__ __launch_bounds__(256) void k_sample_trilinear(Dims g, const uint8_t *__restrict__ active_nodes,
+                                                          const double *__restrict__ nodal, int n_points,
+                                                          const float *__restrict__ pts, float ox, float oy, float oz,
+                                                          float h, double *__restrict__ out,
+                                                          int *__restrict__ fallback) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n_points) return;
+    const float rx = (pts[3 * q] - ox) / h, ry = (pts[3 * q + 1] - oy) / h, rz = (pts[3 * q + 2] - oz) / h;
+    const double fx = floor(static_cast<double>(rx)), fy = floor(static_cast<double>(ry)),
+                 fz = floor(static_cast<double>(rz));
+    int fb = 1;
+    double result = 0.0;
+    // (int)Math.Floor(r) of a finite value inside the grid; anything else (NaN, huge) is outside.
+    if (fx >= 0.0 && fx < g.nx && fy >= 0.0 && fy < g.ny && fz >= 0.0 && fz < g.nz) {
+        const int ix = static_cast<int>(fx), iy = static_cast<int>(fy), iz = static_cast<int>(fz);
+        const int n0 = ix + g.nx1 * (iy + g.ny1 * iz);
+        const double xi = static_cast<double>(rx) - ix, eta = static_cast<double>(ry) - iy,
+                     zeta = static_cast<double>(rz) - iz;
+        const double w[8] = {(1.0 - xi) * (1.0 - eta) * (1.0 - zeta), xi * (1.0 - eta) * (1.0 - zeta),
+                             xi * eta * (1.0 - zeta),                 (1.0 - xi) * eta * (1.0 - zeta),
+                             (1.0 - xi) * (1.0 - eta) * zeta,         xi * (1.0 - eta) * zeta,
+                             xi * eta * zeta,                         (1.0 - xi) * eta * zeta};
+        double ws = 0.0, vs = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const int nd = n0 + corner_x(a) + g.nx1 * (corner_y(a) + g.ny1 * corner_z(a));
+            if (active_nodes[nd]) {
+                ws += w[a];
+                vs += w[a] * nodal[nd];
+            }
+        }
+        if (ws > 1e-12) {
+            result = vs / ws;
+            fb = 0;
+        }
+    }
+    out[q] = result;
+    fallback[q] = fb;
+}
+
+// The fallback (StaticAnalysis.cs:345-365): the value at the active node nearest to the point, distances squared in
+// float32 from the node positions origin + (i h, j h, k h); the first node in index order wins a tie, as the
+// reference's strict < does. One workgroup per point; points that need no fallback return at once.
+__global__ __launch_bounds__(256) void k_sample_nearest(Dims g, const uint8_t *__restrict__ active_nodes,
+                                                        const double *__restrict__ nodal,
+                                                        const float *__restrict__ pts, float ox, float oy, float oz,
+                                                        float h, const int *__restrict__ fallback,
+                                                        double *__restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ float sd[256];
+    __shared__ int si[256];
+    const int q = blockIdx.x;
+    if (!fallback[q]) return;
+    const float px = pts[3 * q], py = pts[3 * q + 1], pz = pts[3 * q + 2];
+    float best = __builtin_huge_valf();
+    int bi = -1;
+    for (int n = threadIdx.x; n < g.n_nodes; n += 256) {
+        if (!active_nodes[n]) continue;
+        const int i = n % g.nx1, t = n / g.nx1, j = t % g.ny1, k = t / g.ny1;
+        const float dx = px - (ox + static_cast<float>(i) * h), dy = py - (oy + static_cast<float>(j) * h),
+                    dz = pz - (oz + static_cast<float>(k) * h);
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < best) {  // ascending n within the thread: the first minimum stays
+            best = d2;
+            bi = n;
+        }
+    }
+    sd[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const float d2 = sd[threadIdx.x + off];
+            const int i2 = si[threadIdx.x + off];
+            // (distance, index) lexicographic: the smallest distance, then the lowest index
+            if (i2 >= 0 && (si[threadIdx.x] < 0 || d2 < sd[threadIdx.x] ||
+                            (d2 == sd[threadIdx.x] && i2 < si[threadIdx.x]))) {
+                sd[threadIdx.x] = d2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = si[0] >= 0 ? nodal[si[0]] : 0.0;
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -393,7 +485,8 @@ static KuForm ku_auto(const mycel_op *op) {
 // thread.
 template <typename T>
 static KuForm ku_form(const mycel_op *op, bool dot) {
-    if (dot || op->ku_form == MYCEL_MATVEC_AUTO) return ku_auto(op);
+    // The reference-order form has an FP64 kernel of its own (launch_apply_k); other launches take the default.
+    if (dot || op->ku_form == MYCEL_MATVEC_AUTO || op->ku_form == MYCEL_MATVEC_REFERENCE_ORDER) return ku_auto(op);
     if (op->ku_form == MYCEL_MATVEC_NODE_GATHER && op->ku_kz && !zm_fits<T>(op, op->ku_kz))
         return {MYCEL_MATVEC_NODE_GATHER, 0};
     return {op->ku_form, op->ku_kz};
@@ -461,6 +554,10 @@ static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const KeMono<T> &
 
 void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s, double *d_dot_partial,
                     const int *d_stop, const Batch &bt) {
+    if (op->ku_form == MYCEL_MATVEC_REFERENCE_ORDER && !d_dot_partial && bt.count == 1) {
+        launch_apply_k_ordered(op, d_x, d_y, mode, s, d_stop);
+        return;
+    }
     launch_apply_k_t<double>(op, op->ks64, op->km64, op->d_ep.get(), d_x, d_y, mode, s, d_dot_partial, d_stop, bt);
 }
 
@@ -488,8 +585,20 @@ void launch_sensitivities(mycel_op *op, const double *d_u, const double *d_rho, 
 }
 
 void launch_avg_e2n(mycel_op *op, const double *d_elem, double *d_nodal, hipStream_t s) {
-    hipLaunchKernelGGL(k_avg_e2n, dim3(cdiv(op->d.n_nodes, 256)), dim3(256), 0, s, op->d, op->d_ep.get(), d_elem,
+    hipLaunchKernelGGL(k_avg_e2n, dim3(cdiv(op->d.n_nodes, 256)), dim3(256), 0, s, op->d, op->d_domain.get(), d_elem,
                        d_nodal);
+    MYCEL_LAUNCH_CHECK();
+}
+
+void launch_sample_nodal(mycel_op *op, const double *d_nodal, int n_points, const float *d_pts, const float *origin,
+                         double *d_out, int *d_fallback, hipStream_t s) {
+    if (n_points <= 0) return;
+    const float h = static_cast<float>(op->h);
+    hipLaunchKernelGGL(k_sample_trilinear, dim3(cdiv(n_points, 256)), dim3(256), 0, s, op->d,
+                       op->d_active_nodes.get(), d_nodal, n_points, d_pts, origin[0], origin[1], origin[2], h, d_out,
+                       d_fallback);
+    hipLaunchKernelGGL(k_sample_nearest, dim3(n_points), dim3(256), 0, s, op->d, op->d_active_nodes.get(), d_nodal,
+                       d_pts, origin[0], origin[1], origin[2], h, d_fallback, d_out);
     MYCEL_LAUNCH_CHECK();
 }
 

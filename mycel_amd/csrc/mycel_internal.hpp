@@ -116,6 +116,7 @@ void constitutive(double e, double nu, double *d36);
 
 struct PcgWork;
 struct Gmg;
+struct OrdWork;
 
 // Several right-hand sides in one launch: grid dimension y is the RHS index; vectors of RHS k start at
 // k * vstride, block partials at k * pstride, and the solver-state word that stops a launch at k * sstride
@@ -162,6 +163,7 @@ struct mycel_op {
     mycel::DevBuf<float> d_ep32;       // FP32 copy of the same
     mycel::DevBuf<uint8_t> d_flags;    // per DOF
     mycel::DevBuf<uint8_t> d_domain;   // per box element (active mask)
+    mycel::DevBuf<uint8_t> d_active_nodes;  // per box node (FeModel.ActiveNodes)
     mycel::DevBuf<int32_t> d_active;   // active list
     mycel::DevBuf<double> d_density;   // box densities (ElementDensity)
     mycel::DevBuf<double> d_diag;      // DiagA
@@ -171,6 +173,7 @@ struct mycel_op {
     mycel::DevBuf<double> d_part;      // block partials for one-off reductions
     mycel::PcgWork *pcg = nullptr;
     mycel::Gmg *gmg = nullptr;         // multigrid hierarchy (built on first use)
+    mycel::OrdWork *ord = nullptr;     // workspace of the reference-order solver (built on first use)
     uint64_t epoch = 0;                // bumped whenever the element moduli change
 };
 
@@ -191,6 +194,9 @@ void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t
 void launch_sensitivities(mycel_op *op, const double *d_u, const double *d_rho, double *d_dc, double penal,
                           hipStream_t s);
 void launch_avg_e2n(mycel_op *op, const double *d_elem, double *d_nodal, hipStream_t s);
+// SampleNodalScalar at n_points points (float32 xyz triples); d_fallback: n_points ints of scratch.
+void launch_sample_nodal(mycel_op *op, const double *d_nodal, int n_points, const float *d_pts, const float *origin,
+                         double *d_out, int *d_fallback, hipStream_t s);
 
 // Deterministic FP64 reductions (fixed block partition, fixed tree) on the op's geometry.
 double reduce_dot(mycel_op *op, const double *d_a, const double *d_b, int n, hipStream_t s);
@@ -209,6 +215,22 @@ void pcg_free(PcgWork *w);
 // own state and stopping test; MYCEL_PRECISION_MIXED_GMG only. res: R results.
 int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, double tol, int max_iter, int precision,
                         mycel_pcg_result *res);
+
+// ---- ordered.hip: the reference's evaluation order (MYCEL_PRECISION_FP64_ORDERED) ---------------
+// K*u with the reference's per-DOF colour order; d_halt: a PCG halt flag that turns the launch into a no-op.
+void launch_apply_k_ordered(mycel_op *op, const double *d_x, double *d_y, int mode, hipStream_t s,
+                            const int *d_halt = nullptr);
+int32_t pcg_solve_ordered(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter,
+                          const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
+                          mycel_pcg_result *res);
+// Sequential sum over i ascending of a_i * b_i (b null: a_i), skipping DOFs d_flags marks constrained.
+double seq_sum(mycel_op *op, const double *d_a, const double *d_b, const uint8_t *d_flags, int n, hipStream_t s);
+double oc_update_ordered(mycel_op *op, const double *d_rho, const double *d_dc, const double *d_s,
+                         const uint8_t *d_passive, double *d_rho_new, int n, double vf, double move, double rho_min,
+                         double lambda_guess, int max_bisect, double vol_tol, hipStream_t s, int *evals);
+void simp_sums_ordered(mycel_op *op, int na, const double *d_rho_bar, const double *d_grey_src, double *out3,
+                       hipStream_t s);
+void ord_free(OrdWork *w);
 
 // ---- gmg.hip ---------------------------------------------------------------------------------
 // Builds the hierarchy on first use and refreshes its moduli, constraints and coarsest inverse when

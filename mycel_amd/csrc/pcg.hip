@@ -56,6 +56,7 @@ struct PcgState {
     int last_iter, pad;
     double best_true;    // mixed: smallest true (FP64) relres seen at a reliable update or at the start
     int no_gain, pad2;   // mixed: consecutive reliable updates that did not improve on best_true
+    double value;        // the quantity a breakdown message quotes (pAp, alpha or beta)
 };
 
 struct PcgWork {
@@ -108,7 +109,8 @@ constexpr int kReliableStall = 3;
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __fma_rn(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __fmaf_rn(a, b, c); }
 
-__device__ __forceinline__ void halt_with(PcgState *st, int message, int iterations, double rel) {
+__device__ __forceinline__ void halt_with(PcgState *st, int message, int iterations, double rel, double value = 0.0) {
+    st->value = value;
     st->message = message;
     st->iterations = iterations;
     st->final_rel = rel;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_xr(int n, int slot, PcgState
     if (curv || step) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             halt_with(st, curv ? MYCEL_PCG_CURVATURE_BREAKDOWN : MYCEL_PCG_STEP_BREAKDOWN, st->iter[slot],
-                      st->prev_rel[slot]);
+                      st->prev_rel[slot], curv ? pap : alpha);
         return;
     }
     const T a = static_cast<T>(alpha);
@@ -233,7 +235,7 @@ __device__ __forceinline__ bool pcg_step4(PcgState *__restrict__ st, const Step4
     }
     beta = rzn / in.rz;
     if (!isfinite(beta) || beta < 0.0) {  // :334
-        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel, beta);
         return false;
     }
     if (lead) {
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, Pcg
     if (curv || step) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             halt_with(st, curv ? MYCEL_PCG_CURVATURE_BREAKDOWN : MYCEL_PCG_STEP_BREAKDOWN, st->iter[slot],
-                      st->prev_rel[slot]);
+                      st->prev_rel[slot], curv ? pap : alpha);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[slot] = alpha;
@@ -369,7 +371,7 @@ __device__ __forceinline__ bool fpcg_step4(PcgState *__restrict__ st, const Step
     }
     beta = -in.alpha * zap / in.rz;
     if (!isfinite(beta)) {
-        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        if (lead) halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel, beta);
         return false;
     }
     beta = fmax(beta, 0.0);
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(256) void k_mixed_resume(PcgState *__restrict__ st,
     }
     const double beta = rzn / st->rz[st->pause_slot];
     if (!isfinite(beta) || beta < 0.0) {
-        halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel);
+        halt_with(st, MYCEL_PCG_DIRECTION_BREAKDOWN, t, rel, beta);
         return;
     }
     st->beta = beta;
@@ -927,6 +929,8 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
     *res = mycel_pcg_result{};
+    if (precision == MYCEL_PRECISION_FP64_ORDERED)
+        return pcg_solve_ordered(op, d_b, d_x, tol, max_iter, cancel, cb, user, res);
     if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32 &&
         precision != MYCEL_PRECISION_MIXED_GMG)
         throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
@@ -1096,6 +1100,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     res->iterations = h.iterations;
     res->final_residual = h.final_rel;
     res->message = h.message;
+    res->message_value = h.value;
     res->converged = h.message == MYCEL_PCG_CONVERGED;
     res->outer_corrections = h.outer;
     if (gmg) {
@@ -1110,6 +1115,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             res->iterations = h.iterations + r64.iterations;
             res->final_residual = r64.final_residual;
             res->message = r64.message;
+            res->message_value = r64.message_value;
             res->converged = r64.converged;
             res->fallback_iterations = r64.iterations;
         }
@@ -1125,6 +1131,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             res->iterations = h.iterations + r64.iterations;
             res->final_residual = r64.final_residual;
             res->message = r64.message;
+            res->message_value = r64.message_value;
             res->converged = r64.converged;
             res->fallback_iterations = r64.iterations;
         } else {  // fold the last FP32 increment (zero after a converging update)
@@ -1239,6 +1246,7 @@ int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, dou
         r.iterations = h.iterations;
         r.final_residual = h.final_rel;
         r.message = h.message;
+        r.message_value = h.value;
         r.converged = h.message == MYCEL_PCG_CONVERGED || h.message == MYCEL_PCG_RHS_ZERO ||
                       h.message == MYCEL_PCG_INITIAL_CONVERGED;
         const bool broke = h.message == MYCEL_PCG_CURVATURE_BREAKDOWN || h.message == MYCEL_PCG_STEP_BREAKDOWN ||
@@ -1250,6 +1258,7 @@ int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, dou
             r.iterations = h.iterations + r64.iterations;
             r.final_residual = r64.final_residual;
             r.message = r64.message;
+            r.message_value = r64.message_value;
             r.converged = r64.converged;
             r.fallback_iterations = r64.iterations;
         }

@@ -349,7 +349,10 @@ class PcgResult:
 
 
 def _pcg_result(r: L.PcgResultC) -> PcgResult:
-    return PcgResult(r.iterations, r.final_residual, bool(r.converged), r.elapsed_ms, L.PCG_MESSAGES.get(r.message),
+    buf = C.create_string_buffer(512)
+    L.check(L.load().mycel_pcg_message(C.byref(r), buf, len(buf)))
+    text = buf.value.decode("utf-8") or None  # PcgResult.Message is null for a converged solve
+    return PcgResult(r.iterations, r.final_residual, bool(r.converged), r.elapsed_ms, text,
                      r.message, r.outer_corrections, r.fallback_iterations)
 
 
@@ -491,6 +494,25 @@ class StaticAnalysis:
         out = np.zeros(op.Model.TotalNodes)
         L.check(L.load().mycel_average_element_to_node(op._h, L.ptr(e), L.ptr(out)))
         return out
+
+    @staticmethod
+    def SampleNodalScalars(op: FemOperator, nodal_scalar, points):
+        """StaticAnalysis.SampleNodalScalar (Fem/StaticAnalysis.cs:278-366) at many points: points is (n, 3), taken
+        as float32 (System.Numerics.Vector3); returns n FP64 values."""
+        nd = L.f64(nodal_scalar)
+        if nd.size != op.Model.TotalNodes:
+            raise L.ArgumentError(L.E_ARGUMENT, "nodalScalar length must equal TotalNodes.")
+        pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+        org = np.ascontiguousarray(np.asarray(op.Model.Grid.Origin, np.float32))
+        out = np.zeros(pts.shape[0])
+        L.check(L.load().mycel_sample_nodal_scalar(op._h, L.ptr(nd), int(pts.shape[0]), L.ptr(pts), L.ptr(org),
+                                                   L.ptr(out)))
+        return out
+
+    @staticmethod
+    def SampleNodalScalar(op: FemOperator, nodal_scalar, point):
+        """StaticAnalysis.SampleNodalScalar at one point (x, y, z)."""
+        return float(StaticAnalysis.SampleNodalScalars(op, nodal_scalar, [point])[0])
 
     @staticmethod
     def ComputeElementVonMises(op: FemOperator, u):

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import Callable, List, Optional
+from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
@@ -146,6 +146,12 @@ class OptimizerOptions:
     Precision: int = L.PRECISION_FP64
     InitialLambda: float = 1.0  # the OC multiplier's first guess (TopologyOptimizer.cs:154); a resume passes the saved one
     InitialDesign64: Optional[np.ndarray] = None  # FP64 initial design (box- or active-sized): an exact resume
+    # The loop state of an exact resume (no reference counterpart): the previous iteration's max change (None:
+    # restart the tolerance schedule as the reference's StartIteration does), the Heaviside beta in effect and
+    # the last <= 6 compliances, oldest first.
+    ResumeMaxChange: Optional[float] = None
+    ResumeBeta: float = 1.0
+    ResumeComplianceHistory: Optional[Sequence[float]] = None
 
 
 @dataclass
@@ -169,6 +175,7 @@ class IterationSnapshot:
     TotalMs: float = 0.0
     PhysicalDensity: Optional[np.ndarray] = None
     DesignDensity: Optional[np.ndarray] = None
+    BetaNext: float = 1.0  # the Heaviside beta of the next iteration (resume state)
 
 
 @dataclass
@@ -190,7 +197,24 @@ class TopologyOptimizationResult:
 def _snap(s: L.SnapshotC) -> IterationSnapshot:
     return IterationSnapshot(s.iteration, s.compliance, s.volume_fraction, s.max_change, s.grey_level,
                              s.solver_iterations, s.solver_residual, s.max_displacement, s.lambda_, s.solve_ms,
-                             s.sensitivity_ms, s.filter_ms, s.update_ms, s.total_ms)
+                             s.sensitivity_ms, s.filter_ms, s.update_ms, s.total_ms, BetaNext=s.beta_next)
+
+
+def simp_options_c(o: "OptimizerOptions") -> L.SimpOptionsC:
+    """The C options struct of an OptimizerOptions (initial_design is set by the caller that owns the array)."""
+    co = L.SimpOptionsC(o.VolumeFraction, o.FilterRadiusMm, o.Penal, o.MoveLimit, o.MaxIterations,
+                        o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
+                        int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
+                        int(o.Continuation), int(o.Precision), int(o.EnableHeaviside), float(o.InitialLambda))
+    if o.ResumeMaxChange is not None:
+        hist = list(o.ResumeComplianceHistory or [])[-6:]
+        co.resume_state = 1
+        co.resume_max_change = float(o.ResumeMaxChange)
+        co.resume_beta = float(o.ResumeBeta)
+        co.resume_history_len = len(hist)
+        for k, c in enumerate(hist):
+            co.resume_history[k] = float(c)
+    return co
 
 
 class TopologyOptimizer:
@@ -221,10 +245,7 @@ class TopologyOptimizer:
     @staticmethod
     def _run(model, f, n_cases, options, progress, cancel, op):
         o = options or OptimizerOptions()
-        co = L.SimpOptionsC(o.VolumeFraction, o.FilterRadiusMm, o.Penal, o.MoveLimit, o.MaxIterations,
-                            o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
-                            int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
-                            int(o.Continuation), int(o.Precision), int(o.EnableHeaviside), float(o.InitialLambda))
+        co = simp_options_c(o)
         design64 = None
         if o.InitialDesign64 is not None:
             design64 = np.ascontiguousarray(o.InitialDesign64, np.float64).ravel()

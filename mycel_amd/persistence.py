@@ -50,6 +50,7 @@ DOMN_HEADER = 20
 FNV_OFFSET = 2166136261
 FNV_PRIME = 16777619
 _EXT_MAGIC = b"FP64"
+_LOOP_MAGIC = b"SIMP"
 
 
 # ---- Brotli ---------------------------------------------------------------------------------------
@@ -390,6 +391,11 @@ class Checkpoint:
     DesignDensity64: Optional[np.ndarray] = None   # FP64, active-compact
     Displacements: Optional[np.ndarray] = None     # FP64 box DOFs (or (n_cases, DOFs))
     Lambda: Optional[float] = None
+    # The loop state after the checkpointed iteration (the "SIMP" block of the extension): its max change, the
+    # Heaviside beta of the next iteration and the last <= 6 compliances, oldest first.
+    MaxChange: Optional[float] = None
+    BetaNext: Optional[float] = None
+    ComplianceHistory: Optional[List[float]] = None
 
 
 # ---- file helpers -----------------------------------------------------------------------------------
@@ -448,9 +454,10 @@ def encode_iteration(nx: int, ny: int, nz: int, iteration: int, compliance: floa
 
 
 def encode_checkpoint(iteration: int, design, physical, design64=None, displacements=None,
-                      lam: Optional[float] = None) -> bytes:
+                      lam: Optional[float] = None, loop_state=None) -> bytes:
     """checkpoint_latest.bin.br (RunWriter.cs:263-295); with design64 / displacements / lam the FP64 block is
-    appended after the two float32 arrays (version 2)."""
+    appended after the two float32 arrays (version 2), and with loop_state = (max_change, beta_next,
+    compliance_history) a "SIMP" block after it."""
     d = np.ascontiguousarray(design, np.float32).ravel()
     p = np.ascontiguousarray(physical, np.float32).ravel()
     if p.size != d.size:
@@ -463,6 +470,11 @@ def encode_checkpoint(iteration: int, design, physical, design64=None, displacem
         cases = u.shape[0] if u.ndim == 2 else 1
         payload += _EXT_MAGIC + struct.pack("<dqqi", float(lam) if lam is not None else float("nan"), g.size,
                                             u.size, cases) + g.tobytes() + u.ravel().tobytes()
+        if loop_state is not None:
+            mc, beta, hist = loop_state
+            hist = [float(c) for c in list(hist)[-6:]]
+            payload += _LOOP_MAGIC + struct.pack("<ddi6d", float(mc), float(beta), len(hist),
+                                                 *(hist + [0.0] * (6 - len(hist))))
     return b"CHKP" + struct.pack("<iii", 2 if ext else 1, iteration, d.size) + brotli_compress(payload)
 
 
@@ -486,6 +498,11 @@ def decode_checkpoint(blob: bytes) -> Checkpoint:
         ck.DesignDensity64 = g if ng else None
         ck.Displacements = (u.reshape(cases, -1) if cases > 1 else u) if nu else None
         ck.Lambda = None if math.isnan(lam) else lam
+        tail = rest[o + 8 * (ng + nu):]
+        if tail[:4] == _LOOP_MAGIC:
+            vals = struct.unpack_from("<ddi6d", tail, 4)
+            ck.MaxChange, ck.BetaNext = vals[0], vals[1]
+            ck.ComplianceHistory = list(vals[3:3 + max(0, min(6, vals[2]))])
     return ck
 
 
@@ -510,6 +527,15 @@ class RunWriter:
         self.Metadata = metadata
         self._keep = max(1, int(metadata.Options.get("KeepEveryN", 1)))
         self.WrittenSnapshotCount = 0
+        # The last six compliances, carried in the checkpoint's loop state; a resumed writer continues them.
+        self._history: List[float] = []
+        self._history_iter = -1
+        if is_resume and os.path.exists(self._checkpoint):
+            try:
+                with open(self._checkpoint, "rb") as fh:
+                    self._history = list(decode_checkpoint(fh.read()).ComplianceHistory or [])
+            except (OSError, ValueError, EOFError, struct.error):
+                self._history = []
         self.RollingStreamChecksum = FNV_OFFSET
         os.makedirs(self._iter_dir, exist_ok=True)
         _cleanup_tmp(self.RunFolder)
@@ -581,8 +607,13 @@ class RunWriter:
         design = getattr(snapshot, "DesignDensity", None)
         d = np.asarray(design if design is not None else np.zeros(0), np.float32)
         p = np.asarray(phys if phys is not None else np.zeros(0), np.float32)
-        lam = getattr(snapshot, "Lambda", None) if (displacements is not None or design64 is not None) else None
-        _atomic_write(self._checkpoint, encode_checkpoint(it, d, p, design64, displacements, lam))
+        if it != self._history_iter:  # a final re-write of the same iteration does not add to the history
+            self._history = (self._history + [float(snapshot.Compliance)])[-6:]
+            self._history_iter = it
+        ext = displacements is not None or design64 is not None
+        lam = getattr(snapshot, "Lambda", None) if ext else None
+        loop = (snapshot.MaxChange, getattr(snapshot, "BetaNext", 1.0), self._history) if ext else None
+        _atomic_write(self._checkpoint, encode_checkpoint(it, d, p, design64, displacements, lam, loop))
         self.RollingStreamChecksum = ((self.RollingStreamChecksum ^ h) * FNV_PRIME) & 0xFFFFFFFF
         self.WrittenSnapshotCount += 1
         total = _dir_bytes(self.RunFolder)
@@ -807,6 +838,10 @@ def resume_options(checkpoint: Checkpoint, options):
 
     o = dataclasses.replace(options)
     o.StartIteration = int(checkpoint.Iteration) + 1
+    if checkpoint.MaxChange is not None:  # continue the tolerance schedule, beta and the stop test exactly
+        o.ResumeMaxChange = float(checkpoint.MaxChange)
+        o.ResumeBeta = float(checkpoint.BetaNext if checkpoint.BetaNext is not None else 1.0)
+        o.ResumeComplianceHistory = list(checkpoint.ComplianceHistory or [])
     if checkpoint.DesignDensity64 is not None:
         o.InitialDesign64 = checkpoint.DesignDensity64
     else:

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib as L
 from .fem import FeModel
-from .optimization import OptimizerOptions
+from .optimization import OptimizerOptions, simp_options_c
 
 JOB_FEA = 0
 JOB_SIMP = 1
@@ -41,10 +41,7 @@ def _info(i: L.JobInfoC) -> JobInfo:
 
 
 def _simp_c(o: OptimizerOptions) -> L.SimpOptionsC:
-    return L.SimpOptionsC(o.VolumeFraction, o.FilterRadiusMm, o.Penal, o.MoveLimit, o.MaxIterations,
-                          o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
-                          int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
-                          int(o.Continuation), int(o.Precision), int(o.EnableHeaviside), float(o.InitialLambda))
+    return simp_options_c(o)
 
 
 class JobServer:
