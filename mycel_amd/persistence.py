@@ -192,23 +192,24 @@ def fmt_exp(x: float, digits: int) -> str:
     return f"{m:.{digits}f}E{'+' if e >= 0 else '-'}{abs(e):03d}"
 
 
-def fmt_g17(x: float) -> str:
-    """'G17': 17 significant digits, trailing zeros dropped; scientific (E+dd) when the exponent is < -5 or
-    >= 17, fixed-point otherwise."""
+def fmt_g(x: float, precision: int) -> str:
+    """'G<precision>': that many significant digits, trailing zeros dropped; fixed-point when the scientific
+    exponent is greater than -5 and less than the precision, scientific (E+dd / E-dd) otherwise."""
     s = _nonfinite(x)
     if s is not None:
         return s
     if x == 0.0:
         return "-0" if math.copysign(1.0, x) < 0 else "0"
+    q = Decimal(1).scaleb(-(precision - 1))
     d = Decimal(x)
     e = d.adjusted()
-    m = d.scaleb(-e).quantize(Decimal("1.0000000000000000"), rounding=ROUND_HALF_UP)
+    m = d.scaleb(-e).quantize(q, rounding=ROUND_HALF_UP)
     if abs(m) >= 10:
         e += 1
-        m = d.scaleb(-e).quantize(Decimal("1.0000000000000000"), rounding=ROUND_HALF_UP)
+        m = d.scaleb(-e).quantize(q, rounding=ROUND_HALF_UP)
     digits = str(abs(m)).replace(".", "").rstrip("0") or "0"
     sign = "-" if x < 0 else ""
-    if e < -5 or e >= 17:
+    if e <= -5 or e >= precision:
         mant = digits[0] + ("." + digits[1:] if len(digits) > 1 else "")
         return f"{sign}{mant}E{'+' if e >= 0 else '-'}{abs(e):02d}"
     if e >= 0:
@@ -216,6 +217,11 @@ def fmt_g17(x: float) -> str:
             return sign + digits + "0" * (e + 1 - len(digits))
         return sign + digits[:e + 1] + "." + digits[e + 1:]
     return sign + "0." + "0" * (-e - 1) + digits
+
+
+def fmt_g17(x: float) -> str:
+    """'G17' (the round-trip form RunWriter uses for the compliance)."""
+    return fmt_g(x, 17)
 
 
 def format_metrics_row(iteration: int, compliance: float, volume_fraction: float, max_change: float,

@@ -120,8 +120,9 @@ def test_apply_k_element_forms(gpu, idx):
 
 
 def test_matvec_form_arguments(gpu):
-    """mycel_op_set_matvec_form accepts the automatic form, gather columns 0/2/4/8 and element layers 2/4/8,
-    and rejects anything else with MYCEL_E_ARGUMENT (ArgumentError) without changing the operator."""
+    """mycel_op_set_matvec_form accepts the automatic form, gather columns 0/2/4/8, element layers 2/4/8 and the
+    reference-order form, and rejects anything else with MYCEL_E_ARGUMENT (ArgumentError) without changing the
+    operator."""
     import mycel_amd as M
     from mycel_amd import ArgumentError
 
@@ -131,14 +132,14 @@ def test_matvec_form_arguments(gpu):
         x = random_field(5, p.n_dofs, -1, 1)
         y0 = op.ApplyK(x)
         for form, cols in ((M.MATVEC_NODE_GATHER, 1), (M.MATVEC_NODE_GATHER, 3), (M.MATVEC_ELEMENT, 0),
-                           (M.MATVEC_ELEMENT, 5), (3, 2), (-1, 0)):
+                           (M.MATVEC_ELEMENT, 5), (4, 2), (-1, 0)):
             with pytest.raises(ArgumentError):
                 op.SetMatvecForm(form, cols)
         with pytest.raises(ArgumentError):
             op.SetMatvecColumns(3)
         assert np.array_equal(op.ApplyK(x), y0)
         for form, cols in ((M.MATVEC_NODE_GATHER, 0), (M.MATVEC_NODE_GATHER, 8), (M.MATVEC_ELEMENT, 8),
-                           (M.MATVEC_AUTO, 0)):
+                           (M.MATVEC_REFERENCE_ORDER, 0), (M.MATVEC_AUTO, 0)):
             op.SetMatvecForm(form, cols)
             assert rel(op.ApplyK(x), y0) < 1e-13
     finally:
