@@ -67,8 +67,9 @@ extern "C" {
  *   the OC volumes, the compliance and the snapshot sums sequentially.
  * MIXED_GMG: FP64 flexible PCG (FP64 operator, residual, dot products, convergence test and update)
  *   preconditioned by one FP32 geometric-multigrid V-cycle per iteration in place of Jacobi: a second
- *   IPreconditioner (Fem/IPreconditioner.cs:9-20). Needs a grid that coarsens by 2 per axis down to
- *   <= 140 DOFs; otherwise MYCEL_E_ARGUMENT. */
+ *   IPreconditioner (Fem/IPreconditioner.cs:9-20). On a grid that does not coarsen by 2 per axis down to
+ *   <= 140 DOFs (an axis reaches one element first) the solve runs the FP64 Jacobi iteration instead and reports
+ *   all its iterations as fallback_iterations. */
 #define MYCEL_PRECISION_FP64 0
 #define MYCEL_PRECISION_MIXED_FP32 1
 #define MYCEL_PRECISION_MIXED_BF16 2
@@ -110,9 +111,10 @@ typedef struct {
 } mycel_pcg_result;
 
 /* IProgress<PcgProgress> (Fem/PcgResult.cs:5). Called on the calling thread between iterations, once per batch
- * of iterations the device ran without a host round trip (32 Jacobi or 4 multigrid iterations), with the
- * latest iteration and relative residual; after a solve in the same FP64 mode converged, the first batches
- * of the next solve (covering up to 3/4 of that solve's count) run without a report. */
+ * of iterations the device ran without a host round trip (32 Jacobi, 4 multigrid or 16 reference-order
+ * iterations), with the latest iteration and relative residual. A solve given a progress callback or a cancel
+ * flag reads the device state after every batch; one given neither runs the first batches of a solve blind
+ * (covering up to 3/4 of the previous converged Jacobi solve's count, 1/2 of the multigrid one's). */
 typedef void (*mycel_pcg_progress_cb)(int32_t iteration, double relative_residual, void *user);
 
 /* ---- library ------------------------------------------------------------------------------ */
@@ -169,6 +171,13 @@ MYCEL_API int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *
  * 8 node layers per tile (all bit-identical to each other). MYCEL_MATVEC_AUTO (columns ignored): by grid size.
  * The two families agree to rounding (both within 1e-12 of the FP64 reference). */
 MYCEL_API int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns);
+/* Island deflation of the multigrid preconditioner (no counterpart in the reference): with solid_ratio > 0, every
+ * 26-connected cluster of elements with E >= solid_ratio * E0 that touches no fixed DOF (a floating island, held
+ * only by the void around it) adds its six rigid-body modes to the V-cycle as an additive coarse correction
+ * z = V(r) + Z (Z^T A Z)^-1 Z^T r, formed in FP64 (up to 64 islands, the largest). Those modes are near-null
+ * modes of A at high stiffness contrast that the smoother and the geometric coarse levels do not represent. The
+ * islands are found again whenever the densities change. 0 (the default) turns it off. */
+MYCEL_API int32_t mycel_op_set_island_deflation(mycel_op *op, double solid_ratio);
 /* The gather form with columns 0 / 2 / 4 / 8, or -1 for MYCEL_MATVEC_AUTO. */
 MYCEL_API int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns);
 /* FemOperator.DiagA / Ke / ConstrainedDofs / ActiveElements properties. */
@@ -182,7 +191,7 @@ MYCEL_API void *mycel_op_stream(mycel_op *op);
 /* ---- PcgSolver (Fem/PcgSolver.cs) + JacobiPreconditioner (Fem/JacobiPreconditioner.cs) ---- */
 /* PcgSolver.Solve(op, b, x, new JacobiPreconditioner(), tol, maxIterations, progress, ct)
  * (Fem/PcgSolver.cs:80-370). x is the warm start on entry and the solution on return.
- * cancel: optional host flag polled between device batches (nonzero = cancel -> MYCEL_E_CANCELED).
+ * cancel: optional host flag polled after every device batch (nonzero = cancel -> MYCEL_E_CANCELED).
  * Returns MYCEL_E_INVALID_OPERATION when fewer than 6 fixed DOFs sit on active nodes. */
 MYCEL_API int32_t mycel_pcg_solve(mycel_op *op, const double *b, double *x, double tol, int32_t max_iter,
                         int32_t precision, const volatile int32_t *cancel,

@@ -934,6 +934,13 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32 &&
         precision != MYCEL_PRECISION_MIXED_GMG)
         throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
+    if (precision == MYCEL_PRECISION_MIXED_GMG && !gmg_supported(op)) {
+        // A grid too thin for the hierarchy (an axis reaches one element above the dense coarsest size): the
+        // reference's FP64 Jacobi iteration, reported as fallback iterations.
+        const int32_t st = pcg_solve_device(op, d_b, d_x, tol, max_iter, MYCEL_PRECISION_FP64, cancel, cb, user, res);
+        res->fallback_iterations = res->iterations;
+        return st;
+    }
     const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32;
     const bool gmg = precision == MYCEL_PRECISION_MIXED_GMG;
     if (op->active_fixed < 6)  // PcgSolver.cs:95-100
@@ -1062,7 +1069,8 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     // of the previous converged solve's iterations and the Jacobi mode three quarters.
     int &last_iters = gmg ? w->last_iters_gmg : w->last_iters_64;
     const long long prior = std::min(last_iters, max_iter);
-    const long long blind = mixed ? 0 : gmg ? prior / 2 / kGmgGraphIters : prior * 3 / 4 / kGraphIters;
+    // A caller that can cancel or watches progress gets a state read after every batch.
+    const long long blind = (mixed || cancel || cb) ? 0 : gmg ? prior / 2 / kGmgGraphIters : prior * 3 / 4 / kGraphIters;
     for (long long batch = 0;; batch++) {
         if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "PCG batch loop did not terminate");
         if (cancel && *cancel) {
@@ -1153,6 +1161,13 @@ int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, dou
     if (R < 1) throw Error(MYCEL_E_ARGUMENT, "at least one right-hand side is required");
     if (precision != MYCEL_PRECISION_MIXED_GMG)
         throw Error(MYCEL_E_ARGUMENT, "several right-hand sides in one solve need MYCEL_PRECISION_MIXED_GMG");
+    if (!gmg_supported(op)) {  // no hierarchy on this grid: one FP64 Jacobi solve per right-hand side
+        for (int k = 0; k < R; k++) {
+            pcg_solve_device(op, d_B + static_cast<size_t>(k) * op->d.n_dofs, d_X + static_cast<size_t>(k) * op->d.n_dofs,
+                             tol, max_iter, MYCEL_PRECISION_MIXED_GMG, nullptr, nullptr, nullptr, &res[k]);
+        }
+        return MYCEL_OK;
+    }
     if (op->active_fixed < 6)  // PcgSolver.cs:95-100
         throw Error(MYCEL_E_INVALID_OPERATION,
                     "Model has fewer than 6 unique fixed DOFs on active nodes (found " +

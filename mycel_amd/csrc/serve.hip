@@ -30,6 +30,15 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// What a run produces; a worker fills its own copy and publishes it into the job under the server lock.
+struct JobResult {
+    std::vector<double> u, rho_bar;
+    double compliance = 0, max_disp = 0;
+    int32_t iterations = 0, simp_iterations = 0, converged = 0;
+    int32_t status = MYCEL_OK;
+    std::string error;
+};
+
 struct Job {
     int64_t id = 0;
     mycel_job_desc desc{};
@@ -37,12 +46,7 @@ struct Job {
     std::vector<int32_t> fixed, patch;
     std::vector<double> forces, density;
     double footprint = 0;
-    // results
-    std::vector<double> u, rho_bar;
-    double compliance = 0, max_disp = 0;
-    int32_t iterations = 0, simp_iterations = 0, converged = 0;
-    int32_t status = MYCEL_OK;
-    std::string error;
+    JobResult res;  // written only under the server lock, when the job finishes
     int32_t state = MYCEL_JOB_QUEUED;
     Clock::time_point t_submit, t_start, t_end;
 };
@@ -85,7 +89,8 @@ struct mycel_server {
 namespace mycel {
 namespace {
 
-void run_job(Job &j, int device) {
+// Runs a job from its own input copies (read-only while it runs) into r.
+void run_job(const Job &j, int device, JobResult &r) {
     MYCEL_HIP(hipSetDevice(device));
     const mycel_job_desc &d = j.desc;
     std::unique_ptr<mycel_op, void (*)(mycel_op *)> op(
@@ -93,27 +98,27 @@ void run_job(Job &j, int device) {
                   d.emin_ratio, d.penal, j.density.empty() ? nullptr : j.density.data()),
         op_destroy);
     const size_t ndof = static_cast<size_t>(op->d.n_dofs);
-    j.u.assign(ndof, 0.0);
+    r.u.assign(ndof, 0.0);
     if (d.kind == MYCEL_JOB_FEA) {
-        mycel_pcg_result r{};
-        static_analysis_run(op.get(), j.forces.data(), j.u.data(), d.tol, d.max_iter, true, d.precision,
-                            &j.compliance, &j.max_disp, nullptr, nullptr, &r);
-        j.iterations = r.iterations;
-        j.converged = r.converged;
+        mycel_pcg_result pr{};
+        static_analysis_run(op.get(), j.forces.data(), r.u.data(), d.tol, d.max_iter, true, d.precision,
+                            &r.compliance, &r.max_disp, nullptr, nullptr, &pr);
+        r.iterations = pr.iterations;
+        r.converged = pr.converged;
     } else {
         mycel_simp_result res{};
         const int na = static_cast<int>(op->active.size());
-        j.rho_bar.assign(std::max(1, na), 0.0);
+        r.rho_bar.assign(std::max(1, na), 0.0);
         std::vector<mycel_snapshot> snaps(std::max(1, d.simp.max_iterations - d.simp.start_iteration + 1));
         simp_run(op.get(), j.forces.data(), 1, d.simp, nullptr, j.patch.data(), static_cast<int>(j.patch.size()),
                  nullptr, 0, nullptr, nullptr, nullptr, &res, snaps.data(), static_cast<int>(snaps.size()), nullptr,
-                 j.rho_bar.data(), j.u.data());
-        j.rho_bar.resize(na);
-        j.compliance = res.final_compliance;
-        j.simp_iterations = res.total_iterations;
-        j.converged = res.converged;
+                 r.rho_bar.data(), r.u.data());
+        r.rho_bar.resize(na);
+        r.compliance = res.final_compliance;
+        r.simp_iterations = res.total_iterations;
+        r.converged = res.converged;
         for (int i = 0; i < std::min<int>(res.total_iterations, static_cast<int>(snaps.size())); i++)
-            j.iterations += snaps[i].solver_iterations;
+            r.iterations += snaps[i].solver_iterations;
     }
 }
 
@@ -137,20 +142,25 @@ void worker_loop(mycel_server *sv) {
             job->state = MYCEL_JOB_RUNNING;
             job->t_start = Clock::now();
         }
+        JobResult r;
         try {
-            run_job(*job, sv->device);
-            job->status = MYCEL_OK;
+            run_job(*job, sv->device, r);
+            r.status = MYCEL_OK;
         } catch (const Error &e) {
-            job->status = e.status;
-            job->error = e.what();
+            r.status = e.status;
+            r.error = e.what();
+        } catch (const std::bad_alloc &) {
+            r.status = MYCEL_E_OUT_OF_MEMORY;
+            r.error = "host allocation failed";
         } catch (const std::exception &e) {
-            job->status = MYCEL_E_DEVICE;
-            job->error = e.what();
+            r.status = MYCEL_E_DEVICE;
+            r.error = e.what();
         }
         {
             std::lock_guard<std::mutex> lk(sv->mu);
+            job->res = std::move(r);
             job->t_end = Clock::now();
-            job->state = job->status == MYCEL_OK ? MYCEL_JOB_DONE : MYCEL_JOB_FAILED;
+            job->state = job->res.status == MYCEL_OK ? MYCEL_JOB_DONE : MYCEL_JOB_FAILED;
             sv->in_use -= job->footprint;
             sv->active--;
         }
@@ -162,12 +172,12 @@ void worker_loop(mycel_server *sv) {
 void fill_info(const Job &j, mycel_job_info *info) {
     *info = mycel_job_info{};
     info->state = j.state;
-    info->status = j.status;
-    info->compliance = j.compliance;
-    info->max_displacement = j.max_disp;
-    info->pcg_iterations = j.iterations;
-    info->simp_iterations = j.simp_iterations;
-    info->converged = j.converged;
+    info->status = j.res.status;
+    info->compliance = j.res.compliance;
+    info->max_displacement = j.res.max_disp;
+    info->pcg_iterations = j.res.iterations;
+    info->simp_iterations = j.res.simp_iterations;
+    info->converged = j.res.converged;
     info->footprint_bytes = j.footprint;
     const auto now = Clock::now();
     const auto start = j.state == MYCEL_JOB_QUEUED ? now : j.t_start;
@@ -224,12 +234,15 @@ int32_t mycel_server_submit(mycel_server *sv, const mycel_job_desc *d, int64_t *
         if (d->kind != MYCEL_JOB_FEA && d->kind != MYCEL_JOB_SIMP) throw Error(MYCEL_E_ARGUMENT, "unknown job kind");
         if (d->grid.nx <= 0 || d->grid.ny <= 0 || d->grid.nz <= 0 || !(d->grid.h > 0))
             throw Error(MYCEL_E_OUT_OF_RANGE, "grid dimensions and h must be positive");
+        if (d->n_fixed < 0 || d->n_patch_nodes < 0) throw Error(MYCEL_E_ARGUMENT, "negative array length");
+        if ((d->n_fixed > 0 && !d->fixed_dofs) || (d->n_patch_nodes > 0 && !d->patch_nodes))
+            throw Error(MYCEL_E_ARGUMENT_NULL, "fixed_dofs / patch_nodes");
         auto j = std::make_unique<Job>();
         j->desc = *d;
         const size_t ne = static_cast<size_t>(d->grid.nx) * d->grid.ny * d->grid.nz;
         const size_t ndof = 3 * static_cast<size_t>(d->grid.nx + 1) * (d->grid.ny + 1) * (d->grid.nz + 1);
         j->domain.assign(d->domain, d->domain + ne);
-        j->fixed.assign(d->fixed_dofs, d->fixed_dofs + d->n_fixed);
+        if (d->n_fixed > 0) j->fixed.assign(d->fixed_dofs, d->fixed_dofs + d->n_fixed);
         j->forces.assign(d->forces, d->forces + ndof);
         if (d->density_box) j->density.assign(d->density_box, d->density_box + ne);
         if (d->n_patch_nodes > 0) j->patch.assign(d->patch_nodes, d->patch_nodes + d->n_patch_nodes);
@@ -281,7 +294,7 @@ int32_t mycel_server_wait(mycel_server *sv, int64_t id, double timeout_s, mycel_
     if (timeout_s < 0) sv->cv_done.wait(lk, finished);
     else sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
     fill_info(*j, info);
-    if (j->state == MYCEL_JOB_FAILED) set_last_error(j->error);
+    if (j->state == MYCEL_JOB_FAILED) set_last_error(j->res.error);
     return MYCEL_OK;
 }
 
@@ -295,11 +308,11 @@ int32_t mycel_server_fetch(mycel_server *sv, int64_t id, double *u_out, double *
     }
     Job &j = *it->second;
     if (j.state != MYCEL_JOB_DONE) {
-        set_last_error(j.state == MYCEL_JOB_FAILED ? j.error : "job not finished");
-        return j.state == MYCEL_JOB_FAILED ? j.status : MYCEL_E_INVALID_OPERATION;
+        set_last_error(j.state == MYCEL_JOB_FAILED ? j.res.error : "job not finished");
+        return j.state == MYCEL_JOB_FAILED ? j.res.status : MYCEL_E_INVALID_OPERATION;
     }
-    if (u_out) std::copy(j.u.begin(), j.u.end(), u_out);
-    if (rho_bar_out) std::copy(j.rho_bar.begin(), j.rho_bar.end(), rho_bar_out);
+    if (u_out) std::copy(j.res.u.begin(), j.res.u.end(), u_out);
+    if (rho_bar_out) std::copy(j.res.rho_bar.begin(), j.res.rho_bar.end(), rho_bar_out);
     return MYCEL_OK;
 }
 
