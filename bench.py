@@ -18,6 +18,9 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   independent load cases (1-4 seeded 4x4-node patches each), 20 SIMP iterations on the summed compliance
   (vf 0.25, rmin 2.0, p = 3, solves to 1e-8 warm-started per case) with the multigrid solver; one SIMP
   iteration with the plain FP64 Jacobi solver is timed beside it. Reports SIMP iterations/s.
+--mode precision: the 60x20x20 cantilever with a bimodal (~1e-9 contrast) density field, one cold solve to 1e-8
+  per step in plain FP64 Jacobi-PCG, FP32 Jacobi-PCG with FP64 reliable updates, the multigrid solver, and the
+  multigrid solver with island deflation; errors against an FP64 solve to 1e-12.
 --mode sweep: K*u (FP64 and FP32) on the holed box and on solid boxes up to 256x128x128 (12.7M DOFs), 100
   launches each timed with HIP events on the operator's stream; reports time per launch and the fraction of
   the HBM and FP64 rooflines at each size (the fixed launch and latency cost of one K*u is amortised as the
@@ -50,7 +53,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "sweep", "summarize"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "sweep", "precision", "summarize"),
+                    default="solve")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
@@ -77,6 +81,18 @@ def traffic_from_profiles(kernel: str, workload: str):
             return json.load(fh).get(f"{kernel}@{workload}")
     except (OSError, ValueError):
         return None
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(p, gpu_iters: int, budget_s: float):
@@ -108,6 +124,7 @@ def cpu_baseline(p, gpu_iters: int, budget_s: float):
         "value": 1.0 / (per_it * gpu_iters),
         "unit": "solves/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"{n_mv} K*u + {res.iterations} Jacobi-PCG iterations of the same problem on the FP64 oracle "
                   f"({threads} threads); solves/s = 1 / (s per iteration x {gpu_iters} iterations of the FP64 "
@@ -173,17 +190,63 @@ def timed_solves(op, b, steps, warmup, tol, precision, rho=None, penal=3.0):
             op.update_densities_device(rho, penal)
         res = solver.solve_device(op, b, x, tol=tol, maxIterations=500_000, precision=precision)
     torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(op.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
+    e0.record(stream)
+    iters = 0
     for _ in range(steps):
         x.zero_()
         if rho is not None:
             op.update_densities_device(rho, penal)
         res = solver.solve_device(op, b, x, tol=tol, maxIterations=500_000, precision=precision)
+        iters += res.Iterations
+    e1.record(stream)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     if not res.Converged:
         raise SystemExit(f"PCG did not converge: {res.Message}")
+    timed_solves.last_event_ms = e0.elapsed_time(e1)
+    timed_solves.last_iterations = iters
     return el, res, x
+
+
+def gmg_levels(nx, ny, nz, coarse_max_dofs=140):
+    """(nodes, elements) of each multigrid level: 2x coarsening per axis (ceil) down to <= 140 DOFs."""
+    levels = []
+    while True:
+        nodes = (nx + 1) * (ny + 1) * (nz + 1)
+        levels.append((nodes, nx * ny * nz))
+        if 3 * nodes <= coarse_max_dofs or min(nx, ny, nz) < 2:
+            return levels
+        nx, ny, nz = (nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2
+
+
+def mg_iteration_bytes(nx, ny, nz, active_nodes, active_elements, fine_degree=3):
+    """Algorithmic HBM bytes of one multigrid outer iteration: every stream read or written once per phase.
+    FP64 outer part: the fused x/r update (read x, r, p, ap; write x, r: 48 B per DOF) and the fused p update with
+    K p (read z, p; write p', ap; the K*u bytes: 32 B per DOF + 8 B per element). FP32 V-cycle, per level with
+    D DOFs and E elements: each Chebyshev step reads x, b, d, 1/diag and the moduli and writes x', d (24 B per DOF
+    + 4 B per element; fine_degree steps before and after the coarse correction on the finest level, 2 below),
+    the residual reads x, b, 1/diag and writes r (16 B/DOF + 4 B/element), the restriction reads r and writes the
+    coarse b, x, d, the prolongation reads the coarse x and updates the fine x; the finest level reads r and writes
+    its b, x, d on entry (20 B per DOF) and its last step writes z in FP64 and reads r, ap for the partials
+    (24 B per DOF more). Levels below the finest are counted on their full box grids."""
+    lv = gmg_levels(nx, ny, nz)
+    total = 0.0
+    for l, (nodes, elems) in enumerate(lv):
+        if l == 0:
+            nodes, elems = active_nodes, active_elements
+        D = 3 * nodes
+        steps = 2 * (fine_degree if l == 0 else 2)
+        total += steps * (24 * D + 4 * elems) + 16 * D + 4 * elems
+        if l + 1 < len(lv):
+            Dc = 3 * lv[l + 1][0]
+            total += 4 * D + 12 * Dc + 4 * Dc + 8 * D
+    D0 = 3 * active_nodes
+    total += 20 * D0 + 24 * D0  # V-cycle entry and FP64 exit
+    total += 48 * D0 + 32 * D0 + 8 * active_elements  # outer x/r update, p update fused with K p
+    return total
 
 
 def mode_solve(args):
@@ -200,6 +263,7 @@ def mode_solve(args):
     el64, r64, x64 = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_FP64, rho, p.penal)
     elmx, rmx, xmx = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_FP32, rho, p.penal)
     elmg, rmg, xmg = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_GMG, rho, p.penal)
+    mg_event_ms, mg_iters = timed_solves.last_event_ms, timed_solves.last_iterations
     # The same step (UpdateDensities + multigrid solve) through the host-pointer entry points (densities, b and
     # x in host memory: PCIe copies in and out included); reported beside the device-resident value, never
     # as it.
@@ -226,6 +290,11 @@ def mode_solve(args):
     bytes_mv = 2 * 3 * 8 * nn + 8 * na
     flops_mv = 1152 * na
     achieved = bytes_mv / (mv_ms * 1e-3) / 1e9
+    # The step's unit of work: one multigrid outer iteration (HIP events on the operator's stream around the timed
+    # solves; the step's UpdateDensities and hierarchy refresh are included in the time).
+    it_ms = mg_event_ms / max(1, mg_iters)
+    it_bytes = mg_iteration_bytes(p.nx, p.ny, p.nz, nn, na)
+    it_gbs = it_bytes / (it_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": solvesmg,
@@ -274,6 +343,17 @@ def mode_solve(args):
         "ms_per_matvec": {"fp64": mv_ms, "fp32": mv32_ms},
         "gdof_per_s_matvec": {"active": 3 * nn / (mv_ms * 1e-3) / 1e9, "box": p.n_dofs / (mv_ms * 1e-3) / 1e9},
         "roofline": {
+            "kernel": "multigrid outer iteration (fused x/r update, FP32 V-cycle: ~19 launches, fused p update + K p)",
+            "bound": "hbm",
+            "achieved": it_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": it_gbs / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("mg_iteration", WORKLOAD),
+            "algorithmic_bytes_per_launch": it_bytes,
+            "avg_launch_ms": it_ms,
+        },
+        "ku_roofline": {
             "kernel": KU_KERNEL,
             "bound": "hbm",
             "achieved": achieved,
@@ -315,6 +395,54 @@ def mode_kernels(args):
         el, r, _ = timed_solves(op, b, 1, 0, args.tol, prec)
         print(f"[kernels] solve precision={prec} {el * 1e3:.2f} ms, {r.Iterations} it", flush=True)
     op.close()
+
+
+def mode_precision(args):
+    """The precision configuration: the 60x20x20 cantilever with a spatially correlated bimodal density field
+    (30% rho = 1, 70% rho = 1e-3: ~1e-9 stiffness contrast), one cold solve to 1e-8 per step in each solver: plain
+    FP64 Jacobi-PCG, FP32 Jacobi-PCG with FP64 reliable updates, FP64 flexible PCG with the FP32 multigrid V-cycle,
+    and the same with island deflation; errors against an FP64 solve to 1e-12 (untimed)."""
+    import numpy as np
+    import torch
+
+    import mycel_amd as M
+    from mycel_amd import FemOperator
+    from mycel_amd import workloads as W
+
+    p = W.precision_variant()
+    fm = p.fe_model()
+    op = FemOperator(fm)
+    f = fm.AssembleForces()
+    dev = torch.device("cuda:0")
+    b = torch.tensor(np.where(fm.ConstrainedDofs, 0.0, f), dtype=torch.float64, device=dev)
+    ft = torch.tensor(f, dtype=torch.float64, device=dev)
+    xr = torch.zeros_like(b)
+    M.PcgSolver().solve_device(op, b, xr, tol=1e-12, maxIterations=2_000_000)
+    c_ref = float(ft @ xr)
+    legs = {}
+    for name, prec, islands in (("fp64", M.PRECISION_FP64, 0.0), ("mixed_fp32_reliable_updates", M.PRECISION_MIXED_FP32, 0.0),
+                                ("gmg", M.PRECISION_MIXED_GMG, 0.0), ("gmg_islands", M.PRECISION_MIXED_GMG, 1e-3)):
+        op.SetIslandDeflation(islands)
+        el, r, x = timed_solves(op, b, args.steps, args.warmup, args.tol, prec)
+        legs[name] = {"solves_per_s": args.steps / el, "ms_per_solve": el / args.steps * 1e3,
+                      "pcg_iterations": r.Iterations, "fp64_fallback_iterations": r.FallbackIterations,
+                      "compliance_rel_err_vs_fp64_1e-12": abs(float(ft @ x) - c_ref) / abs(c_ref),
+                      "u_rel_err": float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr))}
+        print(f"[precision] {name}: {legs[name]}", file=sys.stderr, flush=True)
+    op.close()
+    best = legs["gmg_islands"]
+    out = {
+        "metric": METRIC, "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "bimodal-cantilever-60x20x20", "grid": [p.nx, p.ny, p.nz], "box_dofs": p.n_dofs,
+                   "density": "seeded N(0,1) noise, Gaussian blur sigma 2, top 30% rho = 1, else 1e-3",
+                   "emin_ratio": p.emin_ratio, "tol": args.tol,
+                   "precision": "mixed: FP64 flexible PCG, FP32 V-cycle + FP64 island deflation"},
+        "legs": legs,
+        "mixed_speedup_vs_fp64": best["solves_per_s"] / legs["fp64"]["solves_per_s"],
+    }
+    print(json.dumps(out), flush=True)
 
 
 SWEEP_GRIDS = ((80, 40, 40, True), (100, 50, 50, False), (160, 80, 80, False), (200, 100, 100, False),
@@ -618,7 +746,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve, "cases": mode_cases,
-     "sweep": mode_sweep}[args.mode](args)
+     "sweep": mode_sweep, "precision": mode_precision}[args.mode](args)
 
 
 if __name__ == "__main__":

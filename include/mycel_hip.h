@@ -200,6 +200,12 @@ MYCEL_API int32_t mycel_pcg_solve(mycel_op *op, const double *b, double *x, doub
 MYCEL_API int32_t mycel_pcg_solve_d(mycel_op *op, const double *d_b, double *d_x, double tol, int32_t max_iter,
                           int32_t precision, mycel_pcg_result *result);
 
+/* IPreconditioner.Setup + Apply (Fem/IPreconditioner.cs:9-20) on device vectors: z = M^-1 r with the Jacobi
+ * preconditioner (precision FP64 or FP64_ORDERED; JacobiPreconditioner.cs:15-35) or one multigrid V-cycle
+ * (MIXED_GMG; its hierarchy is set up or refreshed first). r is a projected residual (zero on constrained DOFs).
+ * Enqueued on `stream` (NULL = the handle's stream). */
+MYCEL_API int32_t mycel_precond_apply_d(mycel_op *op, int32_t precision, const double *d_r, double *d_z, void *stream);
+
 /* Several right-hand sides of one operator solved in lockstep (the multi-RHS form of PcgSolver.Solve; no
  * counterpart in the reference, which solves one RHS at a time): d_b and d_x hold n_rhs box-DOF vectors
  * back to back (x: warm starts in, solutions out); every RHS keeps its own stopping test and result. Kernels
@@ -302,6 +308,8 @@ typedef struct {
     double resume_beta;
     int32_t resume_history_len;
     double resume_history[6];
+    double island_deflation; /* 0: the multigrid solver's island deflation (mycel_op_set_island_deflation) for this
+                                run's solves, with this solid ratio; 0 keeps the operator's setting */
 } mycel_simp_options;
 
 /* Fills the reference defaults (OptimizerOptions.cs:5-26). */

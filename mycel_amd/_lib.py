@@ -108,7 +108,8 @@ class SimpOptionsC(C.Structure):
                 ("enforce_min_filter_radius", C.c_int32), ("continuation", C.c_int32), ("precision", C.c_int32),
                 ("enable_heaviside", C.c_int32), ("initial_lambda", C.c_double), ("initial_design", C.c_void_p),
                 ("initial_design_len", C.c_int32), ("resume_state", C.c_int32), ("resume_max_change", C.c_double),
-                ("resume_beta", C.c_double), ("resume_history_len", C.c_int32), ("resume_history", C.c_double * 6)]
+                ("resume_beta", C.c_double), ("resume_history_len", C.c_int32), ("resume_history", C.c_double * 6),
+                ("island_deflation", C.c_double)]
 
 
 class SnapshotC(C.Structure):
@@ -175,6 +176,7 @@ SIGNATURES = {
     "mycel_op_stream": (_vp, [_vp]),
     "mycel_pcg_solve": (_i, [_vp, _vp, _vp, _d, _i, _i, _vp, PROGRESS_CB, _vp, C.POINTER(PcgResultC)]),
     "mycel_pcg_solve_d": (_i, [_vp, _vp, _vp, _d, _i, _i, C.POINTER(PcgResultC)]),
+    "mycel_precond_apply_d": (_i, [_vp, _i, _vp, _vp, _vp]),
     "mycel_static_analysis": (_i, [_vp, _vp, _vp, _d, _i, _i, _i, C.POINTER(_d), C.POINTER(_d), _vp, _vp,
                                    C.POINTER(PcgResultC)]),
     "mycel_von_mises": (_i, [_vp, _vp, _vp]),
@@ -192,6 +194,7 @@ SIGNATURES = {
     "mycel_simp_options_default": (None, [C.POINTER(SimpOptionsC)]),
     "mycel_op_set_matvec_columns": (_i, [_vp, _i]),
     "mycel_op_set_matvec_form": (_i, [_vp, _i, _i]),
+    "mycel_op_set_island_deflation": (_i, [_vp, _d]),
     "mycel_run_quantize": (_i, [_vp, C.c_int64, _vp, C.POINTER(C.c_uint32)]),
     "mycel_run_checksum": (_i, [_vp, C.c_int64, C.POINTER(C.c_uint32)]),
     "mycel_server_create": (_i, [_d, _i, C.POINTER(_vp)]),

@@ -307,6 +307,21 @@ void simp_run(mycel_op *op, const double *f, int n_cases, const mycel_simp_optio
     DevBuf<double> d_dc_k(n_cases > 1 ? nab : 0);
     std::vector<mycel_pcg_result> case_res(n_cases);
     const bool heav = o.enable_heaviside != 0;
+    // Island deflation for this run's multigrid solves; the operator's own setting comes back afterwards.
+    struct RatioGuard {
+        mycel_op *op;
+        double saved;
+        ~RatioGuard() {
+            if (op->island_ratio != saved) {
+                op->island_ratio = saved;
+                op->epoch++;  // the next solve sets the hierarchy up again without this run's islands
+            }
+        }
+    } ratio_guard{op, op->island_ratio};
+    if (o.island_deflation > 0.0) {
+        if (!(o.island_deflation <= 1.0)) throw Error(MYCEL_E_OUT_OF_RANGE, "island_deflation must be in [0, 1]");
+        op->island_ratio = o.island_deflation;
+    }
     // The reference's evaluation order throughout (PCG, compliance, OC volumes, snapshot sums).
     const bool ordered = o.precision == MYCEL_PRECISION_FP64_ORDERED;
     DevBuf<double> d_rho_tilde(heav ? nab : 0);
@@ -764,6 +779,18 @@ int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns) {
     });
 }
 
+int32_t mycel_op_set_island_deflation(mycel_op *op, double solid_ratio) {
+    return guarded([&] {
+        require(op, MYCEL_E_ARGUMENT_NULL, "op");
+        require(std::isfinite(solid_ratio) && solid_ratio >= 0.0 && solid_ratio <= 1.0, MYCEL_E_OUT_OF_RANGE,
+                "solid_ratio must be in [0, 1]");
+        if (op->island_ratio != solid_ratio) {
+            op->island_ratio = solid_ratio;
+            op->epoch++;  // the multigrid set-up runs again before the next solve
+        }
+    });
+}
+
 int32_t mycel_op_set_matvec_columns(mycel_op *op, int32_t columns) {
     if (columns == -1) return mycel_op_set_matvec_form(op, MYCEL_MATVEC_AUTO, 0);
     return mycel_op_set_matvec_form(op, MYCEL_MATVEC_NODE_GATHER, columns);
@@ -813,6 +840,34 @@ int32_t mycel_pcg_solve_d(mycel_op *op, const double *d_b, double *d_x, double t
         require(op && d_b && d_x && result, MYCEL_E_ARGUMENT_NULL, "op/b/x/result");
         DeviceGuard dg(op->device);
         pcg_solve_device(op, d_b, d_x, tol, max_iter, precision, nullptr, nullptr, nullptr, result);
+    });
+}
+
+// IPreconditioner.Setup + Apply (Fem/IPreconditioner.cs:9-20): Jacobi (JacobiPreconditioner.cs:15-35) or one
+// multigrid V-cycle.
+__global__ void k_jacobi_apply(int n, const double *__restrict__ r, const double *__restrict__ inv,
+                               double *__restrict__ z) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = r[i] * inv[i];
+}
+
+int32_t mycel_precond_apply_d(mycel_op *op, int32_t precision, const double *d_r, double *d_z, void *stream) {
+    return guarded([&] {
+        require(op && d_r && d_z, MYCEL_E_ARGUMENT_NULL, "op/r/z");
+        DeviceGuard dg(op->device);
+        hipStream_t s = pick(op, stream);
+        const int n = op->d.n_dofs;
+        if (precision == MYCEL_PRECISION_FP64 || precision == MYCEL_PRECISION_FP64_ORDERED) {
+            hipLaunchKernelGGL(k_jacobi_apply, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d_r, op->d_inv_diag.get(), d_z);
+            MYCEL_LAUNCH_CHECK();
+            return;
+        }
+        require(precision == MYCEL_PRECISION_MIXED_GMG, MYCEL_E_ARGUMENT, "preconditioner: FP64 (Jacobi) or MIXED_GMG");
+        require(gmg_supported(op), MYCEL_E_ARGUMENT, "the grid has no multigrid hierarchy");
+        gmg_prepare(op, s);
+        const size_t np = static_cast<size_t>(gmg_partials(op));
+        if (op->d_part.size() < 2 * np) op->d_part.resize(2 * np);
+        gmg_enqueue_vcycle(op, d_r, d_r, d_z, op->d_part.get(), op->d_part.get() + np, s);
     });
 }
 

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <memory>
+#include <vector>
 
 #include "apply_k.hpp"
 #include "ku_element.hpp"
@@ -207,6 +208,75 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, co
     if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, n, first_layer(g, n0));
 }
 
+// Island deflation (enabled per operator): every floating solid island (a 26-connected cluster of elements with
+// E >= ratio E0 that touches no fixed DOF) adds its six rigid-body modes Z to the preconditioner as an additive
+// coarse correction, z = V(r) + Z (Z^T A Z)^-1 Z^T r. Such an island is held only by the void around it, so its
+// rigid motions are near-null modes of A (eigenvalues of the order of the stiffness contrast) that neither the
+// smoother nor the geometric coarse levels represent. The correction coefficients of island c are
+// (t, w): t the translation, w the rotation about the island's node centroid.
+struct Defl {
+    const int *ncl;      // per node: island or -1
+    const double *coef;  // 6 per island, kDeflStride per right-hand side
+    const double *cen;   // island centroids
+    double h;
+    int nx1, ny1;
+    int m;  // islands in use (0: off)
+};
+constexpr int kMaxIslands = 64;
+constexpr int kDeflStride = 6 * kMaxIslands;
+
+// z (FP64, the node's three components) += t + w x (p - centroid) of the node's island.
+__device__ __forceinline__ void defl_add(const Defl &df, const double *coef, int n, int cl, double (&z)[3]) {
+    const int i = n % df.nx1, t = n / df.nx1, j = t % df.ny1, k = t / df.ny1;
+    const double dx = i * df.h - df.cen[3 * cl], dy = j * df.h - df.cen[3 * cl + 1], dz = k * df.h - df.cen[3 * cl + 2];
+    const double *c = coef + 6 * cl;
+    z[0] += c[0] + (c[4] * dz - c[5] * dy);
+    z[1] += c[1] + (c[5] * dx - c[3] * dz);
+    z[2] += c[2] + (c[3] * dy - c[4] * dx);
+}
+
+// y_c = Z_c^T r for island c (grid x) and right-hand side (grid y): the sums over the island's nodes (ascending)
+// of r and of (p - centroid) x r, each in a fixed block tree.
+__global__ __launch_bounds__(256) void k_defl_zt(int nx1, int ny1, double h, const int *__restrict__ off,
+                                                 const int *__restrict__ nodes, const double *__restrict__ cen,
+                                                 const double *__restrict__ r64, long long rstride,
+                                                 double *__restrict__ y) {
+    __shared__ double red[4];
+    const int c = blockIdx.x;
+    r64 += blockIdx.y * rstride;
+    y += static_cast<long long>(blockIdx.y) * kDeflStride;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = off[c] + threadIdx.x; q < off[c + 1]; q += 256) {
+        const int n = nodes[q];
+        const int i = n % nx1, t = n / nx1, j = t % ny1, k = t / ny1;
+        const double dx = i * h - cen[3 * c], dy = j * h - cen[3 * c + 1], dz = k * h - cen[3 * c + 2];
+        const double rx = r64[3 * n], ry = r64[3 * n + 1], rz = r64[3 * n + 2];
+        s[0] += rx;
+        s[1] += ry;
+        s[2] += rz;
+        s[3] += dy * rz - dz * ry;  // rotation about x: (0, -dz, dy)
+        s[4] += dz * rx - dx * rz;  // about y: (dz, 0, -dx)
+        s[5] += dx * ry - dy * rx;  // about z: (-dy, dx, 0)
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const double v = block_sum256(s[q], red);
+        if (threadIdx.x == 0) y[6 * c + q] = v;
+    }
+}
+
+// coef = (Z^T A Z)^-1 y per right-hand side (grid y), rows in a fixed order.
+__global__ __launch_bounds__(256) void k_defl_solve(int m6, const double *__restrict__ einv,
+                                                    const double *__restrict__ y, double *__restrict__ coef) {
+    y += static_cast<long long>(blockIdx.y) * kDeflStride;
+    coef += static_cast<long long>(blockIdx.y) * kDeflStride;
+    for (int q = threadIdx.x; q < m6; q += 256) {
+        double acc = 0.0;
+        for (int k = 0; k < m6; k++) acc = __fma_rn(einv[q * m6 + k], y[k], acc);
+        coef[q] = acc;
+    }
+}
+
 // The last post-smoothing step on the finest level: writes x' as the FP64 preconditioned residual z and
 // per-block partials of z.r64 and z.ap64 (the flexible-CG inner products).
 __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> ks, const float *__restrict__ ep,
@@ -215,7 +285,8 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
                                                            float cd, float cr, double *__restrict__ z64,
                                                            const double *__restrict__ r64,
                                                            const double *__restrict__ ap64,
-                                                           double *__restrict__ part_zr, double *__restrict__ part_zap) {
+                                                           double *__restrict__ part_zr, double *__restrict__ part_zap,
+                                                           Defl df) {
     __shared__ double red[8];
     const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
@@ -233,15 +304,22 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
         node_ijk(g, n, i, j, k);
         float kx[3], xc[3];
         ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, first_layer(g, n0), kx, xc);
+        double z[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int dof = 3 * n + c;
             const float r = b[dof] - kx[c];
             const float dn = cd * d[dof] + cr * (inv[dof] * r);
-            const double z = static_cast<double>(xc[c] + dn);
-            z64[dof] = z;
-            zr = __fma_rn(z, r64[dof], zr);
-            zap = __fma_rn(z, ap64[dof], zap);
+            z[c] = static_cast<double>(xc[c] + dn);
+        }
+        const int cl = df.m ? df.ncl[n] : -1;
+        if (cl >= 0) defl_add(df, df.coef + static_cast<long long>(blockIdx.y) * kDeflStride, n, cl, z);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int dof = 3 * n + c;
+            z64[dof] = z[c];
+            zr = __fma_rn(z[c], r64[dof], zr);
+            zap = __fma_rn(z[c], ap64[dof], zap);
         }
     }
     block_sum256x2(zr, zap, red);
@@ -320,10 +398,13 @@ struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.
     double *z64;
     const double *r64, *ap64;
     float cd, cr;
+    Defl df;
+    const double *coef;  // this right-hand side's island coefficients
     double zr = 0.0, zap = 0.0;
     struct Pre {
         float b[3], d[3], inv[3];
         double r[3], ap[3];
+        int cl;
     };
     __device__ Pre pre(int n) const {
         Pre p;
@@ -335,17 +416,23 @@ struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.
             p.r[c] = r64[3 * n + c];
             p.ap[c] = ap64[3 * n + c];
         }
+        p.cl = df.m ? df.ncl[n] : -1;
         return p;
     }
     __device__ void operator()(int n, float (&kx)[3], const float (&xc)[3], const Pre &p) {
+        double z[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const float r = p.b[c] - kx[c];
             const float dn = cd * p.d[c] + cr * (p.inv[c] * r);
-            const double z = static_cast<double>(xc[c] + dn);
-            z64[3 * n + c] = z;
-            zr = __fma_rn(z, p.r[c], zr);
-            zap = __fma_rn(z, p.ap[c], zap);
+            z[c] = static_cast<double>(xc[c] + dn);
+        }
+        if (p.cl >= 0) defl_add(df, coef, n, p.cl, z);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            z64[3 * n + c] = z[c];
+            zr = __fma_rn(z[c], p.r[c], zr);
+            zap = __fma_rn(z[c], p.ap[c], zap);
         }
     }
 };
@@ -413,10 +500,11 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float>
                                                            const double *__restrict__ ap64,
                                                            double *__restrict__ part_zr, double *__restrict__ part_zap,
                                                            int n_tiles, const int *__restrict__ halt,
-                                                           int halt_stride) {
+                                                           int halt_stride, Defl df) {
     __shared__ double red[8];
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    FinalFin fin{b + o, d + o, inv, z64 + o, r64 + o, ap64 + o, cd, cr};
+    FinalFin fin{b + o, d + o, inv, z64 + o, r64 + o, ap64 + o, cd, cr, df,
+                 df.coef + static_cast<long long>(blockIdx.y) * kDeflStride};
     const int tile = ec_wave_tile();
     if (tile < n_tiles) fine_march<HALT, KZ>(g, km, ep, x + o, halt, halt_stride, tile, fin);
     double zr = fin.zr, zap = fin.zap;
@@ -1119,7 +1207,19 @@ struct GmgLevel {
     int blocks_s = 0;  // kSplitNodes per block (the slab-split gathers)
 };
 
+// Island deflation state (host set-up on every density update when the operator enables it).
+struct Deflation {
+    int m = 0;                  // islands in use
+    DevBuf<int> ncl;            // per node: island or -1
+    DevBuf<int> off, nodes;     // each island's nodes, ascending (CSR)
+    DevBuf<double> cen;         // 3 per island
+    DevBuf<double> einv;        // (6m)^2, row-major inverse of Z^T A Z
+    DevBuf<double> y, coef;     // kDeflStride per right-hand side
+    int rhs_cap = 0;
+};
+
 struct Gmg {
+    Deflation defl;
     GmgLevel lv[kMaxLevels];
     int levels = 0;
     DevBuf<float> ainv;  // coarsest inverse, column-major
@@ -1134,6 +1234,12 @@ struct Gmg {
 
 void gmg_reserve(mycel_op *op, int n_rhs) {
     Gmg *g = op->gmg;
+    if (n_rhs > g->defl.rhs_cap) {
+        g->defl.y.resize(static_cast<size_t>(kDeflStride) * n_rhs);
+        g->defl.coef.resize(static_cast<size_t>(kDeflStride) * n_rhs);
+        g->defl.rhs_cap = n_rhs;
+        g->gen++;
+    }
     if (n_rhs <= g->rhs_cap) return;
     for (int l = 0; l < g->levels; l++) {
         GmgLevel &L = g->lv[l];
@@ -1165,6 +1271,18 @@ static int fine_ec_kz(const mycel_op *op) {
 int gmg_partials(const mycel_op *op) {
     const int kz = fine_ec_kz(op);
     return kz ? cdiv(ec_tiles(op->d, kz), 4) : cdiv(op->d.n_nodes, 256);
+}
+
+// Whether the grid coarsens by 2 per axis (every axis keeping >= 2 elements) down to <= kCoarseMaxDofs DOFs, the
+// hierarchy gmg_build makes.
+bool gmg_supported(const mycel_op *op) {
+    Dims d = op->d;
+    for (int l = 0; l < kMaxLevels; l++) {
+        if (d.n_dofs <= kCoarseMaxDofs) return true;
+        if (d.nx < 2 || d.ny < 2 || d.nz < 2) return false;
+        d = make_dims((d.nx + 1) / 2, (d.ny + 1) / 2, (d.nz + 1) / 2);
+    }
+    return false;
 }
 
 // Builds the level structure; op->gmg is set only once the whole hierarchy exists.
@@ -1240,9 +1358,191 @@ static void gmg_build(mycel_op *op) {
     op->gmg = owner.release();
 }
 
+// Finds the floating solid islands of the current moduli and forms the coarse correction's operator
+// (Z^T A Z, inverted by Cholesky) on the host from the element moduli, in a fixed order. Islands beyond
+// kMaxIslands (the smallest ones) are left to the iteration.
+static void gmg_setup_deflation(mycel_op *op, Gmg *g, hipStream_t s) {
+    Deflation &D = g->defl;
+    const Dims &d = op->d;
+    const int m_old = D.m;
+    D.m = 0;
+    std::vector<double> ep(d.n_pad);
+    MYCEL_HIP(hipMemcpyAsync(ep.data(), op->d_ep.get(), sizeof(double) * d.n_pad, hipMemcpyDeviceToHost, s));
+    MYCEL_HIP(hipStreamSynchronize(s));
+    auto E = [&](int ex, int ey, int ez) { return ep[(ex + 1) + d.px * ((ey + 1) + d.py * (ez + 1))]; };
+    const double thr = op->island_ratio * op->e0;
+    // 26-connected clusters of solid elements, labelled in ascending element order.
+    std::vector<int> lab(d.n_elem, -1), size, stack;
+    int ncomp = 0;
+    for (int e = 0; e < d.n_elem; e++) {
+        const int ex = e % d.nx, ey = (e / d.nx) % d.ny, ez = e / (d.nx * d.ny);
+        if (lab[e] >= 0 || !(E(ex, ey, ez) >= thr) || E(ex, ey, ez) == 0.0) continue;
+        int cnt = 0;
+        lab[e] = ncomp;
+        stack.assign(1, e);
+        while (!stack.empty()) {
+            const int q = stack.back();
+            stack.pop_back();
+            cnt++;
+            const int qx = q % d.nx, qy = (q / d.nx) % d.ny, qz = q / (d.nx * d.ny);
+            for (int oz = -1; oz <= 1; oz++)
+                for (int oy = -1; oy <= 1; oy++)
+                    for (int ox = -1; ox <= 1; ox++) {
+                        const int nx_ = qx + ox, ny_ = qy + oy, nz_ = qz + oz;
+                        if (nx_ < 0 || ny_ < 0 || nz_ < 0 || nx_ >= d.nx || ny_ >= d.ny || nz_ >= d.nz) continue;
+                        const int r = nx_ + d.nx * (ny_ + d.ny * nz_);
+                        if (lab[r] >= 0 || !(E(nx_, ny_, nz_) >= thr) || E(nx_, ny_, nz_) == 0.0) continue;
+                        lab[r] = ncomp;
+                        stack.push_back(r);
+                    }
+        }
+        size.push_back(cnt);
+        ncomp++;
+    }
+    std::vector<int> ncl(d.n_nodes, -1);
+    for (int e = 0; e < d.n_elem; e++) {
+        if (lab[e] < 0) continue;
+        const int ex = e % d.nx, ey = (e / d.nx) % d.ny, ez = e / (d.nx * d.ny);
+        const int n0 = ex + d.nx1 * (ey + d.ny1 * ez);
+        for (int a = 0; a < 8; a++) ncl[n0 + corner_x(a) + d.nx1 * (corner_y(a) + d.ny1 * corner_z(a))] = lab[e];
+    }
+    std::vector<char> anchored(ncomp, 0);
+    for (int n = 0; n < d.n_nodes; n++)
+        if (ncl[n] >= 0)
+            for (int c = 0; c < 3; c++)
+                if (op->dof_flags[3 * n + c] & kFlagConstrained) anchored[ncl[n]] = 1;
+    std::vector<int> floating;
+    for (int c = 0; c < ncomp; c++)
+        if (!anchored[c]) floating.push_back(c);
+    std::stable_sort(floating.begin(), floating.end(), [&](int a, int b) { return size[a] > size[b]; });
+    if (floating.size() > static_cast<size_t>(kMaxIslands)) floating.resize(kMaxIslands);
+    std::vector<int> newid(ncomp, -1);
+    std::sort(floating.begin(), floating.end());  // islands numbered in order of their first element
+    const int m = static_cast<int>(floating.size());
+    for (int k = 0; k < m; k++) newid[floating[k]] = k;
+    for (int n = 0; n < d.n_nodes; n++) ncl[n] = ncl[n] >= 0 ? newid[ncl[n]] : -1;
+    if (m > 0) {
+        const double h = op->h;
+        std::vector<int> off(m + 1, 0), nodes;
+        std::vector<double> cen(3 * m, 0.0);
+        for (int n = 0; n < d.n_nodes; n++)
+            if (ncl[n] >= 0) off[ncl[n] + 1]++;
+        for (int k = 0; k < m; k++) off[k + 1] += off[k];
+        nodes.assign(off[m], 0);
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int n = 0; n < d.n_nodes; n++) {
+            const int c = ncl[n];
+            if (c < 0) continue;
+            nodes[fill[c]++] = n;
+            cen[3 * c] += (n % d.nx1) * h;
+            cen[3 * c + 1] += ((n / d.nx1) % d.ny1) * h;
+            cen[3 * c + 2] += (n / (d.nx1 * d.ny1)) * h;
+        }
+        for (int c = 0; c < m; c++)
+            for (int q = 0; q < 3; q++) cen[3 * c + q] /= (off[c + 1] - off[c]);
+        // Z^T K Z element by element: sum over the elements touching an island of R_a^T (E_e Ke_ab) R_b.
+        const int m6 = 6 * m;
+        std::vector<double> A(static_cast<size_t>(m6) * m6, 0.0);
+        auto rbm = [&](int n, int c, double (&R)[3][6]) {
+            const double dx = (n % d.nx1) * h - cen[3 * c], dy = ((n / d.nx1) % d.ny1) * h - cen[3 * c + 1],
+                         dz = (n / (d.nx1 * d.ny1)) * h - cen[3 * c + 2];
+            const double R_[3][6] = {{1, 0, 0, 0, dz, -dy}, {0, 1, 0, -dz, 0, dx}, {0, 0, 1, dy, -dx, 0}};
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 6; j++) R[i][j] = R_[i][j];
+        };
+        for (int e = 0; e < d.n_elem; e++) {
+            const int ex = e % d.nx, ey = (e / d.nx) % d.ny, ez = e / (d.nx * d.ny);
+            const double ee = E(ex, ey, ez);
+            if (ee == 0.0) continue;
+            const int n0 = ex + d.nx1 * (ey + d.ny1 * ez);
+            int nd[8], cl[8];
+            bool any = false;
+            for (int a = 0; a < 8; a++) {
+                nd[a] = n0 + corner_x(a) + d.nx1 * (corner_y(a) + d.ny1 * corner_z(a));
+                cl[a] = ncl[nd[a]];
+                any |= cl[a] >= 0;
+            }
+            if (!any) continue;
+            for (int a = 0; a < 8; a++) {
+                if (cl[a] < 0) continue;
+                double Ra[3][6];
+                rbm(nd[a], cl[a], Ra);
+                for (int b = 0; b < 8; b++) {
+                    if (cl[b] < 0) continue;
+                    double Rb[3][6], KR[3][6];
+                    rbm(nd[b], cl[b], Rb);
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 6; j++) {
+                            double acc = 0.0;
+                            for (int k = 0; k < 3; k++) acc += op->ke[(3 * a + i) * 24 + 3 * b + k] * Rb[k][j];
+                            KR[i][j] = ee * acc;
+                        }
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) {
+                            double acc = 0.0;
+                            for (int k = 0; k < 3; k++) acc += Ra[k][i] * KR[k][j];
+                            A[static_cast<size_t>(6 * cl[a] + i) * m6 + 6 * cl[b] + j] += acc;
+                        }
+                }
+            }
+        }
+        // Cholesky A = L L^T, then A^-1 column by column.
+        std::vector<double> L(A);
+        bool ok = true;
+        for (int j = 0; j < m6 && ok; j++) {
+            double dsum = L[static_cast<size_t>(j) * m6 + j];
+            for (int k = 0; k < j; k++) dsum -= L[static_cast<size_t>(j) * m6 + k] * L[static_cast<size_t>(j) * m6 + k];
+            if (!(dsum > 0.0)) {
+                ok = false;
+                break;
+            }
+            const double ljj = std::sqrt(dsum);
+            L[static_cast<size_t>(j) * m6 + j] = ljj;
+            for (int i = j + 1; i < m6; i++) {
+                double v = L[static_cast<size_t>(i) * m6 + j];
+                for (int k = 0; k < j; k++) v -= L[static_cast<size_t>(i) * m6 + k] * L[static_cast<size_t>(j) * m6 + k];
+                L[static_cast<size_t>(i) * m6 + j] = v / ljj;
+            }
+        }
+        if (ok) {
+            std::vector<double> inv(static_cast<size_t>(m6) * m6, 0.0), col(m6);
+            for (int c = 0; c < m6; c++) {
+                for (int i = 0; i < m6; i++) {  // L y = e_c
+                    double v = i == c ? 1.0 : 0.0;
+                    for (int k = 0; k < i; k++) v -= L[static_cast<size_t>(i) * m6 + k] * col[k];
+                    col[i] = v / L[static_cast<size_t>(i) * m6 + i];
+                }
+                for (int i = m6 - 1; i >= 0; i--) {  // L^T x = y
+                    double v = col[i];
+                    for (int k = i + 1; k < m6; k++) v -= L[static_cast<size_t>(k) * m6 + i] * col[k];
+                    col[i] = v / L[static_cast<size_t>(i) * m6 + i];
+                }
+                for (int i = 0; i < m6; i++) inv[static_cast<size_t>(i) * m6 + c] = col[i];
+            }
+            if (D.ncl.size() == 0) {  // fixed capacities: the captured graphs keep valid pointers
+                D.ncl.resize(d.n_nodes);
+                D.nodes.resize(d.n_nodes);
+                D.off.resize(kMaxIslands + 1);
+                D.cen.resize(3 * kMaxIslands);
+                D.einv.resize(static_cast<size_t>(kDeflStride) * kDeflStride);
+                g->gen++;
+            }
+            D.ncl.upload(ncl.data(), d.n_nodes, s);
+            D.off.upload(off.data(), m + 1, s);
+            D.nodes.upload(nodes.data(), nodes.size(), s);
+            D.cen.upload(cen.data(), 3 * m, s);
+            D.einv.upload(inv.data(), inv.size(), s);
+            MYCEL_HIP(hipStreamSynchronize(s));
+            D.m = m;
+        }
+    }
+    if (D.m != m_old) g->gen++;  // the island count is a kernel argument of the captured iteration
+}
+
 void gmg_prepare(mycel_op *op, hipStream_t s) {
     if (!op->gmg) gmg_build(op);
     Gmg *g = op->gmg;
+    gmg_reserve(op, 1);
     if (g->epoch == op->epoch) return;
     const dim3 b256(256);
     GmgLevel &F = g->lv[0];
@@ -1272,6 +1572,12 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
                            C.inv.get(), C.d, static_cast<const double *>(nullptr), g->ainv.get());
     }
     MYCEL_LAUNCH_CHECK();
+    if (op->island_ratio > 0.0) {
+        gmg_setup_deflation(op, g, s);
+    } else if (g->defl.m != 0) {
+        g->defl.m = 0;
+        g->gen++;
+    }
     g->epoch = op->epoch;
 }
 
@@ -1295,6 +1601,15 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     for (int l = 0; l <= top; l++) {
         cur[l] = g->lv[l].x.get();
         nxt[l] = g->lv[l].x2.get();
+    }
+    // Island deflation: the correction's coefficients from r, ahead of the V-cycle; its last step adds Z coef.
+    Deflation &D = g->defl;
+    if (D.m > 0 && D.rhs_cap < R) throw Error(MYCEL_E_ARGUMENT, "multigrid: deflation storage not reserved");
+    Defl df{D.ncl.get(), D.coef.get(), D.cen.get(), op->h, op->d.nx1, op->d.ny1, D.m};
+    if (D.m > 0) {
+        hipLaunchKernelGGL(k_defl_zt, dim3(D.m, R), b256, 0, s, op->d.nx1, op->d.ny1, op->h, D.off.get(),
+                           D.nodes.get(), D.cen.get(), r64, static_cast<long long>(op->d.n_dofs), D.y.get());
+        hipLaunchKernelGGL(k_defl_solve, dim3(1, R), b256, 0, s, 6 * D.m, D.einv.get(), D.y.get(), D.coef.get());
     }
     const int fkz = fine_ec_kz(op);  // finest level: element-centric kernels with fkz layers per tile (0: gathers)
     const bool fh = n_rhs > 1 && halt;  // the finest level's kernels test the halt flag (several right-hand sides)
@@ -1463,25 +1778,25 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<2, true> : k_gmg_cheb_final_ec<2, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
                            F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
-                           halt_stride);
+                           halt_stride, df);
     else if (fkz == 3)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<3, true> : k_gmg_cheb_final_ec<3, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
                            F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
-                           halt_stride);
+                           halt_stride, df);
     else if (fkz == 4)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<4, true> : k_gmg_cheb_final_ec<4, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
                            F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
-                           halt_stride);
+                           halt_stride, df);
     else if (fkz == 8)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<8, true> : k_gmg_cheb_final_ec<8, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
                            F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
-                           halt_stride);
+                           halt_stride, df);
     else
         hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
-                           F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap);
+                           F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, df);
     MYCEL_LAUNCH_CHECK();
 }
 

@@ -175,6 +175,7 @@ struct mycel_op {
     mycel::Gmg *gmg = nullptr;         // multigrid hierarchy (built on first use)
     mycel::OrdWork *ord = nullptr;     // workspace of the reference-order solver (built on first use)
     uint64_t epoch = 0;                // bumped whenever the element moduli change
+    double island_ratio = 0.0;         // multigrid island deflation: solid threshold E >= ratio E0 (0: off)
 };
 
 namespace mycel {
@@ -236,6 +237,9 @@ void ord_free(OrdWork *w);
 // Builds the hierarchy on first use and refreshes its moduli, constraints and coarsest inverse when
 // op->epoch changed (enqueued on s, no host synchronisation).
 void gmg_prepare(mycel_op *op, hipStream_t s);
+// Whether the grid has the multigrid hierarchy (2x coarsening per axis down to a dense coarsest level); the
+// solvers fall back to the FP64 Jacobi iteration on grids that do not.
+bool gmg_supported(const mycel_op *op);
 // z = M(r) (one FP32 V-cycle) plus per-block partials of z.r and z.ap, gmg_partials(op) of each.
 // fine_ready: the finest level's b, x = d = D^-1 b / theta were written by the caller (gmg_fine_view).
 // n_rhs: right-hand sides in one pass (grid dimension y; vectors of RHS k at k * n_dofs of each level;

@@ -296,6 +296,10 @@ class FemOperator:
         family give bit-identical results; the two families agree to rounding."""
         L.check(L.load().mycel_op_set_matvec_form(self._h, int(form), int(columns)))
 
+    def SetIslandDeflation(self, solid_ratio: float) -> None:
+        """Multigrid island deflation (mycel_op_set_island_deflation): 0 turns it off."""
+        L.check(L.load().mycel_op_set_island_deflation(self._h, float(solid_ratio)))
+
     def SetMatvecColumns(self, columns: int) -> None:
         """The gather form with columns 0/2/4/8, or -1 for the automatic form."""
         L.check(L.load().mycel_op_set_matvec_columns(self._h, int(columns)))
@@ -332,6 +336,11 @@ class MultigridPreconditioner:
     """A second IPreconditioner (Fem/IPreconditioner.cs:9-20): one FP32 geometric-multigrid V-cycle per
     iteration (2x coarsening, Chebyshev-Jacobi smoothing, dense coarsest solve) inside an FP64 flexible PCG.
     Passing it to PcgSolver.Solve selects PRECISION_MIXED_GMG."""
+
+
+def precond_apply_device(op: "FemOperator", r_t, z_t, precision=L.PRECISION_MIXED_GMG, stream=None):
+    """IPreconditioner.Apply on device tensors (mycel_precond_apply_d): z = M^-1 r, Jacobi or one V-cycle."""
+    L.check(L.load().mycel_precond_apply_d(op._h, int(precision), L.device_ptr(r_t), L.device_ptr(z_t), stream))
 
 
 @dataclass

@@ -152,6 +152,7 @@ class OptimizerOptions:
     ResumeMaxChange: Optional[float] = None
     ResumeBeta: float = 1.0
     ResumeComplianceHistory: Optional[Sequence[float]] = None
+    IslandDeflation: float = 0.0  # multigrid island deflation's solid ratio for this run (0: the operator's setting)
 
 
 @dataclass
@@ -206,6 +207,7 @@ def simp_options_c(o: "OptimizerOptions") -> L.SimpOptionsC:
                         o.SolverToleranceInitial, o.SolverToleranceFinal, o.SolverMaxIterations, o.RhoMin,
                         int(o.ProtectBoundaryHinges), o.StartIteration, int(o.EnforceMinimumFilterRadius),
                         int(o.Continuation), int(o.Precision), int(o.EnableHeaviside), float(o.InitialLambda))
+    co.island_deflation = float(o.IslandDeflation)
     if o.ResumeMaxChange is not None:
         hist = list(o.ResumeComplianceHistory or [])[-6:]
         co.resume_state = 1

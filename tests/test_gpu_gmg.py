@@ -266,3 +266,52 @@ def test_large_grid_column_matvec_in_solvers(gpu, shape):
     assert res[0] <= 2e-8 and res[1] <= 2e-8
     assert abs(cg - c64) / abs(c64) <= 1e-7
     assert np.array_equal(X[0], singles[0]) and np.array_equal(X[1], singles[1])
+
+
+def _c3_reference():
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_solve.npz")
+    with np.load(path) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_gmg_island_deflation_bimodal(gpu):
+    """The bimodal 1e-9-contrast cantilever (C3): its floating solid islands make near-null modes that the V-cycle
+    does not represent; with island deflation the multigrid solve needs under a quarter of the iterations, with no
+    FP64 fallback, and its compliance agrees with the oracle solved to 1e-12 within 1e-8 (displacements within
+    1e-7, every 8th DOF of the committed oracle solution), its true FP64 residual within 2x the tolerance. Without
+    islands (the holed box) the option changes nothing: the solution is bit-identical."""
+    ref = _c3_reference()
+    p = W.precision_variant()
+    m = oracle_model(p)
+    f = oracle_forces(p)
+    b = np.where(m.constrained, 0.0, f)
+    op = FemOperator(p.fe_model())
+    try:
+        u0 = np.zeros(p.n_dofs)
+        r0 = PcgSolver().Solve(op, b, u0, MultigridPreconditioner(), tol=1e-8, maxIterations=100000)
+        op.SetIslandDeflation(1e-3)
+        u1 = np.zeros(p.n_dofs)
+        r1 = PcgSolver().Solve(op, b, u1, MultigridPreconditioner(), tol=1e-8, maxIterations=100000)
+    finally:
+        op.close()
+    c_err = abs(f @ u1 - ref["compliance"][0]) / abs(ref["compliance"][0])
+    u_err = rel(u1[::8], ref["u_sample"])
+    print(f"[islands] plain {r0.Iterations} it, deflated {r1.Iterations} it, compliance {c_err:.2e}, u {u_err:.2e}",
+          flush=True)
+    assert r1.Converged and r1.FallbackIterations == 0
+    assert r1.Iterations * 4 < r0.Iterations
+    assert c_err <= 1e-8 and u_err <= 1e-7
+    assert _true_relres(m, b, u1) <= 2e-8
+    q = W.matvec_pcg_workstation()
+    bq = np.where(oracle_model(q).constrained, 0.0, oracle_forces(q))
+    op = FemOperator(q.fe_model())
+    try:
+        ua, ub = np.zeros(q.n_dofs), np.zeros(q.n_dofs)
+        PcgSolver().Solve(op, bq, ua, MultigridPreconditioner(), tol=1e-8)
+        op.SetIslandDeflation(1e-3)
+        PcgSolver().Solve(op, bq, ub, MultigridPreconditioner(), tol=1e-8)
+        assert np.array_equal(ua, ub)
+    finally:
+        op.close()
