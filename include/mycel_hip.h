@@ -172,7 +172,7 @@ MYCEL_API int32_t mycel_op_apply_k_f32_d(mycel_op *op, const float *d_x, float *
  * The two families agree to rounding (both within 1e-12 of the FP64 reference). */
 MYCEL_API int32_t mycel_op_set_matvec_form(mycel_op *op, int32_t form, int32_t columns);
 /* Island deflation of the multigrid preconditioner (no counterpart in the reference): with solid_ratio > 0, every
- * 26-connected cluster of elements with E >= solid_ratio * E0 that touches no fixed DOF (a floating island, held
+ * face-connected cluster of elements with E >= solid_ratio * E0 that touches no fixed DOF (a floating island, held
  * only by the void around it) adds its six rigid-body modes to the V-cycle as an additive coarse correction
  * z = V(r) + Z (Z^T A Z)^-1 Z^T r, formed in FP64 (up to 64 islands, the largest). Those modes are near-null
  * modes of A at high stiffness contrast that the smoother and the geometric coarse levels do not represent. The

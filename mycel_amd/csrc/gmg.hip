@@ -208,8 +208,8 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, co
     if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, n, first_layer(g, n0));
 }
 
-// Island deflation (enabled per operator): every floating solid island (a 26-connected cluster of elements with
-// E >= ratio E0 that touches no fixed DOF) adds its six rigid-body modes Z to the preconditioner as an additive
+// Island deflation (enabled per operator): every floating solid island (a face-connected cluster of elements
+// with E >= ratio E0 that touches no fixed DOF) adds its six rigid-body modes Z to the preconditioner as an additive
 // coarse correction, z = V(r) + Z (Z^T A Z)^-1 Z^T r. Such an island is held only by the void around it, so its
 // rigid motions are near-null modes of A (eigenvalues of the order of the stiffness contrast) that neither the
 // smoother nor the geometric coarse levels represent. The correction coefficients of island c are
@@ -1371,7 +1371,10 @@ static void gmg_setup_deflation(mycel_op *op, Gmg *g, hipStream_t s) {
     MYCEL_HIP(hipStreamSynchronize(s));
     auto E = [&](int ex, int ey, int ez) { return ep[(ex + 1) + d.px * ((ey + 1) + d.py * (ez + 1))]; };
     const double thr = op->island_ratio * op->e0;
-    // 26-connected clusters of solid elements, labelled in ascending element order.
+    // Face-connected clusters of solid elements, labelled in ascending element order. Clusters that meet only at
+    // an edge or a corner node stay apart: such a joint is a hinge with almost no stiffness, so each side's rigid
+    // modes are near-null modes of their own (the bimodal cantilever of seed 11: 266 iterations against 700 with
+    // 26-connected clusters). A node shared by two clusters belongs to the one labelled last.
     std::vector<int> lab(d.n_elem, -1), size, stack;
     int ncomp = 0;
     for (int e = 0; e < d.n_elem; e++) {
@@ -1389,6 +1392,7 @@ static void gmg_setup_deflation(mycel_op *op, Gmg *g, hipStream_t s) {
                 for (int oy = -1; oy <= 1; oy++)
                     for (int ox = -1; ox <= 1; ox++) {
                         const int nx_ = qx + ox, ny_ = qy + oy, nz_ = qz + oz;
+                        if (std::abs(ox) + std::abs(oy) + std::abs(oz) != 1) continue;
                         if (nx_ < 0 || ny_ < 0 || nz_ < 0 || nx_ >= d.nx || ny_ >= d.ny || nz_ >= d.nz) continue;
                         const int r = nx_ + d.nx * (ny_ + d.ny * nz_);
                         if (lab[r] >= 0 || !(E(nx_, ny_, nz_) >= thr) || E(nx_, ny_, nz_) == 0.0) continue;
