@@ -222,31 +222,50 @@ def gmg_levels(nx, ny, nz, coarse_max_dofs=140):
         nx, ny, nz = (nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2
 
 
-def mg_iteration_bytes(nx, ny, nz, active_nodes, active_elements, fine_degree=3):
-    """Algorithmic HBM bytes of one multigrid outer iteration: every stream read or written once per phase.
-    FP64 outer part: the fused x/r update (read x, r, p, ap; write x, r: 48 B per DOF) and the fused p update with
-    K p (read z, p; write p', ap; the K*u bytes: 32 B per DOF + 8 B per element). FP32 V-cycle, per level with
-    D DOFs and E elements: each Chebyshev step reads x, b, d, 1/diag and the moduli and writes x', d (24 B per DOF
-    + 4 B per element; fine_degree steps before and after the coarse correction on the finest level, 2 below),
-    the residual reads x, b, 1/diag and writes r (16 B/DOF + 4 B/element), the restriction reads r and writes the
-    coarse b, x, d, the prolongation reads the coarse x and updates the fine x; the finest level reads r and writes
-    its b, x, d on entry (20 B per DOF) and its last step writes z in FP64 and reads r, ap for the partials
-    (24 B per DOF more). Levels below the finest are counted on their full box grids."""
+def mg_iteration_model(nx, ny, nz, active_nodes, active_elements, tail_nodes=512):
+    """Launches and algorithmic HBM bytes of one multigrid outer iteration, per kernel: [(kernel name prefix,
+    launches per iteration, bytes per iteration)]. Every stream a launch reads or writes counts once (FP64 8 B,
+    FP32 4 B per DOF; moduli 8 B (FP64) or 4 B (FP32) per element). The finest level counts its active DOFs D0 and
+    elements E0, the coarse levels their box grids. Per launch:
+      k_fpcg_update_xr_gmg   read x, p, r, Ap, D^-1; write x, r (FP64) and the V-cycle's b, x0, d0 (FP32): 64 D0
+      k_gmg_cheb_ec (x4)     read x, b, d, D^-1; write x', d; the moduli: 24 D0 + 4 E0
+      k_gmg_residual_ec      read x, b, D^-1; write r: 16 D0 + 4 E0
+      k_gmg_restrict_dof     read r; read the coarse D^-1 and write its b, x, d: 4 D0 + 16 D1
+      k_gmg_cheb_s (x3/level) levels 1 .. t0-1, as the Chebyshev step above: 24 D + 4 E
+      k_gmg_residual_restrict read x, b, D^-1 of level l and write level l+1's entry: 12 D_l + 4 E_l + 16 D_l+1
+      k_gmg_tail             stage level t0's b, x, d and every tail level's moduli and D^-1, the coarsest inverse
+                             (4 n^2 B), prolong into level t0-1 (read D^-1, read and write x): 12 D_t0 +
+                             sum(4 D + 4 E) + 4 n^2 + 12 D_t0-1
+      k_gmg_prolong          levels t0-2 .. 0: read the coarse x, read D^-1 and x, write x: 12 D_l + 4 D_l+1
+      k_gmg_cheb_final_ec    read x, b, d, D^-1 (FP32), r and Ap (FP64); write z (FP64); moduli: 40 D0 + 4 E0
+      k_pcg_step4_ap<double, true  read z, p; write p', Ap' (FP64); the FP64 moduli: 32 D0 + 8 E0
+    Levels below t0 (the first level from which every level has <= 512 nodes) run inside the tail."""
     lv = gmg_levels(nx, ny, nz)
-    total = 0.0
-    for l, (nodes, elems) in enumerate(lv):
-        if l == 0:
-            nodes, elems = active_nodes, active_elements
-        D = 3 * nodes
-        steps = 2 * (fine_degree if l == 0 else 2)
-        total += steps * (24 * D + 4 * elems) + 16 * D + 4 * elems
-        if l + 1 < len(lv):
-            Dc = 3 * lv[l + 1][0]
-            total += 4 * D + 12 * Dc + 4 * Dc + 8 * D
-    D0 = 3 * active_nodes
-    total += 20 * D0 + 24 * D0  # V-cycle entry and FP64 exit
-    total += 48 * D0 + 32 * D0 + 8 * active_elements  # outer x/r update, p update fused with K p
-    return total
+    D = [3 * n for n, _ in lv]
+    E = [e for _, e in lv]
+    D[0], E[0] = 3 * active_nodes, active_elements
+    top = len(lv) - 1
+    t0 = top
+    while t0 > 1 and lv[t0 - 1][0] <= tail_nodes:
+        t0 -= 1
+    nc = D[top]
+    rows = [("k_fpcg_update_xr_gmg", 1, 64 * D[0]),
+            ("k_gmg_cheb_ec<", 4, 4 * (24 * D[0] + 4 * E[0])),
+            ("k_gmg_residual_ec<", 1, 16 * D[0] + 4 * E[0]),
+            ("k_gmg_restrict_dof", 1, 4 * D[0] + 16 * D[1])]
+    mid = range(1, t0)
+    rows.append(("k_gmg_cheb_s", 3 * len(mid), sum(3 * (24 * D[l] + 4 * E[l]) for l in mid)))
+    rows.append(("k_gmg_residual_restrict", len(mid), sum(12 * D[l] + 4 * E[l] + 16 * D[l + 1] for l in mid)))
+    rows.append(("k_gmg_tail", 1, 12 * D[t0] + sum(4 * D[l] + 4 * E[l] for l in range(t0, top + 1)) + 4 * nc * nc
+                 + 12 * D[t0 - 1]))
+    rows.append(("k_gmg_prolong", t0 - 1, sum(12 * D[l] + 4 * D[l + 1] for l in range(0, t0 - 1))))
+    rows.append(("k_gmg_cheb_final_ec<", 1, 40 * D[0] + 4 * E[0]))
+    rows.append(("k_pcg_step4_ap<double, true", 1, 32 * D[0] + 8 * E[0]))
+    return rows
+
+
+def mg_iteration_bytes(nx, ny, nz, active_nodes, active_elements):
+    return sum(b for _, _, b in mg_iteration_model(nx, ny, nz, active_nodes, active_elements))
 
 
 def mode_solve(args):
@@ -343,7 +362,7 @@ def mode_solve(args):
         "ms_per_matvec": {"fp64": mv_ms, "fp32": mv32_ms},
         "gdof_per_s_matvec": {"active": 3 * nn / (mv_ms * 1e-3) / 1e9, "box": p.n_dofs / (mv_ms * 1e-3) / 1e9},
         "roofline": {
-            "kernel": "multigrid outer iteration (fused x/r update, FP32 V-cycle: ~19 launches, fused p update + K p)",
+            "kernel": "multigrid outer iteration: 20 launches (fused x/r update, FP32 V-cycle, fused p update + K p), bytes per mg_iteration_model",
             "bound": "hbm",
             "achieved": it_gbs,
             "peak": HBM_PEAK_GBS,
@@ -698,7 +717,47 @@ def _pmc_bytes(directory, counter):
         else:
             per[_short(r["Kernel_Name"])].append(v)
     scale = (COPY_BYTES / 1024.0) / copy_kib if copy_kib > 0 else float("nan")
-    return {k: (statistics.median(v), len(v)) for k, v in per.items()}, scale, copy_kib
+    return {k: (statistics.median(v), len(v), statistics.fmean(v)) for k, v in per.items()}, scale, copy_kib
+
+
+def _mg_table(args, lines, fetch=None, write=None, fs=1.0, ws=1.0):
+    """The multigrid outer iteration of the kernels-mode solve: per kernel, launches per iteration, algorithmic
+    bytes (mg_iteration_model), the kernel trace's average duration and the PMC bytes (mean over dispatches)."""
+    from mycel_amd import workloads as W
+
+    p = W.matvec_pcg_workstation()
+    fm = p.fe_model()
+    na, nn = fm.ActiveElementCount, fm.ActiveNodeCount
+    stats = {}
+    for r in _csv_rows(args.stats_dir, "kernel_stats.csv"):
+        stats[_short(r["Name"])] = (int(r["Calls"]), float(r["AverageNs"]) / 1e3)
+    lines += ["", "Multigrid outer iteration (the solve's kernels; launches per iteration from the V-cycle's structure):",
+              "", "kernel | launches/iteration | algorithmic bytes/launch | avg us (trace) | algorithmic GB/s | "
+              "PMC bytes/launch | PMC / algorithmic", "--- | ---: | ---: | ---: | ---: | ---: | ---:"]
+    tot_us = tot_b = tot_pmc = 0.0
+    pmc_ok = fetch is not None
+    for prefix, n, b in mg_iteration_model(p.nx, p.ny, p.nz, nn, na):
+        def match(k):  # the single-solve instantiations (HALT = false), or the flexible step 4 itself
+            return k.startswith(prefix) and (prefix.endswith("true") or "true>" not in k[len(prefix):])
+        hits = [(c, us) for k, (c, us) in stats.items() if match(k)]
+        calls = sum(c for c, _ in hits)
+        us = sum(c * u for c, u in hits) / calls if calls else float("nan")
+        pmc = float("nan")
+        if pmc_ok:
+            ks = [k for k in set(fetch) & set(write) if match(k)]
+            cnt = sum(fetch[k][1] for k in ks)
+            if cnt:
+                pmc = sum((fetch[k][2] * fs + write[k][2] * ws) * 1024 * fetch[k][1] for k in ks) / cnt
+        per = b / n
+        lines.append(f"{prefix.rstrip('<')} | {n} | {per:,.0f} | {us:.2f} | {per / (us * 1e3):,.0f} | "
+                     f"{pmc:,.0f} | {pmc / per:.2f}")
+        tot_us += n * us
+        tot_b += b
+        tot_pmc += n * pmc
+    lines.append(f"**iteration** | {sum(n for _, n, _ in mg_iteration_model(p.nx, p.ny, p.nz, nn, na))} | "
+                 f"{tot_b:,.0f} | {tot_us:.1f} | {tot_b / (tot_us * 1e3):,.0f} | {tot_pmc:,.0f} | {tot_pmc / tot_b:.2f}")
+    return tot_pmc if pmc_ok else None
+
 
 
 def mode_summarize(args):
@@ -724,6 +783,12 @@ def mode_summarize(args):
             lines.append(f"{k} | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | {b:.0f}")
             if k.startswith("k_apply_k"):
                 traffic[k] = b
+    mg_pmc = None
+    if args.stats_dir:
+        if args.fetch_dir and args.write_dir:
+            mg_pmc = _mg_table(args, lines, fetch, write, fs, ws)
+        else:
+            _mg_table(args, lines)
     print("\n".join(lines))
     if args.traffic_json and traffic:
         # Plain K*u launches (mode 0, no dot partials) of either form, keyed without the template tail.
@@ -733,6 +798,8 @@ def mode_summarize(args):
             parts = [a.strip() for a in targs.rstrip(">").split(",")]
             if len(parts) >= 3 and parts[1] == "0" and parts[2] == "false":
                 ku[f"{name}<{parts[0]}>@{WORKLOAD}"] = v
+        if mg_pmc:
+            ku[f"mg_iteration@{WORKLOAD}"] = mg_pmc
         with open(args.traffic_json, "w") as fh:
             json.dump(ku, fh, indent=1, sort_keys=True)
 
