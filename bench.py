@@ -281,6 +281,7 @@ def mode_solve(args):
     rho = torch.tensor(p.density_box(), dtype=torch.float64, device="cuda:0")
     el64, r64, x64 = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_FP64, rho, p.penal)
     elmx, rmx, xmx = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_FP32, rho, p.penal)
+    elbf, rbf, xbf = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_BF16, rho, p.penal)
     elmg, rmg, xmg = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_GMG, rho, p.penal)
     mg_event_ms, mg_iters = timed_solves.last_event_ms, timed_solves.last_iterations
     # The same step (UpdateDensities + multigrid solve) through the host-pointer entry points (densities, b and
@@ -347,6 +348,11 @@ def mode_solve(args):
             "solves_per_s": solvesmx, "pcg_iterations": rmx.Iterations, "reliable_updates": rmx.OuterCorrections,
             "ms_per_solve": elmx / args.steps * 1e3,
             "compliance_rel_err_vs_fp64_1e-12": abs(cmx - c_ref) / abs(c_ref), "u_rel_err": u_err},
+        "mixed_bf16_search_directions": {
+            "solves_per_s": args.steps / elbf, "pcg_iterations": rbf.Iterations, "reliable_updates": rbf.OuterCorrections,
+            "fp64_fallback_iterations": rbf.FallbackIterations, "ms_per_solve": elbf / args.steps * 1e3,
+            "compliance_rel_err_vs_fp64_1e-12": abs(float(ft @ xbf) - c_ref) / abs(c_ref),
+            "u_rel_err": float(torch.linalg.norm(xbf - xr) / torch.linalg.norm(xr))},
         "fp64": {"solves_per_s": solves64, "pcg_iterations": r64.Iterations, "ms_per_solve": el64 / args.steps * 1e3,
                  "ms_per_pcg_iteration": el64 / args.steps * 1e3 / r64.Iterations,
                  "compliance_rel_err_vs_fp64_1e-12": abs(c64 - c_ref) / abs(c_ref),
@@ -440,11 +446,13 @@ def mode_precision(args):
     c_ref = float(ft @ xr)
     legs = {}
     for name, prec, islands in (("fp64", M.PRECISION_FP64, 0.0), ("mixed_fp32_reliable_updates", M.PRECISION_MIXED_FP32, 0.0),
+                                ("mixed_bf16_search_directions", M.PRECISION_MIXED_BF16, 0.0),
                                 ("gmg", M.PRECISION_MIXED_GMG, 0.0), ("gmg_islands", M.PRECISION_MIXED_GMG, 1e-3)):
         op.SetIslandDeflation(islands)
         el, r, x = timed_solves(op, b, args.steps, args.warmup, args.tol, prec)
         legs[name] = {"solves_per_s": args.steps / el, "ms_per_solve": el / args.steps * 1e3,
                       "pcg_iterations": r.Iterations, "fp64_fallback_iterations": r.FallbackIterations,
+                      "reliable_updates": r.OuterCorrections,
                       "compliance_rel_err_vs_fp64_1e-12": abs(float(ft @ x) - c_ref) / abs(c_ref),
                       "u_rel_err": float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr))}
         print(f"[precision] {name}: {legs[name]}", file=sys.stderr, flush=True)

@@ -60,7 +60,10 @@ extern "C" {
  * FP64: the reference's Jacobi-PCG arithmetic throughout (Fem/PcgSolver.cs:80-370).
  * MIXED_FP32: Jacobi-PCG iterating on FP32 vectors with the FP32 operator and FP64 dot products, with FP64
  *   reliable updates (the true residual r = P(b - K u) in FP64) and convergence on that true residual.
- * MIXED_BF16: reserved; rejected with MYCEL_E_ARGUMENT.
+ * MIXED_BF16: MIXED_FP32 with every search direction rounded to bfloat16 where it is formed (the first, each p
+ *   update, the one after a reliable update), so each K*p contraction takes BF16 inputs and accumulates in FP32;
+ *   the FP64 reliable updates and the convergence test are MIXED_FP32's. A precision study: it converges to
+ *   the same FP64 residual in more iterations.
  * FP64_ORDERED: the reference's Jacobi-PCG with every sum formed in the reference's order (colour-ordered K*u with
  *   4-lane Ke rows, 8,192-DOF blocked dot products with 4 lane chains, sequential ||Pb||) and no contracted
  *   multiply-adds: bit-identical to the reference restatement, for parity runs. In mycel_simp_run it also takes

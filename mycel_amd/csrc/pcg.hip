@@ -34,6 +34,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "ku_element.hpp"
@@ -70,6 +71,7 @@ struct PcgWork {
     DevBuf<PcgState> state;
     PcgState *h_state = nullptr;             // pinned
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+    int bf16 = 0;                            // the FP32 iteration rounds its search directions to bfloat16
     const double *graph64_x = nullptr;       // solution vector the FP64 graph was captured with
     DevBuf<double> part_zr, part_zap;        // V-cycle partials of z.r and z.Ap (multigrid)
     int nb_z = 0;
@@ -252,10 +254,23 @@ __device__ __forceinline__ bool pcg_step4(PcgState *__restrict__ st, const Step4
     return true;
 }
 
+// v rounded to the nearest bfloat16 (ties to even), as a float: the BF16 mode's search directions hold only
+// such values, so every K p of that mode contracts BF16 inputs with FP32 accumulation. Finite v only.
+__device__ __forceinline__ float round_bf16(float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+template <typename T>
+__device__ __forceinline__ T maybe_bf16(T v, int bf16) {
+    if constexpr (std::is_same<T, float>::value) return bf16 ? round_bf16(v) : v;
+    return v;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState *__restrict__ st,
                                                       const double *__restrict__ part2, int nb_vec,
-                                                      T *__restrict__ p, const T *__restrict__ z) {
+                                                      T *__restrict__ p, const T *__restrict__ z, int bf16) {
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
     T pv[4], zv[4];  // prefetched before the dependent reductions
 #pragma unroll
@@ -271,7 +286,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
-        if (i < n) p[i] = fmad(b, pv[q], zv[q]);
+        if (i < n) p[i] = maybe_bf16(fmad(b, pv[q], zv[q]), bf16);
     }
 }
 
@@ -437,6 +452,7 @@ struct PcgNextP {
     double beta;
     T bt;        // beta in the vector type, as the separate p update rounds it
     Step4In in;  // loaded at kernel entry (step4_in)
+    int bf16;    // FP32 only: p' rounded to bfloat16 (the BF16 mode)
     struct Raw {
         T p, z;
     };
@@ -444,7 +460,7 @@ struct PcgNextP {
         constexpr int sT = static_cast<int>(sizeof(T));
         return {ld_off(p + at, o, sT * d), ld_off(z + at, o, sT * d)};
     }
-    __device__ T value(Raw r) const { return fmad(bt, r.p, r.z); }
+    __device__ T value(Raw r) const { return maybe_bf16(fmad(bt, r.p, r.z), bf16); }
     // Called once per wave after its first loads are issued, so the partials' round trip overlaps them; false
     // (the solve halted before or halts here) ends the wave's march.
     template <typename Fin>
@@ -573,7 +589,7 @@ __global__ __launch_bounds__(256) void k_pcg_init_r(int n, const uint8_t *__rest
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_init_z(int n, const double *__restrict__ r64, T *__restrict__ r,
                                                     const T *__restrict__ inv, T *__restrict__ z, T *__restrict__ p,
-                                                    double *__restrict__ part1) {
+                                                    double *__restrict__ part1, int bf16) {
     __shared__ double red[4];
     double v = 0.0;
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
@@ -585,7 +601,7 @@ __global__ __launch_bounds__(256) void k_pcg_init_z(int n, const double *__restr
             if (r != reinterpret_cast<const T *>(r64)) r[i] = ri;
             const T zi = ri * inv[i];
             z[i] = zi;
-            if (p) p[i] = zi;
+            if (p) p[i] = maybe_bf16(zi, bf16);
             v = __fma_rn(static_cast<double>(ri), static_cast<double>(zi), v);
         }
     }
@@ -652,14 +668,14 @@ __global__ __launch_bounds__(256) void k_mixed_resume(PcgState *__restrict__ st,
 // p_out = z + beta p_in (p_out may be p_in).
 __global__ __launch_bounds__(256) void k_mixed_update_p(int n, const PcgState *__restrict__ st,
                                                         const float *p_in, float *p_out,
-                                                        const float *__restrict__ z) {
+                                                        const float *__restrict__ z, int bf16) {
     if (st->done) return;
     const float b = static_cast<float>(st->beta);
     const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int i = base + 256 * q;
-        if (i < n) p_out[i] = __fmaf_rn(b, p_in[i], z[i]);
+        if (i < n) p_out[i] = maybe_bf16(__fmaf_rn(b, p_in[i], z[i]), bf16);
     }
 }
 
@@ -756,7 +772,7 @@ void launch_step4_ap(mycel_op *op, const KeMono<T> &km, const T *ep, const T *p,
     // update_xr reads the partials of this launch as those of the automatic K*u form
     if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "fused step 4: partial count mismatch");
     const PcgNextP<T, FLEX> src{p, z, w->state.get(), w->part2.get(), w->part_zr.get(), w->part_zap.get(),
-                                w->nb_vec, w->nb_z, slot, 0.0, T(0)};
+                                w->nb_vec, w->nb_z, slot, 0.0, T(0), Step4In{}, w->bf16};
     const dim3 grid(cdiv(nt, 4)), b256(256);
     const uint8_t *fl = op->d_flags.get();
     double *part = w->part_pap.get();
@@ -798,7 +814,7 @@ void enqueue_iteration64(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStr
                        w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->z.get(), op->d_inv_diag.get(),
                        w->part2.get());
     hipLaunchKernelGGL(k_pcg_update_p<double>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(),
-                       w->nb_vec, w->p.get(), w->z.get());
+                       w->nb_vec, w->p.get(), w->z.get(), 0);
 }
 
 // One FP32 iteration of the reliable-update solver. Fused: its A p was formed by the step 4 before it (or, for
@@ -819,7 +835,7 @@ void enqueue_iteration32(mycel_op *op, PcgWork *w, int slot, hipStream_t s) {
                        w->nb_mv, w->x32.get(), w->r32.get(), w->p32.get(), w->ap32.get(), w->z32.get(),
                        op->d_inv_diag32.get(), w->part2.get());
     hipLaunchKernelGGL(k_pcg_update_p<float>, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(),
-                       w->nb_vec, w->p32.get(), w->z32.get());
+                       w->nb_vec, w->p32.get(), w->z32.get(), w->bf16);
 }
 
 void ensure_gmg(mycel_op *op, PcgWork *w) {
@@ -932,7 +948,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     if (precision == MYCEL_PRECISION_FP64_ORDERED)
         return pcg_solve_ordered(op, d_b, d_x, tol, max_iter, cancel, cb, user, res);
     if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32 &&
-        precision != MYCEL_PRECISION_MIXED_GMG)
+        precision != MYCEL_PRECISION_MIXED_BF16 && precision != MYCEL_PRECISION_MIXED_GMG)
         throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
     if (precision == MYCEL_PRECISION_MIXED_GMG && !gmg_supported(op)) {
         // A grid too thin for the hierarchy (an axis reaches one element above the dense coarsest size): the
@@ -941,7 +957,7 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         res->fallback_iterations = res->iterations;
         return st;
     }
-    const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32;
+    const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32 || precision == MYCEL_PRECISION_MIXED_BF16;
     const bool gmg = precision == MYCEL_PRECISION_MIXED_GMG;
     if (op->active_fixed < 6)  // PcgSolver.cs:95-100
         throw Error(MYCEL_E_INVALID_OPERATION,
@@ -951,6 +967,12 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
     ensure_work(op);
     PcgWork *w = op->pcg;
     if (mixed) ensure_mixed(w);
+    const int bf16 = precision == MYCEL_PRECISION_MIXED_BF16;
+    if (w->graph32 && w->bf16 != bf16) {  // the captured FP32 batch bakes in the rounding mode
+        MYCEL_HIP(hipGraphExecDestroy(w->graph32));
+        w->graph32 = nullptr;
+    }
+    w->bf16 = bf16;
     if (gmg) {
         gmg_prepare(op, op->stream);
         ensure_gmg(op, w);
@@ -985,11 +1007,11 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part_zr.get(), w->nb_z, &st->rz0, 0);
     } else if (mixed) {
         hipLaunchKernelGGL(k_pcg_init_z<float>, gv, b256, 0, s, n, w->r.get(), w->r32.get(), op->d_inv_diag32.get(),
-                           w->z32.get(), w->p32.get(), w->part1.get());
+                           w->z32.get(), w->p32.get(), w->part1.get(), w->bf16);
         MYCEL_HIP(hipMemsetAsync(w->x32.get(), 0, sizeof(float) * n, s));
     } else {
         hipLaunchKernelGGL(k_pcg_init_z<double>, gv, b256, 0, s, n, w->r.get(), w->r.get(), op->d_inv_diag.get(),
-                           w->z.get(), w->p.get(), w->part1.get());
+                           w->z.get(), w->p.get(), w->part1.get(), 0);
     }
     // rz0 = r.z: the Jacobi forms left its partials in part1; the multigrid form summed them above.
     if (!gmg) hipLaunchKernelGGL(k_pcg_sum_to, dim3(1), b256, 0, s, w->part1.get(), w->nb_vec, &st->rz0, 0);
@@ -1090,11 +1112,11 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
                                w->part1.get());
             hipLaunchKernelGGL(k_pcg_init_z<float>, gv, b256, 0, s, n, w->r.get(), w->r32.get(),
                                op->d_inv_diag32.get(), w->z32.get(), static_cast<float *>(nullptr),
-                               w->part1.get() + w->nb_vec);
+                               w->part1.get() + w->nb_vec, 0);
             hipLaunchKernelGGL(k_mixed_resume, dim3(1), b256, 0, s, st, w->part1.get(), w->part1.get() + w->nb_vec,
                                w->nb_vec);
             const float *p_in = fuse_step4() && w->h_state->pause_slot ? w->p32b.get() : w->p32.get();
-            hipLaunchKernelGGL(k_mixed_update_p, gv, b256, 0, s, n, st, p_in, w->p32.get(), w->z32.get());
+            hipLaunchKernelGGL(k_mixed_update_p, gv, b256, 0, s, n, st, p_in, w->p32.get(), w->z32.get(), w->bf16);
             if (fuse_step4())
                 launch_apply_k_f32(op, w->p32.get(), w->ap32.get(), kApplyAProjected, s, w->part_pap.get(),
                                    &st->halt);
