@@ -12,15 +12,18 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   Prints one JSON line.
 --mode kernels: a short fixed run of each kernel (for rocprofv3 kernel traces and PMC counters),
   including a device copy of known size that calibrates the FETCH_SIZE / WRITE_SIZE counters.
---mode serve: a stream of FEA and SIMP jobs with Poisson arrivals through the job server, against the
-  same stream run one job at a time in plain FP64; reports jobs/s and p50/p99 latency.
+--mode serve: a stream of 48 fresh full-box jobs (20k-250k elements; FEA solves and 40-iteration SIMP runs):
+  every job alone in plain FP64 and in the multigrid solver, the job server's capacity with every job submitted
+  at once (value, jobs/s), and its p50/p99 latency under Poisson arrivals at 80% of that capacity, beside a
+  replay of the same arrivals through one-at-a-time service.
 --mode cases: the batched multi-RHS SIMP configuration: 100x50x50 grid (788,103 DOFs), x = 0 clamped, 16
   independent load cases (1-4 seeded 4x4-node patches each), 20 SIMP iterations on the summed compliance
   (vf 0.25, rmin 2.0, p = 3, solves to 1e-8 warm-started per case) with the multigrid solver; one SIMP
   iteration with the plain FP64 Jacobi solver is timed beside it. Reports SIMP iterations/s.
 --mode precision: the 60x20x20 cantilever with a bimodal (~1e-9 contrast) density field, one cold solve to 1e-8
-  per step in plain FP64 Jacobi-PCG, FP32 Jacobi-PCG with FP64 reliable updates, the multigrid solver, and the
-  multigrid solver with island deflation; errors against an FP64 solve to 1e-12.
+  per step in plain FP64 Jacobi-PCG, FP32 Jacobi-PCG with FP64 reliable updates, the same with bfloat16 search
+  directions, the multigrid solver, and the multigrid solver with island deflation; errors against an FP64
+  solve to 1e-12.
 --mode sweep: K*u (FP64 and FP32) on the holed box and on solid boxes up to 256x128x128 (12.7M DOFs), 100
   launches each timed with HIP events on the operator's stream; reports time per launch and the fraction of
   the HBM and FP64 rooflines at each size (the fixed launch and latency cost of one K*u is amortised as the
@@ -28,8 +31,9 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
 --mode summarize: no GPU; reads rocprofv3 CSV output (--stats-dir, --fetch-dir, --write-dir) of a
   `--mode kernels` run and prints per-kernel durations and HBM bytes per launch. FETCH_SIZE and WRITE_SIZE
   (KiB per dispatch) are each scaled by the factor that makes the largest device copy of the run read and
-  write exactly its 256 MiB; --traffic-json writes the K*u entries that --mode solve reports as
-  roofline.traffic.
+  write exactly its 256 MiB; it also prints the multigrid outer iteration kernel by kernel (launches, algorithmic
+  bytes, durations, PMC bytes); --traffic-json writes the K*u and the multigrid iteration entries that --mode
+  solve reports as ku_roofline.traffic and roofline.traffic.
 """
 from __future__ import annotations
 
@@ -62,7 +66,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--jobs", type=int, default=48, help="serve mode: jobs in the stream")
     ap.add_argument("--workers", type=int, default=4, help="serve mode: resident jobs")
-    ap.add_argument("--simp-iterations", type=int, default=10, help="serve mode: SIMP iterations per job")
+    ap.add_argument("--simp-iterations", type=int, default=40, help="serve mode: SIMP iterations per job")
     ap.add_argument("--max-elements", type=int, default=250_000, help="serve mode: largest job")
     ap.add_argument("--cases-iterations", type=int, default=20, help="cases mode: SIMP iterations")
     ap.add_argument("--serve-precision", choices=("gmg", "fp64"), default="gmg",
@@ -538,6 +542,11 @@ def mode_sweep(args):
 
 
 def mode_serve(args):
+    """The serving configuration: a stream of fresh full-box jobs (20k-250k elements, half FEA solves, half SIMP
+    runs of --simp-iterations iterations). Legs: (1) every job alone, one at a time, in plain FP64 and in the
+    multigrid solver (service times); (2) the job server with every job submitted at once (its capacity at
+    saturation, jobs/s); (3) the server under Poisson arrivals at 80% of that measured capacity (p50/p99 latency),
+    beside a replay of the same arrivals through one-at-a-time service with the service times of (1)."""
     import numpy as np
 
     from mycel_amd import OptimizerOptions, StaticAnalysis, StaticAnalysisOptions, TopologyOptimizer
@@ -558,51 +567,66 @@ def mode_serve(args):
                               simp_iterations=args.simp_iterations)
     models = [j.problem.fe_model() for j in stream]
     forces = [m.AssembleForces() for m in models]
-    # 1. One job at a time, plain FP64 (the baseline): measures service times.
-    service = []
-    for j, m, f in zip(stream, models, forces):
-        t0 = time.perf_counter()
-        if j.kind == "fea":
-            StaticAnalysis.Run(m, StaticAnalysisOptions(Tolerance=1e-8, MaxIterations=500_000))
-        else:
-            TopologyOptimizer.Run(m, options(j))
-        service.append(time.perf_counter() - t0)
-    capacity = len(service) / sum(service)  # jobs/s one at a time
-    rate = 0.8 * capacity * args.workers / 2.0  # offered load: 80% of the measured server capacity estimate
+
+    def alone(precision):
+        service = []
+        for j, m in zip(stream, models):
+            t0 = time.perf_counter()
+            if j.kind == "fea":
+                StaticAnalysis.Run(m, StaticAnalysisOptions(Tolerance=1e-8, MaxIterations=500_000,
+                                                            Precision=precision))
+            else:
+                TopologyOptimizer.Run(m, options(j, precision))
+            service.append(time.perf_counter() - t0)
+        return service
+
+    def replay(arrivals, service):  # one at a time in arrival order: queueing behind earlier jobs + service
+        t_free, lat = 0.0, []
+        for a, sv in zip(arrivals, service):
+            t_free = max(a, t_free) + sv
+            lat.append(t_free - a)
+        return lat, t_free
+
+    def served_run(arrivals):
+        ids, lat = [], []
+        with JobServer(workers=args.workers) as sv:
+            t0 = time.perf_counter()
+            for j, m, f, a in zip(stream, models, forces, arrivals):
+                now = time.perf_counter() - t0
+                if a > now:
+                    time.sleep(a - now)
+                ids.append(sv.submit_fea(m, tol=1e-8, max_iter=500_000, precision=served, forces=f)
+                           if j.kind == "fea" else sv.submit_simp(m, options(j, served), forces=f))
+            for jid in ids:
+                info = sv.wait(jid)
+                if info.state != DONE:
+                    raise SystemExit(f"job {jid} failed with status {info.status}")
+                lat.append(info.latency_ms / 1e3)
+            return lat, time.perf_counter() - t0
+
+    service64 = alone(M.PRECISION_FP64)
+    print(f"[serve] one at a time, FP64: {sum(service64):.1f} s", file=sys.stderr, flush=True)
+    servicemg = alone(M.PRECISION_MIXED_GMG)
+    print(f"[serve] one at a time, multigrid: {sum(servicemg):.1f} s", file=sys.stderr, flush=True)
+    _, burst = served_run(np.zeros(len(stream)))
+    capacity = len(stream) / burst
+    print(f"[serve] server at saturation: {capacity:.2f} jobs/s", file=sys.stderr, flush=True)
+    rate = 0.8 * capacity
     rng = W.SplitMix64(args.seed + 1)
-    gaps = -np.log1p(-rng.uniform(len(stream))) / rate
-    arrivals = np.cumsum(gaps)
-    # Sequential replay of the same arrivals: latency = queueing behind earlier jobs + service.
-    t_free, seq_lat = 0.0, []
-    for a, s in zip(arrivals, service):
-        start = max(a, t_free)
-        t_free = start + s
-        seq_lat.append(t_free - a)
-    seq_makespan = t_free
-    # 2. Continuous batching through the job server with the same arrivals.
-    ids, lat = [], []
-    with JobServer(workers=args.workers) as sv:
-        t0 = time.perf_counter()
-        for j, m, f, a in zip(stream, models, forces, arrivals):
-            now = time.perf_counter() - t0
-            if a > now:
-                time.sleep(a - now)
-            ids.append(sv.submit_fea(m, tol=1e-8, max_iter=500_000, precision=served, forces=f) if j.kind == "fea"
-                       else sv.submit_simp(m, options(j, served), forces=f))
-        for jid in ids:
-            info = sv.wait(jid)
-            if info.state != DONE:
-                raise SystemExit(f"job {jid} failed with status {info.status}")
-            lat.append(info.latency_ms / 1e3)
-        makespan = time.perf_counter() - t0
+    arrivals = np.cumsum(-np.log1p(-rng.uniform(len(stream))) / rate)
+    lat, makespan = served_run(arrivals)
+    print(f"[serve] server at 80% of capacity: p99 {np.percentile(lat, 99):.3f} s", file=sys.stderr, flush=True)
+    pct = lambda v: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
+    lat64, mk64 = replay(arrivals, service64)
+    latmg, mkmg = replay(arrivals, servicemg)
     out = {
         "metric": METRIC,
-        "value": len(stream) / makespan,
+        "value": capacity,
         "unit": "jobs/s",
         "n_gpus": 1,
         "steps": len(stream),
         "warmup": 0,
-        "ms_per_step": makespan / len(stream) * 1e3,
+        "ms_per_step": burst / len(stream) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -611,14 +635,16 @@ def mode_serve(args):
         "config": {"workload": f"serving-stream-{len(stream)}-jobs", "workers": args.workers,
                    "server_solver": {"fp64": "plain FP64 Jacobi-PCG",
                                      "gmg": "FP64 flexible PCG + FP32 multigrid V-cycle"}[args.serve_precision],
-                   "baseline_solver": "plain FP64 Jacobi-PCG, one job at a time",
                    "elements": [20_000, args.max_elements], "simp_iterations": args.simp_iterations,
-                   "offered_rate_jobs_per_s": rate},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "one_at_a_time": {"jobs_per_s": len(stream) / seq_makespan,
-                          "latency_s": {"p50": float(np.percentile(seq_lat, 50)),
-                                        "p99": float(np.percentile(seq_lat, 99))},
-                          "mean_service_s": float(np.mean(service))},
+                   "value": "server capacity: every job submitted at once, jobs / makespan"},
+        "at_80pct_capacity": {"offered_rate_jobs_per_s": rate, "jobs_per_s": len(stream) / makespan,
+                              "latency_s": pct(lat)},
+        "one_at_a_time_fp64": {"capacity_jobs_per_s": len(stream) / sum(service64),
+                               "mean_service_s": float(np.mean(service64)),
+                               "replay_at_same_arrivals": {"latency_s": pct(lat64), "makespan_s": mk64}},
+        "one_at_a_time_multigrid": {"capacity_jobs_per_s": len(stream) / sum(servicemg),
+                                    "mean_service_s": float(np.mean(servicemg)),
+                                    "replay_at_same_arrivals": {"latency_s": pct(latmg), "makespan_s": mkmg}},
     }
     print(json.dumps(out), flush=True)
 
