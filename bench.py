@@ -286,6 +286,7 @@ def mode_solve(args):
     el64, r64, x64 = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_FP64, rho, p.penal)
     elmx, rmx, xmx = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_FP32, rho, p.penal)
     elbf, rbf, xbf = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_BF16, rho, p.penal)
+    elor, ror, xor_ = timed_solves(op, b, 1, 0, args.tol, M.PRECISION_FP64_ORDERED, rho, p.penal)
     elmg, rmg, xmg = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_GMG, rho, p.penal)
     mg_event_ms, mg_iters = timed_solves.last_event_ms, timed_solves.last_iterations
     # The same step (UpdateDensities + multigrid solve) through the host-pointer entry points (densities, b and
@@ -352,6 +353,8 @@ def mode_solve(args):
             "solves_per_s": solvesmx, "pcg_iterations": rmx.Iterations, "reliable_updates": rmx.OuterCorrections,
             "ms_per_solve": elmx / args.steps * 1e3,
             "compliance_rel_err_vs_fp64_1e-12": abs(cmx - c_ref) / abs(c_ref), "u_rel_err": u_err},
+        "fp64_reference_order": {"solves_per_s": 1.0 / elor, "pcg_iterations": ror.Iterations, "ms_per_solve": elor * 1e3,
+                                 "compliance_rel_err_vs_fp64_1e-12": abs(float(ft @ xor_) - c_ref) / abs(c_ref)},
         "mixed_bf16_search_directions": {
             "solves_per_s": args.steps / elbf, "pcg_iterations": rbf.Iterations, "reliable_updates": rbf.OuterCorrections,
             "fp64_fallback_iterations": rbf.FallbackIterations, "ms_per_solve": elbf / args.steps * 1e3,

@@ -170,7 +170,7 @@ def main(argv=None) -> int:
         if a.command == "analyze":
             return run_analyze(a.model, PRECISIONS[a.precision])
         return run_optimize(a.model, a.out, PRECISIONS[a.precision])
-    except Exception as e:  # Program.cs:45-49
+    except Exception as e:  # Program.cs:48-52
         print(f"Error: {e}", file=sys.stderr)
         return 2
 

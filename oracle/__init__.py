@@ -79,6 +79,8 @@ _vp = C.c_void_p
 
 
 def _declare(L):
+    L.orc_pow_penal_n.argtypes = [_dp, C.c_int, C.c_double, _dp]
+    L.orc_pow_penal_n.restype = None
     L.orc_hex8_ke.argtypes = [C.c_double, C.c_double, _dp]
     L.orc_hex8_bc.argtypes = [C.c_double, _dp]
     L.orc_constitutive.argtypes = [C.c_double, C.c_double, _dp]
@@ -136,6 +138,14 @@ class OracleError(RuntimeError):
     def __init__(self, code, what):
         super().__init__(f"oracle {what} failed with status {code}")
         self.code = code
+
+
+def pow_penal(r, p):
+    """rho^p as the oracle's stiffness and sensitivities form it (orc_pow_penal_n)."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty_like(r)
+    lib().orc_pow_penal_n(r, r.size, float(p), out)
+    return out
 
 
 def hex8_ke(h=1.0, nu=0.3):

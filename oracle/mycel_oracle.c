@@ -214,6 +214,10 @@ static double pow_penal(double r, double p) {
     return pow(r, p);
 }
 
+void orc_pow_penal_n(const double *r, int n, double p, double *out) {
+    for (int i = 0; i < n; i++) out[i] = pow_penal(r[i], p);
+}
+
 static void fill_color_e(orc_model *m, const double *density, double penal) {
     double e0 = m->e0, emin = m->emin;
     for (int c = 0; c < 8; c++)

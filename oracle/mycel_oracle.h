@@ -68,6 +68,11 @@ int orc_assemble_forces(int n_nodes_total, int n_cases, const int32_t *case_offs
                         const int32_t *nodes, const double *areas, const float *force_xyz,
                         double *f_out);
 
+/* rho^p as the SIMP stiffness and sensitivities take it (Fem/FemOperator.cs:100,133,
+ * Optimization/SensitivityAnalysis.cs:94,141): rho^3 and rho^2 correctly rounded (error-free products),
+ * other exponents through the C runtime's pow. out[i] = r[i]^p. */
+void orc_pow_penal_n(const double *r, int n, double p, double *out);
+
 /* FemOperator.UpdateDensities + AssembleDiagonal (Fem/FemOperator.cs:114-172). */
 void orc_update_densities(orc_model *m, const double *density_box, double penal);
 
