@@ -8,7 +8,9 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   resident in HBM, in the mixed-precision solver: FP64 flexible PCG (FP64 operator, residual, dot products
   and convergence test) preconditioned by one FP32 geometric-multigrid V-cycle per iteration. The plain
   FP64 Jacobi-PCG solver (the reference's algorithm) and the FP32-iteration solver with FP64 reliable
-  updates are timed the same way beside it; all three are checked against an FP64 solve to 1e-12.
+  updates are timed the same way beside it; all three are checked against an FP64 solve to 1e-12. The
+  metric's job-latency part follows (unless --no-serve): the serving stream of --mode serve through the job
+  server, its capacity at saturation and its p50/p99 latency at 80% of that capacity ("serving").
   Prints one JSON line.
 --mode kernels: a short fixed run of each kernel (for rocprofv3 kernel traces and PMC counters),
   including a device copy of known size that calibrates the FETCH_SIZE / WRITE_SIZE counters.
@@ -63,6 +65,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="solve mode: skip the serving-stream latency leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--jobs", type=int, default=48, help="serve mode: jobs in the stream")
     ap.add_argument("--workers", type=int, default=4, help="serve mode: resident jobs")
@@ -401,6 +404,10 @@ def mode_solve(args):
         "cpu_baseline": None,
     }
     op.close()
+    if not args.no_serve:
+        # The metric's job-latency part: the serving stream through the job server (capacity at saturation,
+        # p50/p99 at 80% of it); --mode serve adds the one-at-a-time legs.
+        out["serving"] = serving(args, one_at_a_time=False)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, r64.Iterations, args.cpu_seconds)
     print(json.dumps(out), flush=True)
@@ -544,12 +551,13 @@ def mode_sweep(args):
                       "fp64_peak_tflops": FP64_PEAK_TFLOPS, "sizes": rows}), flush=True)
 
 
-def mode_serve(args):
+def serving(args, one_at_a_time=True):
     """The serving configuration: a stream of fresh full-box jobs (20k-250k elements, half FEA solves, half SIMP
     runs of --simp-iterations iterations). Legs: (1) every job alone, one at a time, in plain FP64 and in the
-    multigrid solver (service times); (2) the job server with every job submitted at once (its capacity at
-    saturation, jobs/s); (3) the server under Poisson arrivals at 80% of that measured capacity (p50/p99 latency),
-    beside a replay of the same arrivals through one-at-a-time service with the service times of (1)."""
+    multigrid solver (service times; skipped without one_at_a_time); (2) the job server with every job submitted
+    at once (its capacity at saturation, jobs/s); (3) the server under Poisson arrivals at 80% of that measured
+    capacity (p50/p99 latency), beside a replay of the same arrivals through one-at-a-time service with the
+    service times of (1)."""
     import numpy as np
 
     from mycel_amd import OptimizerOptions, StaticAnalysis, StaticAnalysisOptions, TopologyOptimizer
@@ -607,10 +615,16 @@ def mode_serve(args):
                 lat.append(info.latency_ms / 1e3)
             return lat, time.perf_counter() - t0
 
-    service64 = alone(M.PRECISION_FP64)
-    print(f"[serve] one at a time, FP64: {sum(service64):.1f} s", file=sys.stderr, flush=True)
-    servicemg = alone(M.PRECISION_MIXED_GMG)
-    print(f"[serve] one at a time, multigrid: {sum(servicemg):.1f} s", file=sys.stderr, flush=True)
+    pct = lambda v: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
+    out = {"jobs": len(stream), "workers": args.workers, "elements": [20_000, args.max_elements],
+           "simp_iterations": args.simp_iterations,
+           "server_solver": {"fp64": "plain FP64 Jacobi-PCG",
+                             "gmg": "FP64 flexible PCG + FP32 multigrid V-cycle"}[args.serve_precision]}
+    if one_at_a_time:
+        service64 = alone(M.PRECISION_FP64)
+        print(f"[serve] one at a time, FP64: {sum(service64):.1f} s", file=sys.stderr, flush=True)
+        servicemg = alone(M.PRECISION_MIXED_GMG)
+        print(f"[serve] one at a time, multigrid: {sum(servicemg):.1f} s", file=sys.stderr, flush=True)
     _, burst = served_run(np.zeros(len(stream)))
     capacity = len(stream) / burst
     print(f"[serve] server at saturation: {capacity:.2f} jobs/s", file=sys.stderr, flush=True)
@@ -619,36 +633,43 @@ def mode_serve(args):
     arrivals = np.cumsum(-np.log1p(-rng.uniform(len(stream))) / rate)
     lat, makespan = served_run(arrivals)
     print(f"[serve] server at 80% of capacity: p99 {np.percentile(lat, 99):.3f} s", file=sys.stderr, flush=True)
-    pct = lambda v: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
-    lat64, mk64 = replay(arrivals, service64)
-    latmg, mkmg = replay(arrivals, servicemg)
+    out["capacity_jobs_per_s"] = capacity
+    out["saturation_makespan_s"] = burst
+    out["at_80pct_capacity"] = {"offered_rate_jobs_per_s": rate, "jobs_per_s": len(stream) / makespan,
+                                "latency_s": pct(lat)}
+    if one_at_a_time:
+        lat64, mk64 = replay(arrivals, service64)
+        latmg, mkmg = replay(arrivals, servicemg)
+        out["one_at_a_time_fp64"] = {"capacity_jobs_per_s": len(stream) / sum(service64),
+                                     "mean_service_s": float(np.mean(service64)),
+                                     "replay_at_same_arrivals": {"latency_s": pct(lat64), "makespan_s": mk64}}
+        out["one_at_a_time_multigrid"] = {"capacity_jobs_per_s": len(stream) / sum(servicemg),
+                                          "mean_service_s": float(np.mean(servicemg)),
+                                          "replay_at_same_arrivals": {"latency_s": pct(latmg), "makespan_s": mkmg}}
+    return out
+
+
+def mode_serve(args):
+    sv = serving(args)
     out = {
         "metric": METRIC,
-        "value": capacity,
+        "value": sv["capacity_jobs_per_s"],
         "unit": "jobs/s",
         "n_gpus": 1,
-        "steps": len(stream),
+        "steps": sv["jobs"],
         "warmup": 0,
-        "ms_per_step": burst / len(stream) * 1e3,
+        "ms_per_step": sv["saturation_makespan_s"] / sv["jobs"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.serve_precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"serving-stream-{len(stream)}-jobs", "workers": args.workers,
-                   "server_solver": {"fp64": "plain FP64 Jacobi-PCG",
-                                     "gmg": "FP64 flexible PCG + FP32 multigrid V-cycle"}[args.serve_precision],
-                   "elements": [20_000, args.max_elements], "simp_iterations": args.simp_iterations,
+        "config": {"workload": f"serving-stream-{sv['jobs']}-jobs", "workers": args.workers,
+                   "server_solver": sv["server_solver"], "elements": sv["elements"],
+                   "simp_iterations": sv["simp_iterations"],
                    "value": "server capacity: every job submitted at once, jobs / makespan"},
-        "at_80pct_capacity": {"offered_rate_jobs_per_s": rate, "jobs_per_s": len(stream) / makespan,
-                              "latency_s": pct(lat)},
-        "one_at_a_time_fp64": {"capacity_jobs_per_s": len(stream) / sum(service64),
-                               "mean_service_s": float(np.mean(service64)),
-                               "replay_at_same_arrivals": {"latency_s": pct(lat64), "makespan_s": mk64}},
-        "one_at_a_time_multigrid": {"capacity_jobs_per_s": len(stream) / sum(servicemg),
-                                    "mean_service_s": float(np.mean(servicemg)),
-                                    "replay_at_same_arrivals": {"latency_s": pct(latmg), "makespan_s": mkmg}},
     }
+    out.update({k: sv[k] for k in ("at_80pct_capacity", "one_at_a_time_fp64", "one_at_a_time_multigrid")})
     print(json.dumps(out), flush=True)
 
 

@@ -287,6 +287,48 @@ def test_solver_special_cases(gpu):
         op.close()
 
 
+@pytest.mark.parametrize("where", ["forces", "density"])
+def test_nan_input_surfaces_as_breakdown(gpu, where):
+    """A NaN injected into one load or one density (the reference validates neither) surfaces as the reference's
+    curvature breakdown at iteration 1 (pAp is NaN: Fem/PcgSolver.cs:245-252), as in the oracle: same message
+    code, iteration count and final residual, in the reference-order and fast FP64 solvers; the multigrid and
+    mixed solvers also stop with a breakdown status instead of converging; with the NaN density StaticAnalysis
+    raises the reference's non-convergence InvalidOperationException."""
+    import mycel_amd as M
+
+    p = W.cantilever_simp(12, 4, 4)
+    if where == "density":
+        d = np.ones(p.nx * p.ny * p.nz)
+        d[5] = np.nan
+        p.density = d
+    m = oracle_model(p)
+    f = oracle_forces(p)
+    b = np.where(m.constrained, 0.0, f)
+    if where == "forces":
+        b[np.flatnonzero(b)[0]] = np.nan
+    _, ro = m.pcg(b, tol=1e-8, max_iter=1000)
+    assert ro.message == 3 and not ro.converged
+    fm = p.fe_model()
+    op = FemOperator(fm)
+    try:
+        for prec in (M.PRECISION_FP64_ORDERED, M.PRECISION_FP64):
+            r = PcgSolver().Solve(op, b, np.zeros(p.n_dofs), tol=1e-8, maxIterations=1000, precision=prec)
+            print(f"[nan {where} {prec}] it={r.Iterations} code={r.MessageCode} res={r.FinalResidual} "
+                  f"msg={r.Message!r}", flush=True)
+            assert (r.MessageCode, r.Iterations, r.Converged) == (ro.message, ro.iterations, False)
+            assert r.FinalResidual == ro.final_residual or (np.isnan(r.FinalResidual) and np.isnan(ro.final_residual))
+            assert r.Message and "breakdown" in r.Message.lower()
+        for prec in (M.PRECISION_MIXED_GMG, M.PRECISION_MIXED_FP32):
+            r = PcgSolver().Solve(op, b, np.zeros(p.n_dofs), tol=1e-8, maxIterations=1000, precision=prec)
+            print(f"[nan {where} {prec}] it={r.Iterations} code={r.MessageCode} msg={r.Message!r}", flush=True)
+            assert not r.Converged and r.MessageCode in (3, 4, 5, 6)
+        if where == "density":
+            with pytest.raises(InvalidOperationError):
+                StaticAnalysis.Run(fm, StaticAnalysisOptions(Tolerance=1e-8), op)
+    finally:
+        op.close()
+
+
 def _step4_solves():
     """FP64 Jacobi, multigrid and mixed FP32 (reliable updates) solves on one operator: converged, and stopped by
     the iteration limit at an odd and an even count (the fused step 4 alternates two p buffers; the mixed solver
