@@ -180,6 +180,25 @@ def time_matvecs(op, n, fp32=False):
     return e0.elapsed_time(e1) / n
 
 
+def time_copy(nbytes, n=100):
+    """Average time of a plain device-to-device copy moving nbytes in total (nbytes / 2 read, nbytes / 2
+    written), HIP events over n back-to-back launches: what any kernel moving these bytes in one launch can
+    expect at this size."""
+    import torch
+
+    x = torch.randn(max(1, int(nbytes) // 16), dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(n):
+        y.copy_(x)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
 def timed_solves(op, b, steps, warmup, tol, precision, rho=None, penal=3.0):
     """A step is UpdateDensities(rho) (when given: the moduli, the Jacobi diagonal and, for the multigrid
     solver, the whole hierarchy are rebuilt) followed by one cold solve."""
@@ -285,6 +304,7 @@ def mode_solve(args):
     info = op.info()
     mv_ms = time_matvecs(op, args.matvecs)
     mv32_ms = time_matvecs(op, args.matvecs, fp32=True)
+    copy_ms = time_copy(2 * 3 * 8 * op.info().active_nodes + 8 * op.info().active_elements, args.matvecs)
     rho = torch.tensor(p.density_box(), dtype=torch.float64, device="cuda:0")
     el64, r64, x64 = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_FP64, rho, p.penal)
     elmx, rmx, xmx = timed_solves(op, b, args.steps, args.warmup, args.tol, M.PRECISION_MIXED_FP32, rho, p.penal)
@@ -400,6 +420,10 @@ def mode_solve(args):
             "avg_launch_ms": mv_ms,
             "fp64": {"achieved_tflops": flops_mv / (mv_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
                      "frac": flops_mv / (mv_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "flops_per_launch": flops_mv},
+            # A plain device copy of the same bytes (half read, half written), one launch, HIP events: the
+            # practical floor for one launch of this size.
+            "copy_same_bytes": {"avg_launch_ms": copy_ms, "gbs": bytes_mv / (copy_ms * 1e-3) / 1e9,
+                                "frac": bytes_mv / (copy_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         },
         "cpu_baseline": None,
     }
@@ -534,14 +558,17 @@ def mode_sweep(args):
             ms = time_matvecs(op, args.matvecs, fp32=fp32)
             byt = 2 * 3 * sz * nn + sz * na
             gbs = byt / (ms * 1e-3) / 1e9
+            cms = time_copy(byt, args.matvecs)
             row[name] = {"us_per_launch": ms * 1e3, "algorithmic_bytes": byt, "achieved_gbs": gbs,
                          "hbm_frac": gbs / HBM_PEAK_GBS,
-                         "tflops": 1152.0 * na / (ms * 1e-3) / 1e12}
+                         "tflops": 1152.0 * na / (ms * 1e-3) / 1e12,
+                         "copy_same_bytes_us": cms * 1e3, "copy_hbm_frac": byt / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS}
             if not fp32:
                 row[name]["fp64_frac"] = row[name]["tflops"] / FP64_PEAK_TFLOPS
         op.close()
         print(f"[sweep] {nx}x{ny}x{nz}{' holed' if holed else ''}: fp64 {row['fp64']['us_per_launch']:.2f} us "
-              f"(hbm {row['fp64']['hbm_frac']:.3f}, fp64 {row['fp64']['fp64_frac']:.3f}), fp32 "
+              f"(hbm {row['fp64']['hbm_frac']:.3f}, fp64 {row['fp64']['fp64_frac']:.3f}; copy of the same bytes "
+              f"{row['fp64']['copy_same_bytes_us']:.2f} us, hbm {row['fp64']['copy_hbm_frac']:.3f}), fp32 "
               f"{row['fp32']['us_per_launch']:.2f} us (hbm {row['fp32']['hbm_frac']:.3f}); by form "
               + " ".join(f"{k}:{v['fp64_us']:.1f}/{v['fp32_us']:.1f}" for k, v in cols.items()), file=sys.stderr,
               flush=True)
