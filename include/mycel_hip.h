@@ -218,6 +218,13 @@ MYCEL_API int32_t mycel_pcg_solve_batch(mycel_op *op, const double *b, double *x
                                         int32_t max_iter, int32_t precision, mycel_pcg_result *results);
 MYCEL_API int32_t mycel_pcg_solve_batch_d(mycel_op *op, const double *d_b, double *d_x, int32_t n_rhs, double tol,
                                           int32_t max_iter, int32_t precision, mycel_pcg_result *results);
+/* Independent systems on different operators (each its own grid and handle) solved together: every multigrid
+ * iteration is one launch per kernel for all systems (no counterpart in the reference, which solves one model at
+ * a time; the job server packs its jobs' solves the same way). count <= 16 distinct handles; d_b[i], d_x[i]:
+ * device box-DOF vectors of ops[i] (x: warm start in, solution out). Precision MYCEL_PRECISION_MIXED_GMG; each
+ * result and solution is bit-identical to mycel_pcg_solve_d of that system alone. Synchronous. */
+MYCEL_API int32_t mycel_pcg_solve_packed_d(mycel_op *const *ops, const double *const *d_b, double *const *d_x,
+                                           int32_t count, double tol, int32_t max_iter, mycel_pcg_result *results);
 
 /* ---- StaticAnalysis (Fem/StaticAnalysis.cs) ----------------------------------------------- */
 /* StaticAnalysis.Run(model, options) (Fem/StaticAnalysis.cs:35-142). f: assembled forces (box DOFs),
@@ -441,6 +448,23 @@ MYCEL_API int32_t mycel_server_fetch(mycel_server *server, int64_t job_id, doubl
 MYCEL_API int32_t mycel_server_release(mycel_server *server, int64_t job_id);
 MYCEL_API int32_t mycel_server_stats(mycel_server *server, double *budget_bytes, double *in_use_bytes,
                                      int32_t *queued, int32_t *active);
+
+/* Packing (no counterpart in the reference): the multigrid solves of the resident jobs (FEA jobs and every SIMP
+ * iteration's solve, MYCEL_PRECISION_MIXED_GMG) advance together, each iteration one launch per kernel for all
+ * of them (up to 16), 4 iterations per batch; solves join and leave between batches, and a membership that
+ * stays the same for a batch is captured as a hipGraph. A job's result is bit-identical packed or not.
+ * On by default; set before the first submit. Jobs whose solve cannot be packed (island deflation, grids
+ * without a multigrid hierarchy, other precisions) run on their worker's stream as before. */
+MYCEL_API int32_t mycel_server_set_packing(mycel_server *server, int32_t enable);
+typedef struct {
+    int64_t batches;        /* packed batches run */
+    int64_t member_batches; /* sum over batches of the solves in the batch */
+    int64_t launches;       /* kernel launches of those batches */
+    int64_t graph_batches;  /* batches replayed from a captured graph */
+    int64_t rebuilds;       /* membership changes (tables rebuilt) */
+    int64_t solves;         /* solves that left the pack */
+} mycel_pack_stats;
+MYCEL_API int32_t mycel_server_pack_stats(mycel_server *server, mycel_pack_stats *out);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,11 @@ class JobDescC(C.Structure):
                 ("precision", C.c_int32), ("simp", SimpOptionsC)]
 
 
+class PackStatsC(C.Structure):
+    _fields_ = [("batches", C.c_int64), ("member_batches", C.c_int64), ("launches", C.c_int64),
+                ("graph_batches", C.c_int64), ("rebuilds", C.c_int64), ("solves", C.c_int64)]
+
+
 class JobInfoC(C.Structure):
     _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("compliance", C.c_double),
                 ("max_displacement", C.c_double), ("pcg_iterations", C.c_int32), ("simp_iterations", C.c_int32),
@@ -209,6 +214,9 @@ SIGNATURES = {
                             C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
     "mycel_pcg_solve_batch": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, C.POINTER(PcgResultC)]),
     "mycel_pcg_solve_batch_d": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, C.POINTER(PcgResultC)]),
+    "mycel_pcg_solve_packed_d": (_i, [_vp, _vp, _vp, _i, _d, _i, C.POINTER(PcgResultC)]),
+    "mycel_server_set_packing": (_i, [_vp, _i]),
+    "mycel_server_pack_stats": (_i, [_vp, C.POINTER(PackStatsC)]),
     "mycel_simp_run_cases": (_i, [_vp, _vp, _i, C.POINTER(SimpOptionsC), _vp, _vp, _i, _vp, _i, _vp, SNAPSHOT_CB,
                                   _vp, C.POINTER(SimpResultC), C.POINTER(SnapshotC), _i, _vp, _vp, _vp]),
 }

@@ -1149,6 +1149,20 @@ int32_t mycel_pcg_solve_batch_d(mycel_op *op, const double *d_b, double *d_x, in
     });
 }
 
+int32_t mycel_pcg_solve_packed_d(mycel_op *const *ops, const double *const *d_b, double *const *d_x, int32_t count,
+                                 double tol, int32_t max_iter, mycel_pcg_result *results) {
+    return guarded([&] {
+        require(ops && d_b && d_x && results, MYCEL_E_ARGUMENT_NULL, "ops/b/x/results");
+        require(count >= 1 && count <= 16, MYCEL_E_OUT_OF_RANGE, "count must be in [1, 16]");
+        for (int i = 0; i < count; i++) {
+            require(ops[i] && d_b[i] && d_x[i], MYCEL_E_ARGUMENT_NULL, "ops/b/x entry");
+            require(ops[i]->device == ops[0]->device, MYCEL_E_ARGUMENT, "every operator must be on one device");
+        }
+        DeviceGuard dg(ops[0]->device);
+        pcg_solve_packed(ops, d_b, d_x, count, tol, max_iter, results);
+    });
+}
+
 int32_t mycel_simp_run_cases(mycel_op *op, const double *f, int32_t n_cases, const mycel_simp_options *opt,
                              const uint8_t *passive_mask_box, const int32_t *patch_nodes, int32_t n_patch_nodes,
                              const float *initial_density, int32_t initial_density_len, const double *warm_u,

@@ -540,18 +540,14 @@ __device__ __forceinline__ bool split_gather(const Dims &g, const KeSym<float> &
     return half == 0 && n < g.n_nodes;
 }
 
-__global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
-                                                    const float *__restrict__ x, const float *__restrict__ b,
-                                                    float *__restrict__ d, const float *__restrict__ inv, float cd,
-                                                    float cr, float *__restrict__ x_out) {
-    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    x += o;
-    b += o;
-    d += o;
-    x_out += o;
+// Block bx of nb (the XCD remap runs over nb) of one right-hand side's vectors.
+__device__ __forceinline__ void cheb_s_body(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
+                                            const float *__restrict__ x, const float *__restrict__ b,
+                                            float *__restrict__ d, const float *__restrict__ inv, float cd, float cr,
+                                            float *__restrict__ x_out, int bx, int nb) {
     float kx[3], xc[3];
     int n;
-    if (split_gather(g, ks, ep, x, xcd_block(blockIdx.x, gridDim.x) * kSplitNodes, kx, xc, n)) {
+    if (split_gather(g, ks, ep, x, xcd_block(bx, nb) * kSplitNodes, kx, xc, n)) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int dof = 3 * n + c;
@@ -561,6 +557,14 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, con
             d[dof] = dn;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                    const float *__restrict__ x, const float *__restrict__ b,
+                                                    float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                    float cr, float *__restrict__ x_out) {
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    cheb_s_body(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(256) void k_gmg_residual_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
@@ -612,17 +616,12 @@ __global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__re
 
 // Restriction with one thread per coarse DOF (the large levels: three times the parallelism of the node
 // form, the same sums in the same order).
-__global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *__restrict__ rf, Dims gc,
-                                                          const float *__restrict__ invc, float theta_inv,
-                                                          float *__restrict__ bc, float *__restrict__ xcv,
-                                                          float *__restrict__ dc) {
-    const int dof = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void restrict_dof_body(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
+                                                  const float *__restrict__ invc, float theta_inv,
+                                                  float *__restrict__ bc, float *__restrict__ xcv,
+                                                  float *__restrict__ dc, int bx) {
+    const int dof = bx * 256 + threadIdx.x;
     if (dof >= gc.n_dofs) return;
-    rf += static_cast<long long>(blockIdx.y) * gf.n_dofs;
-    const long long oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
-    bc += oc;
-    xcv += oc;
-    dc += oc;
     const int n = dof / 3, c = dof - 3 * n;
     int I, J, K;
     node_ijk(gc, n, I, J, K);
@@ -648,26 +647,28 @@ __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *
     dc[dof] = xi;
 }
 
+__global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *__restrict__ rf, Dims gc,
+                                                          const float *__restrict__ invc, float theta_inv,
+                                                          float *__restrict__ bc, float *__restrict__ xcv,
+                                                          float *__restrict__ dc) {
+    const long long of = static_cast<long long>(blockIdx.y) * gf.n_dofs;
+    const long long oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
+    restrict_dof_body(gf, rf + of, gc, invc, theta_inv, bc + oc, xcv + oc, dc + oc, blockIdx.x);
+}
+
 // The residual of a coarse level fused with the restriction to the next one (the kernels of those levels are
 // bound by their fixed cost, not by arithmetic). Half-wave h of a block owns coarse node 8 blockIdx.x + h; its
 // lane t < 27 forms the residual at restriction tap t (the fine node's own gather, so taps shared by neighbouring
 // coarse nodes are formed again), weighted; the half-wave adds its 32 lanes in a fixed butterfly, and the
 // restriction's coarse pre-smoothing first step follows (x = d = theta^-1 D^-1 b).
-__global__ __launch_bounds__(256) void k_gmg_residual_restrict(Dims gf, KeSym<float> ks, const float *__restrict__ ep,
-                                                               const float *__restrict__ xf,
-                                                               const float *__restrict__ bf,
-                                                               const float *__restrict__ invf, Dims gc,
-                                                               const float *__restrict__ invc, float theta_inv,
-                                                               float *__restrict__ bc, float *__restrict__ xcv,
-                                                               float *__restrict__ dc) {
-    const long long of = static_cast<long long>(blockIdx.y) * gf.n_dofs, oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
-    xf += of;
-    bf += of;
-    bc += oc;
-    xcv += oc;
-    dc += oc;
+__device__ __forceinline__ void residual_restrict_body(const Dims &gf, const KeSym<float> &ks,
+                                                       const float *__restrict__ ep, const float *__restrict__ xf,
+                                                       const float *__restrict__ bf, const float *__restrict__ invf,
+                                                       const Dims &gc, const float *__restrict__ invc,
+                                                       float theta_inv, float *__restrict__ bc,
+                                                       float *__restrict__ xcv, float *__restrict__ dc, int bx) {
     const int t = threadIdx.x & 31;
-    const int n = (blockIdx.x * 256 + threadIdx.x) >> 5;
+    const int n = (bx * 256 + threadIdx.x) >> 5;
     float v[3] = {0.0f, 0.0f, 0.0f};
     if (n < gc.n_nodes && t < 27) {
         int I, J, K;
@@ -699,6 +700,18 @@ __global__ __launch_bounds__(256) void k_gmg_residual_restrict(Dims gf, KeSym<fl
             dc[dof] = xi;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_gmg_residual_restrict(Dims gf, KeSym<float> ks, const float *__restrict__ ep,
+                                                               const float *__restrict__ xf,
+                                                               const float *__restrict__ bf,
+                                                               const float *__restrict__ invf, Dims gc,
+                                                               const float *__restrict__ invc, float theta_inv,
+                                                               float *__restrict__ bc, float *__restrict__ xcv,
+                                                               float *__restrict__ dc) {
+    const long long of = static_cast<long long>(blockIdx.y) * gf.n_dofs, oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
+    residual_restrict_body(gf, ks, ep, xf + of, bf + of, invf, gc, invc, theta_inv, bc + oc, xcv + oc, dc + oc,
+                           blockIdx.x);
 }
 
 // Whether the coarse levels use k_gmg_residual_restrict (MYCEL_GMG_FUSE_RR=0: separate residual and restriction).
@@ -778,7 +791,9 @@ struct TailArgs {
 
 static_assert(sizeof(TailArgs) % 4 == 0 && sizeof(TailArgs) / 4 <= kTailThreads, "tail arguments: one word per thread");
 
-__global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
+// The tail of right-hand side rhs (its vectors at rhs * n_dofs of each level) from the arguments at targ: the
+// kernel's own arguments (one workgroup per right-hand side) or a pack member's table entry.
+__device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int rhs) {
     extern __shared__ __align__(16) float sm[];
     constexpr int kParts = kTailThreads / kCoarseMaxDofs, kSpan = (kCoarseMaxDofs + kParts - 1) / kParts;
     __shared__ double bsh[kCoarseMaxDofs];
@@ -787,10 +802,9 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
     // are scalar loads from the kernel-argument segment, each wait a memory latency, in every phase.
     __shared__ TailArgs t;
     if (threadIdx.x < sizeof(TailArgs) / 4)
-        reinterpret_cast<uint32_t *>(&t)[threadIdx.x] = reinterpret_cast<const uint32_t *>(&targ)[threadIdx.x];
+        reinterpret_cast<uint32_t *>(&t)[threadIdx.x] = reinterpret_cast<const uint32_t *>(targ)[threadIdx.x];
     __syncthreads();
-    // One workgroup per right-hand side (blockIdx.x).
-    const long long o0 = static_cast<long long>(blockIdx.x) * t.lv[0].d.n_dofs;
+    const long long o0 = static_cast<long long>(rhs) * t.lv[0].d.n_dofs;
     {
         // Every segment holds <= kTailPer * kTailThreads floats (checked on the host); the coarsest inverse
         // (n^2 doubles) joins the same round trip when it fits in LDS.
@@ -817,7 +831,7 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
             const int i = threadIdx.x + u * kTailThreads;
             va[u] = t.ainv[i < nn ? i : 0];
         }
-        const int halted = t.halt ? t.halt[blockIdx.x * t.halt_stride] : 0;
+        const int halted = t.halt ? t.halt[rhs * t.halt_stride] : 0;
 #pragma unroll
         for (int sg = 0; sg < kTailSegs; sg++) {
             const TailSeg g = t.seg[sg];
@@ -953,12 +967,141 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) {
     }
     const float *res = cur(0);
     if (t.pro_x) {
-        float *xf = t.pro_x + static_cast<long long>(blockIdx.x) * t.pro_d.n_dofs;
+        float *xf = t.pro_x + static_cast<long long>(rhs) * t.pro_d.n_dofs;
         for (int n = threadIdx.x; n < t.pro_d.n_nodes; n += kTailThreads)
             prolong_node(t.pro_d, xf, t.pro_inv, t.lv[0].d, res, n);
     } else {
         for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) t.out0[o0 + q] = res[q];
     }
+}
+
+// One workgroup per right-hand side (blockIdx.x).
+__global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) { tail_run(&targ, blockIdx.x); }
+
+// ---- packed V-cycles -------------------------------------------------------------------------------
+// The V-cycles of several operators (each with its own grid, hierarchy and vectors) in one launch per phase:
+// grid dimension y is the member, its arguments are its entry of a device table, and blocks beyond the member's
+// own grid return at entry. A member's blocks do exactly what the blocks of its own launch do (the same block
+// indices, the XCD remap over its own block count, the same partial slots), so its results are bit-identical to
+// its V-cycle alone. The finest-level kernels and the tail test the member's halt flag, so a member that
+// converged within a batch skips the rest of it.
+
+struct PkEc {  // finest level, element-centric: Chebyshev step, residual, last step (z and its partials)
+    Dims g;
+    KeMono<float> km;
+    const float *ep, *x, *b, *inv;
+    float *d, *out;  // Chebyshev: d and x'; residual: out = r
+    float cd, cr;
+    double *z64;
+    const double *r64, *ap64;
+    double *part_zr, *part_zap;
+    const int *halt;
+    int n_tiles, nb;
+};
+struct PkS {  // slab-split Chebyshev step of a coarse level
+    Dims g;
+    KeSym<float> ks;
+    const float *ep, *x, *b, *inv;
+    float *d, *x_out;
+    float cd, cr;
+    int nb;
+};
+struct PkRR {  // coarse residual fused with the restriction
+    Dims gf, gc;
+    KeSym<float> ks;
+    const float *ep, *xf, *bf, *invf, *invc;
+    float *bc, *xcv, *dc;
+    float theta_inv;
+    int nb;
+};
+struct PkRD {  // restriction from the finest level, one thread per coarse DOF
+    Dims gf, gc;
+    const float *rf, *invc;
+    float *bc, *xcv, *dc;
+    float theta_inv;
+    int nb;
+};
+struct PkPro {  // prolongation
+    Dims gf, gc;
+    float *xf;
+    const float *invf, *xcv;
+    int nb;
+};
+
+__device__ __forceinline__ int pk_wave_tile(int nb) {
+    return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, nb) * 4 + (threadIdx.x >> 6));
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256) void k_pk_cheb_ec(const PkEc *__restrict__ tab) {
+    const PkEc a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;
+    const int tile = pk_wave_tile(a.nb);
+    if (tile >= a.n_tiles) return;  // wave-uniform; no barrier follows
+    ChebFin fin{a.b, a.inv, a.d, a.out, a.cd, a.cr};
+    FineX src{a.x, *a.halt};
+    ec_tile_march<float, KZ>(a.g, a.km, a.ep, src, tile, fin);
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256) void k_pk_residual_ec(const PkEc *__restrict__ tab) {
+    const PkEc a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;
+    const int tile = pk_wave_tile(a.nb);
+    if (tile >= a.n_tiles) return;
+    ResidualFin fin{a.b, a.inv, a.out};
+    FineX src{a.x, *a.halt};
+    ec_tile_march<float, KZ>(a.g, a.km, a.ep, src, tile, fin);
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256) void k_pk_cheb_final_ec(const PkEc *__restrict__ tab) {
+    __shared__ double red[8];
+    const PkEc a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;  // block-uniform, ahead of the block sum's barriers
+    const Defl none{nullptr, nullptr, nullptr, 0.0, 0, 0, 0};
+    FinalFin fin{a.b, a.d, a.inv, a.z64, a.r64, a.ap64, a.cd, a.cr, none, nullptr};
+    const int tile = pk_wave_tile(a.nb);
+    if (tile < a.n_tiles) {
+        FineX src{a.x, *a.halt};
+        ec_tile_march<float, KZ>(a.g, a.km, a.ep, src, tile, fin);
+    }
+    double zr = fin.zr, zap = fin.zap;
+    block_sum256x2(zr, zap, red);
+    if (threadIdx.x == 0) {
+        a.part_zr[blockIdx.x] = zr;
+        a.part_zap[blockIdx.x] = zap;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pk_cheb_s(const PkS *__restrict__ tab) {
+    const PkS a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;
+    cheb_s_body(a.g, a.ks, a.ep, a.x, a.b, a.d, a.inv, a.cd, a.cr, a.x_out, blockIdx.x, a.nb);
+}
+
+__global__ __launch_bounds__(256) void k_pk_residual_restrict(const PkRR *__restrict__ tab) {
+    const PkRR a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;
+    residual_restrict_body(a.gf, a.ks, a.ep, a.xf, a.bf, a.invf, a.gc, a.invc, a.theta_inv, a.bc, a.xcv, a.dc,
+                           blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_pk_restrict_dof(const PkRD *__restrict__ tab) {
+    const PkRD a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;
+    restrict_dof_body(a.gf, a.rf, a.gc, a.invc, a.theta_inv, a.bc, a.xcv, a.dc, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_pk_prolong(const PkPro *__restrict__ tab) {
+    const PkPro a = tab[blockIdx.y];
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (static_cast<int>(blockIdx.x) < a.nb && n < a.gf.n_nodes) prolong_node(a.gf, a.xf, a.invf, a.gc, a.xcv, n);
+}
+
+// One workgroup per member (blockIdx.x), its arguments from the table.
+__global__ __launch_bounds__(kTailThreads) void k_pk_tail(const TailArgs *__restrict__ tab) {
+    tail_run(tab + blockIdx.x, 0);
 }
 
 // ---- set-up ----------------------------------------------------------------------------------------
@@ -1590,6 +1733,76 @@ GmgFineView gmg_fine_view(mycel_op *op) {
     return GmgFineView{F.b.get(), F.x.get(), F.dv.get(), F.inv.get(), op->gmg->theta_inv};
 }
 
+// The tail kernel's arguments: levels [t0, top] run in the single-workgroup tail (the finest level never does: it
+// has the FP64 entry and exit kernels), as many as fit its node limit, its per-array limit and its LDS. cur / nxt:
+// each level's current iterate and scratch before the tail; tail_lds: the dynamic LDS bytes of its launch.
+static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nxt, const int *halt, int halt_stride,
+                               int &t0, size_t &tail_lds) {
+    auto tail_floats = [&](int l) { return g->lv[l].d.n_pad + 6 * g->lv[l].d.n_dofs; };
+    auto tail_fits = [&](int l) {
+        const Dims &d = g->lv[l].d;
+        return d.n_dofs <= kTailPer * kTailThreads && d.n_pad <= kTailPer * kTailThreads;
+    };
+    const int top = g->levels - 1;
+    t0 = top;
+    long long lds_floats = tail_floats(top);
+    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= tail_nodes() && tail_fits(t0 - 1) &&
+           4 * (lds_floats + tail_floats(t0 - 1)) <= kTailLdsMax) {
+        t0--;
+        lds_floats += tail_floats(t0);
+    }
+    if (top == 0) t0 = 0;
+    TailArgs ta{};
+    ta.count = top - t0 + 1;
+    if (ta.count > kMaxTail || !tail_fits(top) || 4 * lds_floats > kTailLdsMax)
+        throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: coarsest level does not fit the tail kernel");
+    int off = 0;
+    for (int l = t0; l <= top; l++) {
+        const GmgLevel &L = g->lv[l];
+        TailLevel &T = ta.lv[l - t0];
+        T.d = L.d;
+        T.ks = L.ks;
+        T.ep = L.ep;
+        T.inv = L.inv.get();
+        const int nd = L.d.n_dofs;
+        T.o_e = off;
+        off += L.d.n_pad;
+        T.o_inv = off;
+        T.o_b = off + nd;
+        T.o_x = off + 2 * nd;
+        T.o_x2 = off + 3 * nd;
+        T.o_d = off + 4 * nd;
+        T.o_r = off + 5 * nd;
+        off += 6 * nd;
+    }
+    // The coarsest inverse (FP32) joins the LDS image when it fits.
+    const int nc = g->lv[top].d.n_dofs;
+    ta.ainv_lds = 4 * (static_cast<long long>(off) + static_cast<long long>(nc) * nc) <= kTailLdsMax ? off : -1;
+    if (ta.ainv_lds >= 0) off += nc * nc;
+    tail_lds = sizeof(float) * static_cast<size_t>(off);
+    ta.ainv_nn = ta.ainv_lds >= 0 ? nc * nc : 0;
+    {
+        const TailLevel &T0 = ta.lv[0];
+        ta.seg[0] = TailSeg{g->lv[t0].b.get(), T0.d.n_dofs, T0.o_b};
+        ta.seg[1] = TailSeg{cur[t0], T0.d.n_dofs, T0.o_x};
+        ta.seg[2] = TailSeg{g->lv[t0].dv.get(), T0.d.n_dofs, T0.o_d};
+        for (int l = 0; l < ta.count; l++) {
+            const TailLevel &T = ta.lv[l];
+            ta.seg[3 + 2 * l] = TailSeg{T.ep, T.d.n_pad, T.o_e};
+            ta.seg[4 + 2 * l] = TailSeg{T.inv, T.d.n_dofs, T.o_inv};
+        }
+        for (int sg = 3 + 2 * ta.count; sg < kTailSegs; sg++) ta.seg[sg] = TailSeg{ta.seg[0].src, 0, 0};
+    }
+    ta.out0 = ta.count > 1 ? nxt[t0] : cur[t0];  // the host-side swap below names it the current iterate
+    ta.ainv = g->ainv.get();
+    ta.theta_inv = g->theta_inv;
+    ta.cd1 = g->cd[1];
+    ta.cr1 = g->cr[1];
+    ta.halt = halt;
+    ta.halt_stride = halt_stride;
+    return ta;
+}
+
 // z = M(r): one V-cycle. Also writes per-block partials of z.r and z.ap (gmg_partials(op) of each).
 // fine_ready: the caller has already written the finest level's b, x = d = D^-1 b / theta (gmg_fine_view).
 void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, double *z64, double *part_zr,
@@ -1652,69 +1865,9 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
                                L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
         std::swap(cur[l], nxt[l]);
     };
-    // Levels [t0, top] run in the single-workgroup tail kernel (the finest level never does: it has the FP64
-    // entry and exit kernels), as many as fit its node limit, its per-array limit and its LDS.
-    auto tail_floats = [&](int l) { return g->lv[l].d.n_pad + 6 * g->lv[l].d.n_dofs; };
-    auto tail_fits = [&](int l) {
-        const Dims &d = g->lv[l].d;
-        return d.n_dofs <= kTailPer * kTailThreads && d.n_pad <= kTailPer * kTailThreads;
-    };
-    int t0 = top;
-    long long lds_floats = tail_floats(top);
-    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= tail_nodes() && tail_fits(t0 - 1) &&
-           4 * (lds_floats + tail_floats(t0 - 1)) <= kTailLdsMax) {
-        t0--;
-        lds_floats += tail_floats(t0);
-    }
-    if (top == 0) t0 = 0;
-    TailArgs ta{};
-    ta.count = top - t0 + 1;
-    if (ta.count > kMaxTail || !tail_fits(top) || 4 * lds_floats > kTailLdsMax)
-        throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: coarsest level does not fit the tail kernel");
-    int off = 0;
-    for (int l = t0; l <= top; l++) {
-        GmgLevel &L = g->lv[l];
-        TailLevel &T = ta.lv[l - t0];
-        T.d = L.d;
-        T.ks = L.ks;
-        T.ep = L.ep;
-        T.inv = L.inv.get();
-        const int nd = L.d.n_dofs;
-        T.o_e = off;
-        off += L.d.n_pad;
-        T.o_inv = off;
-        T.o_b = off + nd;
-        T.o_x = off + 2 * nd;
-        T.o_x2 = off + 3 * nd;
-        T.o_d = off + 4 * nd;
-        T.o_r = off + 5 * nd;
-        off += 6 * nd;
-    }
-    // The coarsest inverse (FP32) joins the LDS image when it fits.
-    const int nc = g->lv[top].d.n_dofs;
-    ta.ainv_lds = 4 * (static_cast<long long>(off) + static_cast<long long>(nc) * nc) <= kTailLdsMax ? off : -1;
-    if (ta.ainv_lds >= 0) off += nc * nc;
-    const size_t tail_lds = sizeof(float) * static_cast<size_t>(off);
-    ta.ainv_nn = ta.ainv_lds >= 0 ? nc * nc : 0;
-    {
-        const TailLevel &T0 = ta.lv[0];
-        ta.seg[0] = TailSeg{g->lv[t0].b.get(), T0.d.n_dofs, T0.o_b};
-        ta.seg[1] = TailSeg{cur[t0], T0.d.n_dofs, T0.o_x};
-        ta.seg[2] = TailSeg{g->lv[t0].dv.get(), T0.d.n_dofs, T0.o_d};
-        for (int l = 0; l < ta.count; l++) {
-            const TailLevel &T = ta.lv[l];
-            ta.seg[3 + 2 * l] = TailSeg{T.ep, T.d.n_pad, T.o_e};
-            ta.seg[4 + 2 * l] = TailSeg{T.inv, T.d.n_dofs, T.o_inv};
-        }
-        for (int sg = 3 + 2 * ta.count; sg < kTailSegs; sg++) ta.seg[sg] = TailSeg{ta.seg[0].src, 0, 0};
-    }
-    ta.out0 = ta.count > 1 ? nxt[t0] : cur[t0];  // the host-side swap below names it the current iterate
-    ta.ainv = g->ainv.get();
-    ta.theta_inv = g->theta_inv;
-    ta.cd1 = g->cd[1];
-    ta.cr1 = g->cr[1];
-    ta.halt = halt;
-    ta.halt_stride = halt_stride;
+    int t0 = 0;
+    size_t tail_lds = 0;
+    TailArgs ta = make_tail_args(g, cur, nxt, halt, halt_stride, t0, tail_lds);
 
     GmgLevel &F = g->lv[0];
     if (!fine_ready)
@@ -1801,6 +1954,280 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     else
         hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
                            F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, df);
+    MYCEL_LAUNCH_CHECK();
+}
+
+
+// ---- packed V-cycles: the host side ----------------------------------------------------------------
+
+namespace {
+
+enum PkKind : int { kPkChebEc, kPkResidualEc, kPkFinalEc, kPkChebS, kPkResidualRestrict, kPkRestrictDof, kPkProlong,
+                    kPkTail };
+
+// One launch of one member: its place in the V-cycle (key: the members' sequences are merged by key), the packed
+// kernel, its tile depth (finest level), its own grid and its table entry.
+struct PkRec {
+    long long key;
+    int kind, kz, nb;
+    size_t lds;
+    std::vector<uint8_t> entry;
+};
+
+template <typename A>
+PkRec pk_rec(long long key, int kind, int kz, int nb, const A &a, size_t lds = 0) {
+    PkRec r{key, kind, kz, nb, lds, std::vector<uint8_t>(sizeof(A))};
+    std::memcpy(r.entry.data(), &a, sizeof(A));
+    return r;
+}
+
+// Keys: the down sweep by level, then the tail, then the up sweep from the coarse levels to the finest; within a
+// level the steps in their order. A member with fewer launched levels has a subsequence of a deeper one's keys.
+constexpr long long kKeyTail = 1LL << 20, kKeyUp = 2LL << 20;
+long long key_down(int l, int k) { return static_cast<long long>(l) * 64 + k; }
+long long key_up(int l, int k) { return kKeyUp + static_cast<long long>(kMaxLevels - l) * 64 + k; }
+
+// The launches of gmg_enqueue_vcycle (fine_ready, one right-hand side, the default kernel set) for one member,
+// with the same buffers in the same order.
+void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
+    Gmg *g = mb.op->gmg;
+    const int top = g->levels - 1;
+    float *cur[kMaxLevels], *nxt[kMaxLevels];
+    for (int l = 0; l <= top; l++) {
+        cur[l] = g->lv[l].x.get();
+        nxt[l] = g->lv[l].x2.get();
+    }
+    const GmgLevel &F = g->lv[0];
+    const int fkz = fine_ec_kz(mb.op);
+    const int f_tiles = ec_tiles(F.d, fkz), f_nb = cdiv(f_tiles, 4);
+    auto fine = [&](float cd, float cr, float *outv) {
+        PkEc a{};
+        a.g = F.d;
+        a.km = F.km;
+        a.ep = F.ep;
+        a.x = cur[0];
+        a.b = F.b.get();
+        a.inv = F.inv.get();
+        a.d = F.dv.get();
+        a.out = outv;
+        a.cd = cd;
+        a.cr = cr;
+        a.halt = mb.halt;
+        a.n_tiles = f_tiles;
+        a.nb = f_nb;
+        return a;
+    };
+    auto cheb = [&](int l, float cd, float cr, long long key) {
+        const GmgLevel &L = g->lv[l];
+        if (l == 0) {
+            out.push_back(pk_rec(key, kPkChebEc, fkz, f_nb, fine(cd, cr, nxt[0])));
+        } else {
+            PkS a{};
+            a.g = L.d;
+            a.ks = L.ks;
+            a.ep = L.ep;
+            a.x = cur[l];
+            a.b = L.b.get();
+            a.inv = L.inv.get();
+            a.d = L.dv.get();
+            a.x_out = nxt[l];
+            a.cd = cd;
+            a.cr = cr;
+            a.nb = L.blocks_s;
+            out.push_back(pk_rec(key, kPkChebS, 0, L.blocks_s, a));
+        }
+        std::swap(cur[l], nxt[l]);
+    };
+    int t0 = 0;
+    size_t tail_lds = 0;
+    TailArgs ta = make_tail_args(g, cur, nxt, mb.halt, 0, t0, tail_lds);
+    for (int l = 0; l < t0; l++) {
+        const GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
+        for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k], key_down(l, k));
+        if (l > 0) {
+            PkRR a{};
+            a.gf = L.d;
+            a.gc = C.d;
+            a.ks = L.ks;
+            a.ep = L.ep;
+            a.xf = cur[l];
+            a.bf = L.b.get();
+            a.invf = L.inv.get();
+            a.invc = C.inv.get();
+            a.bc = C.b.get();
+            a.xcv = cur[l + 1];
+            a.dc = C.dv.get();
+            a.theta_inv = g->theta_inv;
+            a.nb = cdiv(C.d.n_nodes, 8);
+            out.push_back(pk_rec(key_down(l, 40), kPkResidualRestrict, 0, a.nb, a));
+        } else {
+            out.push_back(pk_rec(key_down(0, 40), kPkResidualEc, fkz, f_nb, fine(0.0f, 0.0f, L.r.get())));
+            PkRD a{};
+            a.gf = L.d;
+            a.gc = C.d;
+            a.rf = L.r.get();
+            a.invc = C.inv.get();
+            a.bc = C.b.get();
+            a.xcv = cur[1];
+            a.dc = C.dv.get();
+            a.theta_inv = g->theta_inv;
+            a.nb = cdiv(C.d.n_dofs, 256);
+            out.push_back(pk_rec(key_down(0, 41), kPkRestrictDof, 0, a.nb, a));
+        }
+    }
+    ta.pro_x = cur[t0 - 1];  // t0 >= 1: every packable hierarchy has at least two levels
+    ta.pro_inv = g->lv[t0 - 1].inv.get();
+    ta.pro_d = g->lv[t0 - 1].d;
+    out.push_back(pk_rec(kKeyTail, kPkTail, 0, 1, ta, tail_lds));
+    if (ta.count > 1) std::swap(cur[t0], nxt[t0]);
+    for (int l = t0 - 1; l >= 0; l--) {
+        const GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
+        if (l != t0 - 1) {
+            PkPro a{};
+            a.gf = L.d;
+            a.gc = C.d;
+            a.xf = cur[l];
+            a.invf = L.inv.get();
+            a.xcv = cur[l + 1];
+            a.nb = L.blocks;
+            out.push_back(pk_rec(key_up(l, 0), kPkProlong, 0, L.blocks, a));
+        }
+        cheb(l, 0.0f, g->theta_inv, key_up(l, 1));
+        for (int k = 1; k < (l == 0 ? g->fine_degree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k], key_up(l, 1 + k));
+    }
+    const int fd = g->fine_degree;
+    PkEc a = fine(g->cd[fd - 1], g->cr[fd - 1], nullptr);
+    a.z64 = mb.z64;
+    a.r64 = mb.r64;
+    a.ap64 = mb.ap64;
+    a.part_zr = mb.part_zr;
+    a.part_zap = mb.part_zap;
+    out.push_back(pk_rec(key_up(0, 63), kPkFinalEc, fkz, f_nb, a));
+}
+
+}  // namespace
+
+struct GmgPack {
+    struct Launch {
+        int kind, kz, members, grid_x;
+        size_t off, lds;
+    };
+    std::vector<Launch> launches;
+    DevBuf<uint8_t> dev;       // the tables
+    uint8_t *host = nullptr;   // pinned staging of the tables
+    size_t host_cap = 0;
+    ~GmgPack() {
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+bool gmg_packable(const mycel_op *op) {
+    const Gmg *g = op->gmg;
+    return g && g->levels >= 2 && g->defl.m == 0 && op->island_ratio == 0.0 && fine_ec_kz(op) != 0 && gmg_split() &&
+           fuse_rr() && g->rhs_cap >= 1;
+}
+
+GmgPack *gmg_pack_create() {
+    MYCEL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pk_tail),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsMax));
+    return new GmgPack();
+}
+
+void gmg_pack_free(GmgPack *p) { delete p; }
+
+int gmg_pack_launches(const GmgPack *p) { return p ? static_cast<int>(p->launches.size()) : 0; }
+
+void gmg_pack_build(GmgPack *p, const GmgPackMember *m, int count, hipStream_t s) {
+    MYCEL_HIP(hipStreamSynchronize(s));  // nothing in flight still reads the old tables
+    p->launches.clear();
+    std::vector<std::vector<PkRec>> recs(count);
+    std::vector<long long> keys;
+    for (int i = 0; i < count; i++) {
+        if (!gmg_packable(m[i].op)) throw Error(MYCEL_E_INVALID_OPERATION, "multigrid pack: member not packable");
+        pk_member_records(m[i], recs[i]);
+        for (const PkRec &r : recs[i]) keys.push_back(r.key);
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<uint8_t> bytes;
+    for (long long key : keys) {
+        // One launch per (kernel, tile depth) among the members that have this step, members in order.
+        std::vector<std::pair<int, int>> groups;
+        for (int i = 0; i < count; i++)
+            for (const PkRec &r : recs[i])
+                if (r.key == key && std::find(groups.begin(), groups.end(), std::make_pair(r.kind, r.kz)) == groups.end())
+                    groups.emplace_back(r.kind, r.kz);
+        for (const auto &gk : groups) {
+            GmgPack::Launch L{gk.first, gk.second, 0, 0, (bytes.size() + 63) / 64 * 64, 0};
+            bytes.resize(L.off);
+            for (int i = 0; i < count; i++)
+                for (const PkRec &r : recs[i])
+                    if (r.key == key && r.kind == gk.first && r.kz == gk.second) {
+                        bytes.insert(bytes.end(), r.entry.begin(), r.entry.end());
+                        L.members++;
+                        L.grid_x = std::max(L.grid_x, r.nb);
+                        L.lds = std::max(L.lds, r.lds);
+                    }
+            p->launches.push_back(L);
+        }
+    }
+    if (bytes.size() > p->host_cap) {
+        if (p->host) (void)hipHostFree(p->host);
+        p->host = nullptr;
+        p->host_cap = 0;
+        MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&p->host), bytes.size(), hipHostMallocDefault));
+        p->host_cap = bytes.size();
+    }
+    if (bytes.size() > p->dev.size()) p->dev.resize(bytes.size());
+    std::memcpy(p->host, bytes.data(), bytes.size());
+    MYCEL_HIP(hipMemcpyAsync(p->dev.get(), p->host, bytes.size(), hipMemcpyHostToDevice, s));
+}
+
+void gmg_pack_enqueue(const GmgPack *p, hipStream_t s) {
+    const dim3 b256(256);
+    for (const GmgPack::Launch &L : p->launches) {
+        const void *t = p->dev.get() + L.off;
+        const dim3 grid(L.grid_x, L.members);
+        const PkEc *ec = static_cast<const PkEc *>(t);
+        switch (L.kind) {
+            case kPkChebEc:
+                if (L.kz == 2) hipLaunchKernelGGL(k_pk_cheb_ec<2>, grid, b256, 0, s, ec);
+                else if (L.kz == 3) hipLaunchKernelGGL(k_pk_cheb_ec<3>, grid, b256, 0, s, ec);
+                else if (L.kz == 4) hipLaunchKernelGGL(k_pk_cheb_ec<4>, grid, b256, 0, s, ec);
+                else hipLaunchKernelGGL(k_pk_cheb_ec<8>, grid, b256, 0, s, ec);
+                break;
+            case kPkResidualEc:
+                if (L.kz == 2) hipLaunchKernelGGL(k_pk_residual_ec<2>, grid, b256, 0, s, ec);
+                else if (L.kz == 3) hipLaunchKernelGGL(k_pk_residual_ec<3>, grid, b256, 0, s, ec);
+                else if (L.kz == 4) hipLaunchKernelGGL(k_pk_residual_ec<4>, grid, b256, 0, s, ec);
+                else hipLaunchKernelGGL(k_pk_residual_ec<8>, grid, b256, 0, s, ec);
+                break;
+            case kPkFinalEc:
+                if (L.kz == 2) hipLaunchKernelGGL(k_pk_cheb_final_ec<2>, grid, b256, 0, s, ec);
+                else if (L.kz == 3) hipLaunchKernelGGL(k_pk_cheb_final_ec<3>, grid, b256, 0, s, ec);
+                else if (L.kz == 4) hipLaunchKernelGGL(k_pk_cheb_final_ec<4>, grid, b256, 0, s, ec);
+                else hipLaunchKernelGGL(k_pk_cheb_final_ec<8>, grid, b256, 0, s, ec);
+                break;
+            case kPkChebS:
+                hipLaunchKernelGGL(k_pk_cheb_s, grid, b256, 0, s, static_cast<const PkS *>(t));
+                break;
+            case kPkResidualRestrict:
+                hipLaunchKernelGGL(k_pk_residual_restrict, grid, b256, 0, s, static_cast<const PkRR *>(t));
+                break;
+            case kPkRestrictDof:
+                hipLaunchKernelGGL(k_pk_restrict_dof, grid, b256, 0, s, static_cast<const PkRD *>(t));
+                break;
+            case kPkProlong:
+                hipLaunchKernelGGL(k_pk_prolong, grid, b256, 0, s, static_cast<const PkPro *>(t));
+                break;
+            case kPkTail:
+                hipLaunchKernelGGL(k_pk_tail, dim3(L.members), dim3(kTailThreads), L.lds, s,
+                                   static_cast<const TailArgs *>(t));
+                break;
+            default:
+                throw Error(MYCEL_E_DEVICE, "multigrid pack: unknown launch kind");
+        }
+    }
     MYCEL_LAUNCH_CHECK();
 }
 

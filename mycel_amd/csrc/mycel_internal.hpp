@@ -117,6 +117,7 @@ void constitutive(double e, double nu, double *d36);
 struct PcgWork;
 struct Gmg;
 struct OrdWork;
+struct SolveBroker;
 
 // Several right-hand sides in one launch: grid dimension y is the RHS index; vectors of RHS k start at
 // k * vstride, block partials at k * pstride, and the solver-state word that stops a launch at k * sstride
@@ -176,6 +177,7 @@ struct mycel_op {
     mycel::OrdWork *ord = nullptr;     // workspace of the reference-order solver (built on first use)
     uint64_t epoch = 0;                // bumped whenever the element moduli change
     double island_ratio = 0.0;         // multigrid island deflation: solid threshold E >= ratio E0 (0: off)
+    mycel::SolveBroker *broker = nullptr;  // job server: multigrid solves run in its packed launches (not owned)
 };
 
 namespace mycel {
@@ -216,6 +218,15 @@ void pcg_free(PcgWork *w);
 // own state and stopping test; MYCEL_PRECISION_MIXED_GMG only. res: R results.
 int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, double tol, int max_iter, int precision,
                         mycel_pcg_result *res);
+// Packed multigrid solves: a broker thread advances every multigrid solve handed to it (op->broker set, no
+// cancel flag or progress callback) in launches shared by all of them, joining and leaving between batches.
+SolveBroker *broker_create(int device);
+void broker_destroy(SolveBroker *b);
+mycel_pack_stats broker_stats(SolveBroker *b);
+// count (<= 16) independent multigrid systems on distinct operators, solved in packed launches by the calling
+// thread; each result is bit-identical to the system's multigrid solve alone.
+void pcg_solve_packed(mycel_op *const *ops, const double *const *d_b, double *const *d_x, int count, double tol,
+                      int max_iter, mycel_pcg_result *res);
 
 // ---- ordered.hip: the reference's evaluation order (MYCEL_PRECISION_FP64_ORDERED) ---------------
 // K*u with the reference's per-DOF colour order; d_halt: a PCG halt flag that turns the launch into a no-op.
@@ -260,6 +271,25 @@ struct GmgFineView {
 GmgFineView gmg_fine_view(mycel_op *op);
 int gmg_partials(const mycel_op *op);
 void gmg_free(Gmg *g);
+// Packed V-cycles: several operators' V-cycles (different grids) in one launch per phase, grid dimension y the
+// member, each member's arguments in a device table; each member's results are bit-identical to its V-cycle alone.
+struct GmgPack;
+struct GmgPackMember {
+    mycel_op *op;
+    const double *r64, *ap64;        // the outer iteration's residual and A p (FP64)
+    double *z64, *part_zr, *part_zap;  // z and the z.r, z.Ap partials (gmg_partials(op) each)
+    const int *halt;                 // the member's halt flag (its V-cycle is skipped once set)
+};
+// Whether op's V-cycle can join a pack: a prepared hierarchy of at least two levels, no island deflation and the
+// default kernel set (element-centric finest level, slab-split coarse gathers, fused coarse residual-restriction).
+bool gmg_packable(const mycel_op *op);
+GmgPack *gmg_pack_create();
+void gmg_pack_free(GmgPack *p);
+// (Re)builds the launch sequence and the device tables for these members; synchronises s, then uploads on s.
+void gmg_pack_build(GmgPack *p, const GmgPackMember *m, int count, hipStream_t s);
+// One V-cycle of every member (each member's finest-level b, x, d written by the caller, as fine_ready).
+void gmg_pack_enqueue(const GmgPack *p, hipStream_t s);
+int gmg_pack_launches(const GmgPack *p);
 
 // ---- capi.hip (internal forms of the entry points, used by the job server) -------------------
 mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,

@@ -35,6 +35,13 @@
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "device_common.hpp"
 #include "ku_element.hpp"
@@ -294,27 +301,14 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
 // PcgSolver.cs:245-270; alpha is kept in the state for the flexible beta); x += alpha p; r -= alpha Ap;
 // per-block partials of r.r; then, on the updated residual, b0 = float(r) (0 on constrained DOFs) and
 // x0 = d0 = D^-1 b0 / theta, the finest Chebyshev smoother's first step.
-__global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, PcgState *__restrict__ st,
-                                                            const double *__restrict__ part_pap, int nb_mv,
-                                                            double *__restrict__ x, double *__restrict__ r,
-                                                            const double *__restrict__ p,
-                                                            const double *__restrict__ ap,
-                                                            double *__restrict__ part_rr, GmgFineView fv) {
+// Block bx of one right-hand side.
+__device__ __forceinline__ void fpcg_update_xr_body(int n, int slot, PcgState *__restrict__ st,
+                                                    const double *__restrict__ part_pap, int nb_mv,
+                                                    double *__restrict__ x, double *__restrict__ r,
+                                                    const double *__restrict__ p, const double *__restrict__ ap,
+                                                    double *__restrict__ part_rr, const GmgFineView &fv, int bx) {
     __shared__ double red[4];
-    {  // grid dimension y: the right-hand side (vectors at y * n, partials at y * blocks, one state each)
-        const long long o = static_cast<long long>(blockIdx.y) * n;
-        st += blockIdx.y;
-        part_pap += blockIdx.y * nb_mv;
-        x += o;
-        r += o;
-        p += o;
-        ap += o;
-        part_rr += blockIdx.y * gridDim.x;
-        fv.b += o;
-        fv.x += o;
-        fv.d += o;
-    }
-    const int base = blockIdx.x * kVecPerBlock + threadIdx.x;
+    const int base = bx * kVecPerBlock + threadIdx.x;
     double xv[4], pv[4], rv[4], av[4];
     float iv[4];
 #pragma unroll
@@ -334,12 +328,12 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, Pcg
     const double alpha = rz0 / pap;
     const bool step = !curv && (!isfinite(alpha) || alpha <= 0.0);
     if (curv || step) {
-        if (blockIdx.x == 0 && threadIdx.x == 0)
+        if (bx == 0 && threadIdx.x == 0)
             halt_with(st, curv ? MYCEL_PCG_CURVATURE_BREAKDOWN : MYCEL_PCG_STEP_BREAKDOWN, st->iter[slot],
                       st->prev_rel[slot], curv ? pap : alpha);
         return;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[slot] = alpha;
+    if (bx == 0 && threadIdx.x == 0) st->alpha[slot] = alpha;
     double rr = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -357,7 +351,22 @@ __global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, Pcg
         }
     }
     rr = block_sum256(rr, red);
-    if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+    if (threadIdx.x == 0) part_rr[bx] = rr;
+}
+
+__global__ __launch_bounds__(256) void k_fpcg_update_xr_gmg(int n, int slot, PcgState *__restrict__ st,
+                                                            const double *__restrict__ part_pap, int nb_mv,
+                                                            double *__restrict__ x, double *__restrict__ r,
+                                                            const double *__restrict__ p,
+                                                            const double *__restrict__ ap,
+                                                            double *__restrict__ part_rr, GmgFineView fv) {
+    // grid dimension y: the right-hand side (vectors at y * n, partials at y * blocks, one state each)
+    const long long o = static_cast<long long>(blockIdx.y) * n;
+    fv.b += o;
+    fv.x += o;
+    fv.d += o;
+    fpcg_update_xr_body(n, slot, st + blockIdx.y, part_pap + blockIdx.y * nb_mv, nb_mv, x + o, r + o, p + o, ap + o,
+                        part_rr + blockIdx.y * gridDim.x, fv, blockIdx.x);
 }
 
 // Flexible PCG, step 4 decisions from the per-wave sums of the r.r, z.r and z.Ap partials: relres, convergence
@@ -517,6 +526,66 @@ __global__ __launch_bounds__(256) void k_pcg_step4_ap(Dims g, const KeMono<T> km
     if (tile < n_tiles) ec_tile_march<T, KZ>(g, km, ep, src, tile, fin);
     const double sum = block_sum256(fin.dot, red);
     if (threadIdx.x == 0 && fin.go) part_pap[blockIdx.x] = sum;  // wave 0 of every block has a tile
+}
+
+// ---- packed multigrid iterations: the outer kernels of several solves (different grids) in one launch ----
+// Grid dimension y is the member, its arguments its entry of a device table, blocks beyond the member's own grid
+// return at entry; each member's blocks do exactly what its own launch's blocks do. slot: the iteration parity
+// (the same for every member: each joins a pack at the start of a batch, in slot 0).
+struct PkXr {
+    int n, nb_mv, nb;
+    PcgState *st;
+    const double *part_pap;
+    double *x, *r;
+    const double *p[2];  // p of the even / odd iterations
+    const double *ap;
+    double *part_rr;
+    GmgFineView fv;
+};
+struct PkStep4 {
+    Dims g;
+    KeMono<double> km;
+    const double *ep;
+    const uint8_t *flags;
+    double *p[2];        // p of the even / odd iterations; step 4 writes the other one
+    const double *z;
+    PcgState *st;
+    const double *part_rr, *part_zr, *part_zap;
+    int nb_vec, nb_z;
+    double *ap, *part_pap;
+    int n_tiles, nb;
+};
+
+__global__ __launch_bounds__(256) void k_pk_fpcg_update_xr(const PkXr *__restrict__ tab, int slot) {
+    const PkXr a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;  // block-uniform, ahead of the block sum's barriers
+    fpcg_update_xr_body(a.n, slot, a.st, a.part_pap, a.nb_mv, a.x, a.r, slot ? a.p[1] : a.p[0], a.ap, a.part_rr,
+                        a.fv, blockIdx.x);
+}
+
+template <int KZ>
+__global__ __launch_bounds__(256, 2) void k_pk_step4_ap(const PkStep4 *__restrict__ tab, int slot) {
+    __shared__ double red[4];
+    const PkStep4 a = tab[blockIdx.y];
+    if (static_cast<int>(blockIdx.x) >= a.nb) return;
+    double *p_cur = slot ? a.p[1] : a.p[0], *p_next = slot ? a.p[0] : a.p[1];
+    PcgNextP<double, true> src{p_cur, a.z, a.st, a.part_rr, a.part_zr, a.part_zap, a.nb_vec, a.nb_z, slot, 0.0,
+                               0.0, Step4In{}, 0};
+    src.in = step4_in(a.st, slot);
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, a.nb) * 4 + (threadIdx.x >> 6));
+    PcgNextFin<double> fin{a.flags, a.ap, p_next};
+    if (tile < a.n_tiles) ec_tile_march<double, KZ>(a.g, a.km, a.ep, src, tile, fin);
+    const double sum = block_sum256(fin.dot, red);
+    if (threadIdx.x == 0 && fin.go) a.part_pap[blockIdx.x] = sum;
+}
+
+// The members' states after a batch, gathered for one copy to the host (one block per member).
+__global__ __launch_bounds__(64) void k_pk_gather_states(PcgState *const *__restrict__ src,
+                                                         PcgState *__restrict__ dst) {
+    constexpr int kWords = static_cast<int>(sizeof(PcgState) / sizeof(uint32_t));
+    const uint32_t *from = reinterpret_cast<const uint32_t *>(src[blockIdx.x]);
+    uint32_t *to = reinterpret_cast<uint32_t *>(dst + blockIdx.x);
+    for (int i = threadIdx.x; i < kWords; i += 64) to[i] = from[i];
 }
 
 // p = z (the first search direction of the preconditioned iteration).
@@ -937,26 +1006,19 @@ void pcg_free(PcgWork *w) {
     delete w;
 }
 
-int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
-                         const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
-                         mycel_pcg_result *res) {
-    const auto t0 = std::chrono::steady_clock::now();
-    auto elapsed = [&] {
-        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    };
-    *res = mycel_pcg_result{};
-    if (precision == MYCEL_PRECISION_FP64_ORDERED)
-        return pcg_solve_ordered(op, d_b, d_x, tol, max_iter, cancel, cb, user, res);
-    if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32 &&
-        precision != MYCEL_PRECISION_MIXED_BF16 && precision != MYCEL_PRECISION_MIXED_GMG)
-        throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
-    if (precision == MYCEL_PRECISION_MIXED_GMG && !gmg_supported(op)) {
-        // A grid too thin for the hierarchy (an axis reaches one element above the dense coarsest size): the
-        // reference's FP64 Jacobi iteration, reported as fallback iterations.
-        const int32_t st = pcg_solve_device(op, d_b, d_x, tol, max_iter, MYCEL_PRECISION_FP64, cancel, cb, user, res);
-        res->fallback_iterations = res->iterations;
-        return st;
-    }
+namespace {
+
+using SolveClock = std::chrono::steady_clock;
+
+double ms_since(SolveClock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(SolveClock::now() - t0).count();
+}
+
+// A solve's set-up, up to its first batch (PcgSolver.cs:80-233): ||Pb||, the zero right-hand side, the warm start,
+// the initial residual and preconditioned residual, the initial convergence test, the state of slot 0 and (fused
+// step 4) the first A p. Returns false when the solve ended here (res filled).
+bool solve_setup(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
+                 mycel_pcg_result *res, SolveClock::time_point t0) {
     const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32 || precision == MYCEL_PRECISION_MIXED_BF16;
     const bool gmg = precision == MYCEL_PRECISION_MIXED_GMG;
     if (op->active_fixed < 6)  // PcgSolver.cs:95-100
@@ -993,8 +1055,8 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         MYCEL_HIP(hipStreamSynchronize(s));
         res->converged = 1;
         res->message = MYCEL_PCG_RHS_ZERO;
-        res->elapsed_ms = elapsed();
-        return MYCEL_OK;
+        res->elapsed_ms = ms_since(t0);
+        return false;
     }
     const bool warm = w->h_state->nonzero_x != 0;
     if (warm) launch_apply_k(op, d_x, w->ap.get(), kApplyAProjected, s);
@@ -1024,14 +1086,14 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
         res->final_residual = rel0;
         res->converged = 1;
         res->message = MYCEL_PCG_INITIAL_CONVERGED;
-        res->elapsed_ms = elapsed();
-        return MYCEL_OK;
+        res->elapsed_ms = ms_since(t0);
+        return false;
     }
     if (max_iter <= 0) {  // the reference's loop body never runs (PcgSolver.cs:235,359-368)
         res->final_residual = rel0;
         res->message = MYCEL_PCG_MAX_ITERATIONS;
-        res->elapsed_ms = elapsed();
-        return MYCEL_OK;
+        res->elapsed_ms = ms_since(t0);
+        return false;
     }
     hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1), 0, s, st, tol, bnorm, rel0, max_iter, mixed ? 1 : 0,
                        kReliableDelta);
@@ -1043,7 +1105,20 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
     }
     MYCEL_LAUNCH_CHECK();
+    return true;
+}
 
+// The iteration loop of one solve on its own: batches of iterations captured once as a hipGraph and replayed,
+// the state read between batches; leaves the final state in w->h_state.
+void solve_loop(mycel_op *op, const double *d_b, double *d_x, int max_iter, int precision,
+                const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user) {
+    const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32 || precision == MYCEL_PRECISION_MIXED_BF16;
+    const bool gmg = precision == MYCEL_PRECISION_MIXED_GMG;
+    PcgWork *w = op->pcg;
+    hipStream_t s = op->stream;
+    PcgState *st = w->state.get();
+    const int n = w->n;
+    const dim3 gv(w->nb_vec), b256(256);
     hipGraphExec_t graph;
     if (gmg) {
         if (w->graph_gmg && (w->graph_gmg_x != d_x || w->graph_gmg_h != op->gmg ||
@@ -1125,7 +1200,21 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             if (w->h_state->done) break;
         }
     }
+}
+
+// The result from the final state in w->h_state, and the FP64 finish after a breakdown of the FP32
+// preconditioner or iteration.
+void solve_finish(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
+                  const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user, mycel_pcg_result *res,
+                  SolveClock::time_point t0) {
+    const bool mixed = precision == MYCEL_PRECISION_MIXED_FP32 || precision == MYCEL_PRECISION_MIXED_BF16;
+    const bool gmg = precision == MYCEL_PRECISION_MIXED_GMG;
+    PcgWork *w = op->pcg;
+    hipStream_t s = op->stream;
+    const int n = w->n;
+    const dim3 gv(w->nb_vec), b256(256);
     const PcgState h = *w->h_state;
+    int &last_iters = gmg ? w->last_iters_gmg : w->last_iters_64;
     if (!mixed) last_iters = h.message == MYCEL_PCG_CONVERGED ? h.iterations : 0;
     res->iterations = h.iterations;
     res->final_residual = h.final_rel;
@@ -1170,8 +1259,392 @@ int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double to
             MYCEL_HIP(hipStreamSynchronize(s));
         }
     }
-    res->elapsed_ms = elapsed();
+    res->elapsed_ms = ms_since(t0);
+}
+
+// Whether a multigrid solve on op goes to its operator's pack broker.
+bool solve_packed(const mycel_op *op, int precision, const volatile int32_t *cancel, mycel_pcg_progress_cb cb) {
+    return op->broker && precision == MYCEL_PRECISION_MIXED_GMG && !cancel && !cb && fuse_step4() &&
+           gmg_packable(op);
+}
+
+void broker_solve(SolveBroker *b, mycel_op *op, double *d_x);
+
+}  // namespace
+
+int32_t pcg_solve_device(mycel_op *op, const double *d_b, double *d_x, double tol, int max_iter, int precision,
+                         const volatile int32_t *cancel, mycel_pcg_progress_cb cb, void *user,
+                         mycel_pcg_result *res) {
+    const auto t0 = SolveClock::now();
+    *res = mycel_pcg_result{};
+    if (precision == MYCEL_PRECISION_FP64_ORDERED)
+        return pcg_solve_ordered(op, d_b, d_x, tol, max_iter, cancel, cb, user, res);
+    if (precision != MYCEL_PRECISION_FP64 && precision != MYCEL_PRECISION_MIXED_FP32 &&
+        precision != MYCEL_PRECISION_MIXED_BF16 && precision != MYCEL_PRECISION_MIXED_GMG)
+        throw Error(MYCEL_E_ARGUMENT, "unknown precision mode " + std::to_string(precision));
+    if (precision == MYCEL_PRECISION_MIXED_GMG && !gmg_supported(op)) {
+        // A grid too thin for the hierarchy (an axis reaches one element above the dense coarsest size): the
+        // reference's FP64 Jacobi iteration, reported as fallback iterations.
+        const int32_t st = pcg_solve_device(op, d_b, d_x, tol, max_iter, MYCEL_PRECISION_FP64, cancel, cb, user, res);
+        res->fallback_iterations = res->iterations;
+        return st;
+    }
+    if (!solve_setup(op, d_b, d_x, tol, max_iter, precision, res, t0)) return MYCEL_OK;
+    if (solve_packed(op, precision, cancel, cb))
+        broker_solve(op->broker, op, d_x);  // the same iterations, in launches shared with other resident solves
+    else
+        solve_loop(op, d_b, d_x, max_iter, precision, cancel, cb, user);
+    solve_finish(op, d_b, d_x, tol, max_iter, precision, cancel, cb, user, res, t0);
     return MYCEL_OK;
+}
+
+// ---- packed multigrid solves -------------------------------------------------------------------------
+// The iterations of several resident multigrid solves (different grids: jobs of the server, or the systems of one
+// mycel_pcg_solve_packed call) run together: per iteration one launch of each outer kernel and one per V-cycle
+// phase for all members (k_pk_*, gmg_pack_enqueue), 4 iterations per batch, the members' states read once per
+// batch. Members join and leave between batches (continuous batching); a member's arithmetic is its solve's alone.
+
+constexpr int kPackMax = 16;  // members of one pack
+
+struct PackTables {
+    GmgPack *gp = nullptr;
+    DevBuf<uint8_t> dev;      // outer-kernel tables and the members' state pointers
+    uint8_t *host = nullptr;  // pinned staging of the same
+    size_t host_cap = 0;
+    DevBuf<PcgState> d_states;
+    PcgState *h_states = nullptr;  // pinned, kPackMax
+    int m = 0, xr_grid = 0;
+    size_t off_xr = 0, off_src = 0;
+    struct Step4Group {
+        int kz, count, grid;
+        size_t off;
+    };
+    std::vector<Step4Group> s4;
+    int launches_per_batch = 0;
+
+    PackTables() {
+        gp = gmg_pack_create();
+        d_states.resize(kPackMax);
+        MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_states), sizeof(PcgState) * kPackMax,
+                                hipHostMallocDefault));
+    }
+    ~PackTables() {
+        gmg_pack_free(gp);
+        if (host) (void)hipHostFree(host);
+        if (h_states) (void)hipHostFree(h_states);
+    }
+    PackTables(const PackTables &) = delete;
+    PackTables &operator=(const PackTables &) = delete;
+
+    // Members i: operator ops[i] (set up by solve_setup, its state in slot 0) with solution xs[i].
+    void build(mycel_op *const *ops, double *const *xs, int count, hipStream_t s) {
+        if (count < 1 || count > kPackMax) throw Error(MYCEL_E_OUT_OF_RANGE, "pack: 1 to 16 members");
+        MYCEL_HIP(hipStreamSynchronize(s));
+        m = count;
+        std::vector<GmgPackMember> gm(count);
+        std::vector<PkXr> xr(count);
+        std::vector<std::pair<int, PkStep4>> st4(count);
+        std::vector<PcgState *> src(count);
+        xr_grid = 0;
+        for (int i = 0; i < count; i++) {
+            mycel_op *op = ops[i];
+            PcgWork *w = op->pcg;
+            PcgState *st = w->state.get();
+            gm[i] = GmgPackMember{op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), &st->halt};
+            xr[i] = PkXr{w->n, w->nb_mv, w->nb_vec, st, w->part_pap.get(), xs[i], w->r.get(), {w->p.get(), w->p2.get()},
+                         w->ap.get(), w->part2.get(), gmg_fine_view(op)};
+            xr_grid = std::max(xr_grid, w->nb_vec);
+            const int kz = ku_element_layers(op), nt = ec_tiles(op->d, kz);
+            if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "pack: partial count mismatch");
+            st4[i] = {kz, PkStep4{op->d, op->km64, op->d_ep.get(), op->d_flags.get(), {w->p.get(), w->p2.get()},
+                                  w->z.get(), st, w->part2.get(), w->part_zr.get(), w->part_zap.get(), w->nb_vec,
+                                  w->nb_z, w->ap.get(), w->part_pap.get(), nt, cdiv(nt, 4)}};
+            src[i] = st;
+        }
+        gmg_pack_build(gp, gm.data(), count, s);
+        std::vector<uint8_t> bytes;
+        auto put = [&](const void *p, size_t n) {
+            const size_t off = (bytes.size() + 63) / 64 * 64;
+            bytes.resize(off);
+            bytes.insert(bytes.end(), static_cast<const uint8_t *>(p), static_cast<const uint8_t *>(p) + n);
+            return off;
+        };
+        off_xr = put(xr.data(), sizeof(PkXr) * count);
+        off_src = put(src.data(), sizeof(PcgState *) * count);
+        s4.clear();
+        for (int i = 0; i < count; i++) {
+            const int kz = st4[i].first;
+            if (std::any_of(s4.begin(), s4.end(), [&](const Step4Group &g) { return g.kz == kz; })) continue;
+            std::vector<PkStep4> grp;
+            int grid = 0;
+            for (int j = i; j < count; j++)
+                if (st4[j].first == kz) {
+                    grp.push_back(st4[j].second);
+                    grid = std::max(grid, st4[j].second.nb);
+                }
+            s4.push_back(Step4Group{kz, static_cast<int>(grp.size()), grid, put(grp.data(), sizeof(PkStep4) * grp.size())});
+        }
+        if (bytes.size() > host_cap) {
+            if (host) (void)hipHostFree(host);
+            host = nullptr;
+            host_cap = 0;
+            MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), bytes.size(), hipHostMallocDefault));
+            host_cap = bytes.size();
+        }
+        if (bytes.size() > dev.size()) dev.resize(bytes.size());
+        std::memcpy(host, bytes.data(), bytes.size());
+        MYCEL_HIP(hipMemcpyAsync(dev.get(), host, bytes.size(), hipMemcpyHostToDevice, s));
+        launches_per_batch = kGmgGraphIters * (1 + gmg_pack_launches(gp) + static_cast<int>(s4.size()));
+    }
+
+    // One batch: kGmgGraphIters iterations of every member.
+    void enqueue_batch(hipStream_t s) const {
+        const dim3 b256(256);
+        const PkXr *txr = reinterpret_cast<const PkXr *>(dev.get() + off_xr);
+        for (int t = 0; t < kGmgGraphIters; t++) {
+            const int slot = t & 1;
+            hipLaunchKernelGGL(k_pk_fpcg_update_xr, dim3(xr_grid, m), b256, 0, s, txr, slot);
+            gmg_pack_enqueue(gp, s);
+            for (const Step4Group &g : s4) {
+                const PkStep4 *ts = reinterpret_cast<const PkStep4 *>(dev.get() + g.off);
+                const dim3 grid(g.grid, g.count);
+                if (g.kz == 2) hipLaunchKernelGGL(k_pk_step4_ap<2>, grid, b256, 0, s, ts, slot);
+                else if (g.kz == 3) hipLaunchKernelGGL(k_pk_step4_ap<3>, grid, b256, 0, s, ts, slot);
+                else if (g.kz == 4) hipLaunchKernelGGL(k_pk_step4_ap<4>, grid, b256, 0, s, ts, slot);
+                else if (g.kz == 8) hipLaunchKernelGGL(k_pk_step4_ap<8>, grid, b256, 0, s, ts, slot);
+                else throw Error(MYCEL_E_INVALID_OPERATION, "pack: unsupported tile depth");
+            }
+        }
+        MYCEL_LAUNCH_CHECK();
+    }
+
+    // The members' states after the batches enqueued so far, into h_states (synchronises s).
+    void read_states(hipStream_t s) const {
+        hipLaunchKernelGGL(k_pk_gather_states, dim3(m), dim3(64), 0, s,
+                           reinterpret_cast<PcgState *const *>(dev.get() + off_src), d_states.get());
+        MYCEL_LAUNCH_CHECK();
+        MYCEL_HIP(hipMemcpyAsync(h_states, d_states.get(), sizeof(PcgState) * m, hipMemcpyDeviceToHost, s));
+        MYCEL_HIP(hipStreamSynchronize(s));
+    }
+};
+
+struct PackReq {
+    mycel_op *op;
+    double *d_x;
+    bool done = false;
+    int32_t status = MYCEL_OK;
+    std::string error;
+    PcgState final_state{};
+};
+
+struct SolveBroker {
+    int device = 0;
+    hipStream_t s = nullptr;
+    std::mutex mu;
+    std::condition_variable cv_in, cv_done;
+    std::deque<PackReq *> incoming;
+    bool stop = false;
+    std::thread th;
+    mycel_pack_stats stats{};  // under mu
+};
+
+namespace {
+
+void broker_loop(SolveBroker *b) {
+    std::vector<PackReq *> members;
+    std::unique_ptr<PackTables> T;
+    hipGraphExec_t graph = nullptr;
+    bool dirty = true;
+    int same = 0;  // batches run since the last change of members
+    auto drop_graph = [&] {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    };
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(b->mu);
+            b->cv_in.wait(lk, [&] { return b->stop || !b->incoming.empty() || !members.empty(); });
+            if (members.empty() && b->incoming.empty()) break;  // stopping, nothing in flight
+            while (!b->incoming.empty() && static_cast<int>(members.size()) < kPackMax) {
+                members.push_back(b->incoming.front());
+                b->incoming.pop_front();
+                dirty = true;
+            }
+        }
+        std::vector<PackReq *> finished;
+        int32_t status = MYCEL_OK;
+        std::string error;
+        bool graph_used = false;
+        int launches = 0;
+        try {
+            if (!T) {
+                MYCEL_HIP(hipSetDevice(b->device));
+                T.reset(new PackTables());
+            }
+            if (dirty) {
+                std::vector<mycel_op *> ops;
+                std::vector<double *> xs;
+                for (PackReq *r : members) {
+                    ops.push_back(r->op);
+                    xs.push_back(r->d_x);
+                }
+                drop_graph();
+                T->build(ops.data(), xs.data(), static_cast<int>(members.size()), b->s);
+                dirty = false;
+                same = 0;
+            }
+            // A membership that has run one batch unchanged is captured as a hipGraph and replayed from then on.
+            if (same >= 1 && !graph) graph = capture(b->s, [&] { T->enqueue_batch(b->s); });
+            if (graph) {
+                MYCEL_HIP(hipGraphLaunch(graph, b->s));
+                graph_used = true;
+            } else {
+                T->enqueue_batch(b->s);
+            }
+            launches = T->launches_per_batch;
+            T->read_states(b->s);
+            same++;
+            for (int i = 0; i < static_cast<int>(members.size()); i++)
+                if (T->h_states[i].done) {
+                    members[i]->final_state = T->h_states[i];
+                    finished.push_back(members[i]);
+                }
+        } catch (const Error &e) {
+            status = e.status;
+            error = e.what();
+            finished = members;
+        } catch (const std::exception &e) {
+            status = MYCEL_E_DEVICE;
+            error = e.what();
+            finished = members;
+        }
+        {
+            std::lock_guard<std::mutex> lk(b->mu);
+            if (status == MYCEL_OK) {
+                b->stats.batches++;
+                b->stats.member_batches += static_cast<int64_t>(members.size());
+                b->stats.launches += launches;
+                if (graph_used) b->stats.graph_batches++;
+            }
+            for (PackReq *r : finished) {
+                r->status = status;
+                r->error = error;
+                r->done = true;
+                members.erase(std::find(members.begin(), members.end(), r));
+                b->stats.solves++;
+                dirty = true;
+            }
+            if (dirty) b->stats.rebuilds++;
+        }
+        if (!finished.empty()) b->cv_done.notify_all();
+    }
+    drop_graph();
+    T.reset();
+}
+
+// Hands a solve (set up by solve_setup) to the broker and waits for it; leaves the final state in w->h_state.
+void broker_solve(SolveBroker *b, mycel_op *op, double *d_x) {
+    MYCEL_HIP(hipStreamSynchronize(op->stream));  // the set-up is complete before the broker's stream reads it
+    PackReq r{op, d_x};
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->incoming.push_back(&r);
+    }
+    b->cv_in.notify_all();
+    {
+        std::unique_lock<std::mutex> lk(b->mu);
+        b->cv_done.wait(lk, [&] { return r.done; });
+    }
+    if (r.status != MYCEL_OK) throw Error(r.status, r.error);
+    *op->pcg->h_state = r.final_state;
+}
+
+}  // namespace
+
+SolveBroker *broker_create(int device) {
+    std::unique_ptr<SolveBroker> b(new SolveBroker());
+    b->device = device;
+    MYCEL_HIP(hipSetDevice(device));
+    MYCEL_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+    b->th = std::thread(broker_loop, b.get());
+    return b.release();
+}
+
+void broker_destroy(SolveBroker *b) {
+    if (!b) return;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->stop = true;
+    }
+    b->cv_in.notify_all();
+    if (b->th.joinable()) b->th.join();
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
+mycel_pack_stats broker_stats(SolveBroker *b) {
+    std::lock_guard<std::mutex> lk(b->mu);
+    return b->stats;
+}
+
+// Several independent systems on different operators, solved together in packed launches (the caller's thread
+// drives the batches). Each system's result is bit-identical to its multigrid solve alone.
+void pcg_solve_packed(mycel_op *const *ops, const double *const *d_b, double *const *d_x, int count, double tol,
+                      int max_iter, mycel_pcg_result *res) {
+    if (count < 1 || count > kPackMax) throw Error(MYCEL_E_OUT_OF_RANGE, "count must be in [1, 16]");
+    const auto t0 = SolveClock::now();
+    const int P = MYCEL_PRECISION_MIXED_GMG;
+    std::vector<int> live;  // members that iterate (set-up did not end them)
+    for (int i = 0; i < count; i++) {
+        for (int j = 0; j < i; j++)
+            if (ops[j] == ops[i]) throw Error(MYCEL_E_ARGUMENT, "each system needs its own operator handle");
+        res[i] = mycel_pcg_result{};
+        if (!gmg_supported(ops[i])) {  // as pcg_solve_device: the FP64 Jacobi iteration
+            pcg_solve_device(ops[i], d_b[i], d_x[i], tol, max_iter, P, nullptr, nullptr, nullptr, &res[i]);
+            continue;
+        }
+        if (!solve_setup(ops[i], d_b[i], d_x[i], tol, max_iter, P, &res[i], t0)) continue;
+        if (!fuse_step4() || !gmg_packable(ops[i])) {
+            solve_loop(ops[i], d_b[i], d_x[i], max_iter, P, nullptr, nullptr, nullptr);
+            solve_finish(ops[i], d_b[i], d_x[i], tol, max_iter, P, nullptr, nullptr, nullptr, &res[i], t0);
+            continue;
+        }
+        MYCEL_HIP(hipStreamSynchronize(ops[i]->stream));
+        live.push_back(i);
+    }
+    if (live.empty()) return;
+    hipStream_t s = ops[live[0]]->stream;
+    PackTables T;
+    std::vector<int> cur = live;
+    bool dirty = true;
+    const long long max_batches = static_cast<long long>(max_iter) + 64;
+    for (long long batch = 0; !cur.empty(); batch++) {
+        if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "packed PCG batch loop did not terminate");
+        if (dirty) {
+            std::vector<mycel_op *> o;
+            std::vector<double *> x;
+            for (int i : cur) {
+                o.push_back(ops[i]);
+                x.push_back(d_x[i]);
+            }
+            T.build(o.data(), x.data(), static_cast<int>(cur.size()), s);
+            dirty = false;
+        }
+        T.enqueue_batch(s);
+        T.read_states(s);
+        std::vector<int> next;
+        for (int k = 0; k < static_cast<int>(cur.size()); k++) {
+            const int i = cur[k];
+            if (!T.h_states[k].done) {
+                next.push_back(i);
+                continue;
+            }
+            *ops[i]->pcg->h_state = T.h_states[k];
+            solve_finish(ops[i], d_b[i], d_x[i], tol, max_iter, P, nullptr, nullptr, nullptr, &res[i], t0);
+            dirty = true;
+        }
+        cur.swap(next);
+    }
 }
 
 int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, double tol, int max_iter, int precision,

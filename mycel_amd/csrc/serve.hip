@@ -84,19 +84,23 @@ struct mycel_server {
     int64_t next_id = 1;
     int active = 0;
     bool stopping = false;
+    bool packing = true;                   // the jobs' multigrid solves go to the broker
+    mycel::SolveBroker *broker = nullptr;  // created with the server; destroyed after the workers
 };
 
 namespace mycel {
 namespace {
 
-// Runs a job from its own input copies (read-only while it runs) into r.
-void run_job(const Job &j, int device, JobResult &r) {
+// Runs a job from its own input copies (read-only while it runs) into r; broker: where its multigrid solves go
+// (null: on the worker's own stream).
+void run_job(const Job &j, int device, SolveBroker *broker, JobResult &r) {
     MYCEL_HIP(hipSetDevice(device));
     const mycel_job_desc &d = j.desc;
     std::unique_ptr<mycel_op, void (*)(mycel_op *)> op(
         op_create(d.grid, j.domain.data(), j.fixed.data(), static_cast<int>(j.fixed.size()), d.e0, d.nu,
                   d.emin_ratio, d.penal, j.density.empty() ? nullptr : j.density.data()),
         op_destroy);
+    op->broker = broker;
     const size_t ndof = static_cast<size_t>(op->d.n_dofs);
     r.u.assign(ndof, 0.0);
     if (d.kind == MYCEL_JOB_FEA) {
@@ -144,7 +148,7 @@ void worker_loop(mycel_server *sv) {
         }
         JobResult r;
         try {
-            run_job(*job, sv->device, r);
+            run_job(*job, sv->device, sv->packing ? sv->broker : nullptr, r);
             r.status = MYCEL_OK;
         } catch (const Error &e) {
             r.status = e.status;
@@ -206,6 +210,7 @@ int32_t mycel_server_create(double hbm_budget_bytes, int32_t workers, mycel_serv
         const double cap = 0.95 * static_cast<double>(free_b);
         sv->budget = hbm_budget_bytes > 0 ? std::min(hbm_budget_bytes, cap) : cap;
         sv->n_workers = workers;
+        sv->broker = broker_create(sv->device);
         for (int i = 0; i < workers; i++) sv->workers.emplace_back(worker_loop, sv.get());
         *out = sv.release();
         return MYCEL_OK;
@@ -224,7 +229,25 @@ int32_t mycel_server_destroy(mycel_server *sv) {
     }
     sv->cv_work.notify_all();
     for (auto &t : sv->workers) t.join();
+    broker_destroy(sv->broker);
     delete sv;
+    return MYCEL_OK;
+}
+
+int32_t mycel_server_set_packing(mycel_server *sv, int32_t enable) {
+    if (!sv) return MYCEL_E_ARGUMENT_NULL;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (!sv->jobs.empty()) {
+        set_last_error("packing is set before the first submit");
+        return MYCEL_E_INVALID_OPERATION;
+    }
+    sv->packing = enable != 0;
+    return MYCEL_OK;
+}
+
+int32_t mycel_server_pack_stats(mycel_server *sv, mycel_pack_stats *out) {
+    if (!sv || !out) return MYCEL_E_ARGUMENT_NULL;
+    *out = broker_stats(sv->broker);
     return MYCEL_OK;
 }
 

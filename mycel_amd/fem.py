@@ -423,6 +423,20 @@ class PcgSolver:
                                                  int(maxIterations), int(precision), res))
         return [_pcg_result(r) for r in res]
 
+    def solve_packed_device(self, ops, bs, xs, tol=1e-6, maxIterations=5000) -> List["PcgResult"]:
+        """Independent multigrid systems on distinct operators (different grids) solved together in shared
+        launches (mycel_pcg_solve_packed_d): bs[i], xs[i] are float64 device tensors of ops[i]'s box DOFs (xs:
+        warm starts in, solutions out). Each result is bit-identical to solve_device of that system alone."""
+        n = len(ops)
+        if not (len(bs) == n and len(xs) == n):
+            raise L.ArgumentError(L.E_ARGUMENT, "ops, bs and xs must have the same length")
+        res = (L.PcgResultC * n)()
+        o = (C.c_void_p * n)(*[op._h.value if isinstance(op._h, C.c_void_p) else op._h for op in ops])
+        b = (C.c_void_p * n)(*[t.data_ptr() for t in bs])
+        x = (C.c_void_p * n)(*[t.data_ptr() for t in xs])
+        L.check(L.load().mycel_pcg_solve_packed_d(o, b, x, n, float(tol), int(maxIterations), res))
+        return [_pcg_result(r) for r in res]
+
     def solve_device(self, op: FemOperator, b_t, x_t, tol=1e-6, maxIterations=5000, precision=L.PRECISION_FP64):
         res = L.PcgResultC()
         L.check(L.load().mycel_pcg_solve_d(op._h, L.device_ptr(b_t), L.device_ptr(x_t), float(tol), int(maxIterations),

@@ -121,6 +121,15 @@ class JobServer:
         L.check(L.load().mycel_server_release(self._h, int(jid)))
         self._sizes.pop(jid, None)
 
+    def set_packing(self, enable: bool):
+        """Pack the resident jobs' multigrid solves into shared launches (default on); before the first submit."""
+        L.check(L.load().mycel_server_set_packing(self._h, 1 if enable else 0))
+
+    def pack_stats(self):
+        st = L.PackStatsC()
+        L.check(L.load().mycel_server_pack_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in L.PackStatsC._fields_}
+
     def stats(self):
         b, u = C.c_double(), C.c_double()
         q, a = C.c_int32(), C.c_int32()
