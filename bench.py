@@ -74,6 +74,10 @@ def parse():
     ap.add_argument("--cases-iterations", type=int, default=20, help="cases mode: SIMP iterations")
     ap.add_argument("--serve-precision", choices=("gmg", "fp64"), default="gmg",
                     help="serve mode: solver of the job server (the one-at-a-time baseline is always plain FP64)")
+    ap.add_argument("--packing", choices=("on", "off"), default="on",
+                    help="serve: the server packs its jobs' multigrid solves into shared launches (on) or runs each "
+                         "job's solves on its worker's stream (off)")
+    ap.add_argument("--no-alone", action="store_true", help="serve mode: skip the one-at-a-time legs")
     ap.add_argument("--stats-dir", help="summarize: rocprofv3 --kernel-trace --stats output directory")
     ap.add_argument("--fetch-dir", help="summarize: rocprofv3 --pmc FETCH_SIZE output directory")
     ap.add_argument("--write-dir", help="summarize: rocprofv3 --pmc WRITE_SIZE output directory")
@@ -625,9 +629,12 @@ def serving(args, one_at_a_time=True):
             lat.append(t_free - a)
         return lat, t_free
 
+    pack_stats = {}
+
     def served_run(arrivals):
         ids, lat = [], []
         with JobServer(workers=args.workers) as sv:
+            sv.set_packing(args.packing == "on")
             t0 = time.perf_counter()
             for j, m, f, a in zip(stream, models, forces, arrivals):
                 now = time.perf_counter() - t0
@@ -640,7 +647,9 @@ def serving(args, one_at_a_time=True):
                 if info.state != DONE:
                     raise SystemExit(f"job {jid} failed with status {info.status}")
                 lat.append(info.latency_ms / 1e3)
-            return lat, time.perf_counter() - t0
+            elapsed = time.perf_counter() - t0
+            pack_stats.update(sv.pack_stats())
+            return lat, elapsed
 
     pct = lambda v: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
     out = {"jobs": len(stream), "workers": args.workers, "elements": [20_000, args.max_elements],
@@ -654,6 +663,10 @@ def serving(args, one_at_a_time=True):
         print(f"[serve] one at a time, multigrid: {sum(servicemg):.1f} s", file=sys.stderr, flush=True)
     _, burst = served_run(np.zeros(len(stream)))
     capacity = len(stream) / burst
+    st = dict(pack_stats)
+    out["packing"] = {"on": args.packing == "on", "batches": st.get("batches", 0),
+                      "mean_solves_per_batch": st["member_batches"] / st["batches"] if st.get("batches") else 0.0,
+                      "graph_batches": st.get("graph_batches", 0), "solves": st.get("solves", 0)}
     print(f"[serve] server at saturation: {capacity:.2f} jobs/s", file=sys.stderr, flush=True)
     rate = 0.8 * capacity
     rng = W.SplitMix64(args.seed + 1)
@@ -677,7 +690,7 @@ def serving(args, one_at_a_time=True):
 
 
 def mode_serve(args):
-    sv = serving(args)
+    sv = serving(args, one_at_a_time=not args.no_alone)
     out = {
         "metric": METRIC,
         "value": sv["capacity_jobs_per_s"],
@@ -696,7 +709,8 @@ def mode_serve(args):
                    "simp_iterations": sv["simp_iterations"],
                    "value": "server capacity: every job submitted at once, jobs / makespan"},
     }
-    out.update({k: sv[k] for k in ("at_80pct_capacity", "one_at_a_time_fp64", "one_at_a_time_multigrid")})
+    out.update({k: sv[k] for k in ("at_80pct_capacity", "packing", "one_at_a_time_fp64", "one_at_a_time_multigrid")
+                if k in sv})
     print(json.dumps(out), flush=True)
 
 

@@ -10,8 +10,8 @@
 // so its stiffness is the same Ke scaled by h_l / h (Hex8 elasticity scales linearly with h), and its
 // modulus is the mean of its 8 children on the zero-padded fine element grid (rediscretisation). A coarse
 // DOF is constrained when no element with E > 0 touches its node or when the collocated fine DOF
-// (2I, 2J, 2K) is constrained. Constrained DOFs carry inv = 0, and every vector of the V-cycle is zero on
-// them, so the smoother, residual, restriction and prolongation never branch on flags.
+// (2I, 2J, 2K), clamped to the fine grid, is constrained. Constrained DOFs carry inv = 0, and every vector of
+// the V-cycle is zero on them, so the smoother, residual, restriction and prolongation never branch on flags.
 //
 // Smoother: Chebyshev iteration of degree 2 on D^-1 A over [lmax / 30, lmax], where lmax is the
 // Gershgorin bound max_r sum_c |Ke_rc| / Ke_rr of the element matrix: a row of D^-1 A is an E-weighted
@@ -1145,11 +1145,16 @@ __global__ __launch_bounds__(256) void k_gmg_inv_coarse(Dims gc, const float *__
     float se = 0.0f;
 #pragma unroll
     for (int a = 0; a < 8; a++) se += epc[pb - corner_x(a) - gc.px * (corner_y(a) + gc.py * corner_z(a))];
-    const bool inside = 2 * I <= gf.nx && 2 * J <= gf.ny && 2 * K <= gf.nz;
-    const int fm = inside ? 3 * (2 * I + gf.nx1 * (2 * J + gf.ny1 * 2 * K)) : 0;
+    // The fine node the coarse node takes its constraint from: the collocated one (2I, 2J, 2K), clamped to the fine
+    // grid. A fine grid with an odd element count along an axis has its last node layer at an odd index, between
+    // the last two coarse layers, the outer of which (2I = n + 1) lies outside it; that outer layer takes the
+    // constraints of the fine grid's last layer, so a support on that face (or on a face that reaches an odd index
+    // further down the hierarchy) still holds every level. Without the clamp such a level had no support at all:
+    // singular, and the FP32 V-cycle broke down within a few iterations.
+    const int fm = 3 * (min(2 * I, gf.nx) + gf.nx1 * (min(2 * J, gf.ny) + gf.ny1 * min(2 * K, gf.nz)));
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const bool cons = !(se > 0.0f) || (inside && !(invf[fm + c] > 0.0f));
+        const bool cons = !(se > 0.0f) || !(invf[fm + c] > 0.0f);
         invc[3 * n + c] = cons ? 0.0f : 1.0f / (kdiag * se);
     }
 }

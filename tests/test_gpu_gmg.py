@@ -49,6 +49,34 @@ def _check_against_oracle(p, tol=1e-8, max_gmg_iters=200):
     return r
 
 
+def _far_face_problem(nx, ny, nz, face):
+    """A box clamped on the far face `face` ("+x", "+y" or "+z") with a tip load on the opposite face's edge."""
+    p = W.Problem(nx, ny, nz, name=f"far-face {face} {nx}x{ny}x{nz}")
+    p.fixed_dofs = W.fix_all(W._face_nodes(p, face))
+    if face == "+x":
+        tip = [p.node(0, j, 0) for j in range(ny + 1)]
+    elif face == "+y":
+        tip = [p.node(i, 0, 0) for i in range(nx + 1)]
+    else:
+        tip = [p.node(i, 0, 0) for i in range(nx + 1)]
+    tip = np.array(tip, np.int32)
+    p.loads = [(tip, np.ones(tip.size), np.array([0.3, -0.2, -1.0], np.float32))]
+    p.density = random_field(nx * ny + nz, p.n_elem, 0.01, 1.0)
+    return p
+
+
+@pytest.mark.parametrize("shape,face", [((45, 15, 11), "+x"), ((56, 28, 28), "+y"), ((24, 16, 13), "+z")],
+                         ids=["odd-finest-x", "odd-at-level-2-y", "odd-finest-z"])
+def test_gmg_support_on_a_far_face(gpu, shape, face):
+    """The only support on a far face whose node index is odd on some level (45: the finest level; 28 -> 14 -> 7:
+    the second coarse level; 13: the finest). Each coarse level takes the support from the fine grid's last node
+    layer, so the V-cycle holds: no breakdown, no FP64 fallback, a few dozen iterations (before the coarse
+    constraints were clamped to the fine grid these levels had no support, and the solve fell back to FP64
+    Jacobi after 1-3 iterations)."""
+    r = _check_against_oracle(_far_face_problem(*shape, face), max_gmg_iters=120)
+    assert r.FallbackIterations == 0
+
+
 def test_gmg_cantilever_random_density(gpu):
     p = W.cantilever_simp()
     p.density = random_field(4, p.n_elem, 0.01, 1.0)
