@@ -14,9 +14,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "apply_k.hpp"
 #include "device_common.hpp"
@@ -499,6 +503,74 @@ static int ku_blocks(const mycel_op *op, const KuForm &f) {
 
 int apply_k_blocks(const mycel_op *op) { return ku_blocks(op, ku_auto(op)); }
 
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 5>, int> g_tuned;  // (family, device, nx, ny, nz) -> chosen variant
+
+bool autotune_enabled() {
+    static const bool on = [] {
+        const char *e = std::getenv("MYCEL_AUTOTUNE");
+        return !(e && *e == '0');
+    }();
+    return on;
+}
+}  // namespace
+
+int tile_depth_candidates(int rule_kz, int *out) {
+    static const int all[4] = {2, 3, 4, 8};
+    int n = 0;
+    out[n++] = rule_kz;
+    for (int k : all)
+        if (k != rule_kz) out[n++] = k;
+    return n;
+}
+
+int autotune(int family, int device, const Dims &d, const int *cand, int n, hipStream_t s,
+             const std::function<void(int)> &launch) {
+    if (n <= 1 || !autotune_enabled()) return cand[0];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MYCEL_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return cand[0];
+    const std::array<int, 5> key{family, device, d.nx, d.ny, d.nz};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
+    }
+    // Two untimed launches, then the mean of kReps launches back to back, per candidate; the smallest mean wins
+    // (ties: the earlier candidate, so the rule's choice unless another is faster).
+    constexpr int kReps = 8;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MYCEL_HIP(hipEventCreate(&e0));
+    MYCEL_HIP(hipEventCreate(&e1));
+    int best = cand[0];
+    float best_ms = 0.0f;
+    try {
+        for (int q = 0; q < n; q++) {
+            launch(cand[q]);
+            launch(cand[q]);
+            MYCEL_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < kReps; r++) launch(cand[q]);
+            MYCEL_HIP(hipEventRecord(e1, s));
+            MYCEL_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            MYCEL_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (q == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = cand[q];
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tuned.emplace(key, best).first->second;  // a concurrent tuner of the same key keeps its entry
+}
+
 int ku_element_layers(const mycel_op *op) { return ku_auto(op).kz; }
 
 template <typename T, int MODE, bool DOT>
@@ -539,7 +611,28 @@ template <typename T>
 static void launch_apply_k_t(mycel_op *op, const KeSym<T> &ke, const KeMono<T> &km, const T *ep, const T *d_x,
                              T *d_y, int mode, hipStream_t s, double *d_dot_partial, const int *d_stop,
                              const Batch &bt) {
-    const KuForm f = ku_form<T>(op, d_dot_partial != nullptr);
+    KuForm f = ku_form<T>(op, d_dot_partial != nullptr);
+    if (!d_dot_partial && op->ku_form == MYCEL_MATVEC_AUTO && f.form == MYCEL_MATVEC_ELEMENT && bt.count == 1) {
+        // Launches without dot partials: the tile depth is the autotuner's (every depth gives the same bits).
+        int &tuned = op->ku_tuned[std::is_same<T, float>::value ? 1 : 0];
+        if (!tuned) {
+            int cand[4];
+            const int nc = tile_depth_candidates(f.kz, cand);
+            tuned = autotune(std::is_same<T, float>::value ? kTuneKu32 : kTuneKu64, op->device, op->d, cand, nc, s,
+                             [&](int kz) {
+                                 const KuForm c{MYCEL_MATVEC_ELEMENT, kz};
+                                 if (mode == kApplyK)
+                                     launch_ku<T, kApplyK, false>(op, c, ke, km, ep, d_x, d_y, s, nullptr, nullptr, bt);
+                                 else if (mode == kApplyAProjected)
+                                     launch_ku<T, kApplyAProjected, false>(op, c, ke, km, ep, d_x, d_y, s, nullptr,
+                                                                          nullptr, bt);
+                                 else
+                                     launch_ku<T, kApplyAGeneral, false>(op, c, ke, km, ep, d_x, d_y, s, nullptr,
+                                                                        nullptr, bt);
+                             });
+        }
+        f.kz = tuned;
+    }
     if (d_dot_partial) {
         if (mode != kApplyAProjected) throw Error(MYCEL_E_ARGUMENT, "fused dot only on the projected operator");
         launch_ku<T, kApplyAProjected, true>(op, f, ke, km, ep, d_x, d_y, s, d_dot_partial, d_stop, bt);

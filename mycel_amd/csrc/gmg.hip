@@ -1376,6 +1376,7 @@ struct Gmg {
     float theta_inv = 0;
     float cd[kMaxDegree], cr[kMaxDegree];  // Chebyshev step k >= 1: d' = cd[k] d + cr[k] D^-1 r
     int fine_degree = kFineDegree;
+    int smooth_kz = 0;  // tile depth of the finest level's Chebyshev and residual launches (autotuned)
     int rhs_cap = 1;    // right-hand sides the level vectors hold
     uint64_t gen = 0;   // bumped when the level vectors move (captured graphs must be rebuilt)
 };
@@ -1691,6 +1692,41 @@ static void gmg_setup_deflation(mycel_op *op, Gmg *g, hipStream_t s) {
     if (D.m != m_old) g->gen++;  // the island count is a kernel argument of the captured iteration
 }
 
+// The finest level's Chebyshev and residual launches write no partials, so every tile depth gives the same bits:
+// the autotuner picks the fastest for this grid (the last smoothing step, which writes the z.r partials, keeps the
+// size rule's depth). Timed on the level's own vectors, zeroed first (the V-cycle writes them before reading).
+static void gmg_tune_smoothing(mycel_op *op, Gmg *g, hipStream_t s) {
+    const int rule = fine_ec_kz(op);
+    if (!rule) return;
+    GmgLevel &F = g->lv[0];
+    const size_t bytes = sizeof(float) * static_cast<size_t>(F.d.n_dofs);
+    MYCEL_HIP(hipMemsetAsync(F.b.get(), 0, bytes, s));
+    MYCEL_HIP(hipMemsetAsync(F.x.get(), 0, bytes, s));
+    MYCEL_HIP(hipMemsetAsync(F.x2.get(), 0, bytes, s));
+    MYCEL_HIP(hipMemsetAsync(F.dv.get(), 0, bytes, s));
+    int cand[4];
+    const int nc = tile_depth_candidates(rule, cand);
+    const dim3 b256(256);
+    g->smooth_kz = autotune(kTuneSmooth, op->device, F.d, cand, nc, s, [&](int kz) {
+        const int nt = ec_tiles(F.d, kz);
+        const dim3 grid(cdiv(nt, 4));
+        const float cd = g->cd[1], cr = g->cr[1];
+        if (kz == 2)
+            hipLaunchKernelGGL((k_gmg_cheb_ec<2, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
+                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+        else if (kz == 3)
+            hipLaunchKernelGGL((k_gmg_cheb_ec<3, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
+                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+        else if (kz == 4)
+            hipLaunchKernelGGL((k_gmg_cheb_ec<4, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
+                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+        else
+            hipLaunchKernelGGL((k_gmg_cheb_ec<8, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
+                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+        MYCEL_LAUNCH_CHECK();
+    });
+}
+
 void gmg_prepare(mycel_op *op, hipStream_t s) {
     if (!op->gmg) gmg_build(op);
     Gmg *g = op->gmg;
@@ -1731,6 +1767,7 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
         g->gen++;
     }
     g->epoch = op->epoch;
+    if (!g->smooth_kz) gmg_tune_smoothing(op, g, s);
 }
 
 GmgFineView gmg_fine_view(mycel_op *op) {
@@ -1840,7 +1877,9 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     // Element-centric kernels on the finest level only: on the coarse levels, whose few waves each march a
     // chain of dependent z-steps, they measured slower than the slab-split gathers (holed box: 126.1 solves/s
     // with the finest level alone, 123.7 with level 1 as well, 122.5 with levels 1 and 2).
-    auto lkz = [&](int l) { return l == 0 ? fkz : 0; };
+    // The finest level's smoothing and residual launches take the autotuned depth (no partials: the same bits at
+    // every depth); the last step, which writes the partials, takes fkz.
+    auto lkz = [&](int l) { return l == 0 && fkz ? (g->smooth_kz ? g->smooth_kz : fkz) : 0; };
     auto cheb = [&](int l, float cd, float cr) {
         GmgLevel &L = g->lv[l];
         if (const int kz = lkz(l)) {
@@ -2003,9 +2042,10 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
         nxt[l] = g->lv[l].x2.get();
     }
     const GmgLevel &F = g->lv[0];
-    const int fkz = fine_ec_kz(mb.op);
+    const int fkz = fine_ec_kz(mb.op), skz = g->smooth_kz ? g->smooth_kz : fkz;
     const int f_tiles = ec_tiles(F.d, fkz), f_nb = cdiv(f_tiles, 4);
-    auto fine = [&](float cd, float cr, float *outv) {
+    const int s_tiles = ec_tiles(F.d, skz), s_nb = cdiv(s_tiles, 4);
+    auto fine = [&](float cd, float cr, float *outv, bool smoothing) {
         PkEc a{};
         a.g = F.d;
         a.km = F.km;
@@ -2018,14 +2058,14 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
         a.cd = cd;
         a.cr = cr;
         a.halt = mb.halt;
-        a.n_tiles = f_tiles;
-        a.nb = f_nb;
+        a.n_tiles = smoothing ? s_tiles : f_tiles;
+        a.nb = smoothing ? s_nb : f_nb;
         return a;
     };
     auto cheb = [&](int l, float cd, float cr, long long key) {
         const GmgLevel &L = g->lv[l];
         if (l == 0) {
-            out.push_back(pk_rec(key, kPkChebEc, fkz, f_nb, fine(cd, cr, nxt[0])));
+            out.push_back(pk_rec(key, kPkChebEc, skz, s_nb, fine(cd, cr, nxt[0], true)));
         } else {
             PkS a{};
             a.g = L.d;
@@ -2066,7 +2106,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.nb = cdiv(C.d.n_nodes, 8);
             out.push_back(pk_rec(key_down(l, 40), kPkResidualRestrict, 0, a.nb, a));
         } else {
-            out.push_back(pk_rec(key_down(0, 40), kPkResidualEc, fkz, f_nb, fine(0.0f, 0.0f, L.r.get())));
+            out.push_back(pk_rec(key_down(0, 40), kPkResidualEc, skz, s_nb, fine(0.0f, 0.0f, L.r.get(), true)));
             PkRD a{};
             a.gf = L.d;
             a.gc = C.d;
@@ -2101,7 +2141,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
         for (int k = 1; k < (l == 0 ? g->fine_degree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k], key_up(l, 1 + k));
     }
     const int fd = g->fine_degree;
-    PkEc a = fine(g->cd[fd - 1], g->cr[fd - 1], nullptr);
+    PkEc a = fine(g->cd[fd - 1], g->cr[fd - 1], nullptr, false);
     a.z64 = mb.z64;
     a.r64 = mb.r64;
     a.ap64 = mb.ap64;

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -139,6 +140,8 @@ struct mycel_op {
     hipStream_t stream = nullptr;
     int ku_form = MYCEL_MATVEC_AUTO;  // K*u form of the launches without dot partials (mycel_op_set_matvec_form)
     int ku_kz = 0;                    // its node layers per thread or tile
+    int ku_tuned[2] = {0, 0};         // autotuned tile depth of the automatic form's launches without dot partials
+                                      // (FP64, FP32; 0 until the first such launch)
     mycel::Dims d{};
     double h = 1, e0 = 1, nu = 0.3, emin = 0, penal = 3;
 
@@ -191,6 +194,16 @@ void launch_apply_k(mycel_op *op, const double *d_x, double *d_y, int mode, hipS
 int apply_k_blocks(const mycel_op *op);
 // Node layers per tile of the automatic element-centric K*u form on this operator's grid.
 int ku_element_layers(const mycel_op *op);
+// Autotuner. Among launch variants whose results are bit-identical (the tile depth of the element-centric K*u
+// march in launches without dot partials: each node's sum is formed in the same order at every depth), the fastest
+// on this device for this grid, timed with HIP events on s the first time a (family, device, grid) key is seen in
+// the process and reused afterwards. cand[0] is the rule's choice: returned when MYCEL_AUTOTUNE=0 or while s is
+// being captured. launch(c) enqueues one launch of variant c on s.
+enum TuneFamily : int { kTuneKu64 = 0, kTuneKu32 = 1, kTuneSmooth = 2 };
+int autotune(int family, int device, const Dims &d, const int *cand, int n, hipStream_t s,
+             const std::function<void(int)> &launch);
+// The tile depths the autotuner tries for an element-centric march: the size rule's first, then the others.
+int tile_depth_candidates(int rule_kz, int *out);
 void launch_apply_k_f32(mycel_op *op, const float *d_x, float *d_y, int mode, hipStream_t s,
                         double *d_dot_partial = nullptr, const int *d_stop = nullptr);
 void launch_von_mises(mycel_op *op, const double *d_u, double *d_vm, hipStream_t s);
