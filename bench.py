@@ -67,8 +67,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serve", action="store_true", help="solve mode: skip the serving-stream latency leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--jobs", type=int, default=48, help="serve mode: jobs in the stream")
-    ap.add_argument("--workers", type=int, default=4, help="serve mode: resident jobs")
+    ap.add_argument("--jobs", type=int, default=96, help="serve mode: jobs in the stream")
+    ap.add_argument("--workers", type=int, default=16, help="serve mode: resident jobs (and packed solves)")
     ap.add_argument("--simp-iterations", type=int, default=40, help="serve mode: SIMP iterations per job")
     ap.add_argument("--max-elements", type=int, default=250_000, help="serve mode: largest job")
     ap.add_argument("--cases-iterations", type=int, default=20, help="cases mode: SIMP iterations")
@@ -630,9 +630,10 @@ def serving(args, one_at_a_time=True):
         return lat, t_free
 
     pack_stats = {}
+    longest = []
 
     def served_run(arrivals):
-        ids, lat = [], []
+        ids, lat, run = [], [], []
         with JobServer(workers=args.workers) as sv:
             sv.set_packing(args.packing == "on")
             t0 = time.perf_counter()
@@ -647,8 +648,10 @@ def serving(args, one_at_a_time=True):
                 if info.state != DONE:
                     raise SystemExit(f"job {jid} failed with status {info.status}")
                 lat.append(info.latency_ms / 1e3)
+                run.append(info.run_ms / 1e3)
             elapsed = time.perf_counter() - t0
             pack_stats.update(sv.pack_stats())
+            longest[:] = [max(run)]
             return lat, elapsed
 
     pct = lambda v: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
@@ -664,9 +667,11 @@ def serving(args, one_at_a_time=True):
     _, burst = served_run(np.zeros(len(stream)))
     capacity = len(stream) / burst
     st = dict(pack_stats)
+    out["saturation_longest_job_run_s"] = longest[0]
     out["packing"] = {"on": args.packing == "on", "batches": st.get("batches", 0),
                       "mean_solves_per_batch": st["member_batches"] / st["batches"] if st.get("batches") else 0.0,
-                      "graph_batches": st.get("graph_batches", 0), "solves": st.get("solves", 0)}
+                      "graph_batches": st.get("graph_batches", 0), "solves": st.get("solves", 0),
+                      "broker_host_ms": {k: st.get(k, 0.0) for k in ("build_ms", "enqueue_ms", "wait_ms")}}
     print(f"[serve] server at saturation: {capacity:.2f} jobs/s", file=sys.stderr, flush=True)
     rate = 0.8 * capacity
     rng = W.SplitMix64(args.seed + 1)
@@ -709,8 +714,8 @@ def mode_serve(args):
                    "simp_iterations": sv["simp_iterations"],
                    "value": "server capacity: every job submitted at once, jobs / makespan"},
     }
-    out.update({k: sv[k] for k in ("at_80pct_capacity", "packing", "one_at_a_time_fp64", "one_at_a_time_multigrid")
-                if k in sv})
+    out.update({k: sv[k] for k in ("saturation_longest_job_run_s", "at_80pct_capacity", "packing",
+                                   "one_at_a_time_fp64", "one_at_a_time_multigrid") if k in sv})
     print(json.dumps(out), flush=True)
 
 

@@ -463,6 +463,9 @@ typedef struct {
     int64_t graph_batches;  /* batches replayed from a captured graph */
     int64_t rebuilds;       /* membership changes (tables rebuilt) */
     int64_t solves;         /* solves that left the pack */
+    double build_ms;        /* broker host time: rebuilding tables and capturing graphs */
+    double enqueue_ms;      /* broker host time: enqueuing batches */
+    double wait_ms;         /* broker host time: waiting for batches to finish on the device */
 } mycel_pack_stats;
 MYCEL_API int32_t mycel_server_pack_stats(mycel_server *server, mycel_pack_stats *out);
 

@@ -135,7 +135,8 @@ class JobDescC(C.Structure):
 
 class PackStatsC(C.Structure):
     _fields_ = [("batches", C.c_int64), ("member_batches", C.c_int64), ("launches", C.c_int64),
-                ("graph_batches", C.c_int64), ("rebuilds", C.c_int64), ("solves", C.c_int64)]
+                ("graph_batches", C.c_int64), ("rebuilds", C.c_int64), ("solves", C.c_int64),
+                ("build_ms", C.c_double), ("enqueue_ms", C.c_double), ("wait_ms", C.c_double)]
 
 
 class JobInfoC(C.Structure):

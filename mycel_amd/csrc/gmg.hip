@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <memory>
+#include <tuple>
 #include <vector>
 
 #include "apply_k.hpp"
@@ -804,6 +805,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
     if (threadIdx.x < sizeof(TailArgs) / 4)
         reinterpret_cast<uint32_t *>(&t)[threadIdx.x] = reinterpret_cast<const uint32_t *>(targ)[threadIdx.x];
     __syncthreads();
+    if (t.count == 0) return;  // an empty slot of a pack (the same value in every thread)
     const long long o0 = static_cast<long long>(rhs) * t.lv[0].d.n_dofs;
     {
         // Every segment holds <= kTailPer * kTailThreads floats (checked on the host); the coarsest inverse
@@ -1099,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_pk_prolong(const PkPro *__restrict__ ta
     if (static_cast<int>(blockIdx.x) < a.nb && n < a.gf.n_nodes) prolong_node(a.gf, a.xf, a.invf, a.gc, a.xcv, n);
 }
 
-// One workgroup per member (blockIdx.x), its arguments from the table.
+// One workgroup per slot (blockIdx.x), its arguments from the table (an empty slot's entry has count 0).
 __global__ __launch_bounds__(kTailThreads) void k_pk_tail(const TailArgs *__restrict__ tab) {
     tail_run(tab + blockIdx.x, 0);
 }
@@ -2152,15 +2154,22 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
 
 }  // namespace
 
+// A pack has a fixed number of member slots (grid dimension y of every launch) and a launch list that only grows:
+// a launch per (step key, kernel, tile depth) that any member has brought, with the largest grid and LDS seen.
+// A membership change then rewrites only the tables (empty entries: zero, so every block returns at entry), and a
+// graph captured for the current shape stays valid until the shape grows.
 struct GmgPack {
     struct Launch {
-        int kind, kz, members, grid_x;
-        size_t off, lds;
+        long long key;
+        int kind, kz, grid_x;
+        size_t lds, entry, off;  // entry: table entry size; off: the table's offset (slots entries)
     };
-    std::vector<Launch> launches;
-    DevBuf<uint8_t> dev;       // the tables
-    uint8_t *host = nullptr;   // pinned staging of the tables
-    size_t host_cap = 0;
+    std::vector<Launch> launches;  // in (key, kind, tile depth) order
+    int slots = 0;
+    uint64_t shape = 0;            // bumped whenever launches, grid sizes, LDS or table offsets change
+    DevBuf<uint8_t> dev;           // the tables
+    uint8_t *host = nullptr;       // pinned staging of the tables
+    size_t host_cap = 0, bytes = 0;
     ~GmgPack() {
         if (host) (void)hipHostFree(host);
     }
@@ -2172,67 +2181,84 @@ bool gmg_packable(const mycel_op *op) {
            fuse_rr() && g->rhs_cap >= 1;
 }
 
-GmgPack *gmg_pack_create() {
+GmgPack *gmg_pack_create(int slots) {
+    if (slots < 1) throw Error(MYCEL_E_OUT_OF_RANGE, "multigrid pack: at least one slot");
     MYCEL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pk_tail),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsMax));
-    return new GmgPack();
+    GmgPack *p = new GmgPack();
+    p->slots = slots;
+    return p;
 }
 
 void gmg_pack_free(GmgPack *p) { delete p; }
 
 int gmg_pack_launches(const GmgPack *p) { return p ? static_cast<int>(p->launches.size()) : 0; }
 
+uint64_t gmg_pack_shape(const GmgPack *p) { return p->shape; }
+
 void gmg_pack_build(GmgPack *p, const GmgPackMember *m, int count, hipStream_t s) {
+    if (count != p->slots) throw Error(MYCEL_E_INVALID_OPERATION, "multigrid pack: one entry per slot");
     MYCEL_HIP(hipStreamSynchronize(s));  // nothing in flight still reads the old tables
-    p->launches.clear();
     std::vector<std::vector<PkRec>> recs(count);
-    std::vector<long long> keys;
-    for (int i = 0; i < count; i++) {
-        if (!gmg_packable(m[i].op)) throw Error(MYCEL_E_INVALID_OPERATION, "multigrid pack: member not packable");
-        pk_member_records(m[i], recs[i]);
-        for (const PkRec &r : recs[i]) keys.push_back(r.key);
-    }
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    std::vector<uint8_t> bytes;
-    for (long long key : keys) {
-        // One launch per (kernel, tile depth) among the members that have this step, members in order.
-        std::vector<std::pair<int, int>> groups;
-        for (int i = 0; i < count; i++)
-            for (const PkRec &r : recs[i])
-                if (r.key == key && std::find(groups.begin(), groups.end(), std::make_pair(r.kind, r.kz)) == groups.end())
-                    groups.emplace_back(r.kind, r.kz);
-        for (const auto &gk : groups) {
-            GmgPack::Launch L{gk.first, gk.second, 0, 0, (bytes.size() + 63) / 64 * 64, 0};
-            bytes.resize(L.off);
-            for (int i = 0; i < count; i++)
-                for (const PkRec &r : recs[i])
-                    if (r.key == key && r.kind == gk.first && r.kz == gk.second) {
-                        bytes.insert(bytes.end(), r.entry.begin(), r.entry.end());
-                        L.members++;
-                        L.grid_x = std::max(L.grid_x, r.nb);
-                        L.lds = std::max(L.lds, r.lds);
-                    }
-            p->launches.push_back(L);
+    auto find = [&](const PkRec &r) {
+        return std::find_if(p->launches.begin(), p->launches.end(), [&](const GmgPack::Launch &L) {
+            return L.key == r.key && L.kind == r.kind && L.kz == r.kz;
+        });
+    };
+    bool grown = false;
+    for (int q = 0; q < count; q++) {
+        if (!m[q].op) continue;
+        if (!gmg_packable(m[q].op)) throw Error(MYCEL_E_INVALID_OPERATION, "multigrid pack: member not packable");
+        pk_member_records(m[q], recs[q]);
+        for (const PkRec &r : recs[q]) {
+            auto it = find(r);
+            if (it == p->launches.end()) {
+                p->launches.push_back(GmgPack::Launch{r.key, r.kind, r.kz, r.nb, r.lds, r.entry.size(), 0});
+                grown = true;
+            } else {
+                if (r.nb > it->grid_x || r.lds > it->lds) grown = true;
+                it->grid_x = std::max(it->grid_x, r.nb);
+                it->lds = std::max(it->lds, r.lds);
+            }
         }
     }
-    if (bytes.size() > p->host_cap) {
+    if (grown) {
+        std::sort(p->launches.begin(), p->launches.end(), [](const GmgPack::Launch &a, const GmgPack::Launch &b) {
+            return std::make_tuple(a.key, a.kind, a.kz) < std::make_tuple(b.key, b.kind, b.kz);
+        });
+        size_t off = 0;
+        for (GmgPack::Launch &L : p->launches) {
+            L.off = off;
+            off = (off + L.entry * static_cast<size_t>(p->slots) + 63) / 64 * 64;
+        }
+        p->bytes = off;
+        p->shape++;
+    }
+    if (p->bytes > p->host_cap) {
         if (p->host) (void)hipHostFree(p->host);
         p->host = nullptr;
         p->host_cap = 0;
-        MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&p->host), bytes.size(), hipHostMallocDefault));
-        p->host_cap = bytes.size();
+        MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&p->host), p->bytes, hipHostMallocDefault));
+        p->host_cap = p->bytes;
     }
-    if (bytes.size() > p->dev.size()) p->dev.resize(bytes.size());
-    std::memcpy(p->host, bytes.data(), bytes.size());
-    MYCEL_HIP(hipMemcpyAsync(p->dev.get(), p->host, bytes.size(), hipMemcpyHostToDevice, s));
+    if (p->bytes > p->dev.size()) {
+        p->dev.resize(p->bytes);
+        p->shape++;  // the tables moved
+    }
+    std::memset(p->host, 0, p->bytes);
+    for (int q = 0; q < count; q++)
+        for (const PkRec &r : recs[q]) {
+            const auto it = find(r);
+            std::memcpy(p->host + it->off + it->entry * static_cast<size_t>(q), r.entry.data(), r.entry.size());
+        }
+    MYCEL_HIP(hipMemcpyAsync(p->dev.get(), p->host, p->bytes, hipMemcpyHostToDevice, s));
 }
 
 void gmg_pack_enqueue(const GmgPack *p, hipStream_t s) {
     const dim3 b256(256);
     for (const GmgPack::Launch &L : p->launches) {
         const void *t = p->dev.get() + L.off;
-        const dim3 grid(L.grid_x, L.members);
+        const dim3 grid(L.grid_x, p->slots);
         const PkEc *ec = static_cast<const PkEc *>(t);
         switch (L.kind) {
             case kPkChebEc:
@@ -2266,7 +2292,7 @@ void gmg_pack_enqueue(const GmgPack *p, hipStream_t s) {
                 hipLaunchKernelGGL(k_pk_prolong, grid, b256, 0, s, static_cast<const PkPro *>(t));
                 break;
             case kPkTail:
-                hipLaunchKernelGGL(k_pk_tail, dim3(L.members), dim3(kTailThreads), L.lds, s,
+                hipLaunchKernelGGL(k_pk_tail, dim3(p->slots), dim3(kTailThreads), L.lds, s,
                                    static_cast<const TailArgs *>(t));
                 break;
             default:

@@ -233,7 +233,7 @@ int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, dou
                         mycel_pcg_result *res);
 // Packed multigrid solves: a broker thread advances every multigrid solve handed to it (op->broker set, no
 // cancel flag or progress callback) in launches shared by all of them, joining and leaving between batches.
-SolveBroker *broker_create(int device);
+SolveBroker *broker_create(int device, int slots);  // slots: members of its pack (<= 16)
 void broker_destroy(SolveBroker *b);
 mycel_pack_stats broker_stats(SolveBroker *b);
 // count (<= 16) independent multigrid systems on distinct operators, solved in packed launches by the calling
@@ -296,13 +296,18 @@ struct GmgPackMember {
 // Whether op's V-cycle can join a pack: a prepared hierarchy of at least two levels, no island deflation and the
 // default kernel set (element-centric finest level, slab-split coarse gathers, fused coarse residual-restriction).
 bool gmg_packable(const mycel_op *op);
-GmgPack *gmg_pack_create();
+// A pack of `slots` member slots: grid dimension y of every launch; its launch list (one launch per step, kernel
+// and tile depth any member has brought, with the largest grid and LDS seen) only grows.
+GmgPack *gmg_pack_create(int slots);
 void gmg_pack_free(GmgPack *p);
-// (Re)builds the launch sequence and the device tables for these members; synchronises s, then uploads on s.
+// Rewrites the device tables for the members in m[0 .. slots) (m[q].op null: slot q is empty, its entries zero),
+// growing the launch list when a member brings a new step or a larger grid; synchronises s, then uploads on s.
 void gmg_pack_build(GmgPack *p, const GmgPackMember *m, int count, hipStream_t s);
 // One V-cycle of every member (each member's finest-level b, x, d written by the caller, as fine_ready).
 void gmg_pack_enqueue(const GmgPack *p, hipStream_t s);
 int gmg_pack_launches(const GmgPack *p);
+// Bumped whenever the launch list, a grid, an LDS size or the table buffer changes (a captured graph is stale).
+uint64_t gmg_pack_shape(const GmgPack *p);
 
 // ---- capi.hip (internal forms of the entry points, used by the job server) -------------------
 mycel_op *op_create(const mycel_grid &g, const uint8_t *domain, const int32_t *fixed, int n_fixed, double e0,

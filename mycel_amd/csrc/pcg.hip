@@ -579,11 +579,13 @@ __global__ __launch_bounds__(256, 2) void k_pk_step4_ap(const PkStep4 *__restric
     if (threadIdx.x == 0 && fin.go) a.part_pap[blockIdx.x] = sum;
 }
 
-// The members' states after a batch, gathered for one copy to the host (one block per member).
+// The members' states after a batch, gathered for one copy to the host (one block per slot).
 __global__ __launch_bounds__(64) void k_pk_gather_states(PcgState *const *__restrict__ src,
                                                          PcgState *__restrict__ dst) {
     constexpr int kWords = static_cast<int>(sizeof(PcgState) / sizeof(uint32_t));
-    const uint32_t *from = reinterpret_cast<const uint32_t *>(src[blockIdx.x]);
+    const PcgState *from_state = src[blockIdx.x];
+    if (!from_state) return;  // an empty slot
+    const uint32_t *from = reinterpret_cast<const uint32_t *>(from_state);
     uint32_t *to = reinterpret_cast<uint32_t *>(dst + blockIdx.x);
     for (int i = threadIdx.x; i < kWords; i += 64) to[i] = from[i];
 }
@@ -1310,23 +1312,26 @@ struct PackTables {
     GmgPack *gp = nullptr;
     DevBuf<uint8_t> dev;      // outer-kernel tables and the members' state pointers
     uint8_t *host = nullptr;  // pinned staging of the same
-    size_t host_cap = 0;
+    size_t host_cap = 0, bytes = 0;
     DevBuf<PcgState> d_states;
-    PcgState *h_states = nullptr;  // pinned, kPackMax
-    int m = 0, xr_grid = 0;
+    PcgState *h_states = nullptr;  // pinned, one per slot
+    int xr_grid = 0;
     size_t off_xr = 0, off_src = 0;
+    // Step 4 launches, one per tile depth any member has brought (each a table of one entry per slot).
     struct Step4Group {
-        int kz, count, grid;
+        int kz, grid;
         size_t off;
     };
     std::vector<Step4Group> s4;
+    uint64_t shape = 0, gmg_shape = 0;  // the outer launches' shape, and the V-cycle pack's at the last build
     int launches_per_batch = 0;
+    int slots = 0;  // member slots: grid dimension y of every launch
 
-    PackTables() {
-        gp = gmg_pack_create();
-        d_states.resize(kPackMax);
-        MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_states), sizeof(PcgState) * kPackMax,
-                                hipHostMallocDefault));
+    explicit PackTables(int n_slots) : slots(n_slots) {
+        if (slots < 1 || slots > kPackMax) throw Error(MYCEL_E_OUT_OF_RANGE, "pack: 1 to 16 slots");
+        gp = gmg_pack_create(slots);
+        d_states.resize(slots);
+        MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_states), sizeof(PcgState) * slots, hipHostMallocDefault));
     }
     ~PackTables() {
         gmg_pack_free(gp);
@@ -1336,65 +1341,83 @@ struct PackTables {
     PackTables(const PackTables &) = delete;
     PackTables &operator=(const PackTables &) = delete;
 
-    // Members i: operator ops[i] (set up by solve_setup, its state in slot 0) with solution xs[i].
-    void build(mycel_op *const *ops, double *const *xs, int count, hipStream_t s) {
-        if (count < 1 || count > kPackMax) throw Error(MYCEL_E_OUT_OF_RANGE, "pack: 1 to 16 members");
+    // Slot q (of `slots`) holds operator ops[q] (set up by solve_setup, its state in slot 0) with solution xs[q],
+    // or nothing (ops[q] null). Returns true when the launch shape changed (a captured batch is stale).
+    bool build(mycel_op *const *ops, double *const *xs, hipStream_t s) {
         MYCEL_HIP(hipStreamSynchronize(s));
-        m = count;
-        std::vector<GmgPackMember> gm(count);
-        std::vector<PkXr> xr(count);
-        std::vector<std::pair<int, PkStep4>> st4(count);
-        std::vector<PcgState *> src(count);
-        xr_grid = 0;
-        for (int i = 0; i < count; i++) {
-            mycel_op *op = ops[i];
+        std::vector<GmgPackMember> gm(slots, GmgPackMember{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                           nullptr});
+        std::vector<PkXr> xr(slots, PkXr{});
+        std::vector<std::pair<int, PkStep4>> st4(slots, {0, PkStep4{}});
+        std::vector<PcgState *> src(slots, nullptr);
+        bool grown = false;
+        for (int q = 0; q < slots; q++) {
+            mycel_op *op = ops[q];
+            if (!op) continue;
             PcgWork *w = op->pcg;
             PcgState *st = w->state.get();
-            gm[i] = GmgPackMember{op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), &st->halt};
-            xr[i] = PkXr{w->n, w->nb_mv, w->nb_vec, st, w->part_pap.get(), xs[i], w->r.get(), {w->p.get(), w->p2.get()},
+            gm[q] = GmgPackMember{op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(),
+                                  &st->halt};
+            xr[q] = PkXr{w->n, w->nb_mv, w->nb_vec, st, w->part_pap.get(), xs[q], w->r.get(), {w->p.get(), w->p2.get()},
                          w->ap.get(), w->part2.get(), gmg_fine_view(op)};
-            xr_grid = std::max(xr_grid, w->nb_vec);
+            if (w->nb_vec > xr_grid) {
+                xr_grid = w->nb_vec;
+                grown = true;
+            }
             const int kz = ku_element_layers(op), nt = ec_tiles(op->d, kz);
             if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "pack: partial count mismatch");
-            st4[i] = {kz, PkStep4{op->d, op->km64, op->d_ep.get(), op->d_flags.get(), {w->p.get(), w->p2.get()},
+            st4[q] = {kz, PkStep4{op->d, op->km64, op->d_ep.get(), op->d_flags.get(), {w->p.get(), w->p2.get()},
                                   w->z.get(), st, w->part2.get(), w->part_zr.get(), w->part_zap.get(), w->nb_vec,
                                   w->nb_z, w->ap.get(), w->part_pap.get(), nt, cdiv(nt, 4)}};
-            src[i] = st;
+            auto g = std::find_if(s4.begin(), s4.end(), [&](const Step4Group &x) { return x.kz == kz; });
+            if (g == s4.end()) {
+                s4.push_back(Step4Group{kz, cdiv(nt, 4), 0});
+                grown = true;
+            } else if (cdiv(nt, 4) > g->grid) {
+                g->grid = cdiv(nt, 4);
+                grown = true;
+            }
+            src[q] = st;
         }
-        gmg_pack_build(gp, gm.data(), count, s);
-        std::vector<uint8_t> bytes;
-        auto put = [&](const void *p, size_t n) {
-            const size_t off = (bytes.size() + 63) / 64 * 64;
-            bytes.resize(off);
-            bytes.insert(bytes.end(), static_cast<const uint8_t *>(p), static_cast<const uint8_t *>(p) + n);
-            return off;
-        };
-        off_xr = put(xr.data(), sizeof(PkXr) * count);
-        off_src = put(src.data(), sizeof(PcgState *) * count);
-        s4.clear();
-        for (int i = 0; i < count; i++) {
-            const int kz = st4[i].first;
-            if (std::any_of(s4.begin(), s4.end(), [&](const Step4Group &g) { return g.kz == kz; })) continue;
-            std::vector<PkStep4> grp;
-            int grid = 0;
-            for (int j = i; j < count; j++)
-                if (st4[j].first == kz) {
-                    grp.push_back(st4[j].second);
-                    grid = std::max(grid, st4[j].second.nb);
-                }
-            s4.push_back(Step4Group{kz, static_cast<int>(grp.size()), grid, put(grp.data(), sizeof(PkStep4) * grp.size())});
+        gmg_pack_build(gp, gm.data(), slots, s);
+        if (grown) {
+            std::sort(s4.begin(), s4.end(), [](const Step4Group &a, const Step4Group &b) { return a.kz < b.kz; });
+            size_t off = 0;
+            auto place = [&](size_t n) {
+                const size_t at = off;
+                off = (off + n + 63) / 64 * 64;
+                return at;
+            };
+            off_xr = place(sizeof(PkXr) * slots);
+            off_src = place(sizeof(PcgState *) * slots);
+            for (Step4Group &g : s4) g.off = place(sizeof(PkStep4) * slots);
+            bytes = off;
+            shape++;
         }
-        if (bytes.size() > host_cap) {
+        if (bytes > host_cap) {
             if (host) (void)hipHostFree(host);
             host = nullptr;
             host_cap = 0;
-            MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), bytes.size(), hipHostMallocDefault));
-            host_cap = bytes.size();
+            MYCEL_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), bytes, hipHostMallocDefault));
+            host_cap = bytes;
         }
-        if (bytes.size() > dev.size()) dev.resize(bytes.size());
-        std::memcpy(host, bytes.data(), bytes.size());
-        MYCEL_HIP(hipMemcpyAsync(dev.get(), host, bytes.size(), hipMemcpyHostToDevice, s));
+        if (bytes > dev.size()) {
+            dev.resize(bytes);
+            shape++;
+        }
+        std::memset(host, 0, bytes);
+        std::memcpy(host + off_xr, xr.data(), sizeof(PkXr) * slots);
+        std::memcpy(host + off_src, src.data(), sizeof(PcgState *) * slots);
+        for (int q = 0; q < slots; q++) {
+            if (!ops[q]) continue;
+            const auto g = std::find_if(s4.begin(), s4.end(), [&](const Step4Group &x) { return x.kz == st4[q].first; });
+            std::memcpy(host + g->off + sizeof(PkStep4) * q, &st4[q].second, sizeof(PkStep4));
+        }
+        MYCEL_HIP(hipMemcpyAsync(dev.get(), host, bytes, hipMemcpyHostToDevice, s));
         launches_per_batch = kGmgGraphIters * (1 + gmg_pack_launches(gp) + static_cast<int>(s4.size()));
+        const bool changed = grown || gmg_pack_shape(gp) != gmg_shape;
+        gmg_shape = gmg_pack_shape(gp);
+        return changed;
     }
 
     // One batch: kGmgGraphIters iterations of every member.
@@ -1403,11 +1426,11 @@ struct PackTables {
         const PkXr *txr = reinterpret_cast<const PkXr *>(dev.get() + off_xr);
         for (int t = 0; t < kGmgGraphIters; t++) {
             const int slot = t & 1;
-            hipLaunchKernelGGL(k_pk_fpcg_update_xr, dim3(xr_grid, m), b256, 0, s, txr, slot);
+            hipLaunchKernelGGL(k_pk_fpcg_update_xr, dim3(xr_grid, slots), b256, 0, s, txr, slot);
             gmg_pack_enqueue(gp, s);
             for (const Step4Group &g : s4) {
                 const PkStep4 *ts = reinterpret_cast<const PkStep4 *>(dev.get() + g.off);
-                const dim3 grid(g.grid, g.count);
+                const dim3 grid(g.grid, slots);
                 if (g.kz == 2) hipLaunchKernelGGL(k_pk_step4_ap<2>, grid, b256, 0, s, ts, slot);
                 else if (g.kz == 3) hipLaunchKernelGGL(k_pk_step4_ap<3>, grid, b256, 0, s, ts, slot);
                 else if (g.kz == 4) hipLaunchKernelGGL(k_pk_step4_ap<4>, grid, b256, 0, s, ts, slot);
@@ -1418,12 +1441,13 @@ struct PackTables {
         MYCEL_LAUNCH_CHECK();
     }
 
-    // The members' states after the batches enqueued so far, into h_states (synchronises s).
+    // The states of every slot after the batches enqueued so far, into h_states (synchronises s; an empty slot's
+    // entry is left as it was).
     void read_states(hipStream_t s) const {
-        hipLaunchKernelGGL(k_pk_gather_states, dim3(m), dim3(64), 0, s,
+        hipLaunchKernelGGL(k_pk_gather_states, dim3(slots), dim3(64), 0, s,
                            reinterpret_cast<PcgState *const *>(dev.get() + off_src), d_states.get());
         MYCEL_LAUNCH_CHECK();
-        MYCEL_HIP(hipMemcpyAsync(h_states, d_states.get(), sizeof(PcgState) * m, hipMemcpyDeviceToHost, s));
+        MYCEL_HIP(hipMemcpyAsync(h_states, d_states.get(), sizeof(PcgState) * slots, hipMemcpyDeviceToHost, s));
         MYCEL_HIP(hipStreamSynchronize(s));
     }
 };
@@ -1439,6 +1463,7 @@ struct PackReq {
 
 struct SolveBroker {
     int device = 0;
+    int slots = 1;  // members of its pack (the server's workers)
     hipStream_t s = nullptr;
     std::mutex mu;
     std::condition_variable cv_in, cv_done;
@@ -1451,90 +1476,107 @@ struct SolveBroker {
 namespace {
 
 void broker_loop(SolveBroker *b) {
-    std::vector<PackReq *> members;
+    std::vector<PackReq *> slot(b->slots, nullptr);  // a member keeps its slot until it leaves
     std::unique_ptr<PackTables> T;
     hipGraphExec_t graph = nullptr;
     bool dirty = true;
-    int same = 0;  // batches run since the last change of members
     auto drop_graph = [&] {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
     };
+    auto count = [&] { return static_cast<int>(std::count_if(slot.begin(), slot.end(), [](PackReq *r) { return r; })); };
     for (;;) {
+        int members = 0;
         {
             std::unique_lock<std::mutex> lk(b->mu);
-            b->cv_in.wait(lk, [&] { return b->stop || !b->incoming.empty() || !members.empty(); });
-            if (members.empty() && b->incoming.empty()) break;  // stopping, nothing in flight
-            while (!b->incoming.empty() && static_cast<int>(members.size()) < kPackMax) {
-                members.push_back(b->incoming.front());
-                b->incoming.pop_front();
-                dirty = true;
-            }
+            b->cv_in.wait(lk, [&] { return b->stop || !b->incoming.empty() || count() > 0; });
+            if (count() == 0 && b->incoming.empty()) break;  // stopping, nothing in flight
+            for (int q = 0; q < b->slots && !b->incoming.empty(); q++)
+                if (!slot[q]) {
+                    slot[q] = b->incoming.front();
+                    b->incoming.pop_front();
+                    dirty = true;
+                }
+            members = count();
         }
         std::vector<PackReq *> finished;
         int32_t status = MYCEL_OK;
         std::string error;
-        bool graph_used = false;
+        bool captured = false;
         int launches = 0;
+        double t_build = 0.0, t_enqueue = 0.0, t_wait = 0.0;
         try {
+            auto t0 = SolveClock::now();
             if (!T) {
                 MYCEL_HIP(hipSetDevice(b->device));
-                T.reset(new PackTables());
+                T.reset(new PackTables(b->slots));
             }
             if (dirty) {
-                std::vector<mycel_op *> ops;
-                std::vector<double *> xs;
-                for (PackReq *r : members) {
-                    ops.push_back(r->op);
-                    xs.push_back(r->d_x);
-                }
-                drop_graph();
-                T->build(ops.data(), xs.data(), static_cast<int>(members.size()), b->s);
+                std::vector<mycel_op *> ops(b->slots, nullptr);
+                std::vector<double *> xs(b->slots, nullptr);
+                for (int q = 0; q < b->slots; q++)
+                    if (slot[q]) {
+                        ops[q] = slot[q]->op;
+                        xs[q] = slot[q]->d_x;
+                    }
+                if (T->build(ops.data(), xs.data(), b->s)) drop_graph();
                 dirty = false;
-                same = 0;
             }
-            // A membership that has run one batch unchanged is captured as a hipGraph and replayed from then on.
-            if (same >= 1 && !graph) graph = capture(b->s, [&] { T->enqueue_batch(b->s); });
-            if (graph) {
-                MYCEL_HIP(hipGraphLaunch(graph, b->s));
-                graph_used = true;
-            } else {
-                T->enqueue_batch(b->s);
+            // The batch is a graph captured for the pack's shape; a new membership of the same shape only rewrites
+            // the tables it reads.
+            if (!graph) {
+                graph = capture(b->s, [&] { T->enqueue_batch(b->s); });
+                captured = true;
             }
+            auto t1 = SolveClock::now();
+            MYCEL_HIP(hipGraphLaunch(graph, b->s));
             launches = T->launches_per_batch;
+            auto t2 = SolveClock::now();
             T->read_states(b->s);
-            same++;
-            for (int i = 0; i < static_cast<int>(members.size()); i++)
-                if (T->h_states[i].done) {
-                    members[i]->final_state = T->h_states[i];
-                    finished.push_back(members[i]);
+            auto t3 = SolveClock::now();
+            t_build = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            t_enqueue = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            t_wait = std::chrono::duration<double, std::milli>(t3 - t2).count();
+            for (int q = 0; q < b->slots; q++)
+                if (slot[q] && T->h_states[q].done) {
+                    slot[q]->final_state = T->h_states[q];
+                    finished.push_back(slot[q]);
                 }
         } catch (const Error &e) {
             status = e.status;
             error = e.what();
-            finished = members;
+            for (PackReq *r : slot)
+                if (r) finished.push_back(r);
         } catch (const std::exception &e) {
             status = MYCEL_E_DEVICE;
             error = e.what();
-            finished = members;
+            for (PackReq *r : slot)
+                if (r) finished.push_back(r);
         }
         {
             std::lock_guard<std::mutex> lk(b->mu);
             if (status == MYCEL_OK) {
                 b->stats.batches++;
-                b->stats.member_batches += static_cast<int64_t>(members.size());
+                b->stats.member_batches += members;
                 b->stats.launches += launches;
-                if (graph_used) b->stats.graph_batches++;
+                if (!captured) b->stats.graph_batches++;
+                b->stats.build_ms += t_build;
+                b->stats.enqueue_ms += t_enqueue;
+                b->stats.wait_ms += t_wait;
             }
             for (PackReq *r : finished) {
                 r->status = status;
                 r->error = error;
                 r->done = true;
-                members.erase(std::find(members.begin(), members.end(), r));
+                *std::find(slot.begin(), slot.end(), r) = nullptr;
                 b->stats.solves++;
                 dirty = true;
             }
-            if (dirty) b->stats.rebuilds++;
+            if (!finished.empty()) b->stats.rebuilds++;
+        }
+        if (status != MYCEL_OK) {
+            drop_graph();
+            T.reset();  // rebuilt from scratch for the next members
         }
         if (!finished.empty()) b->cv_done.notify_all();
     }
@@ -1561,9 +1603,11 @@ void broker_solve(SolveBroker *b, mycel_op *op, double *d_x) {
 
 }  // namespace
 
-SolveBroker *broker_create(int device) {
+SolveBroker *broker_create(int device, int slots) {
+    if (slots < 1 || slots > kPackMax) throw Error(MYCEL_E_OUT_OF_RANGE, "broker: 1 to 16 slots");
     std::unique_ptr<SolveBroker> b(new SolveBroker());
     b->device = device;
+    b->slots = slots;
     MYCEL_HIP(hipSetDevice(device));
     MYCEL_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
     b->th = std::thread(broker_loop, b.get());
@@ -1614,37 +1658,49 @@ void pcg_solve_packed(mycel_op *const *ops, const double *const *d_b, double *co
     }
     if (live.empty()) return;
     hipStream_t s = ops[live[0]]->stream;
-    PackTables T;
-    std::vector<int> cur = live;
-    bool dirty = true;
-    const long long max_batches = static_cast<long long>(max_iter) + 64;
-    for (long long batch = 0; !cur.empty(); batch++) {
-        if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "packed PCG batch loop did not terminate");
-        if (dirty) {
-            std::vector<mycel_op *> o;
-            std::vector<double *> x;
-            for (int i : cur) {
-                o.push_back(ops[i]);
-                x.push_back(d_x[i]);
-            }
-            T.build(o.data(), x.data(), static_cast<int>(cur.size()), s);
-            dirty = false;
-        }
-        T.enqueue_batch(s);
-        T.read_states(s);
-        std::vector<int> next;
-        for (int k = 0; k < static_cast<int>(cur.size()); k++) {
-            const int i = cur[k];
-            if (!T.h_states[k].done) {
-                next.push_back(i);
-                continue;
-            }
-            *ops[i]->pcg->h_state = T.h_states[k];
-            solve_finish(ops[i], d_b[i], d_x[i], tol, max_iter, P, nullptr, nullptr, nullptr, &res[i], t0);
-            dirty = true;
-        }
-        cur.swap(next);
+    const int n_slots = static_cast<int>(live.size());
+    PackTables T(n_slots);
+    std::vector<mycel_op *> slot_op(n_slots, nullptr);
+    std::vector<double *> slot_x(n_slots, nullptr);
+    std::vector<int> slot_sys(n_slots, -1);
+    for (int k = 0; k < static_cast<int>(live.size()); k++) {  // system live[k] in slot k
+        slot_op[k] = ops[live[k]];
+        slot_x[k] = d_x[live[k]];
+        slot_sys[k] = live[k];
     }
+    hipGraphExec_t graph = nullptr;
+    bool dirty = true;
+    int left = static_cast<int>(live.size());
+    const long long max_batches = static_cast<long long>(max_iter) + 64;
+    try {
+        for (long long batch = 0; left > 0; batch++) {
+            if (batch > max_batches) throw Error(MYCEL_E_DEVICE, "packed PCG batch loop did not terminate");
+            if (dirty) {
+                if (T.build(slot_op.data(), slot_x.data(), s) && graph) {
+                    MYCEL_HIP(hipGraphExecDestroy(graph));
+                    graph = nullptr;
+                }
+                dirty = false;
+            }
+            if (!graph) graph = capture(s, [&] { T.enqueue_batch(s); });
+            MYCEL_HIP(hipGraphLaunch(graph, s));
+            T.read_states(s);
+            for (int q = 0; q < n_slots; q++) {
+                if (!slot_op[q] || !T.h_states[q].done) continue;
+                const int i = slot_sys[q];
+                *ops[i]->pcg->h_state = T.h_states[q];
+                solve_finish(ops[i], d_b[i], d_x[i], tol, max_iter, P, nullptr, nullptr, nullptr, &res[i], t0);
+                slot_op[q] = nullptr;
+                slot_x[q] = nullptr;
+                left--;
+                dirty = true;
+            }
+        }
+    } catch (...) {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        throw;
+    }
+    if (graph) (void)hipGraphExecDestroy(graph);
 }
 
 int32_t pcg_solve_batch(mycel_op *op, const double *d_B, double *d_X, int R, double tol, int max_iter, int precision,

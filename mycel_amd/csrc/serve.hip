@@ -210,7 +210,7 @@ int32_t mycel_server_create(double hbm_budget_bytes, int32_t workers, mycel_serv
         const double cap = 0.95 * static_cast<double>(free_b);
         sv->budget = hbm_budget_bytes > 0 ? std::min(hbm_budget_bytes, cap) : cap;
         sv->n_workers = workers;
-        sv->broker = broker_create(sv->device);
+        sv->broker = broker_create(sv->device, workers);  // at most one solve per resident job
         for (int i = 0; i < workers; i++) sv->workers.emplace_back(worker_loop, sv.get());
         *out = sv.release();
         return MYCEL_OK;
