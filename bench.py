@@ -14,10 +14,12 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   Prints one JSON line.
 --mode kernels: a short fixed run of each kernel (for rocprofv3 kernel traces and PMC counters),
   including a device copy of known size that calibrates the FETCH_SIZE / WRITE_SIZE counters.
---mode serve: a stream of 48 fresh full-box jobs (20k-250k elements; FEA solves and 40-iteration SIMP runs):
-  every job alone in plain FP64 and in the multigrid solver, the job server's capacity with every job submitted
-  at once (value, jobs/s), and its p50/p99 latency under Poisson arrivals at 80% of that capacity, beside a
-  replay of the same arrivals through one-at-a-time service.
+--mode serve: a stream of --jobs (default 96; the configuration's 512 with --jobs 512) fresh full-box jobs
+  (20k-250k elements; FEA solves and 40-iteration SIMP runs): every job alone in plain FP64 and in the multigrid
+  solver (skipped with --no-alone), the job server's capacity with every job submitted at once (value, jobs/s;
+  --workers resident jobs whose multigrid solves run packed, --packing off runs each on its worker's stream),
+  and its p50/p99 latency under Poisson arrivals at 80% of that capacity, beside a replay of the same arrivals
+  through one-at-a-time service.
 --mode cases: the batched multi-RHS SIMP configuration: 100x50x50 grid (788,103 DOFs), x = 0 clamped, 16
   independent load cases (1-4 seeded 4x4-node patches each), 20 SIMP iterations on the summed compliance
   (vf 0.25, rmin 2.0, p = 3, solves to 1e-8 warm-started per case) with the multigrid solver; one SIMP
