@@ -32,6 +32,8 @@ clamped, 8 seeded 5x5-node patch loads; E0 = 1, nu = 0.3, Emin/E0 = 1e-9, p = 3.
   launches each timed with HIP events on the operator's stream; reports time per launch and the fraction of
   the HBM and FP64 rooflines at each size (the fixed launch and latency cost of one K*u is amortised as the
   grid grows). Prints one JSON line.
+--mode packed: --pack (16) independent systems of the serving mix, one cold multigrid solve each, one at a time and
+  then together in packed launches (the job server's machinery); solutions checked bit for bit; solves/s both ways.
 --mode summarize: no GPU; reads rocprofv3 CSV output (--stats-dir, --fetch-dir, --write-dir) of a
   `--mode kernels` run and prints per-kernel durations and HBM bytes per launch. FETCH_SIZE and WRITE_SIZE
   (KiB per dispatch) are each scaled by the factor that makes the largest device copy of the run read and
@@ -61,8 +63,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "sweep", "precision", "summarize"),
-                    default="solve")
+    ap.add_argument("--mode", choices=("solve", "kernels", "serve", "cases", "sweep", "precision", "packed",
+                                       "summarize"), default="solve")
+    ap.add_argument("--pack", type=int, default=16, help="packed mode: systems solved together (<= 16)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--matvecs", type=int, default=100)
@@ -696,6 +699,78 @@ def serving(args, one_at_a_time=True):
     return out
 
 
+def mode_packed(args):
+    """--pack systems of the serving mix (fresh full-box grids of 20k-250k elements, densities U[0.01, 1], one random
+    face clamped, 1-4 patch loads): one cold multigrid solve to 1e-8 each, one at a time (mycel_pcg_solve_d), then
+    all of them together in packed launches (mycel_pcg_solve_packed_d), --steps times each after a warm-up. The
+    packed solutions must equal the solutions alone bit for bit. Reports solves/s both ways."""
+    import numpy as np
+    import torch
+
+    import mycel_amd as M
+    from mycel_amd import FemOperator, PcgSolver
+    from mycel_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    stream = W.serving_stream(args.pack, seed=args.seed, max_el=args.max_elements, simp_iterations=1)
+    probs = [j.problem for j in stream]
+    ops, bs, xs = [], [], []
+    for k, p in enumerate(probs):
+        p.density = 0.01 + 0.99 * W.SplitMix64(args.seed + 17 * k).uniform(p.n_elem)
+        fm = p.fe_model()
+        ops.append(FemOperator(fm))
+        bs.append(torch.tensor(np.where(fm.ConstrainedDofs, 0.0, fm.AssembleForces()), device=dev))
+        xs.append(torch.zeros(p.n_dofs, dtype=torch.float64, device=dev))
+    solver = PcgSolver()
+    G = M.PRECISION_MIXED_GMG
+
+    def alone():
+        res = []
+        for op, b, x in zip(ops, bs, xs):
+            x.zero_()
+            res.append(solver.solve_device(op, b, x, tol=args.tol, maxIterations=100000, precision=G))
+        torch.cuda.synchronize()
+        return res
+
+    def packed():
+        for x in xs:
+            x.zero_()
+        res = solver.solve_packed_device(ops, bs, xs, tol=args.tol, maxIterations=100000)
+        torch.cuda.synchronize()
+        return res
+
+    for _ in range(max(1, args.warmup)):
+        alone()
+        packed()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ra = alone()
+    t_alone = (time.perf_counter() - t0) / args.steps
+    xa = [x.cpu().numpy().copy() for x in xs]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rp = packed()
+    t_packed = (time.perf_counter() - t0) / args.steps
+    same = all(np.array_equal(x.cpu().numpy(), y) for x, y in zip(xs, xa)) and \
+        all(a.Iterations == b.Iterations and a.FinalResidual == b.FinalResidual for a, b in zip(ra, rp))
+    if not same:
+        raise SystemExit("packed solutions differ from the solutions alone")
+    n = len(ops)
+    out = {
+        "metric": "multigrid solves/s of independent serving-size systems, packed vs one at a time",
+        "value": n / t_packed, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_packed * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"packed-{n}-serving-systems", "elements": [int(p.n_elem) for p in probs],
+                   "tol": args.tol},
+        "one_at_a_time_solves_per_s": n / t_alone, "packed_speedup": t_alone / t_packed,
+        "iterations": [r.Iterations for r in rp], "bit_identical": same,
+    }
+    for op in ops:
+        op.close()
+    print(json.dumps(out), flush=True)
+
+
 def mode_serve(args):
     sv = serving(args, one_at_a_time=not args.no_alone)
     out = {
@@ -923,7 +998,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     {"solve": mode_solve, "kernels": mode_kernels, "serve": mode_serve, "cases": mode_cases,
-     "sweep": mode_sweep, "precision": mode_precision}[args.mode](args)
+     "sweep": mode_sweep, "precision": mode_precision, "packed": mode_packed}[args.mode](args)
 
 
 if __name__ == "__main__":
