@@ -71,20 +71,60 @@ __device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, i
 // Per-node bodies of the V-cycle phases, shared by the one-launch-per-phase kernels of the large levels and
 // by the single-workgroup tail kernel of the small ones.
 
-// One Chebyshev step at node n: r = b - K x; d' = cd d + cr inv r; x' = x + d' (x' to x_out, d' back to d).
+// The smoother's preconditioner is the nodal block Jacobi B^-1: per node the inverse of the 3x3 diagonal block of
+// the level's operator, stored as (xx, yy, zz, xy, xz, yz), with zero rows and columns on constrained components.
+constexpr int kBinv = 6;  // floats per node
+
+__device__ __forceinline__ void bload(const float *__restrict__ binv, int n, float (&bi)[kBinv]) {
+#pragma unroll
+    for (int q = 0; q < kBinv; q++) bi[q] = binv[kBinv * n + q];
+}
+
+// z = B^-1 r at one node.
+__device__ __forceinline__ void bapply(const float (&bi)[kBinv], const float (&r)[3], float (&z)[3]) {
+    z[0] = (bi[0] * r[0] + bi[3] * r[1]) + bi[4] * r[2];
+    z[1] = (bi[3] * r[0] + bi[1] * r[1]) + bi[5] * r[2];
+    z[2] = (bi[4] * r[0] + bi[5] * r[1]) + bi[2] * r[2];
+}
+
+// The first Chebyshev step of a level at node n from its right-hand side sv (0 on constrained components, whose
+// inv is 0): b = sv, x = d = B^-1 b / theta.
+__device__ __forceinline__ void first_step(const float *__restrict__ inv, const float *__restrict__ binv,
+                                           float theta_inv, const float (&sv)[3], float *__restrict__ b,
+                                           float *__restrict__ x, float *__restrict__ d, int n) {
+    float bi[kBinv], bv[3], z[3];
+    bload(binv, n, bi);
+#pragma unroll
+    for (int c = 0; c < 3; c++) bv[c] = inv[3 * n + c] > 0.0f ? sv[c] : 0.0f;
+    bapply(bi, bv, z);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int dof = 3 * n + c;
+        const float xi = z[c] * theta_inv;
+        b[dof] = bv[c];
+        x[dof] = xi;
+        d[dof] = xi;
+    }
+}
+
+// One Chebyshev step at node n: r = b - K x; d' = cd d + cr B^-1 r; x' = x + d' (x' to x_out, d' back to d).
 __device__ __forceinline__ void cheb_node(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
                                           const float *__restrict__ x, const float *__restrict__ b,
-                                          float *__restrict__ d, const float *__restrict__ inv, float cd, float cr,
+                                          float *__restrict__ d, const float *__restrict__ binv, float cd, float cr,
                                           float *__restrict__ x_out, int n, int k0) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
     float kx[3], xc[3];
     ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, k0, kx, xc);
+    float bi[kBinv], r[3], z[3];
+    bload(binv, n, bi);
+#pragma unroll
+    for (int c = 0; c < 3; c++) r[c] = b[3 * n + c] - kx[c];
+    bapply(bi, r, z);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int dof = 3 * n + c;
-        const float r = b[dof] - kx[c];
-        const float dn = cd * d[dof] + cr * (inv[dof] * r);
+        const float dn = cd * d[dof] + cr * z[c];
         x_out[dof] = xc[c] + dn;
         d[dof] = dn;
     }
@@ -118,11 +158,12 @@ __device__ __forceinline__ void restrict_taps(int C, int nf, int (&idx)[3], floa
     }
 }
 
-// b_c = P^T r_f at coarse node n, then the coarse pre-smoother's first step x_c = d_c = inv_c b_c / theta.
+// b_c = P^T r_f at coarse node n, then the coarse pre-smoother's first step x_c = d_c = B_c^-1 b_c / theta.
 // The taps are summed z-major, then y, then x; taps outside the fine grid add nothing.
 __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
-                                              const float *__restrict__ invc, float theta_inv, float *__restrict__ bc,
-                                              float *__restrict__ xcv, float *__restrict__ dc, int n) {
+                                              const float *__restrict__ invc, const float *__restrict__ binvc,
+                                              float theta_inv, float *__restrict__ bc, float *__restrict__ xcv,
+                                              float *__restrict__ dc, int n) {
     int I, J, K;
     node_ijk(gc, n, I, J, K);
     int ix[3], iy[3], iz[3];
@@ -147,16 +188,52 @@ __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__res
         s2 += w != 0.0f ? w * v[t][2] : 0.0f;
     }
     const float sv[3] = {s0, s1, s2};
+    first_step(invc, binvc, theta_inv, sv, bc, xcv, dc, n);
+}
+
+// B^-1 at node n of a level from its zero-padded moduli ep: the node's 3x3 diagonal block is the sum over its <= 8
+// elements of E_e times that element's corner block, kd on the diagonal and ko s_i s_j off it (s = +1 / -1 for a
+// corner on the low / high side of the element along that axis). Constrained components (inv == 0) get zero rows and
+// columns; the free ones the inverse of their sub-block (formed in FP64, stored in FP32).
+__device__ __forceinline__ void binv_node(const Dims &g, const float *__restrict__ ep, float kd, float ko,
+                                          const float *__restrict__ inv, int n, float (&bi)[kBinv]) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
+    float sd = 0.0f, sxy = 0.0f, sxz = 0.0f, syz = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int dof = 3 * n + c;
-        const float iv = invc[dof];
-        const float bi = iv > 0.0f ? sv[c] : 0.0f;
-        const float xi = (iv * bi) * theta_inv;
-        bc[dof] = bi;
-        xcv[dof] = xi;
-        dc[dof] = xi;
+    for (int a = 0; a < 8; a++) {
+        const float e = ep[pb - corner_x(a) - g.px * (corner_y(a) + g.py * corner_z(a))];
+        const float sx = corner_x(a) ? -1.0f : 1.0f, sy = corner_y(a) ? -1.0f : 1.0f, sz = corner_z(a) ? -1.0f : 1.0f;
+        sd += e;
+        sxy += e * (sx * sy);
+        sxz += e * (sx * sz);
+        syz += e * (sy * sz);
     }
+    bool f[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) f[c] = inv[3 * n + c] > 0.0f;
+    // Constrained rows and columns replaced by the identity, inverted, then zeroed again.
+    const double dg = static_cast<double>(kd) * sd;
+    double m[3][3] = {{dg, static_cast<double>(ko) * sxy, static_cast<double>(ko) * sxz},
+                      {static_cast<double>(ko) * sxy, dg, static_cast<double>(ko) * syz},
+                      {static_cast<double>(ko) * sxz, static_cast<double>(ko) * syz, dg}};
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (!f[r] || !f[c]) m[r][c] = r == c ? 1.0 : 0.0;
+    const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
+                 c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    const double id = det > 0.0 ? 1.0 / det : 0.0;
+    const double q[6] = {c00 * id, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id,
+                         (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id, c01 * id, c02 * id,
+                         (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id};
+    // (xx, yy, zz, xy, xz, yz): entry (r, c) is kept when both components are free.
+    const bool keep[6] = {f[0], f[1], f[2], f[0] && f[1], f[0] && f[2], f[1] && f[2]};
+#pragma unroll
+    for (int t = 0; t < kBinv; t++) bi[t] = keep[t] ? static_cast<float>(q[t]) : 0.0f;
 }
 
 // x_f += P x_c at fine node n, unconstrained DOFs only (trilinear: weight 1 at an even fine index, 1/2 from
@@ -199,14 +276,14 @@ __device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__
 
 // One Chebyshev step (one node per thread).
 // Grid dimension y of every V-cycle kernel is the right-hand side: the level's vectors of RHS k start at
-// k * n_dofs (the moduli and D^-1 are shared).
+// k * n_dofs (the moduli, D^-1 and B^-1 are shared).
 __global__ __launch_bounds__(256, 2) void k_gmg_cheb(Dims g, KeSym<float> ks, const float *__restrict__ ep,
                                                      const float *__restrict__ x, const float *__restrict__ b,
-                                                     float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                     float *__restrict__ d, const float *__restrict__ binv, float cd,
                                                      float cr, float *__restrict__ x_out) {
     const int n0 = xcd_block(blockIdx.x, gridDim.x) * 256, n = n0 + threadIdx.x;
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, n, first_layer(g, n0));
+    if (n < g.n_nodes) cheb_node(g, ks, ep, x + o, b + o, d + o, binv, cd, cr, x_out + o, n, first_layer(g, n0));
 }
 
 // Island deflation (enabled per operator): every floating solid island (a face-connected cluster of elements
@@ -282,7 +359,7 @@ __global__ __launch_bounds__(256) void k_defl_solve(int m6, const double *__rest
 // per-block partials of z.r64 and z.ap64 (the flexible-CG inner products).
 __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> ks, const float *__restrict__ ep,
                                                            const float *__restrict__ x, const float *__restrict__ b,
-                                                           const float *__restrict__ d, const float *__restrict__ inv,
+                                                           const float *__restrict__ d, const float *__restrict__ binv,
                                                            float cd, float cr, double *__restrict__ z64,
                                                            const double *__restrict__ r64,
                                                            const double *__restrict__ ap64,
@@ -306,11 +383,15 @@ __global__ __launch_bounds__(256, 2) void k_gmg_cheb_final(Dims g, KeSym<float> 
         float kx[3], xc[3];
         ku_gather<float, kApplyK>(g, ks, ep, x, nullptr, i, j, k, first_layer(g, n0), kx, xc);
         double z[3];
+        float bi[kBinv], rv[3], zv[3];
+        bload(binv, n, bi);
+#pragma unroll
+        for (int c = 0; c < 3; c++) rv[c] = b[3 * n + c] - kx[c];
+        bapply(bi, rv, zv);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int dof = 3 * n + c;
-            const float r = b[dof] - kx[c];
-            const float dn = cd * d[dof] + cr * (inv[dof] * r);
+            const float dn = cd * d[dof] + cr * zv[c];
             z[c] = static_cast<double>(xc[c] + dn);
         }
         const int cl = df.m ? df.ncl[n] : -1;
@@ -345,12 +426,12 @@ __device__ __forceinline__ int ec_wave_tile() {
     return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
 }
 
-struct ChebFin {  // x' = x + d', d' = cd d + cr D^-1 (b - K x)
-    const float *b, *inv;
+struct ChebFin {  // x' = x + d', d' = cd d + cr B^-1 (b - K x)
+    const float *b, *binv;
     float *d, *x_out;
     float cd, cr;
     struct Pre {
-        float b[3], d[3], inv[3];
+        float b[3], d[3], bi[kBinv];
     };
     __device__ Pre pre(int n) const {
         Pre p;
@@ -358,15 +439,18 @@ struct ChebFin {  // x' = x + d', d' = cd d + cr D^-1 (b - K x)
         for (int c = 0; c < 3; c++) {
             p.b[c] = b[3 * n + c];
             p.d[c] = d[3 * n + c];
-            p.inv[c] = inv[3 * n + c];
         }
+        bload(binv, n, p.bi);
         return p;
     }
     __device__ void operator()(int n, float (&kx)[3], const float (&xc)[3], const Pre &p) const {
+        float r[3], z[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[c] = p.b[c] - kx[c];
+        bapply(p.bi, r, z);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float r = p.b[c] - kx[c];
-            const float dn = cd * p.d[c] + cr * (p.inv[c] * r);
+            const float dn = cd * p.d[c] + cr * z[c];
             x_out[3 * n + c] = xc[c] + dn;
             d[3 * n + c] = dn;
         }
@@ -395,7 +479,7 @@ struct ResidualFin {  // r = b - K x on unconstrained DOFs, 0 on constrained one
 };
 
 struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.r64 and z.ap64 contributions
-    const float *b, *d, *inv;
+    const float *b, *d, *binv;
     double *z64;
     const double *r64, *ap64;
     float cd, cr;
@@ -403,7 +487,7 @@ struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.
     const double *coef;  // this right-hand side's island coefficients
     double zr = 0.0, zap = 0.0;
     struct Pre {
-        float b[3], d[3], inv[3];
+        float b[3], d[3], bi[kBinv];
         double r[3], ap[3];
         int cl;
     };
@@ -413,19 +497,22 @@ struct FinalFin {  // the last smoothing step written as the FP64 z, with the z.
         for (int c = 0; c < 3; c++) {
             p.b[c] = b[3 * n + c];
             p.d[c] = d[3 * n + c];
-            p.inv[c] = inv[3 * n + c];
             p.r[c] = r64[3 * n + c];
             p.ap[c] = ap64[3 * n + c];
         }
+        bload(binv, n, p.bi);
         p.cl = df.m ? df.ncl[n] : -1;
         return p;
     }
     __device__ void operator()(int n, float (&kx)[3], const float (&xc)[3], const Pre &p) {
         double z[3];
+        float rv[3], zv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) rv[c] = p.b[c] - kx[c];
+        bapply(p.bi, rv, zv);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float r = p.b[c] - kx[c];
-            const float dn = cd * p.d[c] + cr * (p.inv[c] * r);
+            const float dn = cd * p.d[c] + cr * zv[c];
             z[c] = static_cast<double>(xc[c] + dn);
         }
         if (p.cl >= 0) defl_add(df, coef, n, p.cl, z);
@@ -469,13 +556,13 @@ __device__ __forceinline__ void fine_march(const Dims &g, const KeMono<float> &k
 template <int KZ, bool HALT>
 __global__ __launch_bounds__(256) void k_gmg_cheb_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
                                                      const float *__restrict__ x, const float *__restrict__ b,
-                                                     float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                     float *__restrict__ d, const float *__restrict__ binv, float cd,
                                                      float cr, float *__restrict__ x_out, int n_tiles,
                                                      const int *__restrict__ halt, int halt_stride) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     const int tile = ec_wave_tile();
     if (tile >= n_tiles) return;  // wave-uniform; no barrier follows
-    ChebFin fin{b + o, inv, d + o, x_out + o, cd, cr};
+    ChebFin fin{b + o, binv, d + o, x_out + o, cd, cr};
     fine_march<HALT, KZ>(g, km, ep, x + o, halt, halt_stride, tile, fin);
 }
 
@@ -495,7 +582,7 @@ __global__ __launch_bounds__(256) void k_gmg_residual_ec(Dims g, KeMono<float> k
 template <int KZ, bool HALT>
 __global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float> km, const float *__restrict__ ep,
                                                            const float *__restrict__ x, const float *__restrict__ b,
-                                                           const float *__restrict__ d, const float *__restrict__ inv,
+                                                           const float *__restrict__ d, const float *__restrict__ binv,
                                                            float cd, float cr, double *__restrict__ z64,
                                                            const double *__restrict__ r64,
                                                            const double *__restrict__ ap64,
@@ -504,7 +591,7 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_final_ec(Dims g, KeMono<float>
                                                            int halt_stride, Defl df) {
     __shared__ double red[8];
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    FinalFin fin{b + o, d + o, inv, z64 + o, r64 + o, ap64 + o, cd, cr, df,
+    FinalFin fin{b + o, d + o, binv, z64 + o, r64 + o, ap64 + o, cd, cr, df,
                  df.coef + static_cast<long long>(blockIdx.y) * kDeflStride};
     const int tile = ec_wave_tile();
     if (tile < n_tiles) fine_march<HALT, KZ>(g, km, ep, x + o, halt, halt_stride, tile, fin);
@@ -544,16 +631,20 @@ __device__ __forceinline__ bool split_gather(const Dims &g, const KeSym<float> &
 // Block bx of nb (the XCD remap runs over nb) of one right-hand side's vectors.
 __device__ __forceinline__ void cheb_s_body(const Dims &g, const KeSym<float> &ks, const float *__restrict__ ep,
                                             const float *__restrict__ x, const float *__restrict__ b,
-                                            float *__restrict__ d, const float *__restrict__ inv, float cd, float cr,
+                                            float *__restrict__ d, const float *__restrict__ binv, float cd, float cr,
                                             float *__restrict__ x_out, int bx, int nb) {
     float kx[3], xc[3];
     int n;
     if (split_gather(g, ks, ep, x, xcd_block(bx, nb) * kSplitNodes, kx, xc, n)) {
+        float bi[kBinv], r[3], z[3];
+        bload(binv, n, bi);
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[c] = b[3 * n + c] - kx[c];
+        bapply(bi, r, z);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int dof = 3 * n + c;
-            const float r = b[dof] - kx[c];
-            const float dn = cd * d[dof] + cr * (inv[dof] * r);
+            const float dn = cd * d[dof] + cr * z[c];
             x_out[dof] = xc[c] + dn;
             d[dof] = dn;
         }
@@ -562,10 +653,10 @@ __device__ __forceinline__ void cheb_s_body(const Dims &g, const KeSym<float> &k
 
 __global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
                                                     const float *__restrict__ x, const float *__restrict__ b,
-                                                    float *__restrict__ d, const float *__restrict__ inv, float cd,
+                                                    float *__restrict__ d, const float *__restrict__ binv, float cd,
                                                     float cr, float *__restrict__ x_out) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
-    cheb_s_body(g, ks, ep, x + o, b + o, d + o, inv, cd, cr, x_out + o, blockIdx.x, gridDim.x);
+    cheb_s_body(g, ks, ep, x + o, b + o, d + o, binv, cd, cr, x_out + o, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(256) void k_gmg_residual_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
@@ -599,62 +690,38 @@ bool gmg_split() {
 }
 
 // b0 = float(r64) on unconstrained DOFs; the finest pre-smoother's first step x = d = inv b0 / theta.
+// One thread per node (the block smoother's first step needs the node's three components). n: nodes.
 __global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__restrict__ r64,
-                                                       const float *__restrict__ inv, float theta_inv,
-                                                       float *__restrict__ b, float *__restrict__ x,
+                                                       const float *__restrict__ inv, const float *__restrict__ binv,
+                                                       float theta_inv, float *__restrict__ b, float *__restrict__ x,
                                                        float *__restrict__ d) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const long long o = static_cast<long long>(blockIdx.y) * n;
-    const float iv = inv[i];
-    const float bi = iv > 0.0f ? static_cast<float>(r64[o + i]) : 0.0f;
-    const float xi = (iv * bi) * theta_inv;
-    b[o + i] = bi;
-    x[o + i] = xi;
-    d[o + i] = xi;
+    const int node = blockIdx.x * 256 + threadIdx.x;
+    if (node >= n) return;
+    const long long o = static_cast<long long>(blockIdx.y) * 3 * n;
+    const float sv[3] = {static_cast<float>(r64[o + 3 * node]), static_cast<float>(r64[o + 3 * node + 1]),
+                         static_cast<float>(r64[o + 3 * node + 2])};
+    first_step(inv, binv, theta_inv, sv, b + o, x + o, d + o, node);
 }
 
 
-// Restriction with one thread per coarse DOF (the large levels: three times the parallelism of the node
-// form, the same sums in the same order).
+// Restriction from the finest level, one thread per coarse node (the block smoother's first step needs the node's
+// three components); the same sums in the same order as the tail's restrict_node.
 __device__ __forceinline__ void restrict_dof_body(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
-                                                  const float *__restrict__ invc, float theta_inv,
-                                                  float *__restrict__ bc, float *__restrict__ xcv,
+                                                  const float *__restrict__ invc, const float *__restrict__ binvc,
+                                                  float theta_inv, float *__restrict__ bc, float *__restrict__ xcv,
                                                   float *__restrict__ dc, int bx) {
-    const int dof = bx * 256 + threadIdx.x;
-    if (dof >= gc.n_dofs) return;
-    const int n = dof / 3, c = dof - 3 * n;
-    int I, J, K;
-    node_ijk(gc, n, I, J, K);
-    int ix[3], iy[3], iz[3];
-    float wx[3], wy[3], wz[3];
-    restrict_taps(I, gf.nx, ix, wx);
-    restrict_taps(J, gf.ny, iy, wy);
-    restrict_taps(K, gf.nz, iz, wz);
-    float v[27];  // all taps loaded first, then summed: one memory round trip
-#pragma unroll
-    for (int t = 0; t < 27; t++) v[t] = rf[3 * (ix[t % 3] + gf.nx1 * (iy[(t / 3) % 3] + gf.ny1 * iz[t / 9])) + c];
-    const float iv = invc[dof];
-    float sum = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 27; t++) {
-        const float w = wz[t / 9] * wy[(t / 3) % 3] * wx[t % 3];
-        sum += w != 0.0f ? w * v[t] : 0.0f;
-    }
-    const float bi = iv > 0.0f ? sum : 0.0f;
-    const float xi = (iv * bi) * theta_inv;
-    bc[dof] = bi;
-    xcv[dof] = xi;
-    dc[dof] = xi;
+    const int n = bx * 256 + threadIdx.x;
+    if (n < gc.n_nodes) restrict_node(gf, rf, gc, invc, binvc, theta_inv, bc, xcv, dc, n);
 }
 
 __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *__restrict__ rf, Dims gc,
-                                                          const float *__restrict__ invc, float theta_inv,
+                                                          const float *__restrict__ invc,
+                                                          const float *__restrict__ binvc, float theta_inv,
                                                           float *__restrict__ bc, float *__restrict__ xcv,
                                                           float *__restrict__ dc) {
     const long long of = static_cast<long long>(blockIdx.y) * gf.n_dofs;
     const long long oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
-    restrict_dof_body(gf, rf + of, gc, invc, theta_inv, bc + oc, xcv + oc, dc + oc, blockIdx.x);
+    restrict_dof_body(gf, rf + of, gc, invc, binvc, theta_inv, bc + oc, xcv + oc, dc + oc, blockIdx.x);
 }
 
 // The residual of a coarse level fused with the restriction to the next one (the kernels of those levels are
@@ -666,6 +733,7 @@ __device__ __forceinline__ void residual_restrict_body(const Dims &gf, const KeS
                                                        const float *__restrict__ ep, const float *__restrict__ xf,
                                                        const float *__restrict__ bf, const float *__restrict__ invf,
                                                        const Dims &gc, const float *__restrict__ invc,
+                                                       const float *__restrict__ binvc,
                                                        float theta_inv, float *__restrict__ bc,
                                                        float *__restrict__ xcv, float *__restrict__ dc, int bx) {
     const int t = threadIdx.x & 31;
@@ -689,29 +757,19 @@ __device__ __forceinline__ void residual_restrict_body(const Dims &gf, const KeS
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int off = 16; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off, 32);
-    if (t == 0 && n < gc.n_nodes) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int dof = 3 * n + c;
-            const float iv = invc[dof];
-            const float bi = iv > 0.0f ? v[c] : 0.0f;
-            const float xi = (iv * bi) * theta_inv;
-            bc[dof] = bi;
-            xcv[dof] = xi;
-            dc[dof] = xi;
-        }
-    }
+    if (t == 0 && n < gc.n_nodes) first_step(invc, binvc, theta_inv, v, bc, xcv, dc, n);
 }
 
 __global__ __launch_bounds__(256) void k_gmg_residual_restrict(Dims gf, KeSym<float> ks, const float *__restrict__ ep,
                                                                const float *__restrict__ xf,
                                                                const float *__restrict__ bf,
                                                                const float *__restrict__ invf, Dims gc,
-                                                               const float *__restrict__ invc, float theta_inv,
+                                                               const float *__restrict__ invc,
+                                                               const float *__restrict__ binvc, float theta_inv,
                                                                float *__restrict__ bc, float *__restrict__ xcv,
                                                                float *__restrict__ dc) {
     const long long of = static_cast<long long>(blockIdx.y) * gf.n_dofs, oc = static_cast<long long>(blockIdx.y) * gc.n_dofs;
-    residual_restrict_body(gf, ks, ep, xf + of, bf + of, invf, gc, invc, theta_inv, bc + oc, xcv + oc, dc + oc,
+    residual_restrict_body(gf, ks, ep, xf + of, bf + of, invf, gc, invc, binvc, theta_inv, bc + oc, xcv + oc, dc + oc,
                            blockIdx.x);
 }
 
@@ -758,8 +816,10 @@ struct TailLevel {
     Dims d;
     KeSym<float> ks;
     const float *ep;   // padded moduli (global)
-    const float *inv;  // inverse diagonal (global)
+    const float *inv;  // inverse diagonal (global): the constraint mask
     int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
+    int o_binv;        // and of its nodal block inverses, formed in LDS from the moduli on entry
+    float kd, ko;      // the level's corner-block values (binv_node)
 };
 constexpr int kTailSegs = 3 + 2 * kMaxTail;
 constexpr int kAinvPer = (kCoarseMaxDofs * kCoarseMaxDofs + kTailThreads - 1) / kTailThreads;
@@ -853,6 +913,17 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
         if (halted) return;  // the same value in every thread of the workgroup
     }
     __syncthreads();
+    // Every tail level's nodal block inverses, from its moduli and constraints now in LDS.
+    for (int l = 0; l < t.count; l++) {
+        const TailLevel &L = t.lv[l];
+        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads) {
+            float bi[kBinv];
+            binv_node(L.d, sm + L.o_e, L.kd, L.ko, sm + L.o_inv, n, bi);
+#pragma unroll
+            for (int q = 0; q < kBinv; q++) sm[L.o_binv + kBinv * n + q] = bi[q];
+        }
+    }
+    __syncthreads();
     // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
     unsigned flip = 0;
     auto S = [&](int off) { return sm + off; };
@@ -883,13 +954,17 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
             int n;
             float kx[3], xc[3];
             if (split_kx(L, l, n, kx, xc)) {
-                const float *b = S(L.o_b), *inv = S(L.o_inv);
+                const float *b = S(L.o_b);
                 float *d = S(L.o_d), *xo = nxt(l);
+                float bi[kBinv], r[3], z[3];
+                bload(S(L.o_binv), n, bi);
+#pragma unroll
+                for (int c = 0; c < 3; c++) r[c] = b[3 * n + c] - kx[c];
+                bapply(bi, r, z);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const int dof = 3 * n + c;
-                    const float r = b[dof] - kx[c];
-                    const float dn = cd * d[dof] + cr * (inv[dof] * r);
+                    const float dn = cd * d[dof] + cr * z[c];
                     xo[dof] = xc[c] + dn;
                     d[dof] = dn;
                 }
@@ -919,7 +994,8 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
         cheb_phase(L, l, t.cd1, t.cr1);
         residual_phase(L, l);
         for (int n = threadIdx.x; n < C.d.n_nodes; n += kTailThreads)
-            restrict_node(L.d, S(L.o_r), C.d, S(C.o_inv), t.theta_inv, S(C.o_b), cur(l + 1), S(C.o_d), n);
+            restrict_node(L.d, S(L.o_r), C.d, S(C.o_inv), S(C.o_binv), t.theta_inv, S(C.o_b), cur(l + 1), S(C.o_d),
+                          n);
         __syncthreads();
     }
     {
@@ -991,7 +1067,7 @@ __global__ __launch_bounds__(kTailThreads) void k_gmg_tail(TailArgs targ) { tail
 struct PkEc {  // finest level, element-centric: Chebyshev step, residual, last step (z and its partials)
     Dims g;
     KeMono<float> km;
-    const float *ep, *x, *b, *inv;
+    const float *ep, *x, *b, *inv, *binv;  // inv: the residual's constraint mask; binv: the smoother's B^-1
     float *d, *out;  // Chebyshev: d and x'; residual: out = r
     float cd, cr;
     double *z64;
@@ -1003,7 +1079,7 @@ struct PkEc {  // finest level, element-centric: Chebyshev step, residual, last 
 struct PkS {  // slab-split Chebyshev step of a coarse level
     Dims g;
     KeSym<float> ks;
-    const float *ep, *x, *b, *inv;
+    const float *ep, *x, *b, *binv;
     float *d, *x_out;
     float cd, cr;
     int nb;
@@ -1011,14 +1087,14 @@ struct PkS {  // slab-split Chebyshev step of a coarse level
 struct PkRR {  // coarse residual fused with the restriction
     Dims gf, gc;
     KeSym<float> ks;
-    const float *ep, *xf, *bf, *invf, *invc;
+    const float *ep, *xf, *bf, *invf, *invc, *binvc;
     float *bc, *xcv, *dc;
     float theta_inv;
     int nb;
 };
 struct PkRD {  // restriction from the finest level, one thread per coarse DOF
     Dims gf, gc;
-    const float *rf, *invc;
+    const float *rf, *invc, *binvc;
     float *bc, *xcv, *dc;
     float theta_inv;
     int nb;
@@ -1040,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_pk_cheb_ec(const PkEc *__restrict__ tab
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     const int tile = pk_wave_tile(a.nb);
     if (tile >= a.n_tiles) return;  // wave-uniform; no barrier follows
-    ChebFin fin{a.b, a.inv, a.d, a.out, a.cd, a.cr};
+    ChebFin fin{a.b, a.binv, a.d, a.out, a.cd, a.cr};
     FineX src{a.x, *a.halt};
     ec_tile_march<float, KZ>(a.g, a.km, a.ep, src, tile, fin);
 }
@@ -1062,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_pk_cheb_final_ec(const PkEc *__restrict
     const PkEc a = tab[blockIdx.y];
     if (static_cast<int>(blockIdx.x) >= a.nb) return;  // block-uniform, ahead of the block sum's barriers
     const Defl none{nullptr, nullptr, nullptr, 0.0, 0, 0, 0};
-    FinalFin fin{a.b, a.d, a.inv, a.z64, a.r64, a.ap64, a.cd, a.cr, none, nullptr};
+    FinalFin fin{a.b, a.d, a.binv, a.z64, a.r64, a.ap64, a.cd, a.cr, none, nullptr};
     const int tile = pk_wave_tile(a.nb);
     if (tile < a.n_tiles) {
         FineX src{a.x, *a.halt};
@@ -1079,20 +1155,20 @@ __global__ __launch_bounds__(256) void k_pk_cheb_final_ec(const PkEc *__restrict
 __global__ __launch_bounds__(256) void k_pk_cheb_s(const PkS *__restrict__ tab) {
     const PkS a = tab[blockIdx.y];
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
-    cheb_s_body(a.g, a.ks, a.ep, a.x, a.b, a.d, a.inv, a.cd, a.cr, a.x_out, blockIdx.x, a.nb);
+    cheb_s_body(a.g, a.ks, a.ep, a.x, a.b, a.d, a.binv, a.cd, a.cr, a.x_out, blockIdx.x, a.nb);
 }
 
 __global__ __launch_bounds__(256) void k_pk_residual_restrict(const PkRR *__restrict__ tab) {
     const PkRR a = tab[blockIdx.y];
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
-    residual_restrict_body(a.gf, a.ks, a.ep, a.xf, a.bf, a.invf, a.gc, a.invc, a.theta_inv, a.bc, a.xcv, a.dc,
-                           blockIdx.x);
+    residual_restrict_body(a.gf, a.ks, a.ep, a.xf, a.bf, a.invf, a.gc, a.invc, a.binvc, a.theta_inv, a.bc, a.xcv,
+                           a.dc, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_pk_restrict_dof(const PkRD *__restrict__ tab) {
     const PkRD a = tab[blockIdx.y];
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
-    restrict_dof_body(a.gf, a.rf, a.gc, a.invc, a.theta_inv, a.bc, a.xcv, a.dc, blockIdx.x);
+    restrict_dof_body(a.gf, a.rf, a.gc, a.invc, a.binvc, a.theta_inv, a.bc, a.xcv, a.dc, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_pk_prolong(const PkPro *__restrict__ tab) {
@@ -1159,6 +1235,17 @@ __global__ __launch_bounds__(256) void k_gmg_inv_coarse(Dims gc, const float *__
         const bool cons = !(se > 0.0f) || !(invf[fm + c] > 0.0f);
         invc[3 * n + c] = cons ? 0.0f : 1.0f / (kdiag * se);
     }
+}
+
+// The nodal block inverses of a level (binv_node), once its moduli and constraints are set.
+__global__ __launch_bounds__(256) void k_gmg_binv(Dims g, const float *__restrict__ ep, float kd, float ko,
+                                                  const float *__restrict__ inv, float *__restrict__ binv) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.n_nodes) return;
+    float bi[kBinv];
+    binv_node(g, ep, kd, ko, inv, n, bi);
+#pragma unroll
+    for (int t = 0; t < kBinv; t++) binv[kBinv * n + t] = bi[t];
 }
 
 // Galerkin blocks of the second-coarsest level: for each of its elements e, B_e = E_e P_e^T Ke P_e (24 x 24),
@@ -1351,8 +1438,10 @@ struct GmgLevel {
     KeMono<float> km{};  // the same Ke in the monomial basis (element-centric kernels)
     double scale = 1.0;  // h_l / h
     float kdiag = 0.0f;  // diagonal entry of the level's Ke
+    float koff = 0.0f;   // off-diagonal entry (x, y) of its corner-0 block
     const float *ep = nullptr;
     DevBuf<float> ep_own, inv, b, x, x2, dv, r;
+    DevBuf<float> binv;  // the smoother's nodal block inverses, kBinv per node
     int blocks = 0;    // 256 nodes per block (one lane per node)
     int blocks_s = 0;  // kSplitNodes per block (the slab-split gathers)
 };
@@ -1440,16 +1529,69 @@ bool gmg_supported(const mycel_op *op) {
 static void gmg_build(mycel_op *op) {
     std::unique_ptr<Gmg> owner(new Gmg());
     Gmg *g = owner.get();
-    // Chebyshev constants from the Gershgorin bound of D^-1 A.
-    double lmax = 0.0;
-    for (int r = 0; r < 24; r++) {
-        double s = 0.0;
-        for (int c = 0; c < 24; c++) s += std::fabs(op->ke[r * 24 + c]);
-        lmax = std::fmax(lmax, s / op->ke[r * 24 + r]);
+    // The smoother's preconditioner is the nodal block Jacobi B^-1 (the 3x3 diagonal blocks of A). Every corner
+    // block of the cube's Ke is kd on the diagonal and ko s_i s_j off it (s: the corner's side per axis), which the
+    // per-node inverses rely on; checked here.
+    const double kd = op->ke[0], ko = op->ke[1];
+    for (int a = 0; a < 8; a++) {
+        const double sgn[3] = {corner_x(a) ? -1.0 : 1.0, corner_y(a) ? -1.0 : 1.0, corner_z(a) ? -1.0 : 1.0};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                const double want = r == c ? kd : ko * sgn[r] * sgn[c];
+                if (std::fabs(op->ke[(3 * a + r) * 24 + 3 * a + c] - want) > 1e-12 * std::fabs(kd))
+                    throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: element matrix blocks are not the cube's");
+            }
     }
-    for (int r = 1; r < 24; r++)
-        if (std::fabs(op->ke[r * 25] - op->ke[0]) > 1e-12 * std::fabs(op->ke[0]))
-            throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: element matrix diagonal is not uniform");
+    // Chebyshev interval [lmax / 30, lmax]: lmax = lmax(B_e^-1 K_e) of one element (B_e its block diagonal) bounds
+    // lmax(B^-1 A) on every level for every density field: x'Ax / x'Bx = sum_e E_e x_e'K_e x_e / sum_e E_e x_e'B_e x_e.
+    // Power iteration on the 24 x 24 pencil; 0.1% added for the iteration's own error.
+    double lmax = 0.0;
+    {
+        double binv_e[8][3][3];
+        for (int a = 0; a < 8; a++) {
+            double m[3][3];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) m[r][c] = op->ke[(3 * a + r) * 24 + 3 * a + c];
+            const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
+                         c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+            const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+            const double adj[3][3] = {{c00, m[0][2] * m[2][1] - m[0][1] * m[2][2], m[0][1] * m[1][2] - m[0][2] * m[1][1]},
+                                      {c01, m[0][0] * m[2][2] - m[0][2] * m[2][0], m[0][2] * m[1][0] - m[0][0] * m[1][2]},
+                                      {c02, m[0][1] * m[2][0] - m[0][0] * m[2][1], m[0][0] * m[1][1] - m[0][1] * m[1][0]}};
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) binv_e[a][r][c] = adj[r][c] / det;
+        }
+        double v[24], w[24];
+        for (int q = 0; q < 24; q++) v[q] = 1.0 + 0.1 * q;
+        for (int it = 0; it < 5000; it++) {
+            double kv[24];
+            for (int r = 0; r < 24; r++) {
+                double t = 0.0;
+                for (int c = 0; c < 24; c++) t += op->ke[r * 24 + c] * v[c];
+                kv[r] = t;
+            }
+            double nrm = 0.0;
+            for (int a = 0; a < 8; a++)
+                for (int r = 0; r < 3; r++) {
+                    double t = 0.0;
+                    for (int c = 0; c < 3; c++) t += binv_e[a][r][c] * kv[3 * a + c];
+                    w[3 * a + r] = t;
+                    nrm += t * t;
+                }
+            nrm = std::sqrt(nrm);
+            for (int q = 0; q < 24; q++) v[q] = w[q] / nrm;
+        }
+        double vkv = 0.0, vbv = 0.0;
+        for (int r = 0; r < 24; r++) {
+            double t = 0.0;
+            for (int c = 0; c < 24; c++) t += op->ke[r * 24 + c] * v[c];
+            vkv += v[r] * t;
+        }
+        for (int a = 0; a < 8; a++)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) vbv += v[3 * a + r] * op->ke[(3 * a + r) * 24 + 3 * a + c] * v[3 * a + c];
+        lmax = 1.001 * vkv / vbv;
+    }
     const double lmin = lmax / kSmoothRange, theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
     g->theta_inv = static_cast<float>(1.0 / theta);
@@ -1479,6 +1621,7 @@ static void gmg_build(mycel_op *op) {
                     static_cast<float>(m.h * scale), static_cast<float>(m.q * scale)};
         }
         L.kdiag = static_cast<float>(op->ke[0] * scale);
+        L.koff = static_cast<float>(op->ke[1] * scale);
         if (l == 0) {
             L.ep = op->d_ep32.get();
         } else {
@@ -1486,6 +1629,7 @@ static void gmg_build(mycel_op *op) {
             L.ep = L.ep_own.get();
         }
         L.inv.resize(d.n_dofs);
+        L.binv.resize(static_cast<size_t>(kBinv) * d.n_nodes);
         L.b.resize(d.n_dofs);
         L.x.resize(d.n_dofs);
         L.x2.resize(d.n_dofs);
@@ -1715,16 +1859,16 @@ static void gmg_tune_smoothing(mycel_op *op, Gmg *g, hipStream_t s) {
         const float cd = g->cd[1], cr = g->cr[1];
         if (kz == 2)
             hipLaunchKernelGGL((k_gmg_cheb_ec<2, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
         else if (kz == 3)
             hipLaunchKernelGGL((k_gmg_cheb_ec<3, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
         else if (kz == 4)
             hipLaunchKernelGGL((k_gmg_cheb_ec<4, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
         else
             hipLaunchKernelGGL((k_gmg_cheb_ec<8, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.inv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
         MYCEL_LAUNCH_CHECK();
     });
 }
@@ -1743,6 +1887,11 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
         hipLaunchKernelGGL(k_gmg_coarsen_e, dim3(cdiv(L.d.n_pad, 256)), b256, 0, s, P.d, P.ep, L.d, L.ep_own.get());
         hipLaunchKernelGGL(k_gmg_inv_coarse, dim3(cdiv(L.d.n_nodes, 256)), b256, 0, s, L.d, L.ep, L.kdiag, P.d,
                            P.inv.get(), L.inv.get());
+    }
+    for (int l = 0; l < g->levels; l++) {
+        GmgLevel &L = g->lv[l];
+        hipLaunchKernelGGL(k_gmg_binv, dim3(cdiv(L.d.n_nodes, 256)), b256, 0, s, L.d, L.ep, L.kdiag, L.koff,
+                           L.inv.get(), L.binv.get());
     }
     GmgLevel &C = g->lv[g->levels - 1];
     const int nc = C.d.n_dofs;
@@ -1774,7 +1923,7 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
 
 GmgFineView gmg_fine_view(mycel_op *op) {
     GmgLevel &F = op->gmg->lv[0];
-    return GmgFineView{F.b.get(), F.x.get(), F.dv.get(), F.inv.get(), op->gmg->theta_inv};
+    return GmgFineView{F.b.get(), F.x.get(), F.dv.get(), F.inv.get(), F.binv.get(), op->gmg->theta_inv};
 }
 
 // The tail kernel's arguments: levels [t0, top] run in the single-workgroup tail (the finest level never does: it
@@ -1782,7 +1931,7 @@ GmgFineView gmg_fine_view(mycel_op *op) {
 // each level's current iterate and scratch before the tail; tail_lds: the dynamic LDS bytes of its launch.
 static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nxt, const int *halt, int halt_stride,
                                int &t0, size_t &tail_lds) {
-    auto tail_floats = [&](int l) { return g->lv[l].d.n_pad + 6 * g->lv[l].d.n_dofs; };
+    auto tail_floats = [&](int l) { return g->lv[l].d.n_pad + 8 * g->lv[l].d.n_dofs; };
     auto tail_fits = [&](int l) {
         const Dims &d = g->lv[l].d;
         return d.n_dofs <= kTailPer * kTailThreads && d.n_pad <= kTailPer * kTailThreads;
@@ -1817,7 +1966,10 @@ static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nx
         T.o_x2 = off + 3 * nd;
         T.o_d = off + 4 * nd;
         T.o_r = off + 5 * nd;
-        off += 6 * nd;
+        T.o_binv = off + 6 * nd;  // kBinv per node = 2 per DOF
+        T.kd = L.kdiag;
+        T.ko = L.koff;
+        off += 8 * nd;
     }
     // The coarsest inverse (FP32) joins the LDS image when it fits.
     const int nc = g->lv[top].d.n_dofs;
@@ -1890,25 +2042,25 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             if (kz == 2)
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<2, true> : k_gmg_cheb_ec<2, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
             else if (kz == 3)
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<3, true> : k_gmg_cheb_ec<3, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
             else if (kz == 4)
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<4, true> : k_gmg_cheb_ec<4, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
             else
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<8, true> : k_gmg_cheb_ec<8, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.inv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
         } else if (split && l > 0)
             hipLaunchKernelGGL(k_gmg_cheb_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
+                               L.dv.get(), L.binv.get(), cd, cr, nxt[l]);
         else
             hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.dv.get(), L.inv.get(), cd, cr, nxt[l]);
+                               L.dv.get(), L.binv.get(), cd, cr, nxt[l]);
         std::swap(cur[l], nxt[l]);
     };
     int t0 = 0;
@@ -1917,14 +2069,15 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
 
     GmgLevel &F = g->lv[0];
     if (!fine_ready)
-        hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_dofs, 256), R), b256, 0, s, F.d.n_dofs, r64,
-                           F.inv.get(), g->theta_inv, F.b.get(), cur[0], F.dv.get());
+        hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_nodes, 256), R), b256, 0, s, F.d.n_nodes, r64,
+                           F.inv.get(), F.binv.get(), g->theta_inv, F.b.get(), cur[0], F.dv.get());
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
         if (l > 0 && fuse_rr()) {
             hipLaunchKernelGGL(k_gmg_residual_restrict, dim3(cdiv(C.d.n_nodes, 8), R), b256, 0, s, L.d, L.ks, L.ep,
-                               cur[l], L.b.get(), L.inv.get(), C.d, C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1],
+                               cur[l], L.b.get(), L.inv.get(), C.d, C.inv.get(), C.binv.get(), g->theta_inv, C.b.get(),
+                               cur[l + 1],
                                C.dv.get());
             continue;
         }
@@ -1953,8 +2106,8 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         else
             hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                                L.inv.get(), L.r.get());
-        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_dofs, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
-                           C.inv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
+        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_nodes, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
+                           C.inv.get(), C.binv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
     if (t0 > 0) {
         // The tail prolongs into level t0 - 1 itself (one launch fewer): into that level's current iterate as
@@ -1980,26 +2133,26 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     if (fkz == 2)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<2, true> : k_gmg_cheb_final_ec<2, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else if (fkz == 3)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<3, true> : k_gmg_cheb_final_ec<3, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else if (fkz == 4)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<4, true> : k_gmg_cheb_final_ec<4, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else if (fkz == 8)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<8, true> : k_gmg_cheb_final_ec<8, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else
         hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
-                           F.dv.get(), F.inv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, df);
+                           F.dv.get(), F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, df);
     MYCEL_LAUNCH_CHECK();
 }
 
@@ -2055,6 +2208,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
         a.x = cur[0];
         a.b = F.b.get();
         a.inv = F.inv.get();
+        a.binv = F.binv.get();
         a.d = F.dv.get();
         a.out = outv;
         a.cd = cd;
@@ -2075,7 +2229,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.ep = L.ep;
             a.x = cur[l];
             a.b = L.b.get();
-            a.inv = L.inv.get();
+            a.binv = L.binv.get();
             a.d = L.dv.get();
             a.x_out = nxt[l];
             a.cd = cd;
@@ -2101,6 +2255,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.bf = L.b.get();
             a.invf = L.inv.get();
             a.invc = C.inv.get();
+            a.binvc = C.binv.get();
             a.bc = C.b.get();
             a.xcv = cur[l + 1];
             a.dc = C.dv.get();
@@ -2114,11 +2269,12 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.gc = C.d;
             a.rf = L.r.get();
             a.invc = C.inv.get();
+            a.binvc = C.binv.get();
             a.bc = C.b.get();
             a.xcv = cur[1];
             a.dc = C.dv.get();
             a.theta_inv = g->theta_inv;
-            a.nb = cdiv(C.d.n_dofs, 256);
+            a.nb = cdiv(C.d.n_nodes, 256);
             out.push_back(pk_rec(key_down(0, 41), kPkRestrictDof, 0, a.nb, a));
         }
     }

@@ -278,7 +278,8 @@ uint64_t gmg_generation(const mycel_op *op);
 // The finest level's V-cycle inputs, for a caller that fuses their initialisation into its own kernel.
 struct GmgFineView {
     float *b, *x, *d;
-    const float *inv;
+    const float *inv;   // D^-1 of the finest level: 0 on constrained DOFs (the mask)
+    const float *binv;  // the smoother's nodal block inverses (6 per node: xx, yy, zz, xy, xz, yz)
     float theta_inv;
 };
 GmgFineView gmg_fine_view(mycel_op *op);

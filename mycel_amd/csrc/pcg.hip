@@ -70,6 +70,7 @@ struct PcgState {
 struct PcgWork {
     int n = 0;
     int nb_mv = 0, nb_vec = 0;
+    int nb_rr = 0;                           // blocks (= r.r partials) of the flexible x/r update: kXrNodes nodes each
     DevBuf<double> r, z, p, ap;              // FP64 iteration vectors (r / ap double as r64 / A x64 when mixed)
     DevBuf<double> p2;                       // FP64 iterations with the fused step 4: p of the odd ones
     DevBuf<float> r32, z32, p32, ap32, x32;  // FP32 iteration vectors of the mixed path
@@ -300,25 +301,34 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
 // Flexible PCG, step 2, fused with the V-cycle's first step: alpha = rz / p.Ap (the breakdown tests of
 // PcgSolver.cs:245-270; alpha is kept in the state for the flexible beta); x += alpha p; r -= alpha Ap;
 // per-block partials of r.r; then, on the updated residual, b0 = float(r) (0 on constrained DOFs) and
-// x0 = d0 = D^-1 b0 / theta, the finest Chebyshev smoother's first step.
-// Block bx of one right-hand side.
+// x0 = d0 = B^-1 b0 / theta, the finest Chebyshev smoother's first step (B^-1: the nodal block inverses).
+// Block bx of one right-hand side: kXrNodes nodes, two per thread (the smoother's first step needs a node's three
+// components together).
+constexpr int kXrNodes = 512;
 __device__ __forceinline__ void fpcg_update_xr_body(int n, int slot, PcgState *__restrict__ st,
                                                     const double *__restrict__ part_pap, int nb_mv,
                                                     double *__restrict__ x, double *__restrict__ r,
                                                     const double *__restrict__ p, const double *__restrict__ ap,
                                                     double *__restrict__ part_rr, const GmgFineView &fv, int bx) {
     __shared__ double red[4];
-    const int base = bx * kVecPerBlock + threadIdx.x;
-    double xv[4], pv[4], rv[4], av[4];
-    float iv[4];
+    const int nn = n / 3;
+    const int base = bx * kXrNodes + threadIdx.x;
+    double xv[2][3], pv[2][3], rv[2][3], av[2][3];
+    float iv[2][3], bi[2][6];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int i = min(base + 256 * q, n - 1);
-        xv[q] = x[i];
-        pv[q] = p[i];
-        rv[q] = r[i];
-        av[q] = ap[i];
-        iv[q] = fv.inv[i];
+    for (int q = 0; q < 2; q++) {
+        const int node = min(base + 256 * q, nn - 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int i = 3 * node + c;
+            xv[q][c] = x[i];
+            pv[q][c] = p[i];
+            rv[q][c] = r[i];
+            av[q][c] = ap[i];
+            iv[q][c] = fv.inv[i];
+        }
+#pragma unroll
+        for (int t = 0; t < 6; t++) bi[q][t] = fv.binv[6 * node + t];
     }
     const int halt = st->halt;  // read ahead of the partial sum, as in k_pcg_update_xr
     const double rz0 = st->rz[slot];
@@ -336,18 +346,31 @@ __device__ __forceinline__ void fpcg_update_xr_body(int n, int slot, PcgState *_
     if (bx == 0 && threadIdx.x == 0) st->alpha[slot] = alpha;
     double rr = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int i = base + 256 * q;
-        if (i < n) {
-            x[i] = fmad(alpha, pv[q], xv[q]);
-            const double ri = fmad(-alpha, av[q], rv[q]);
-            r[i] = ri;
-            rr = __fma_rn(ri, ri, rr);
-            const float bi = iv[q] > 0.0f ? static_cast<float>(ri) : 0.0f;
-            const float x0 = (iv[q] * bi) * fv.theta_inv;
-            fv.b[i] = bi;
-            fv.x[i] = x0;
-            fv.d[i] = x0;
+    for (int q = 0; q < 2; q++) {
+        const int node = base + 256 * q;
+        if (node < nn) {
+            float b0[3], z[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int i = 3 * node + c;
+                x[i] = fmad(alpha, pv[q][c], xv[q][c]);
+                const double ri = fmad(-alpha, av[q][c], rv[q][c]);
+                r[i] = ri;
+                rr = __fma_rn(ri, ri, rr);
+                b0[c] = iv[q][c] > 0.0f ? static_cast<float>(ri) : 0.0f;
+            }
+            // z = B^-1 b0 as the multigrid's bapply forms it (xx, yy, zz, xy, xz, yz)
+            z[0] = (bi[q][0] * b0[0] + bi[q][3] * b0[1]) + bi[q][4] * b0[2];
+            z[1] = (bi[q][3] * b0[0] + bi[q][1] * b0[1]) + bi[q][5] * b0[2];
+            z[2] = (bi[q][4] * b0[0] + bi[q][5] * b0[1]) + bi[q][2] * b0[2];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int i = 3 * node + c;
+                const float x0 = z[c] * fv.theta_inv;
+                fv.b[i] = b0[c];
+                fv.x[i] = x0;
+                fv.d[i] = x0;
+            }
         }
     }
     rr = block_sum256(rr, red);
@@ -799,6 +822,7 @@ void ensure_work(mycel_op *op) {
     w->n = op->d.n_dofs;
     w->nb_mv = apply_k_blocks(op);
     w->nb_vec = cdiv(w->n, kVecPerBlock);
+    w->nb_rr = cdiv(w->n / 3, kXrNodes);
     w->r.resize(w->n);
     w->z.resize(w->n);
     w->p.resize(w->n);
@@ -842,8 +866,9 @@ void launch_step4_ap(mycel_op *op, const KeMono<T> &km, const T *ep, const T *p,
     const int nt = ec_tiles(op->d, kz);
     // update_xr reads the partials of this launch as those of the automatic K*u form
     if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "fused step 4: partial count mismatch");
+    // part2: the interleaved (r.r, r.z) partials of the Jacobi update (nb_vec), or the flexible update's r.r (nb_rr)
     const PcgNextP<T, FLEX> src{p, z, w->state.get(), w->part2.get(), w->part_zr.get(), w->part_zap.get(),
-                                w->nb_vec, w->nb_z, slot, 0.0, T(0), Step4In{}, w->bf16};
+                                FLEX ? w->nb_rr : w->nb_vec, w->nb_z, slot, 0.0, T(0), Step4In{}, w->bf16};
     const dim3 grid(cdiv(nt, 4)), b256(256);
     const uint8_t *fl = op->d_flags.get();
     double *part = w->part_pap.get();
@@ -922,7 +947,7 @@ void ensure_gmg(mycel_op *op, PcgWork *w) {
 void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipStream_t s) {
     PcgState *st = w->state.get();
     if (fuse_step4()) {
-        hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+        hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_rr), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                            w->nb_mv, d_x, w->r.get(), slot ? w->p2.get() : w->p.get(), w->ap.get(), w->part2.get(),
                            gmg_fine_view(op));
         gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true, 1,
@@ -931,11 +956,11 @@ void enqueue_iteration_gmg(mycel_op *op, PcgWork *w, double *d_x, int slot, hipS
         return;
     }
     launch_apply_k(op, w->p.get(), w->ap.get(), kApplyAProjected, s, w->part_pap.get(), &st->halt);
-    hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
+    hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_rr), dim3(256), 0, s, w->n, slot, st, w->part_pap.get(),
                        w->nb_mv, d_x, w->r.get(), w->p.get(), w->ap.get(), w->part2.get(), gmg_fine_view(op));
     gmg_enqueue_vcycle(op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(), s, true, 1,
                        &st->halt);
-    hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(), w->nb_vec,
+    hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec), dim3(256), 0, s, w->n, slot, st, w->part2.get(), w->nb_rr,
                        w->part_zr.get(), w->part_zap.get(), w->nb_z, w->p.get(), w->z.get());
 }
 
@@ -969,13 +994,13 @@ void enqueue_iteration_gmg_batch(mycel_op *op, PcgWork *w, double *d_X, int R, i
     const int words = static_cast<int>(sizeof(PcgState) / sizeof(int));
     const Batch bt{R, static_cast<long long>(w->n), nb_node, words};
     launch_apply_k(op, w->bp.get(), w->bap.get(), kApplyAProjected, s, w->bpart_pap.get(), &st->halt, bt);
-    hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_vec, R), dim3(256), 0, s, w->n, slot, st,
+    hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_rr, R), dim3(256), 0, s, w->n, slot, st,
                        w->bpart_pap.get(), nb_node, d_X, w->br.get(), w->bp.get(), w->bap.get(), w->bpart_rr.get(),
                        gmg_fine_view(op));
     gmg_enqueue_vcycle(op, w->br.get(), w->bap.get(), w->bz.get(), w->bpart_zr.get(), w->bpart_zap.get(), s, true,
                        R, &st->halt, words);
     hipLaunchKernelGGL(k_fpcg_update_p, dim3(w->nb_vec, R), dim3(256), 0, s, w->n, slot, st, w->bpart_rr.get(),
-                       w->nb_vec, w->bpart_zr.get(), w->bpart_zap.get(), w->nb_z, w->bp.get(), w->bz.get());
+                       w->nb_rr, w->bpart_zr.get(), w->bpart_zap.get(), w->nb_z, w->bp.get(), w->bz.get());
 }
 
 void read_state(PcgWork *w, hipStream_t s) {
@@ -1358,16 +1383,16 @@ struct PackTables {
             PcgState *st = w->state.get();
             gm[q] = GmgPackMember{op, w->r.get(), w->ap.get(), w->z.get(), w->part_zr.get(), w->part_zap.get(),
                                   &st->halt};
-            xr[q] = PkXr{w->n, w->nb_mv, w->nb_vec, st, w->part_pap.get(), xs[q], w->r.get(), {w->p.get(), w->p2.get()},
+            xr[q] = PkXr{w->n, w->nb_mv, w->nb_rr, st, w->part_pap.get(), xs[q], w->r.get(), {w->p.get(), w->p2.get()},
                          w->ap.get(), w->part2.get(), gmg_fine_view(op)};
-            if (w->nb_vec > xr_grid) {
-                xr_grid = w->nb_vec;
+            if (w->nb_rr > xr_grid) {
+                xr_grid = w->nb_rr;
                 grown = true;
             }
             const int kz = ku_element_layers(op), nt = ec_tiles(op->d, kz);
             if (cdiv(nt, 4) != w->nb_mv) throw Error(MYCEL_E_INVALID_OPERATION, "pack: partial count mismatch");
             st4[q] = {kz, PkStep4{op->d, op->km64, op->d_ep.get(), op->d_flags.get(), {w->p.get(), w->p2.get()},
-                                  w->z.get(), st, w->part2.get(), w->part_zr.get(), w->part_zap.get(), w->nb_vec,
+                                  w->z.get(), st, w->part2.get(), w->part_zr.get(), w->part_zap.get(), w->nb_rr,
                                   w->nb_z, w->ap.get(), w->part_pap.get(), nt, cdiv(nt, 4)}};
             auto g = std::find_if(s4.begin(), s4.end(), [&](const Step4Group &x) { return x.kz == kz; });
             if (g == s4.end()) {
