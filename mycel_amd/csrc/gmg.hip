@@ -194,7 +194,9 @@ __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__res
 // B^-1 at node n of a level from its zero-padded moduli ep: the node's 3x3 diagonal block is the sum over its <= 8
 // elements of E_e times that element's corner block, kd on the diagonal and ko s_i s_j off it (s = +1 / -1 for a
 // corner on the low / high side of the element along that axis). Constrained components (inv == 0) get zero rows and
-// columns; the free ones the inverse of their sub-block (formed in FP64, stored in FP32).
+// columns; the free ones the inverse of their sub-block (formed in R: FP64 in the set-up kernel, FP32 in the tail,
+// which forms its levels' inverses in LDS on every launch), stored in FP32.
+template <typename R>
 __device__ __forceinline__ void binv_node(const Dims &g, const float *__restrict__ ep, float kd, float ko,
                                           const float *__restrict__ inv, int n, float (&bi)[kBinv]) {
     int i, j, k;
@@ -214,22 +216,21 @@ __device__ __forceinline__ void binv_node(const Dims &g, const float *__restrict
 #pragma unroll
     for (int c = 0; c < 3; c++) f[c] = inv[3 * n + c] > 0.0f;
     // Constrained rows and columns replaced by the identity, inverted, then zeroed again.
-    const double dg = static_cast<double>(kd) * sd;
-    double m[3][3] = {{dg, static_cast<double>(ko) * sxy, static_cast<double>(ko) * sxz},
-                      {static_cast<double>(ko) * sxy, dg, static_cast<double>(ko) * syz},
-                      {static_cast<double>(ko) * sxz, static_cast<double>(ko) * syz, dg}};
+    const R dg = static_cast<R>(kd) * static_cast<R>(sd);
+    R m[3][3] = {{dg, static_cast<R>(ko) * static_cast<R>(sxy), static_cast<R>(ko) * static_cast<R>(sxz)},
+                 {static_cast<R>(ko) * static_cast<R>(sxy), dg, static_cast<R>(ko) * static_cast<R>(syz)},
+                 {static_cast<R>(ko) * static_cast<R>(sxz), static_cast<R>(ko) * static_cast<R>(syz), dg}};
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
         for (int c = 0; c < 3; c++)
-            if (!f[r] || !f[c]) m[r][c] = r == c ? 1.0 : 0.0;
-    const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
-                 c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
-    const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
-    const double id = det > 0.0 ? 1.0 / det : 0.0;
-    const double q[6] = {c00 * id, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id,
-                         (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id, c01 * id, c02 * id,
-                         (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id};
+            if (!f[r] || !f[c]) m[r][c] = r == c ? R(1) : R(0);
+    const R c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
+            c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const R det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    const R id = det > R(0) ? R(1) / det : R(0);
+    const R q[6] = {c00 * id, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id, (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id,
+                    c01 * id, c02 * id, (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id};
     // (xx, yy, zz, xy, xz, yz): entry (r, c) is kept when both components are free.
     const bool keep[6] = {f[0], f[1], f[2], f[0] && f[1], f[0] && f[2], f[1] && f[2]};
 #pragma unroll
@@ -704,14 +705,50 @@ __global__ __launch_bounds__(256) void k_gmg_init_fine(int n, const double *__re
 }
 
 
-// Restriction from the finest level, one thread per coarse node (the block smoother's first step needs the node's
-// three components); the same sums in the same order as the tail's restrict_node.
+// Restriction from the finest level: four lanes per coarse node (64 nodes per block), lane c < 3 summing component
+// c over the 27 taps (all 27 loads first, then the sum: one memory round trip), lane 3 idle; the node's three sums
+// meet by lane shuffles and lane c writes component c of the coarse pre-smoother's first step b, x = d = B^-1 b /
+// theta. The same sums in the same order as the tail's restrict_node.
 __device__ __forceinline__ void restrict_dof_body(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
                                                   const float *__restrict__ invc, const float *__restrict__ binvc,
                                                   float theta_inv, float *__restrict__ bc, float *__restrict__ xcv,
                                                   float *__restrict__ dc, int bx) {
-    const int n = bx * 256 + threadIdx.x;
-    if (n < gc.n_nodes) restrict_node(gf, rf, gc, invc, binvc, theta_inv, bc, xcv, dc, n);
+    const int q = threadIdx.x & 3, c = q < 3 ? q : 2;
+    const int n = (bx * 256 + threadIdx.x) >> 2;
+    const bool live = n < gc.n_nodes;
+    float sum = 0.0f;
+    if (live) {
+        int I, J, K;
+        node_ijk(gc, n, I, J, K);
+        int ix[3], iy[3], iz[3];
+        float wx[3], wy[3], wz[3];
+        restrict_taps(I, gf.nx, ix, wx);
+        restrict_taps(J, gf.ny, iy, wy);
+        restrict_taps(K, gf.nz, iz, wz);
+        float v[27];
+#pragma unroll
+        for (int t = 0; t < 27; t++) v[t] = rf[3 * (ix[t % 3] + gf.nx1 * (iy[(t / 3) % 3] + gf.ny1 * iz[t / 9])) + c];
+#pragma unroll
+        for (int t = 0; t < 27; t++) {
+            const float w = wz[t / 9] * wy[(t / 3) % 3] * wx[t % 3];
+            sum += w != 0.0f ? w * v[t] : 0.0f;
+        }
+    }
+    const int lead = (threadIdx.x & 63) & ~3;
+    float bv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) bv[k] = __shfl(sum, lead + k, 64);  // every lane takes part
+    if (live && q < 3) {
+        float bi[kBinv], z[3];
+        bload(binvc, n, bi);
+#pragma unroll
+        for (int k = 0; k < 3; k++) bv[k] = invc[3 * n + k] > 0.0f ? bv[k] : 0.0f;
+        bapply(bi, bv, z);
+        const float xi = z[q] * theta_inv;
+        bc[3 * n + q] = bv[q];
+        xcv[3 * n + q] = xi;
+        dc[3 * n + q] = xi;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_gmg_restrict_dof(Dims gf, const float *__restrict__ rf, Dims gc,
@@ -798,7 +835,7 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
 // (or, when lv[0] is the finest level, writes lv[0]'s iterate back). A chain of global accesses would cost a
 // memory round trip (~1 us) per phase instead. lv[count-1] is the coarsest level, solved with its dense FP32
 // inverse (column-major; staged into LDS with the rest when it fits).
-constexpr int kMaxTail = 8;
+constexpr int kMaxTail = 4;  // levels in the tail: from <= 512 nodes to the dense coarsest one it is at most 3
 constexpr int kTailThreads = 1024;   // 16 waves (up to 128 VGPRs each): two lanes per node on the tail levels
 constexpr int kTailPer = 2;          // a tail level's arrays hold <= kTailPer * kTailThreads floats each
 constexpr int kTailNodes = 512;      // levels up to this many nodes run in the tail (larger ones are FMA-bound
@@ -818,10 +855,9 @@ struct TailLevel {
     const float *ep;   // padded moduli (global)
     const float *inv;  // inverse diagonal (global): the constraint mask
     int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
-    int o_binv;        // and of its nodal block inverses, formed in LDS from the moduli on entry
-    float kd, ko;      // the level's corner-block values (binv_node)
+    int o_binv;        // and of its nodal block inverses (staged with the rest)
 };
-constexpr int kTailSegs = 3 + 2 * kMaxTail;
+constexpr int kTailSegs = 3 + 4 * kMaxTail;
 constexpr int kAinvPer = (kCoarseMaxDofs * kCoarseMaxDofs + kTailThreads - 1) / kTailThreads;
 // One array staged into LDS on entry: n floats from src (+ the RHS offset for the first three) to LDS offset dst.
 struct TailSeg {
@@ -911,17 +947,6 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
             if (i < nn) al[i] = va[u];
         }
         if (halted) return;  // the same value in every thread of the workgroup
-    }
-    __syncthreads();
-    // Every tail level's nodal block inverses, from its moduli and constraints now in LDS.
-    for (int l = 0; l < t.count; l++) {
-        const TailLevel &L = t.lv[l];
-        for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads) {
-            float bi[kBinv];
-            binv_node(L.d, sm + L.o_e, L.kd, L.ko, sm + L.o_inv, n, bi);
-#pragma unroll
-            for (int q = 0; q < kBinv; q++) sm[L.o_binv + kBinv * n + q] = bi[q];
-        }
     }
     __syncthreads();
     // Bit l of `flip` set: level l's current iterate is in x2 (its scratch in x).
@@ -1243,7 +1268,7 @@ __global__ __launch_bounds__(256) void k_gmg_binv(Dims g, const float *__restric
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= g.n_nodes) return;
     float bi[kBinv];
-    binv_node(g, ep, kd, ko, inv, n, bi);
+    binv_node<double>(g, ep, kd, ko, inv, n, bi);
 #pragma unroll
     for (int t = 0; t < kBinv; t++) binv[kBinv * n + t] = bi[t];
 }
@@ -1967,8 +1992,6 @@ static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nx
         T.o_d = off + 4 * nd;
         T.o_r = off + 5 * nd;
         T.o_binv = off + 6 * nd;  // kBinv per node = 2 per DOF
-        T.kd = L.kdiag;
-        T.ko = L.koff;
         off += 8 * nd;
     }
     // The coarsest inverse (FP32) joins the LDS image when it fits.
@@ -1984,10 +2007,13 @@ static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nx
         ta.seg[2] = TailSeg{g->lv[t0].dv.get(), T0.d.n_dofs, T0.o_d};
         for (int l = 0; l < ta.count; l++) {
             const TailLevel &T = ta.lv[l];
-            ta.seg[3 + 2 * l] = TailSeg{T.ep, T.d.n_pad, T.o_e};
-            ta.seg[4 + 2 * l] = TailSeg{T.inv, T.d.n_dofs, T.o_inv};
+            const float *bv = g->lv[t0 + l].binv.get();  // kBinv per node = 2 n_dofs floats, staged in two halves
+            ta.seg[3 + 4 * l] = TailSeg{T.ep, T.d.n_pad, T.o_e};
+            ta.seg[4 + 4 * l] = TailSeg{T.inv, T.d.n_dofs, T.o_inv};
+            ta.seg[5 + 4 * l] = TailSeg{bv, T.d.n_dofs, T.o_binv};
+            ta.seg[6 + 4 * l] = TailSeg{bv + T.d.n_dofs, T.d.n_dofs, T.o_binv + T.d.n_dofs};
         }
-        for (int sg = 3 + 2 * ta.count; sg < kTailSegs; sg++) ta.seg[sg] = TailSeg{ta.seg[0].src, 0, 0};
+        for (int sg = 3 + 4 * ta.count; sg < kTailSegs; sg++) ta.seg[sg] = TailSeg{ta.seg[0].src, 0, 0};
     }
     ta.out0 = ta.count > 1 ? nxt[t0] : cur[t0];  // the host-side swap below names it the current iterate
     ta.ainv = g->ainv.get();
@@ -2106,7 +2132,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         else
             hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
                                L.inv.get(), L.r.get());
-        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_nodes, 256), R), b256, 0, s, L.d, L.r.get(), C.d,
+        hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_nodes, 64), R), b256, 0, s, L.d, L.r.get(), C.d,
                            C.inv.get(), C.binv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
     if (t0 > 0) {
@@ -2274,7 +2300,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.xcv = cur[1];
             a.dc = C.dv.get();
             a.theta_inv = g->theta_inv;
-            a.nb = cdiv(C.d.n_nodes, 256);
+            a.nb = cdiv(C.d.n_nodes, 64);
             out.push_back(pk_rec(key_down(0, 41), kPkRestrictDof, 0, a.nb, a));
         }
     }

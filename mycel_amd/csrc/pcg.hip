@@ -302,33 +302,35 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
 // PcgSolver.cs:245-270; alpha is kept in the state for the flexible beta); x += alpha p; r -= alpha Ap;
 // per-block partials of r.r; then, on the updated residual, b0 = float(r) (0 on constrained DOFs) and
 // x0 = d0 = B^-1 b0 / theta, the finest Chebyshev smoother's first step (B^-1: the nodal block inverses).
-// Block bx of one right-hand side: kXrNodes nodes, two per thread (the smoother's first step needs a node's three
-// components together).
-constexpr int kXrNodes = 512;
+// Block bx of one right-hand side: kXrNodes nodes, four lanes per node (lane c < 3 owns the node's component c,
+// lane 3 idle) and four nodes per lane group (nodes g, g + 64, g + 128, g + 192 of the block), so the vector loads
+// and stores stay nearly contiguous while the smoother's first step, which needs a node's three components, meets
+// them by lane shuffles.
+constexpr int kXrNodes = 256;
 __device__ __forceinline__ void fpcg_update_xr_body(int n, int slot, PcgState *__restrict__ st,
                                                     const double *__restrict__ part_pap, int nb_mv,
                                                     double *__restrict__ x, double *__restrict__ r,
                                                     const double *__restrict__ p, const double *__restrict__ ap,
                                                     double *__restrict__ part_rr, const GmgFineView &fv, int bx) {
     __shared__ double red[4];
-    const int nn = n / 3;
-    const int base = bx * kXrNodes + threadIdx.x;
-    double xv[2][3], pv[2][3], rv[2][3], av[2][3];
-    float iv[2][3], bi[2][6];
+    constexpr int Q = kXrNodes / 64;
+    const int nn = n / 3, q = threadIdx.x & 3, c = q < 3 ? q : 2;
+    const int g = threadIdx.x >> 2, base = bx * kXrNodes;
+    // Row c of the node's block inverse (xx, yy, zz, xy, xz, yz): (xx, xy, xz), (xy, yy, yz) or (xz, yz, zz).
+    const int k0 = c == 0 ? 0 : (c == 1 ? 3 : 4), k1 = c == 0 ? 3 : (c == 1 ? 1 : 5), k2 = c == 0 ? 4 : (c == 1 ? 5 : 2);
+    double xv[Q], pv[Q], rv[Q], av[Q];
+    float iv[Q], b0r[Q], b1r[Q], b2r[Q];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const int node = min(base + 256 * q, nn - 1);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int i = 3 * node + c;
-            xv[q][c] = x[i];
-            pv[q][c] = p[i];
-            rv[q][c] = r[i];
-            av[q][c] = ap[i];
-            iv[q][c] = fv.inv[i];
-        }
-#pragma unroll
-        for (int t = 0; t < 6; t++) bi[q][t] = fv.binv[6 * node + t];
+    for (int u = 0; u < Q; u++) {
+        const int node = min(base + g + 64 * u, nn - 1), i = 3 * node + c;
+        xv[u] = x[i];
+        pv[u] = p[i];
+        rv[u] = r[i];
+        av[u] = ap[i];
+        iv[u] = fv.inv[i];
+        b0r[u] = fv.binv[6 * node + k0];
+        b1r[u] = fv.binv[6 * node + k1];
+        b2r[u] = fv.binv[6 * node + k2];
     }
     const int halt = st->halt;  // read ahead of the partial sum, as in k_pcg_update_xr
     const double rz0 = st->rz[slot];
@@ -344,33 +346,25 @@ __device__ __forceinline__ void fpcg_update_xr_body(int n, int slot, PcgState *_
         return;
     }
     if (bx == 0 && threadIdx.x == 0) st->alpha[slot] = alpha;
+    const int lead = (threadIdx.x & 63) & ~3;
     double rr = 0.0;
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const int node = base + 256 * q;
-        if (node < nn) {
-            float b0[3], z[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const int i = 3 * node + c;
-                x[i] = fmad(alpha, pv[q][c], xv[q][c]);
-                const double ri = fmad(-alpha, av[q][c], rv[q][c]);
-                r[i] = ri;
-                rr = __fma_rn(ri, ri, rr);
-                b0[c] = iv[q][c] > 0.0f ? static_cast<float>(ri) : 0.0f;
-            }
-            // z = B^-1 b0 as the multigrid's bapply forms it (xx, yy, zz, xy, xz, yz)
-            z[0] = (bi[q][0] * b0[0] + bi[q][3] * b0[1]) + bi[q][4] * b0[2];
-            z[1] = (bi[q][3] * b0[0] + bi[q][1] * b0[1]) + bi[q][5] * b0[2];
-            z[2] = (bi[q][4] * b0[0] + bi[q][5] * b0[1]) + bi[q][2] * b0[2];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const int i = 3 * node + c;
-                const float x0 = z[c] * fv.theta_inv;
-                fv.b[i] = b0[c];
-                fv.x[i] = x0;
-                fv.d[i] = x0;
-            }
+    for (int u = 0; u < Q; u++) {
+        const int node = base + g + 64 * u, i = 3 * node + c;
+        const bool own = q < 3 && node < nn;
+        const double ri = fmad(-alpha, av[u], rv[u]);
+        const float bq = iv[u] > 0.0f ? static_cast<float>(ri) : 0.0f;
+        // b0 of the node's three components (every lane takes part in the shuffles), then z_c = row c . b0
+        const float s0 = __shfl(bq, lead, 64), s1 = __shfl(bq, lead + 1, 64), s2 = __shfl(bq, lead + 2, 64);
+        const float z = (b0r[u] * s0 + b1r[u] * s1) + b2r[u] * s2;
+        if (own) {
+            x[i] = fmad(alpha, pv[u], xv[u]);
+            r[i] = ri;
+            rr = __fma_rn(ri, ri, rr);
+            const float x0 = z * fv.theta_inv;
+            fv.b[i] = bq;
+            fv.x[i] = x0;
+            fv.d[i] = x0;
         }
     }
     rr = block_sum256(rr, red);
