@@ -307,6 +307,9 @@ __global__ __launch_bounds__(256) void k_pcg_update_p(int n, int slot, PcgState 
 // and stores stay nearly contiguous while the smoother's first step, which needs a node's three components, meets
 // them by lane shuffles.
 constexpr int kXrNodes = 256;
+// Its r.r partials (one per block, nb_rr = ceil(n / 3 / kXrNodes)) go to the Jacobi partial buffer (2 per
+// kVecPerBlock DOFs) in single and packed solves, so they must not outnumber that buffer's slots.
+static_assert(2 * 3 * kXrNodes >= kVecPerBlock, "flexible x/r update: partials exceed the Jacobi partial buffer");
 __device__ __forceinline__ void fpcg_update_xr_body(int n, int slot, PcgState *__restrict__ st,
                                                     const double *__restrict__ part_pap, int nb_mv,
                                                     double *__restrict__ x, double *__restrict__ r,
@@ -967,7 +970,7 @@ void ensure_batch(mycel_op *op, PcgWork *w, int R) {
     w->bp.resize(n);
     w->bap.resize(n);
     w->bpart_pap.resize(static_cast<size_t>(nb_node) * R);
-    w->bpart_rr.resize(static_cast<size_t>(w->nb_vec) * R);
+    w->bpart_rr.resize(static_cast<size_t>(w->nb_rr) * R);  // the flexible x/r update's r.r: nb_rr per RHS
     w->bpart1.resize(static_cast<size_t>(w->nb_vec) * R);
     w->bpart_zr.resize(static_cast<size_t>(gmg_partials(op)) * R);
     w->bpart_zap.resize(static_cast<size_t>(gmg_partials(op)) * R);
@@ -987,6 +990,8 @@ void enqueue_iteration_gmg_batch(mycel_op *op, PcgWork *w, double *d_X, int R, i
     const int nb_node = apply_k_blocks(op);  // partials of the batched K*u launch
     const int words = static_cast<int>(sizeof(PcgState) / sizeof(int));
     const Batch bt{R, static_cast<long long>(w->n), nb_node, words};
+    if (w->bpart_rr.size() < static_cast<size_t>(w->nb_rr) * R)
+        throw Error(MYCEL_E_INVALID_OPERATION, "batched flexible update: partial buffer too small");
     launch_apply_k(op, w->bp.get(), w->bap.get(), kApplyAProjected, s, w->bpart_pap.get(), &st->halt, bt);
     hipLaunchKernelGGL(k_fpcg_update_xr_gmg, dim3(w->nb_rr, R), dim3(256), 0, s, w->n, slot, st,
                        w->bpart_pap.get(), nb_node, d_X, w->br.get(), w->bp.get(), w->bap.get(), w->bpart_rr.get(),

@@ -224,6 +224,30 @@ def test_gmg_batch_equals_single_solves(gpu):
     assert np.array_equal(X[3], single_x[3])
 
 
+def test_gmg_batch_of_16_equals_single_solves(gpu):
+    """Sixteen right-hand sides on a 60x30x30 grid (per-RHS partial sums of every batched kernel at full block
+    counts): each equals its single-RHS multigrid solve bit for bit."""
+    p = W.cantilever_simp(nx=60, ny=30, nz=30)
+    p.density = random_field(17, p.n_elem, 0.05, 1.0)
+    m = oracle_model(p)
+    f = np.where(m.constrained, 0.0, oracle_forces(p))
+    g = np.where(m.constrained, 0.0, random_field(18, p.n_dofs) - 0.5)
+    B = np.stack([f * (1.0 + 0.25 * k) + (k % 3) * g for k in range(16)])
+    op = FemOperator(p.fe_model())
+    try:
+        X = np.zeros_like(B)
+        rb = PcgSolver().SolveBatch(op, B, X, tol=1e-8, maxIterations=5000)
+        for k in (0, 7, 15):
+            x = np.zeros(p.n_dofs)
+            r = PcgSolver().Solve(op, B[k], x, MultigridPreconditioner(), tol=1e-8, maxIterations=5000)
+            assert rb[k].Converged and rb[k].Iterations == r.Iterations, k
+            assert np.array_equal(X[k], x), k
+    finally:
+        op.close()
+    print(f"[gmg] batch of 16 it={[r.Iterations for r in rb]}", flush=True)
+    assert all(r.Converged for r in rb)
+
+
 def test_simp_cases_multigrid_batched(gpu):
     """Multi-case SIMP with the multigrid solver (all cases in one lockstep solve per iteration) follows the
     oracle's FP64 Jacobi trajectory: both solve to 1e-10."""
