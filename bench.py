@@ -260,19 +260,21 @@ def gmg_levels(nx, ny, nz, coarse_max_dofs=140):
 def mg_iteration_model(nx, ny, nz, active_nodes, active_elements, tail_nodes=512):
     """Launches and algorithmic HBM bytes of one multigrid outer iteration, per kernel: [(kernel name prefix,
     launches per iteration, bytes per iteration)]. Every stream a launch reads or writes counts once (FP64 8 B,
-    FP32 4 B per DOF; moduli 8 B (FP64) or 4 B (FP32) per element). The finest level counts its active DOFs D0 and
+    FP32 4 B per DOF; moduli 8 B (FP64) or 4 B (FP32) per element; the nodal block inverse B^-1, 6 FP32 values
+    per node, 8 B per DOF; the constraint mask D^-1 4 B per DOF). The finest level counts its active DOFs D0 and
     elements E0, the coarse levels their box grids. Per launch:
-      k_fpcg_update_xr_gmg   read x, p, r, Ap, D^-1; write x, r (FP64) and the V-cycle's b, x0, d0 (FP32): 64 D0
-      k_gmg_cheb_ec (x4)     read x, b, d, D^-1; write x', d; the moduli: 24 D0 + 4 E0
+      k_fpcg_update_xr_gmg   read x, p, r, Ap, D^-1, B^-1; write x, r (FP64) and the V-cycle's b, x0, d0 (FP32):
+                             72 D0
+      k_gmg_cheb_ec (x4)     read x, b, d, B^-1; write x', d; the moduli: 28 D0 + 4 E0
       k_gmg_residual_ec      read x, b, D^-1; write r: 16 D0 + 4 E0
-      k_gmg_restrict_dof     read r; read the coarse D^-1 and write its b, x, d: 4 D0 + 16 D1
-      k_gmg_cheb_s (x3/level) levels 1 .. t0-1, as the Chebyshev step above: 24 D + 4 E
-      k_gmg_residual_restrict read x, b, D^-1 of level l and write level l+1's entry: 12 D_l + 4 E_l + 16 D_l+1
-      k_gmg_tail             stage level t0's b, x, d and every tail level's moduli and D^-1, the coarsest inverse
-                             (4 n^2 B), prolong into level t0-1 (read D^-1, read and write x): 12 D_t0 +
-                             sum(4 D + 4 E) + 4 n^2 + 12 D_t0-1
+      k_gmg_restrict_dof     read r; read the coarse D^-1 and B^-1 and write its b, x, d: 4 D0 + 24 D1
+      k_gmg_cheb_s (x3/level) levels 1 .. t0-1, as the Chebyshev step above: 28 D + 4 E
+      k_gmg_residual_restrict read x, b, D^-1 of level l and write level l+1's entry: 12 D_l + 4 E_l + 24 D_l+1
+      k_gmg_tail             stage level t0's b, x, d and every tail level's moduli, D^-1 and B^-1, the coarsest
+                             inverse (4 n^2 B), prolong into level t0-1 (read D^-1, read and write x): 12 D_t0 +
+                             sum(12 D + 4 E) + 4 n^2 + 12 D_t0-1
       k_gmg_prolong          levels t0-2 .. 0: read the coarse x, read D^-1 and x, write x: 12 D_l + 4 D_l+1
-      k_gmg_cheb_final_ec    read x, b, d, D^-1 (FP32), r and Ap (FP64); write z (FP64); moduli: 40 D0 + 4 E0
+      k_gmg_cheb_final_ec    read x, b, d, B^-1 (FP32), r and Ap (FP64); write z (FP64); moduli: 44 D0 + 4 E0
       k_pcg_step4_ap<double, true  read z, p; write p', Ap' (FP64); the FP64 moduli: 32 D0 + 8 E0
     Levels below t0 (the first level from which every level has <= 512 nodes) run inside the tail."""
     lv = gmg_levels(nx, ny, nz)
@@ -284,17 +286,17 @@ def mg_iteration_model(nx, ny, nz, active_nodes, active_elements, tail_nodes=512
     while t0 > 1 and lv[t0 - 1][0] <= tail_nodes:
         t0 -= 1
     nc = D[top]
-    rows = [("k_fpcg_update_xr_gmg", 1, 64 * D[0]),
-            ("k_gmg_cheb_ec<", 4, 4 * (24 * D[0] + 4 * E[0])),
+    rows = [("k_fpcg_update_xr_gmg", 1, 72 * D[0]),
+            ("k_gmg_cheb_ec<", 4, 4 * (28 * D[0] + 4 * E[0])),
             ("k_gmg_residual_ec<", 1, 16 * D[0] + 4 * E[0]),
-            ("k_gmg_restrict_dof", 1, 4 * D[0] + 16 * D[1])]
+            ("k_gmg_restrict_dof", 1, 4 * D[0] + 24 * D[1])]
     mid = range(1, t0)
-    rows.append(("k_gmg_cheb_s", 3 * len(mid), sum(3 * (24 * D[l] + 4 * E[l]) for l in mid)))
-    rows.append(("k_gmg_residual_restrict", len(mid), sum(12 * D[l] + 4 * E[l] + 16 * D[l + 1] for l in mid)))
-    rows.append(("k_gmg_tail", 1, 12 * D[t0] + sum(4 * D[l] + 4 * E[l] for l in range(t0, top + 1)) + 4 * nc * nc
+    rows.append(("k_gmg_cheb_s", 3 * len(mid), sum(3 * (28 * D[l] + 4 * E[l]) for l in mid)))
+    rows.append(("k_gmg_residual_restrict", len(mid), sum(12 * D[l] + 4 * E[l] + 24 * D[l + 1] for l in mid)))
+    rows.append(("k_gmg_tail", 1, 12 * D[t0] + sum(12 * D[l] + 4 * E[l] for l in range(t0, top + 1)) + 4 * nc * nc
                  + 12 * D[t0 - 1]))
     rows.append(("k_gmg_prolong", t0 - 1, sum(12 * D[l] + 4 * D[l + 1] for l in range(0, t0 - 1))))
-    rows.append(("k_gmg_cheb_final_ec<", 1, 40 * D[0] + 4 * E[0]))
+    rows.append(("k_gmg_cheb_final_ec<", 1, 44 * D[0] + 4 * E[0]))
     rows.append(("k_pcg_step4_ap<double, true", 1, 32 * D[0] + 8 * E[0]))
     return rows
 

@@ -47,7 +47,7 @@ namespace {
 
 constexpr int kCoarseMaxDofs = 140;  // 140^2 doubles = 156,800 B of LDS (gfx950: 160 KiB per workgroup)
 constexpr int kMaxLevels = 16;
-constexpr double kSmoothRange = 30.0;  // Chebyshev interval [lmax / 30, lmax]
+constexpr double kSmoothRange = 30.0;  // Chebyshev interval [lmax / 30, lmax] (MYCEL_GMG_SMOOTH_RANGE: 2 .. 1000)
 constexpr int kFineDegree = 3;          // Chebyshev degree on the finest level (2 on the coarser ones) ...
 constexpr int kMaxDegree = 8;           // ... unless MYCEL_GMG_FINE_DEGREE asks for another in [2, kMaxDegree]
 
@@ -55,6 +55,14 @@ int fine_degree() {
     static const int v = [] {
         const char *e = std::getenv("MYCEL_GMG_FINE_DEGREE");
         return e && *e ? std::min(std::max(2, std::atoi(e)), kMaxDegree) : kFineDegree;
+    }();
+    return v;
+}
+
+double smooth_range() {
+    static const double v = [] {
+        const char *e = std::getenv("MYCEL_GMG_SMOOTH_RANGE");
+        return e && *e ? std::min(std::max(2.0, std::atof(e)), 1000.0) : kSmoothRange;
     }();
     return v;
 }
@@ -194,11 +202,10 @@ __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__res
 // B^-1 at node n of a level from its zero-padded moduli ep: the node's 3x3 diagonal block is the sum over its <= 8
 // elements of E_e times that element's corner block, kd on the diagonal and ko s_i s_j off it (s = +1 / -1 for a
 // corner on the low / high side of the element along that axis). Constrained components (inv == 0) get zero rows and
-// columns; the free ones the inverse of their sub-block (formed in R: FP64 in the set-up kernel, FP32 in the tail,
-// which forms its levels' inverses in LDS on every launch), stored in FP32.
-template <typename R>
+// columns; the free ones the inverse of their sub-block, formed in FP64 and stored in FP32.
 __device__ __forceinline__ void binv_node(const Dims &g, const float *__restrict__ ep, float kd, float ko,
                                           const float *__restrict__ inv, int n, float (&bi)[kBinv]) {
+    using R = double;
     int i, j, k;
     node_ijk(g, n, i, j, k);
     const int pb = (i + 1) + g.px * ((j + 1) + g.py * (k + 1));
@@ -1268,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_gmg_binv(Dims g, const float *__restric
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= g.n_nodes) return;
     float bi[kBinv];
-    binv_node<double>(g, ep, kd, ko, inv, n, bi);
+    binv_node(g, ep, kd, ko, inv, n, bi);
 #pragma unroll
     for (int t = 0; t < kBinv; t++) binv[kBinv * n + t] = bi[t];
 }
@@ -1617,7 +1624,7 @@ static void gmg_build(mycel_op *op) {
                 for (int c = 0; c < 3; c++) vbv += v[3 * a + r] * op->ke[(3 * a + r) * 24 + 3 * a + c] * v[3 * a + c];
         lmax = 1.001 * vkv / vbv;
     }
-    const double lmin = lmax / kSmoothRange, theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double lmin = lmax / smooth_range(), theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
     g->theta_inv = static_cast<float>(1.0 / theta);
     double rho = 1.0 / sigma;
