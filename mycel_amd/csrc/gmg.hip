@@ -835,8 +835,9 @@ __global__ __launch_bounds__(256) void k_gmg_prolong(Dims gf, float *__restrict_
 }
 
 // The small levels of the V-cycle in one workgroup, resident in LDS. On entry every thread loads its share
-// of lv[0]'s b, x, d (written by the restriction from the level above) and of every tail level's moduli and
-// inverse diagonal, all loads in flight together (one memory round trip), and stores them into LDS; every
+// of lv[0]'s b, x, d (written by the restriction from the level above) and of the tail image (every tail level's
+// moduli, inverse diagonal and block inverses, and the coarsest inverse, in one buffer), all loads in flight
+// together (one memory round trip), and stores them into LDS; every
 // phase after that (smoothing, residual, restriction, coarsest solve, prolongation) reads and writes LDS
 // only, separated by workgroup barriers. At the end the tail prolongs lv[0]'s correction into the level above
 // (or, when lv[0] is the finest level, writes lv[0]'s iterate back). A chain of global accesses would cost a
@@ -864,20 +865,28 @@ struct TailLevel {
     int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
     int o_binv;        // and of its nodal block inverses (staged with the rest)
 };
-constexpr int kTailSegs = 3 + 4 * kMaxTail;
-constexpr int kAinvPer = (kCoarseMaxDofs * kCoarseMaxDofs + kTailThreads - 1) / kTailThreads;
-// One array staged into LDS on entry: n floats from src (+ the RHS offset for the first three) to LDS offset dst.
+constexpr int kImgPer = kTailLdsMax / 4 / kTailThreads;  // tail image floats per thread, at most
+// lv[0]'s b, x or d, staged into LDS on entry: n floats from src + the RHS offset to LDS offset dst.
 struct TailSeg {
     const float *src;
     int n, dst;
 };
+// The tail's levels [t0, t0 + count) and its LDS image: first the constant part, which is also the layout of the
+// hierarchy's tail image buffer (per level: padded moduli (not the coarsest level's: the tail never smooths it),
+// D^-1, B^-1; then the coarsest inverse if it fits), then per level the vectors b, x, x2, d, r.
+struct TailLayout {
+    int t0 = 0, count = 0;
+    int o_e[kMaxTail] = {}, o_inv[kMaxTail] = {}, o_binv[kMaxTail] = {}, o_vec[kMaxTail] = {};
+    int ainv = -1;  // offset of the coarsest inverse, or -1: read from global memory
+    int img_n = 0, lds_floats = 0;
+};
 struct TailArgs {
     // The staging table comes first and is contiguous, so its scalar loads merge and are all issued before
     // the first vector load (fields scattered over the levels cost one scalar-load latency each).
-    TailSeg seg[kTailSegs];  // lv[0]'s b, x, d (RHS 0; RHS k at k * n_dofs), then (moduli, D^-1) per level;
-                             // unused entries: a valid pointer and n = 0
+    TailSeg seg[3];          // lv[0]'s b, x, d (RHS 0; RHS k at k * n_dofs)
+    const float *img;        // the constant part of the LDS image (img_n floats to LDS offset 0)
+    int img_n;
     int ainv_lds;            // float offset of the coarsest inverse in LDS, or -1 to read it from global memory
-    int ainv_nn;             // entries of the inverse staged into LDS (n^2, or 0)
     const float *ainv;
     float *out0;             // where lv[0]'s corrected iterate goes (global, RHS layout as above), unless:
     float *pro_x;            // non-null: the level above lv[0] (its iterate, RHS layout as above), into which the
@@ -903,55 +912,54 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
     __shared__ double bsh[kCoarseMaxDofs];
     __shared__ double psum[kParts][kCoarseMaxDofs];
     // The arguments (~1.6 KB) are copied into LDS with one vector load per thread: read field by field they
-    // are scalar loads from the kernel-argument segment, each wait a memory latency, in every phase.
+    // are scalar loads from the kernel-argument segment, each wait a memory latency, in every phase. The few
+    // the staging needs are read as scalar loads first (the constant address space lets the compiler do so for
+    // the kernel arguments and for a pack's table alike), so that the staging loads are in flight before the
+    // copy, and the copy's round trip overlaps theirs.
     __shared__ TailArgs t;
-    if (threadIdx.x < sizeof(TailArgs) / 4)
-        reinterpret_cast<uint32_t *>(&t)[threadIdx.x] = reinterpret_cast<const uint32_t *>(targ)[threadIdx.x];
-    __syncthreads();
-    if (t.count == 0) return;  // an empty slot of a pack (the same value in every thread)
-    const long long o0 = static_cast<long long>(rhs) * t.lv[0].d.n_dofs;
+    const auto *ct = (const __attribute__((address_space(4))) TailArgs *)targ;
+    if (ct->count == 0) return;  // an empty slot of a pack (the same value in every thread)
+    const long long o0 = static_cast<long long>(rhs) * ct->lv[0].d.n_dofs;
     {
-        // Every segment holds <= kTailPer * kTailThreads floats (checked on the host); the coarsest inverse
-        // (n^2 doubles) joins the same round trip when it fits in LDS.
-        // Loads are unconditional at a clamped index (every table entry points at valid memory; unused entries
-        // have n = 0), so the staging is one basic block: the scheduler issues every scalar load of the
-        // table, then every vector load, and waits once.
-        float v[kTailSegs][kTailPer];
+        // The image (<= kImgPer * kTailThreads floats) and lv[0]'s vectors (<= kTailPer * kTailThreads each; both
+        // checked on the host) are loaded before the first LDS store, so every load is in flight together. A chunk
+        // of kTailThreads floats is loaded only if the array reaches it (a uniform branch), at a clamped index.
+        float v[3][kTailPer], w[kImgPer];
+        const float *img = gmem(ct->img);
+        const int img_n = ct->img_n;
 #pragma unroll
-        for (int sg = 0; sg < kTailSegs; sg++) {
-            const TailSeg g = t.seg[sg];
-            const float *src = g.src + (sg < 3 ? o0 : 0);
-            const int last = g.n > 0 ? g.n - 1 : 0;
+        for (int u = 0; u < kImgPer; u++) {
+            const int i = threadIdx.x + u * kTailThreads;
+            if (u * kTailThreads < img_n) w[u] = img[i < img_n - 1 ? i : img_n - 1];
+        }
+        int nv[3];
+#pragma unroll
+        for (int sg = 0; sg < 3; sg++) {
+            const float *src = gmem(ct->seg[sg].src) + o0;
+            const int n = nv[sg] = ct->seg[sg].n;
 #pragma unroll
             for (int u = 0; u < kTailPer; u++) {
                 const int i = threadIdx.x + u * kTailThreads;
-                const float x = src[i < last ? i : last];
-                v[sg][u] = i < g.n ? x : 0.0f;
+                if (u * kTailThreads < n) v[sg][u] = src[i < n - 1 ? i : n - 1];
             }
         }
-        float va[kAinvPer];
-        const int nn = t.ainv_nn;
+        const int *halt = ct->halt;
+        const int halted = halt ? gmem(halt)[rhs * ct->halt_stride] : 0;
+        if (threadIdx.x < sizeof(TailArgs) / 4)
+            reinterpret_cast<uint32_t *>(&t)[threadIdx.x] = reinterpret_cast<const uint32_t *>(targ)[threadIdx.x];
 #pragma unroll
-        for (int u = 0; u < kAinvPer; u++) {
+        for (int u = 0; u < kImgPer; u++) {
             const int i = threadIdx.x + u * kTailThreads;
-            va[u] = t.ainv[i < nn ? i : 0];
+            if (u * kTailThreads < img_n && i < img_n) sm[i] = w[u];
         }
-        const int halted = t.halt ? t.halt[rhs * t.halt_stride] : 0;
 #pragma unroll
-        for (int sg = 0; sg < kTailSegs; sg++) {
-            const TailSeg g = t.seg[sg];
-            const int n = g.n;
+        for (int sg = 0; sg < 3; sg++) {
+            float *dst = sm + ct->seg[sg].dst;
 #pragma unroll
             for (int u = 0; u < kTailPer; u++) {
                 const int i = threadIdx.x + u * kTailThreads;
-                if (i < n) sm[g.dst + i] = v[sg][u];
+                if (u * kTailThreads < nv[sg] && i < nv[sg]) dst[i] = v[sg][u];
             }
-        }
-        float *al = sm + (t.ainv_lds >= 0 ? t.ainv_lds : 0);
-#pragma unroll
-        for (int u = 0; u < kAinvPer; u++) {
-            const int i = threadIdx.x + u * kTailThreads;
-            if (i < nn) al[i] = va[u];
         }
         if (halted) return;  // the same value in every thread of the workgroup
     }
@@ -1054,7 +1062,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
                 for (int q = 0; q < kSpan; q++) {
                     const int j = part * span + q;
                     if (q < span && j < n)
-                        s = __fma_rn(static_cast<double>(t.ainv[static_cast<size_t>(j) * n + row]), bsh[j], s);
+                        s = __fma_rn(static_cast<double>(gmem(t.ainv)[static_cast<size_t>(j) * n + row]), bsh[j], s);
                 }
             }
             psum[part][row] = s;
@@ -1081,7 +1089,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
         for (int n = threadIdx.x; n < t.pro_d.n_nodes; n += kTailThreads)
             prolong_node(t.pro_d, xf, t.pro_inv, t.lv[0].d, res, n);
     } else {
-        for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) t.out0[o0 + q] = res[q];
+        for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) gmem(t.out0)[o0 + q] = res[q];
     }
 }
 
@@ -1138,13 +1146,41 @@ struct PkPro {  // prolongation
     int nb;
 };
 
+// A table entry with every pointer marked global (gmem): read from a device table they would otherwise be generic,
+// and the element-centric march's lane exchanges would wait for every flat access in flight.
+__device__ __forceinline__ PkEc globalized(PkEc a) {
+    a.ep = gmem(a.ep), a.x = gmem(a.x), a.b = gmem(a.b), a.inv = gmem(a.inv), a.binv = gmem(a.binv);
+    a.d = gmem(a.d), a.out = gmem(a.out), a.z64 = gmem(a.z64), a.r64 = gmem(a.r64), a.ap64 = gmem(a.ap64);
+    a.part_zr = gmem(a.part_zr), a.part_zap = gmem(a.part_zap), a.halt = gmem(a.halt);
+    return a;
+}
+__device__ __forceinline__ PkS globalized(PkS a) {
+    a.ep = gmem(a.ep), a.x = gmem(a.x), a.b = gmem(a.b), a.binv = gmem(a.binv), a.d = gmem(a.d);
+    a.x_out = gmem(a.x_out);
+    return a;
+}
+__device__ __forceinline__ PkRR globalized(PkRR a) {
+    a.ep = gmem(a.ep), a.xf = gmem(a.xf), a.bf = gmem(a.bf), a.invf = gmem(a.invf), a.invc = gmem(a.invc);
+    a.binvc = gmem(a.binvc), a.bc = gmem(a.bc), a.xcv = gmem(a.xcv), a.dc = gmem(a.dc);
+    return a;
+}
+__device__ __forceinline__ PkRD globalized(PkRD a) {
+    a.rf = gmem(a.rf), a.invc = gmem(a.invc), a.binvc = gmem(a.binvc), a.bc = gmem(a.bc), a.xcv = gmem(a.xcv);
+    a.dc = gmem(a.dc);
+    return a;
+}
+__device__ __forceinline__ PkPro globalized(PkPro a) {
+    a.xf = gmem(a.xf), a.invf = gmem(a.invf), a.xcv = gmem(a.xcv);
+    return a;
+}
+
 __device__ __forceinline__ int pk_wave_tile(int nb) {
     return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, nb) * 4 + (threadIdx.x >> 6));
 }
 
 template <int KZ>
 __global__ __launch_bounds__(256) void k_pk_cheb_ec(const PkEc *__restrict__ tab) {
-    const PkEc a = tab[blockIdx.y];
+    const PkEc a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     const int tile = pk_wave_tile(a.nb);
     if (tile >= a.n_tiles) return;  // wave-uniform; no barrier follows
@@ -1155,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_pk_cheb_ec(const PkEc *__restrict__ tab
 
 template <int KZ>
 __global__ __launch_bounds__(256) void k_pk_residual_ec(const PkEc *__restrict__ tab) {
-    const PkEc a = tab[blockIdx.y];
+    const PkEc a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     const int tile = pk_wave_tile(a.nb);
     if (tile >= a.n_tiles) return;
@@ -1167,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_pk_residual_ec(const PkEc *__restrict__
 template <int KZ>
 __global__ __launch_bounds__(256) void k_pk_cheb_final_ec(const PkEc *__restrict__ tab) {
     __shared__ double red[8];
-    const PkEc a = tab[blockIdx.y];
+    const PkEc a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;  // block-uniform, ahead of the block sum's barriers
     const Defl none{nullptr, nullptr, nullptr, 0.0, 0, 0, 0};
     FinalFin fin{a.b, a.d, a.binv, a.z64, a.r64, a.ap64, a.cd, a.cr, none, nullptr};
@@ -1185,26 +1221,26 @@ __global__ __launch_bounds__(256) void k_pk_cheb_final_ec(const PkEc *__restrict
 }
 
 __global__ __launch_bounds__(256) void k_pk_cheb_s(const PkS *__restrict__ tab) {
-    const PkS a = tab[blockIdx.y];
+    const PkS a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     cheb_s_body(a.g, a.ks, a.ep, a.x, a.b, a.d, a.binv, a.cd, a.cr, a.x_out, blockIdx.x, a.nb);
 }
 
 __global__ __launch_bounds__(256) void k_pk_residual_restrict(const PkRR *__restrict__ tab) {
-    const PkRR a = tab[blockIdx.y];
+    const PkRR a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     residual_restrict_body(a.gf, a.ks, a.ep, a.xf, a.bf, a.invf, a.gc, a.invc, a.binvc, a.theta_inv, a.bc, a.xcv,
                            a.dc, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_pk_restrict_dof(const PkRD *__restrict__ tab) {
-    const PkRD a = tab[blockIdx.y];
+    const PkRD a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     restrict_dof_body(a.gf, a.rf, a.gc, a.invc, a.binvc, a.theta_inv, a.bc, a.xcv, a.dc, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_pk_prolong(const PkPro *__restrict__ tab) {
-    const PkPro a = tab[blockIdx.y];
+    const PkPro a = globalized(tab[blockIdx.y]);
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (static_cast<int>(blockIdx.x) < a.nb && n < a.gf.n_nodes) prolong_node(a.gf, a.xf, a.invf, a.gc, a.xcv, n);
 }
@@ -1471,9 +1507,12 @@ struct GmgLevel {
     double scale = 1.0;  // h_l / h
     float kdiag = 0.0f;  // diagonal entry of the level's Ke
     float koff = 0.0f;   // off-diagonal entry (x, y) of its corner-0 block
-    const float *ep = nullptr;
-    DevBuf<float> ep_own, inv, b, x, x2, dv, r;
-    DevBuf<float> binv;  // the smoother's nodal block inverses, kBinv per node
+    const float *ep = nullptr;  // padded moduli: the operator's (finest level) or ep_w
+    float *ep_w = nullptr;      // a coarse level's padded moduli (written by the coarsening kernel)
+    float *inv = nullptr;       // inverse diagonal (the constraint mask)
+    float *binv = nullptr;      // the smoother's nodal block inverses, kBinv per node
+    // ep_w, inv and binv live in the tail image (Gmg::tail_img) for the tail's levels, else in these buffers
+    DevBuf<float> ep_own, inv_own, binv_own, b, x, x2, dv, r;
     int blocks = 0;    // 256 nodes per block (one lane per node)
     int blocks_s = 0;  // kSplitNodes per block (the slab-split gathers)
 };
@@ -1493,7 +1532,10 @@ struct Gmg {
     Deflation defl;
     GmgLevel lv[kMaxLevels];
     int levels = 0;
-    DevBuf<float> ainv;  // coarsest inverse, column-major
+    float *ainv = nullptr;  // coarsest inverse, column-major: in the tail image when the tail stages it, else ainv_own
+    DevBuf<float> ainv_own;
+    TailLayout tl;           // the tail's levels and its LDS image
+    DevBuf<float> tail_img;  // the constant part of that image (tl.img_n floats), in the same order
     DevBuf<double> blocks;  // Galerkin element blocks of the second-coarsest level
     uint64_t epoch = ~0ull;
     float theta_inv = 0;
@@ -1555,6 +1597,50 @@ bool gmg_supported(const mycel_op *op) {
         d = make_dims((d.nx + 1) / 2, (d.ny + 1) / 2, (d.nz + 1) / 2);
     }
     return false;
+}
+
+// The tail's levels: from the coarsest up, as many as have <= tail_nodes() nodes (never the finest: it has the
+// FP64 entry and exit kernels) and fit the LDS image, at most kMaxTail; the layout of that image.
+static TailLayout tail_layout(const Gmg *g) {
+    const int top = g->levels - 1;
+    auto img_floats = [&](int l) { return (l < top ? g->lv[l].d.n_pad : 0) + 3 * g->lv[l].d.n_dofs; };
+    auto level_floats = [&](int l) { return img_floats(l) + 5 * g->lv[l].d.n_dofs; };
+    auto fits = [&](int l) { return g->lv[l].d.n_dofs <= kTailPer * kTailThreads; };
+    TailLayout tl;
+    int t0 = top;
+    long long total = level_floats(top);
+    while (t0 > 1 && top - t0 + 1 < kMaxTail && g->lv[t0 - 1].d.n_nodes <= tail_nodes() && fits(t0 - 1) &&
+           4 * (total + level_floats(t0 - 1)) <= kTailLdsMax) {
+        t0--;
+        total += level_floats(t0);
+    }
+    if (top == 0) t0 = 0;
+    tl.t0 = t0;
+    tl.count = top - t0 + 1;
+    if (!fits(top) || 4 * total > kTailLdsMax)
+        throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: coarsest level does not fit the tail kernel");
+    int off = 0;
+    for (int l = t0; l <= top; l++) {
+        const int k = l - t0, nd = g->lv[l].d.n_dofs;
+        tl.o_e[k] = off;
+        if (l < top) off += g->lv[l].d.n_pad;
+        tl.o_inv[k] = off;
+        off += nd;
+        tl.o_binv[k] = off;  // kBinv per node = 2 per DOF
+        off += 2 * nd;
+    }
+    const int nc = g->lv[top].d.n_dofs;
+    if (4 * (total + static_cast<long long>(nc) * nc) <= kTailLdsMax) {
+        tl.ainv = off;
+        off += nc * nc;
+    }
+    tl.img_n = off;
+    for (int l = t0; l <= top; l++) {
+        tl.o_vec[l - t0] = off;
+        off += 5 * g->lv[l].d.n_dofs;
+    }
+    tl.lds_floats = off;
+    return tl;
 }
 
 // Builds the level structure; op->gmg is set only once the whole hierarchy exists.
@@ -1654,14 +1740,6 @@ static void gmg_build(mycel_op *op) {
         }
         L.kdiag = static_cast<float>(op->ke[0] * scale);
         L.koff = static_cast<float>(op->ke[1] * scale);
-        if (l == 0) {
-            L.ep = op->d_ep32.get();
-        } else {
-            L.ep_own.resize(d.n_pad);
-            L.ep = L.ep_own.get();
-        }
-        L.inv.resize(d.n_dofs);
-        L.binv.resize(static_cast<size_t>(kBinv) * d.n_nodes);
         L.b.resize(d.n_dofs);
         L.x.resize(d.n_dofs);
         L.x2.resize(d.n_dofs);
@@ -1679,8 +1757,45 @@ static void gmg_build(mycel_op *op) {
         d = make_dims((d.nx + 1) / 2, (d.ny + 1) / 2, (d.nz + 1) / 2);
         scale *= 2.0;
     }
-    const int nc = g->lv[g->levels - 1].d.n_dofs;
-    g->ainv.resize(static_cast<size_t>(nc) * nc);
+    // The tail's levels keep their constant arrays (moduli, D^-1, B^-1, the coarsest inverse) in one buffer laid out
+    // as the constant part of the tail's LDS image: the tail stages it with one contiguous copy. (Gathered from a
+    // dozen separate arrays, the staging took ~11k cycles, three times a contiguous copy of the same bytes: the
+    // per-array address, clamp and mask arithmetic of 16 waves made it VALU-bound.)
+    g->tl = tail_layout(g);
+    g->tail_img.resize(g->tl.img_n);
+    const int top = g->levels - 1;
+    for (int l = 0; l < g->levels; l++) {
+        GmgLevel &L = g->lv[l];
+        const int k = l - g->tl.t0;
+        float *img = g->tail_img.get();
+        if (l == 0) {
+            L.ep = op->d_ep32.get();
+        } else {
+            if (k >= 0 && l < top) {
+                L.ep_w = img + g->tl.o_e[k];
+            } else {
+                L.ep_own.resize(L.d.n_pad);
+                L.ep_w = L.ep_own.get();
+            }
+            L.ep = L.ep_w;
+        }
+        if (k >= 0) {
+            L.inv = img + g->tl.o_inv[k];
+            L.binv = img + g->tl.o_binv[k];
+        } else {
+            L.inv_own.resize(L.d.n_dofs);
+            L.binv_own.resize(static_cast<size_t>(kBinv) * L.d.n_nodes);
+            L.inv = L.inv_own.get();
+            L.binv = L.binv_own.get();
+        }
+    }
+    const int nc = g->lv[top].d.n_dofs;
+    if (g->tl.ainv >= 0) {
+        g->ainv = g->tail_img.get() + g->tl.ainv;
+    } else {
+        g->ainv_own.resize(static_cast<size_t>(nc) * nc);
+        g->ainv = g->ainv_own.get();
+    }
     if (g->levels >= 2) g->blocks.resize(static_cast<size_t>(g->lv[g->levels - 2].d.n_elem) * 576);
     op->gmg = owner.release();
 }
@@ -1891,16 +2006,16 @@ static void gmg_tune_smoothing(mycel_op *op, Gmg *g, hipStream_t s) {
         const float cd = g->cd[1], cr = g->cr[1];
         if (kz == 2)
             hipLaunchKernelGGL((k_gmg_cheb_ec<2, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv, cd, cr, F.x2.get(), nt, nullptr, 0);
         else if (kz == 3)
             hipLaunchKernelGGL((k_gmg_cheb_ec<3, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv, cd, cr, F.x2.get(), nt, nullptr, 0);
         else if (kz == 4)
             hipLaunchKernelGGL((k_gmg_cheb_ec<4, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv, cd, cr, F.x2.get(), nt, nullptr, 0);
         else
             hipLaunchKernelGGL((k_gmg_cheb_ec<8, false>), grid, b256, 0, s, F.d, F.km, F.ep, F.x.get(), F.b.get(),
-                               F.dv.get(), F.binv.get(), cd, cr, F.x2.get(), nt, nullptr, 0);
+                               F.dv.get(), F.binv, cd, cr, F.x2.get(), nt, nullptr, 0);
         MYCEL_LAUNCH_CHECK();
     });
 }
@@ -1913,17 +2028,17 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
     const dim3 b256(256);
     GmgLevel &F = g->lv[0];
     hipLaunchKernelGGL(k_gmg_inv_fine, dim3(cdiv(F.d.n_dofs, 256)), b256, 0, s, F.d.n_dofs, op->d_flags.get(),
-                       op->d_diag.get(), F.inv.get());
+                       op->d_diag.get(), F.inv);
     for (int l = 1; l < g->levels; l++) {
         GmgLevel &P = g->lv[l - 1], &L = g->lv[l];
-        hipLaunchKernelGGL(k_gmg_coarsen_e, dim3(cdiv(L.d.n_pad, 256)), b256, 0, s, P.d, P.ep, L.d, L.ep_own.get());
+        hipLaunchKernelGGL(k_gmg_coarsen_e, dim3(cdiv(L.d.n_pad, 256)), b256, 0, s, P.d, P.ep, L.d, L.ep_w);
         hipLaunchKernelGGL(k_gmg_inv_coarse, dim3(cdiv(L.d.n_nodes, 256)), b256, 0, s, L.d, L.ep, L.kdiag, P.d,
-                           P.inv.get(), L.inv.get());
+                           P.inv, L.inv);
     }
     for (int l = 0; l < g->levels; l++) {
         GmgLevel &L = g->lv[l];
         hipLaunchKernelGGL(k_gmg_binv, dim3(cdiv(L.d.n_nodes, 256)), b256, 0, s, L.d, L.ep, L.kdiag, L.koff,
-                           L.inv.get(), L.binv.get());
+                           L.inv, L.binv);
     }
     GmgLevel &C = g->lv[g->levels - 1];
     const int nc = C.d.n_dofs;
@@ -1934,13 +2049,13 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsMax));
     if (g->levels >= 2) {  // Galerkin coarsest operator from the second-coarsest level's elements
         GmgLevel &S = g->lv[g->levels - 2];
-        hipLaunchKernelGGL(k_gmg_galerkin_blocks, dim3(S.d.n_elem), b256, 0, s, S.d, S.ep, S.inv.get(),
+        hipLaunchKernelGGL(k_gmg_galerkin_blocks, dim3(S.d.n_elem), b256, 0, s, S.d, S.ep, S.inv,
                            op->d_ke.get(), S.scale, g->blocks.get());
         hipLaunchKernelGGL(k_gmg_coarse_factor, dim3(1), dim3(1024), lds, s, C.d, C.ep, op->d_ke.get(), C.scale,
-                           C.inv.get(), S.d, static_cast<const double *>(g->blocks.get()), g->ainv.get());
+                           C.inv, S.d, static_cast<const double *>(g->blocks.get()), g->ainv);
     } else {
         hipLaunchKernelGGL(k_gmg_coarse_factor, dim3(1), dim3(1024), lds, s, C.d, C.ep, op->d_ke.get(), C.scale,
-                           C.inv.get(), C.d, static_cast<const double *>(nullptr), g->ainv.get());
+                           C.inv, C.d, static_cast<const double *>(nullptr), g->ainv);
     }
     MYCEL_LAUNCH_CHECK();
     if (op->island_ratio > 0.0) {
@@ -1955,7 +2070,7 @@ void gmg_prepare(mycel_op *op, hipStream_t s) {
 
 GmgFineView gmg_fine_view(mycel_op *op) {
     GmgLevel &F = op->gmg->lv[0];
-    return GmgFineView{F.b.get(), F.x.get(), F.dv.get(), F.inv.get(), F.binv.get(), op->gmg->theta_inv};
+    return GmgFineView{F.b.get(), F.x.get(), F.dv.get(), F.inv, F.binv, op->gmg->theta_inv};
 }
 
 // The tail kernel's arguments: levels [t0, top] run in the single-workgroup tail (the finest level never does: it
@@ -1963,67 +2078,39 @@ GmgFineView gmg_fine_view(mycel_op *op) {
 // each level's current iterate and scratch before the tail; tail_lds: the dynamic LDS bytes of its launch.
 static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nxt, const int *halt, int halt_stride,
                                int &t0, size_t &tail_lds) {
-    auto tail_floats = [&](int l) { return g->lv[l].d.n_pad + 8 * g->lv[l].d.n_dofs; };
-    auto tail_fits = [&](int l) {
-        const Dims &d = g->lv[l].d;
-        return d.n_dofs <= kTailPer * kTailThreads && d.n_pad <= kTailPer * kTailThreads;
-    };
-    const int top = g->levels - 1;
-    t0 = top;
-    long long lds_floats = tail_floats(top);
-    while (t0 > 1 && g->lv[t0 - 1].d.n_nodes <= tail_nodes() && tail_fits(t0 - 1) &&
-           4 * (lds_floats + tail_floats(t0 - 1)) <= kTailLdsMax) {
-        t0--;
-        lds_floats += tail_floats(t0);
-    }
-    if (top == 0) t0 = 0;
+    const TailLayout &tl = g->tl;
+    t0 = tl.t0;
     TailArgs ta{};
-    ta.count = top - t0 + 1;
-    if (ta.count > kMaxTail || !tail_fits(top) || 4 * lds_floats > kTailLdsMax)
-        throw Error(MYCEL_E_INVALID_OPERATION, "multigrid: coarsest level does not fit the tail kernel");
-    int off = 0;
-    for (int l = t0; l <= top; l++) {
-        const GmgLevel &L = g->lv[l];
-        TailLevel &T = ta.lv[l - t0];
+    ta.count = tl.count;
+    for (int k = 0; k < tl.count; k++) {
+        const GmgLevel &L = g->lv[t0 + k];
+        TailLevel &T = ta.lv[k];
         T.d = L.d;
         T.ks = L.ks;
         T.ep = L.ep;
-        T.inv = L.inv.get();
-        const int nd = L.d.n_dofs;
-        T.o_e = off;
-        off += L.d.n_pad;
-        T.o_inv = off;
-        T.o_b = off + nd;
-        T.o_x = off + 2 * nd;
-        T.o_x2 = off + 3 * nd;
-        T.o_d = off + 4 * nd;
-        T.o_r = off + 5 * nd;
-        T.o_binv = off + 6 * nd;  // kBinv per node = 2 per DOF
-        off += 8 * nd;
+        T.inv = L.inv;
+        const int nd = L.d.n_dofs, ov = tl.o_vec[k];
+        T.o_e = tl.o_e[k];
+        T.o_inv = tl.o_inv[k];
+        T.o_binv = tl.o_binv[k];
+        T.o_b = ov;
+        T.o_x = ov + nd;
+        T.o_x2 = ov + 2 * nd;
+        T.o_d = ov + 3 * nd;
+        T.o_r = ov + 4 * nd;
     }
-    // The coarsest inverse (FP32) joins the LDS image when it fits.
-    const int nc = g->lv[top].d.n_dofs;
-    ta.ainv_lds = 4 * (static_cast<long long>(off) + static_cast<long long>(nc) * nc) <= kTailLdsMax ? off : -1;
-    if (ta.ainv_lds >= 0) off += nc * nc;
-    tail_lds = sizeof(float) * static_cast<size_t>(off);
-    ta.ainv_nn = ta.ainv_lds >= 0 ? nc * nc : 0;
+    ta.ainv_lds = tl.ainv;
+    tail_lds = sizeof(float) * static_cast<size_t>(tl.lds_floats);
+    ta.img = g->tail_img.get();
+    ta.img_n = tl.img_n;
     {
         const TailLevel &T0 = ta.lv[0];
         ta.seg[0] = TailSeg{g->lv[t0].b.get(), T0.d.n_dofs, T0.o_b};
         ta.seg[1] = TailSeg{cur[t0], T0.d.n_dofs, T0.o_x};
         ta.seg[2] = TailSeg{g->lv[t0].dv.get(), T0.d.n_dofs, T0.o_d};
-        for (int l = 0; l < ta.count; l++) {
-            const TailLevel &T = ta.lv[l];
-            const float *bv = g->lv[t0 + l].binv.get();  // kBinv per node = 2 n_dofs floats, staged in two halves
-            ta.seg[3 + 4 * l] = TailSeg{T.ep, T.d.n_pad, T.o_e};
-            ta.seg[4 + 4 * l] = TailSeg{T.inv, T.d.n_dofs, T.o_inv};
-            ta.seg[5 + 4 * l] = TailSeg{bv, T.d.n_dofs, T.o_binv};
-            ta.seg[6 + 4 * l] = TailSeg{bv + T.d.n_dofs, T.d.n_dofs, T.o_binv + T.d.n_dofs};
-        }
-        for (int sg = 3 + 4 * ta.count; sg < kTailSegs; sg++) ta.seg[sg] = TailSeg{ta.seg[0].src, 0, 0};
     }
     ta.out0 = ta.count > 1 ? nxt[t0] : cur[t0];  // the host-side swap below names it the current iterate
-    ta.ainv = g->ainv.get();
+    ta.ainv = g->ainv;
     ta.theta_inv = g->theta_inv;
     ta.cd1 = g->cd[1];
     ta.cr1 = g->cr[1];
@@ -2075,25 +2162,25 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             if (kz == 2)
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<2, true> : k_gmg_cheb_ec<2, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv, cd, cr, nxt[l], nt, halt, halt_stride);
             else if (kz == 3)
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<3, true> : k_gmg_cheb_ec<3, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv, cd, cr, nxt[l], nt, halt, halt_stride);
             else if (kz == 4)
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<4, true> : k_gmg_cheb_ec<4, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv, cd, cr, nxt[l], nt, halt, halt_stride);
             else
                 hipLaunchKernelGGL((fh ? k_gmg_cheb_ec<8, true> : k_gmg_cheb_ec<8, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.dv.get(), L.binv.get(), cd, cr, nxt[l], nt, halt, halt_stride);
+                                   L.dv.get(), L.binv, cd, cr, nxt[l], nt, halt, halt_stride);
         } else if (split && l > 0)
             hipLaunchKernelGGL(k_gmg_cheb_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.dv.get(), L.binv.get(), cd, cr, nxt[l]);
+                               L.dv.get(), L.binv, cd, cr, nxt[l]);
         else
             hipLaunchKernelGGL(k_gmg_cheb, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.dv.get(), L.binv.get(), cd, cr, nxt[l]);
+                               L.dv.get(), L.binv, cd, cr, nxt[l]);
         std::swap(cur[l], nxt[l]);
     };
     int t0 = 0;
@@ -2103,13 +2190,13 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     GmgLevel &F = g->lv[0];
     if (!fine_ready)
         hipLaunchKernelGGL(k_gmg_init_fine, dim3(cdiv(F.d.n_nodes, 256), R), b256, 0, s, F.d.n_nodes, r64,
-                           F.inv.get(), F.binv.get(), g->theta_inv, F.b.get(), cur[0], F.dv.get());
+                           F.inv, F.binv, g->theta_inv, F.b.get(), cur[0], F.dv.get());
     for (int l = 0; l < t0; l++) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         for (int k = 1; k < (l == 0 ? g->fine_degree : 2); k++) cheb(l, g->cd[k], g->cr[k]);
         if (l > 0 && fuse_rr()) {
             hipLaunchKernelGGL(k_gmg_residual_restrict, dim3(cdiv(C.d.n_nodes, 8), R), b256, 0, s, L.d, L.ks, L.ep,
-                               cur[l], L.b.get(), L.inv.get(), C.d, C.inv.get(), C.binv.get(), g->theta_inv, C.b.get(),
+                               cur[l], L.b.get(), L.inv, C.d, C.inv, C.binv, g->theta_inv, C.b.get(),
                                cur[l + 1],
                                C.dv.get());
             continue;
@@ -2120,33 +2207,33 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
             if (kz == 2)
                 hipLaunchKernelGGL((fh ? k_gmg_residual_ec<2, true> : k_gmg_residual_ec<2, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
+                                   L.inv, L.r.get(), nt, halt, halt_stride);
             else if (kz == 3)
                 hipLaunchKernelGGL((fh ? k_gmg_residual_ec<3, true> : k_gmg_residual_ec<3, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
+                                   L.inv, L.r.get(), nt, halt, halt_stride);
             else if (kz == 4)
                 hipLaunchKernelGGL((fh ? k_gmg_residual_ec<4, true> : k_gmg_residual_ec<4, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
+                                   L.inv, L.r.get(), nt, halt, halt_stride);
             else
                 hipLaunchKernelGGL((fh ? k_gmg_residual_ec<8, true> : k_gmg_residual_ec<8, false>), grid, b256, 0, s,
                                    L.d, L.km, L.ep, cur[l], L.b.get(),
-                                   L.inv.get(), L.r.get(), nt, halt, halt_stride);
+                                   L.inv, L.r.get(), nt, halt, halt_stride);
         } else if (split)
             hipLaunchKernelGGL(k_gmg_residual_s, dim3(L.blocks_s, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.inv.get(), L.r.get());
+                               L.inv, L.r.get());
         else
             hipLaunchKernelGGL(k_gmg_residual, dim3(L.blocks, R), b256, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(),
-                               L.inv.get(), L.r.get());
+                               L.inv, L.r.get());
         hipLaunchKernelGGL(k_gmg_restrict_dof, dim3(cdiv(C.d.n_nodes, 64), R), b256, 0, s, L.d, L.r.get(), C.d,
-                           C.inv.get(), C.binv.get(), g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
+                           C.inv, C.binv, g->theta_inv, C.b.get(), cur[l + 1], C.dv.get());
     }
     if (t0 > 0) {
         // The tail prolongs into level t0 - 1 itself (one launch fewer): into that level's current iterate as
         // the pre-smoothing above has left it (the smoothing steps swap the level's iterate buffers).
         ta.pro_x = cur[t0 - 1];
-        ta.pro_inv = g->lv[t0 - 1].inv.get();
+        ta.pro_inv = g->lv[t0 - 1].inv;
         ta.pro_d = g->lv[t0 - 1].d;
     }
     hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(kTailThreads), tail_lds, s, ta);
@@ -2154,7 +2241,7 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
         if (l != t0 - 1)  // level t0 - 1 was prolonged into by the tail
-            hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv.get(), C.d,
+            hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv, C.d,
                                cur[l + 1]);
         cheb(l, 0.0f, g->theta_inv);
         for (int k = 1; k < (l == 0 ? g->fine_degree - 1 : 2); k++) cheb(l, g->cd[k], g->cr[k]);
@@ -2166,26 +2253,26 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     if (fkz == 2)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<2, true> : k_gmg_cheb_final_ec<2, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv, cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else if (fkz == 3)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<3, true> : k_gmg_cheb_final_ec<3, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv, cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else if (fkz == 4)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<4, true> : k_gmg_cheb_final_ec<4, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv, cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else if (fkz == 8)
         hipLaunchKernelGGL((fh ? k_gmg_cheb_final_ec<8, true> : k_gmg_cheb_final_ec<8, false>), f_grid, b256, 0, s,
                            F.d, F.km, F.ep, cur[0], F.b.get(), F.dv.get(),
-                           F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
+                           F.binv, cd, cr, z64, r64, ap64, part_zr, part_zap, f_tiles, halt,
                            halt_stride, df);
     else
         hipLaunchKernelGGL(k_gmg_cheb_final, dim3(F.blocks, R), b256, 0, s, F.d, F.ks, F.ep, cur[0], F.b.get(),
-                           F.dv.get(), F.binv.get(), cd, cr, z64, r64, ap64, part_zr, part_zap, df);
+                           F.dv.get(), F.binv, cd, cr, z64, r64, ap64, part_zr, part_zap, df);
     MYCEL_LAUNCH_CHECK();
 }
 
@@ -2240,8 +2327,8 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
         a.ep = F.ep;
         a.x = cur[0];
         a.b = F.b.get();
-        a.inv = F.inv.get();
-        a.binv = F.binv.get();
+        a.inv = F.inv;
+        a.binv = F.binv;
         a.d = F.dv.get();
         a.out = outv;
         a.cd = cd;
@@ -2262,7 +2349,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.ep = L.ep;
             a.x = cur[l];
             a.b = L.b.get();
-            a.binv = L.binv.get();
+            a.binv = L.binv;
             a.d = L.dv.get();
             a.x_out = nxt[l];
             a.cd = cd;
@@ -2286,9 +2373,9 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.ep = L.ep;
             a.xf = cur[l];
             a.bf = L.b.get();
-            a.invf = L.inv.get();
-            a.invc = C.inv.get();
-            a.binvc = C.binv.get();
+            a.invf = L.inv;
+            a.invc = C.inv;
+            a.binvc = C.binv;
             a.bc = C.b.get();
             a.xcv = cur[l + 1];
             a.dc = C.dv.get();
@@ -2301,8 +2388,8 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.gf = L.d;
             a.gc = C.d;
             a.rf = L.r.get();
-            a.invc = C.inv.get();
-            a.binvc = C.binv.get();
+            a.invc = C.inv;
+            a.binvc = C.binv;
             a.bc = C.b.get();
             a.xcv = cur[1];
             a.dc = C.dv.get();
@@ -2312,7 +2399,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
         }
     }
     ta.pro_x = cur[t0 - 1];  // t0 >= 1: every packable hierarchy has at least two levels
-    ta.pro_inv = g->lv[t0 - 1].inv.get();
+    ta.pro_inv = g->lv[t0 - 1].inv;
     ta.pro_d = g->lv[t0 - 1].d;
     out.push_back(pk_rec(kKeyTail, kPkTail, 0, 1, ta, tail_lds));
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);
@@ -2323,7 +2410,7 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
             a.gf = L.d;
             a.gc = C.d;
             a.xf = cur[l];
-            a.invf = L.inv.get();
+            a.invf = L.inv;
             a.xcv = cur[l + 1];
             a.nb = L.blocks;
             out.push_back(pk_rec(key_up(l, 0), kPkProlong, 0, L.blocks, a));

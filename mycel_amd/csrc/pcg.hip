@@ -576,8 +576,24 @@ struct PkStep4 {
     int n_tiles, nb;
 };
 
+// Table entries with every pointer marked global (gmem): read from a device table they would otherwise be generic,
+// and every LDS access or lane exchange behind a flat access would wait for it.
+__device__ __forceinline__ PkXr globalized(PkXr a) {
+    a.st = gmem(a.st), a.part_pap = gmem(a.part_pap), a.x = gmem(a.x), a.r = gmem(a.r);
+    a.p[0] = gmem(a.p[0]), a.p[1] = gmem(a.p[1]), a.ap = gmem(a.ap), a.part_rr = gmem(a.part_rr);
+    a.fv.b = gmem(a.fv.b), a.fv.x = gmem(a.fv.x), a.fv.d = gmem(a.fv.d), a.fv.inv = gmem(a.fv.inv);
+    a.fv.binv = gmem(a.fv.binv);
+    return a;
+}
+__device__ __forceinline__ PkStep4 globalized(PkStep4 a) {
+    a.ep = gmem(a.ep), a.flags = gmem(a.flags), a.p[0] = gmem(a.p[0]), a.p[1] = gmem(a.p[1]), a.z = gmem(a.z);
+    a.st = gmem(a.st), a.part_rr = gmem(a.part_rr), a.part_zr = gmem(a.part_zr), a.part_zap = gmem(a.part_zap);
+    a.ap = gmem(a.ap), a.part_pap = gmem(a.part_pap);
+    return a;
+}
+
 __global__ __launch_bounds__(256) void k_pk_fpcg_update_xr(const PkXr *__restrict__ tab, int slot) {
-    const PkXr a = tab[blockIdx.y];
+    const PkXr a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;  // block-uniform, ahead of the block sum's barriers
     fpcg_update_xr_body(a.n, slot, a.st, a.part_pap, a.nb_mv, a.x, a.r, slot ? a.p[1] : a.p[0], a.ap, a.part_rr,
                         a.fv, blockIdx.x);
@@ -586,7 +602,7 @@ __global__ __launch_bounds__(256) void k_pk_fpcg_update_xr(const PkXr *__restric
 template <int KZ>
 __global__ __launch_bounds__(256, 2) void k_pk_step4_ap(const PkStep4 *__restrict__ tab, int slot) {
     __shared__ double red[4];
-    const PkStep4 a = tab[blockIdx.y];
+    const PkStep4 a = globalized(tab[blockIdx.y]);
     if (static_cast<int>(blockIdx.x) >= a.nb) return;
     double *p_cur = slot ? a.p[1] : a.p[0], *p_next = slot ? a.p[0] : a.p[1];
     PcgNextP<double, true> src{p_cur, a.z, a.st, a.part_rr, a.part_zr, a.part_zap, a.nb_vec, a.nb_z, slot, 0.0,
@@ -603,7 +619,7 @@ __global__ __launch_bounds__(256, 2) void k_pk_step4_ap(const PkStep4 *__restric
 __global__ __launch_bounds__(64) void k_pk_gather_states(PcgState *const *__restrict__ src,
                                                          PcgState *__restrict__ dst) {
     constexpr int kWords = static_cast<int>(sizeof(PcgState) / sizeof(uint32_t));
-    const PcgState *from_state = src[blockIdx.x];
+    const PcgState *from_state = gmem(src[blockIdx.x]);
     if (!from_state) return;  // an empty slot
     const uint32_t *from = reinterpret_cast<const uint32_t *>(from_state);
     uint32_t *to = reinterpret_cast<uint32_t *>(dst + blockIdx.x);
