@@ -927,10 +927,13 @@ def _mg_table(args, lines, fetch=None, write=None, fs=1.0, ws=1.0):
         us = sum(c * u for c, u in hits) / calls if calls else float("nan")
         pmc = float("nan")
         if pmc_ok:
-            ks = [k for k in set(fetch) & set(write) if match(k)]
-            cnt = sum(fetch[k][1] for k in ks)
-            if cnt:
-                pmc = sum((fetch[k][2] * fs + write[k][2] * ws) * 1024 * fetch[k][1] for k in ks) / cnt
+            # Each counter over every instantiation of the kernel in its own run: the autotuner may pick another
+            # tile depth in each profiled run (the depths give the same bits and move the same bytes).
+            fk, wk = [k for k in fetch if match(k)], [k for k in write if match(k)]
+            fc, wc = sum(fetch[k][1] for k in fk), sum(write[k][1] for k in wk)
+            if fc and wc:
+                pmc = 1024 * (fs * sum(fetch[k][2] * fetch[k][1] for k in fk) / fc +
+                              ws * sum(write[k][2] * write[k][1] for k in wk) / wc)
         per = b / n
         lines.append(f"{prefix.rstrip('<')} | {n} | {per:,.0f} | {us:.2f} | {per / (us * 1e3):,.0f} | "
                      f"{pmc:,.0f} | {pmc / per:.2f}")
@@ -965,7 +968,7 @@ def mode_summarize(args):
             b = fetch[k][0] * 1024 * fs + write[k][0] * 1024 * ws
             lines.append(f"{k} | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | {b:.0f}")
             if k.startswith("k_apply_k"):
-                traffic[k] = b
+                traffic[k] = (b, fetch[k][1])
     mg_pmc = None
     if args.stats_dir:
         if args.fetch_dir and args.write_dir:
@@ -974,13 +977,17 @@ def mode_summarize(args):
             _mg_table(args, lines)
     print("\n".join(lines))
     if args.traffic_json and traffic:
-        # Plain K*u launches (mode 0, no dot partials) of either form, keyed without the template tail.
-        ku = {}
-        for k, v in traffic.items():
+        # Plain K*u launches (mode 0, no dot partials) of either form, keyed without the template tail: the
+        # dispatch-weighted mean over the tile depths (the autotuner's trials and the depth it picked).
+        acc = {}
+        for k, (v, cnt) in traffic.items():
             name, targs = k.split("<", 1)
             parts = [a.strip() for a in targs.rstrip(">").split(",")]
             if len(parts) >= 3 and parts[1] == "0" and parts[2] == "false":
-                ku[f"{name}<{parts[0]}>@{WORKLOAD}"] = v
+                a = acc.setdefault(f"{name}<{parts[0]}>@{WORKLOAD}", [0.0, 0])
+                a[0] += v * cnt
+                a[1] += cnt
+        ku = {k: a[0] / a[1] for k, a in acc.items()}
         if mg_pmc:
             ku[f"mg_iteration@{WORKLOAD}"] = mg_pmc
         with open(args.traffic_json, "w") as fh:
