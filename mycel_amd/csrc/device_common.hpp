@@ -21,16 +21,6 @@ __device__ __forceinline__ T *gmem(T *p) {
     return (T *)__builtin_bit_cast(__attribute__((address_space(1))) T *, p);
 }
 
-// p, the same in every lane but read from LDS or memory (the compiler cannot tell): in scalar registers once, so
-// the accesses through it take a scalar base instead of re-deriving one in every branch.
-template <typename T>
-__device__ __forceinline__ T *uniform(T *p) {
-    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
-    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
-    return reinterpret_cast<T *>((static_cast<unsigned long long>(hi) << 32) | lo);
-}
-
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each with a private L2.
 // Remap so that each XCD receives a contiguous range of logical tiles: neighbouring node tiles then
 // share halo lines in the same L2. Bijective on [0, G).

@@ -860,10 +860,8 @@ int tail_nodes() {
 struct TailLevel {
     Dims d;
     KeSym<float> ks;
-    const float *ep;   // padded moduli (global)
-    const float *inv;  // inverse diagonal (global): the constraint mask
     int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
-    int o_binv;        // and of its nodal block inverses (staged with the rest)
+    int o_binv;        // and of its nodal block inverses
 };
 constexpr int kImgPer = kTailLdsMax / 4 / kTailThreads;  // tail image floats per thread, at most
 // lv[0]'s b, x or d, staged into LDS on entry: n floats from src + the RHS offset to LDS offset dst.
@@ -2087,8 +2085,6 @@ static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nx
         TailLevel &T = ta.lv[k];
         T.d = L.d;
         T.ks = L.ks;
-        T.ep = L.ep;
-        T.inv = L.inv;
         const int nd = L.d.n_dofs, ov = tl.o_vec[k];
         T.o_e = tl.o_e[k];
         T.o_inv = tl.o_inv[k];
