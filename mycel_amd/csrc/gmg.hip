@@ -76,6 +76,26 @@ __device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, i
     k = t / g.ny1;
 }
 
+// (q, r) = n divided by d, for 0 <= n < 2^16, from rd = 1 / d rounded to FP32: (n + 1/2) / d lies at least
+// 1/(2 d) from an integer, and the product's error, below (n + 1/2) / d * 2^-23, is smaller (n + 1/2 < 2^22).
+// Three operations instead of an integer division's ~30.
+__device__ __forceinline__ void small_divmod(int n, int d, float rd, int &q, int &r) {
+    q = static_cast<int>((static_cast<float>(n) + 0.5f) * rd);
+    r = n - q * d;
+}
+// Node decompositions: node_ijk, or small_divmod with the grid's reciprocal node counts (grids < 2^16 nodes).
+struct NodeIjk {
+    __device__ void operator()(const Dims &g, int n, int &i, int &j, int &k) const { node_ijk(g, n, i, j, k); }
+};
+struct SmallIjk {
+    float rx, ry;  // 1 / nx1, 1 / ny1
+    __device__ void operator()(const Dims &g, int n, int &i, int &j, int &k) const {
+        int t;
+        small_divmod(n, g.nx1, rx, t, i);
+        small_divmod(t, g.ny1, ry, k, j);
+    }
+};
+
 // Per-node bodies of the V-cycle phases, shared by the one-launch-per-phase kernels of the large levels and
 // by the single-workgroup tail kernel of the small ones.
 
@@ -168,12 +188,13 @@ __device__ __forceinline__ void restrict_taps(int C, int nf, int (&idx)[3], floa
 
 // b_c = P^T r_f at coarse node n, then the coarse pre-smoother's first step x_c = d_c = B_c^-1 b_c / theta.
 // The taps are summed z-major, then y, then x; taps outside the fine grid add nothing.
+template <typename IJK = NodeIjk>
 __device__ __forceinline__ void restrict_node(const Dims &gf, const float *__restrict__ rf, const Dims &gc,
                                               const float *__restrict__ invc, const float *__restrict__ binvc,
                                               float theta_inv, float *__restrict__ bc, float *__restrict__ xcv,
-                                              float *__restrict__ dc, int n) {
+                                              float *__restrict__ dc, int n, IJK ijk = {}) {
     int I, J, K;
-    node_ijk(gc, n, I, J, K);
+    ijk(gc, n, I, J, K);
     int ix[3], iy[3], iz[3];
     float wx[3], wy[3], wz[3];
     restrict_taps(I, gf.nx, ix, wx);
@@ -247,10 +268,11 @@ __device__ __forceinline__ void binv_node(const Dims &g, const float *__restrict
 // x_f += P x_c at fine node n, unconstrained DOFs only (trilinear: weight 1 at an even fine index, 1/2 from
 // each neighbour at an odd one). All 8 candidate coarse nodes are read (indices always valid); those that
 // do not contribute have weight 0 and add nothing.
+template <typename IJK = NodeIjk>
 __device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__ xf, const float *__restrict__ invf,
-                                             const Dims &gc, const float *__restrict__ xcv, int n) {
+                                             const Dims &gc, const float *__restrict__ xcv, int n, IJK ijk = {}) {
     int i, j, k;
-    node_ijk(gf, n, i, j, k);
+    ijk(gf, n, i, j, k);
     const int is[2] = {i >> 1, (i + 1) >> 1}, js[2] = {j >> 1, (j + 1) >> 1}, ks_[2] = {k >> 1, (k + 1) >> 1};
     const float wx[2] = {(i & 1) ? 0.5f : 1.0f, (i & 1) ? 0.5f : 0.0f};
     const float wy[2] = {(j & 1) ? 0.5f : 1.0f, (j & 1) ? 0.5f : 0.0f};
@@ -857,11 +879,24 @@ int tail_nodes() {
     }();
     return v;
 }
+// Tail levels with more than kTailThreads / 8 nodes form their products element by element (tail_ec);
+// MYCEL_GMG_TAIL_EC=0 keeps the two-lanes-per-node gathers on every tail level.
+bool tail_ec() {
+    static const bool on = [] {
+        const char *e = std::getenv("MYCEL_GMG_TAIL_EC");
+        return !(e && *e == '0');
+    }();
+    return on;
+}
 struct TailLevel {
     Dims d;
     KeSym<float> ks;
+    KeMono<float> km;  // the same Ke in the monomial basis (ec levels)
     int o_e, o_inv, o_b, o_x, o_x2, o_d, o_r;  // float offsets of the level's arrays in LDS
     int o_binv;        // and of its nodal block inverses
+    int ec;            // 1: the level's products are formed element by element (tail_ec), into the scratch at o_scr
+    int o_scr;
+    float rnx1, rny1, rnx, rny;  // 1 / node and element counts along x and y (small_divmod)
 };
 constexpr int kImgPer = kTailLdsMax / 4 / kTailThreads;  // tail image floats per thread, at most
 // lv[0]'s b, x or d, staged into LDS on entry: n floats from src + the RHS offset to LDS offset dst.
@@ -875,6 +910,8 @@ struct TailSeg {
 struct TailLayout {
     int t0 = 0, count = 0;
     int o_e[kMaxTail] = {}, o_inv[kMaxTail] = {}, o_binv[kMaxTail] = {}, o_vec[kMaxTail] = {};
+    int ec[kMaxTail] = {};  // levels whose products are formed element by element, in the scratch at o_scr
+    int o_scr = 0;
     int ainv = -1;  // offset of the coarsest inverse, or -1: read from global memory
     int img_n = 0, lds_floats = 0;
 };
@@ -890,6 +927,7 @@ struct TailArgs {
     float *pro_x;            // non-null: the level above lv[0] (its iterate, RHS layout as above), into which the
     const float *pro_inv;    //   tail prolongs lv[0]'s correction itself (k_gmg_prolong's arithmetic) from LDS
     Dims pro_d;
+    float pro_rnx1, pro_rny1;
     float theta_inv, cd1, cr1;
     int count;
     // The outer iteration's halt flag of right-hand side k at halt[k * halt_stride], or null. A halted solve's
@@ -980,18 +1018,90 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
         for (int c = 0; c < 3; c++) kx[c] = xc[c] = 0.0f;
         if (n < L.d.n_nodes) {
             int i, j, k;
-            node_ijk(L.d, n, i, j, k);
+            SmallIjk{L.rnx1, L.rny1}(L.d, n, i, j, k);
             ku_gather_half<float>(L.d, L.ks, S(L.o_e), cur(l), i, j, k, 0, half, kx, xc);
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) kx[c] += __shfl_down(kx[c], 32, 64);  // every lane takes part
         return half == 0 && n < L.d.n_nodes;
     };
+    // An ec level: lane e forms element e's product E Ke x_e once, in the monomial basis (the tile march's
+    // transforms for one element: its lower and upper faces, the sparse product, each face's corner values),
+    // and stores its 8 corners' values in the scratch (component-major: scr[(3 a + c) ne + e]); after a barrier
+    // lane n sums node n's corner values of its <= 8 elements in a fixed order (z, y, x side). About 200
+    // operations per element instead of 4 x 72 multiply-adds per half-node lane: these levels' phases are bound
+    // by VALU issue, not by their dependent chains.
+    auto ec_kx = [&](const TailLevel &L, int l, int &n, float (&kx)[3], float (&xc)[3]) {
+        const Dims &g = L.d;
+        const float *x = cur(l), *ep = S(L.o_e);
+        float *scr = S(L.o_scr);
+        const int ne = g.n_elem, e = threadIdx.x;
+        if (e < ne) {
+            int ex, et, ey, ez;
+            small_divmod(e, g.nx, L.rnx, et, ex);
+            small_divmod(et, g.ny, L.rny, ez, ey);
+            const float E = ep[(ex + 1) + g.px * ((ey + 1) + g.py * (ez + 1))];
+            float v[8][3];
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                const int m = 3 * ((ex + corner_x(a)) + g.nx1 * ((ey + corner_y(a)) + g.ny1 * (ez + corner_z(a))));
+#pragma unroll
+                for (int c = 0; c < 3; c++) v[a][c] = x[m + c];
+            }
+            FaceSums<float> lo[3], hi[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                lo[c] = face_sums(v[0][c], v[1][c], v[2][c], v[3][c]);
+                hi[c] = face_sums(v[4][c], v[5][c], v[6][c], v[7][c]);
+            }
+            float cf[3][7], gm[3][7];
+            mono_coefficients(lo, hi, cf);
+            mono_product(L.km, E, cf, gm);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float gx = gm[c][0], gy = gm[c][1], gz = gm[c][2], gxy = gm[c][3], gxz = gm[c][4],
+                            gyz = gm[c][5], gxyz = gm[c][6];
+                const FaceSums<float> flo{-gz, gx - gxz, gy - gyz, gxy - gxyz}, fhi{gz, gx + gxz, gy + gyz, gxy + gxyz};
+                float y[8];
+                face_corners(flo, y[0], y[1], y[2], y[3]);
+                face_corners(fhi, y[4], y[5], y[6], y[7]);
+#pragma unroll
+                for (int a = 0; a < 8; a++) scr[(3 * a + c) * ne + e] = y[a];
+            }
+        }
+        __syncthreads();
+        n = threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < 3; c++) kx[c] = xc[c] = 0.0f;
+        if (n >= g.n_nodes) return false;
+        int i, j, k;
+        SmallIjk{L.rnx1, L.rny1}(g, n, i, j, k);
+#pragma unroll
+        for (int c = 0; c < 3; c++) xc[c] = x[3 * n + c];
+#pragma unroll
+        for (int sz = 0; sz < 2; sz++)
+#pragma unroll
+            for (int sy = 0; sy < 2; sy++)
+#pragma unroll
+                for (int sx = 0; sx < 2; sx++) {
+                    const int ex = i - 1 + sx, ey = j - 1 + sy, ez = k - 1 + sz;
+                    if (ex < 0 || ex >= g.nx || ey < 0 || ey >= g.ny || ez < 0 || ez >= g.nz) continue;
+                    // the node is corner (1 - sx, 1 - sy, 1 - sz) of element (ex, ey, ez)
+                    const int cx = 1 - sx, cy = 1 - sy, a = 4 * (1 - sz) + (cy ? (cx ? 2 : 3) : cx);
+                    const int el = ex + g.nx * (ey + g.ny * ez);
+#pragma unroll
+                    for (int c = 0; c < 3; c++) kx[c] += scr[(3 * a + c) * ne + el];
+                }
+        return true;
+    };
+    auto level_kx = [&](const TailLevel &L, int l, int &n, float (&kx)[3], float (&xc)[3]) {
+        return L.ec ? ec_kx(L, l, n, kx, xc) : split_kx(L, l, n, kx, xc);  // L.ec: the same in every thread
+    };
     auto cheb_phase = [&](const TailLevel &L, int l, float cd, float cr) {
         {
             int n;
             float kx[3], xc[3];
-            if (split_kx(L, l, n, kx, xc)) {
+            if (level_kx(L, l, n, kx, xc)) {
                 const float *b = S(L.o_b);
                 float *d = S(L.o_d), *xo = nxt(l);
                 float bi[kBinv], r[3], z[3];
@@ -1015,7 +1125,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
         {
             int n;
             float kx[3], xc[3];
-            if (split_kx(L, l, n, kx, xc)) {
+            if (level_kx(L, l, n, kx, xc)) {
                 const float *b = S(L.o_b), *inv = S(L.o_inv);
                 float *r = S(L.o_r);
 #pragma unroll
@@ -1033,7 +1143,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
         residual_phase(L, l);
         for (int n = threadIdx.x; n < C.d.n_nodes; n += kTailThreads)
             restrict_node(L.d, S(L.o_r), C.d, S(C.o_inv), S(C.o_binv), t.theta_inv, S(C.o_b), cur(l + 1), S(C.o_d),
-                          n);
+                          n, SmallIjk{C.rnx1, C.rny1});
         __syncthreads();
     }
     {
@@ -1076,7 +1186,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
     for (int l = last - 1; l >= 0; l--) {
         const TailLevel &L = t.lv[l], &C = t.lv[l + 1];
         for (int n = threadIdx.x; n < L.d.n_nodes; n += kTailThreads)
-            prolong_node(L.d, cur(l), S(L.o_inv), C.d, cur(l + 1), n);
+            prolong_node(L.d, cur(l), S(L.o_inv), C.d, cur(l + 1), n, SmallIjk{L.rnx1, L.rny1});
         __syncthreads();
         cheb_phase(L, l, 0.0f, t.theta_inv);
         cheb_phase(L, l, t.cd1, t.cr1);
@@ -1085,7 +1195,7 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
     if (t.pro_x) {
         float *xf = t.pro_x + static_cast<long long>(rhs) * t.pro_d.n_dofs;
         for (int n = threadIdx.x; n < t.pro_d.n_nodes; n += kTailThreads)
-            prolong_node(t.pro_d, xf, t.pro_inv, t.lv[0].d, res, n);
+            prolong_node(t.pro_d, xf, t.pro_inv, t.lv[0].d, res, n, SmallIjk{t.pro_rnx1, t.pro_rny1});
     } else {
         for (int q = threadIdx.x; q < t.lv[0].d.n_dofs; q += kTailThreads) gmem(t.out0)[o0 + q] = res[q];
     }
@@ -1637,7 +1747,19 @@ static TailLayout tail_layout(const Gmg *g) {
         tl.o_vec[l - t0] = off;
         off += 5 * g->lv[l].d.n_dofs;
     }
-    tl.lds_floats = off;
+    // Element-by-element products on the smoothed levels with many nodes (one lane per element, 24 corner values
+    // each in a scratch shared by those levels), where their LDS fits.
+    tl.o_scr = off;
+    int scr = 0;
+    for (int l = t0; l < top && tail_ec(); l++) {
+        const Dims &d = g->lv[l].d;
+        if (d.n_nodes > kTailThreads / 8 && d.n_elem <= kTailThreads &&
+            4LL * (off + std::max(scr, 24 * d.n_elem)) <= kTailLdsMax) {
+            tl.ec[l - t0] = 1;
+            scr = std::max(scr, 24 * d.n_elem);
+        }
+    }
+    tl.lds_floats = off + scr;
     return tl;
 }
 
@@ -2085,6 +2207,13 @@ static TailArgs make_tail_args(const Gmg *g, float *const *cur, float *const *nx
         TailLevel &T = ta.lv[k];
         T.d = L.d;
         T.ks = L.ks;
+        T.km = L.km;
+        T.ec = tl.ec[k];
+        T.o_scr = tl.o_scr;
+        T.rnx1 = 1.0f / static_cast<float>(L.d.nx1);
+        T.rny1 = 1.0f / static_cast<float>(L.d.ny1);
+        T.rnx = 1.0f / static_cast<float>(L.d.nx);
+        T.rny = 1.0f / static_cast<float>(L.d.ny);
         const int nd = L.d.n_dofs, ov = tl.o_vec[k];
         T.o_e = tl.o_e[k];
         T.o_inv = tl.o_inv[k];
@@ -2231,6 +2360,8 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         ta.pro_x = cur[t0 - 1];
         ta.pro_inv = g->lv[t0 - 1].inv;
         ta.pro_d = g->lv[t0 - 1].d;
+        ta.pro_rnx1 = 1.0f / static_cast<float>(ta.pro_d.nx1);
+        ta.pro_rny1 = 1.0f / static_cast<float>(ta.pro_d.ny1);
     }
     hipLaunchKernelGGL(k_gmg_tail, dim3(R), dim3(kTailThreads), tail_lds, s, ta);
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
@@ -2397,6 +2528,8 @@ void pk_member_records(const GmgPackMember &mb, std::vector<PkRec> &out) {
     ta.pro_x = cur[t0 - 1];  // t0 >= 1: every packable hierarchy has at least two levels
     ta.pro_inv = g->lv[t0 - 1].inv;
     ta.pro_d = g->lv[t0 - 1].d;
+    ta.pro_rnx1 = 1.0f / static_cast<float>(ta.pro_d.nx1);
+    ta.pro_rny1 = 1.0f / static_cast<float>(ta.pro_d.ny1);
     out.push_back(pk_rec(kKeyTail, kPkTail, 0, 1, ta, tail_lds));
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);
     for (int l = t0 - 1; l >= 0; l--) {
