@@ -1006,11 +1006,11 @@ __device__ __forceinline__ void tail_run(const TailArgs *__restrict__ targ, int 
     auto cur = [&](int l) { return sm + (((flip >> l) & 1u) ? t.lv[l].o_x2 : t.lv[l].o_x); };
     auto nxt = [&](int l) { return sm + (((flip >> l) & 1u) ? t.lv[l].o_x : t.lv[l].o_x2); };
     const int last = t.count - 1;
-    // One Chebyshev step and the residual of a tail level, one node gather per node. (Forming each element's
-    // product once, in an LDS scratch with a second barrier per phase, measured slower: 122.5 vs 126.8 solves/s.)
-    // Every tail level has at most kTailThreads / 2 nodes (tail_nodes()), so each node gets two lanes (lanes w*64 + l and w*64 + l + 32,
-    // the elements below and above the node: ku_gather_half), which halves each lane's dependent FMA chain;
-    // the phases are bound by that chain, not by the CU's FMA rate. Half 0 finishes the node.
+    // One Chebyshev step and the residual of a tail level. On a level of <= kTailThreads / 8 nodes (and on every
+    // level with MYCEL_GMG_TAIL_EC=0) each node gets two lanes (lanes w*64 + l and w*64 + l + 32, the elements below
+    // and above the node: ku_gather_half), which halves each lane's dependent FMA chain: with a few waves the
+    // phase is bound by that chain. Half 0 finishes the node. Larger levels (L.ec) form each element's product
+    // once instead (ec_kx below): with 12+ waves on the CU their phases are bound by VALU issue.
     auto split_kx = [&](const TailLevel &L, int l, int &n, float (&kx)[3], float (&xc)[3]) {
         const int lane = threadIdx.x & 63, half = lane >> 5;
         n = (threadIdx.x >> 6) * 32 + (lane & 31);
