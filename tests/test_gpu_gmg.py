@@ -5,6 +5,10 @@ FP64 residual, checked with the oracle's operator, is at the requested tolerance
 the oracle solved to 1e-12 within 1e-8 (relative) and whose displacements match within 1e-7. The FP32
 V-cycle only changes how many iterations that takes.
 """
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -119,6 +123,32 @@ def test_gmg_shallow_hierarchies(gpu, shape):
     r = _check_against_oracle(p, max_gmg_iters=60)
     if shape == (3, 2, 2):
         assert r.Iterations <= 2  # the preconditioner is the exact inverse up to FP32 rounding
+
+
+def _tail_form_result(nx, ny, nz):
+    p = W.cantilever_simp(nx, ny, nz)
+    p.density = random_field(41, p.n_elem, 0.01, 1.0)
+    r = _check_against_oracle(p, max_gmg_iters=60)
+    return r.Iterations
+
+
+def test_gmg_tail_gathers_switch(gpu, tmp_path):
+    """MYCEL_GMG_TAIL_EC=0 (every tail level on the two-lanes-per-node gathers, read once per process) solves as
+    the default, whose 200-node tail level (28x14x14 grid: 7x4x4 elements) forms its products element by
+    element: both meet the oracle bars, in iteration counts within 2 of each other (the two forms round
+    differently)."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    out = tmp_path / "it.txt"
+    code = ("import sys; sys.path[:0] = [%r, %r]; import test_gpu_gmg as T; "
+            "open(%r, 'w').write(str(T._tail_form_result(28, 14, 14)))" % (root, here, str(out)))
+    env = dict(os.environ, MYCEL_GMG_TAIL_EC="0")
+    print("[gmg] child process with MYCEL_GMG_TAIL_EC=0", flush=True)
+    subprocess.run([sys.executable, "-c", code], env=env, cwd=root, check=True, timeout=240)
+    it_gather = int(out.read_text())
+    it_ec = _tail_form_result(28, 14, 14)
+    print(f"[gmg] tail forms: element products {it_ec} it, gathers {it_gather} it", flush=True)
+    assert abs(it_ec - it_gather) <= 2
 
 
 def test_gmg_refresh_after_density_update_and_warm_start(gpu):
