@@ -115,6 +115,10 @@ __device__ __forceinline__ void bapply(const float (&bi)[kBinv], const float (&r
     z[2] = (bi[4] * r[0] + bi[5] * r[1]) + bi[2] * r[2];
 }
 
+// The Chebyshev direction update d' = cd d + cr z (two products and a sum, never contracted): the launched
+// coarse levels' one-step and two-step kernels form it alike, so both give the same bits.
+__device__ __forceinline__ float cheb_dn(float cd, float d, float cr, float z) { return cd * d + cr * z; }
+
 // The first Chebyshev step of a level at node n from its right-hand side sv (0 on constrained components, whose
 // inv is 0): b = sv, x = d = B^-1 b / theta.
 __device__ __forceinline__ void first_step(const float *__restrict__ inv, const float *__restrict__ binv,
@@ -268,27 +272,20 @@ __device__ __forceinline__ void binv_node(const Dims &g, const float *__restrict
 // x_f += P x_c at fine node n, unconstrained DOFs only (trilinear: weight 1 at an even fine index, 1/2 from
 // each neighbour at an odd one). All 8 candidate coarse nodes are read (indices always valid); those that
 // do not contribute have weight 0 and add nothing.
-template <typename IJK = NodeIjk>
-__device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__ xf, const float *__restrict__ invf,
-                                             const Dims &gc, const float *__restrict__ xcv, int n, IJK ijk = {}) {
-    int i, j, k;
-    ijk(gf, n, i, j, k);
+// (P x_c) at fine node (i, j, k), into sv.
+__device__ __forceinline__ void prolong_sums(const Dims &gc, const float *__restrict__ xcv, int i, int j, int k,
+                                             float (&sv)[3]) {
     const int is[2] = {i >> 1, (i + 1) >> 1}, js[2] = {j >> 1, (j + 1) >> 1}, ks_[2] = {k >> 1, (k + 1) >> 1};
     const float wx[2] = {(i & 1) ? 0.5f : 1.0f, (i & 1) ? 0.5f : 0.0f};
     const float wy[2] = {(j & 1) ? 0.5f : 1.0f, (j & 1) ? 0.5f : 0.0f};
     const float wz[2] = {(k & 1) ? 0.5f : 1.0f, (k & 1) ? 0.5f : 0.0f};
-    float v[8][3], iv[3], xo[3];  // every load first, then the sums: one memory round trip
+    float v[8][3];
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         const int m = 3 * (is[t & 1] + gc.nx1 * (js[(t >> 1) & 1] + gc.ny1 * ks_[t >> 2]));
         v[t][0] = xcv[m];
         v[t][1] = xcv[m + 1];
         v[t][2] = xcv[m + 2];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        iv[c] = invf[3 * n + c];
-        xo[c] = xf[3 * n + c];
     }
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
@@ -298,7 +295,23 @@ __device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__
         s1 += w != 0.0f ? w * v[t][1] : 0.0f;
         s2 += w != 0.0f ? w * v[t][2] : 0.0f;
     }
-    const float sv[3] = {s0, s1, s2};
+    sv[0] = s0;
+    sv[1] = s1;
+    sv[2] = s2;
+}
+
+template <typename IJK = NodeIjk>
+__device__ __forceinline__ void prolong_node(const Dims &gf, float *__restrict__ xf, const float *__restrict__ invf,
+                                             const Dims &gc, const float *__restrict__ xcv, int n, IJK ijk = {}) {
+    int i, j, k;
+    ijk(gf, n, i, j, k);
+    float iv[3], xo[3], sv[3];  // every load first, then the sums: one memory round trip
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        iv[c] = invf[3 * n + c];
+        xo[c] = xf[3 * n + c];
+    }
+    prolong_sums(gc, xcv, i, j, k, sv);
 #pragma unroll
     for (int c = 0; c < 3; c++)
         if (iv[c] > 0.0f) xf[3 * n + c] = xo[c] + sv[c];
@@ -674,7 +687,7 @@ __device__ __forceinline__ void cheb_s_body(const Dims &g, const KeSym<float> &k
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const int dof = 3 * n + c;
-            const float dn = cd * d[dof] + cr * z[c];
+            const float dn = cheb_dn(cd, d[dof], cr, z[c]);
             x_out[dof] = xc[c] + dn;
             d[dof] = dn;
         }
@@ -687,6 +700,193 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, con
                                                     float cr, float *__restrict__ x_out) {
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
     cheb_s_body(g, ks, ep, x + o, b + o, d + o, binv, cd, cr, x_out + o, blockIdx.x, gridDim.x);
+}
+
+// ---- Two Chebyshev steps of a launched coarse level in one launch (the post-smoothing pair) ----
+// A 512-thread block owns a kC2T^3 tile of nodes. It stages the iterate on the tile and two node layers around
+// it in LDS (the prolongation from the level below, when that level is launched too, applied on the staged
+// copy), forms the first step on the tile and one layer around it (the neighbours the second step reads; the
+// halo's nodes are formed again by the neighbouring tiles) and the second step on the tile. Every global load
+// (the staged iterate, and b, B^-1, d and the moduli of both steps' nodes) is issued before the first barrier.
+// Each node's prolongation sums are k_gmg_prolong's, its half sums ku_gather_half's (from LDS) and its updates
+// cheb_s_body's: the same bits as k_gmg_prolong and two k_gmg_cheb_s launches. The level's d is not written:
+// after the post-smoothing it is dead until the next cycle's restriction sets it again.
+constexpr int kC2T = 4, kC2A = kC2T + 2, kC2X = kC2T + 4;  // tile, first-step frame and staged frame edges
+constexpr int kC2Threads = 512;
+static_assert(kC2X * kC2X * kC2X <= kC2Threads, "one staged node per thread");
+static_assert(kC2A * kC2A * kC2A <= kC2Threads / 2, "one first-step node per lane pair");
+
+// The moduli of the four elements of half `half` at node (i, j, k), as ku_gather_half reads them.
+__device__ __forceinline__ void half_moduli(const Dims &g, const float *__restrict__ ep, int i, int j, int k,
+                                            int half, float (&E)[4]) {
+    const int e0 = (i + 1) + g.px * ((j + 1) + g.py * (k + half));
+#pragma unroll
+    for (int q = 0; q < 4; q++) E[q] = ep[e0 - (corner_x(q) ? 1 : 0) - (corner_y(q) ? g.px : 0)];
+}
+
+// ku_gather_half at global node (i, j, k) (its neighbours clamped to the grid as there) with x from an LDS frame
+// of edge S in which the node sits at (li, lj, lk).
+template <int S>
+__device__ __forceinline__ void gather_half_lds(const Dims &g, const KeSym<float> &ks, const float *xl,
+                                                const float (&E)[4], int i, int j, int k, int li, int lj, int lk,
+                                                int half, float (&kx)[3], float (&xc)[3]) {
+    const int xs[3] = {i > 0 ? li - 1 : li, li, i < g.nx ? li + 1 : li};
+    const int ys[3] = {j > 0 ? lj - 1 : lj, lj, j < g.ny ? lj + 1 : lj};
+    const int kl = half ? (k < g.nz ? lk + 1 : lk) : (k > 0 ? lk - 1 : lk);  // the (mirrored) lower layer
+    float lo[9][3], up[9][3];
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            const int ml = 3 * (xs[dx] + S * (ys[dy] + S * kl)), mu = 3 * (xs[dx] + S * (ys[dy] + S * lk));
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                lo[dx + 3 * dy][c] = xl[ml + c];
+                up[dx + 3 * dy][c] = xl[mu + c];
+            }
+        }
+#pragma unroll
+    for (int c = 0; c < 3; c++) xc[c] = up[4][c];
+    const float sgn = half ? -1.0f : 1.0f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        lo[q][2] *= sgn;
+        up[q][2] *= sgn;
+    }
+    float acc0 = 0, acc1 = 0, acc2 = 0;
+    corner_rows_slab<4>(ks, lo, up, E[0], acc0, acc1, acc2);
+    corner_rows_slab<5>(ks, lo, up, E[1], acc0, acc1, acc2);
+    corner_rows_slab<6>(ks, lo, up, E[2], acc0, acc1, acc2);
+    corner_rows_slab<7>(ks, lo, up, E[3], acc0, acc1, acc2);
+    kx[0] = acc0;
+    kx[1] = acc1;
+    kx[2] = acc2 * sgn;
+}
+
+__device__ __forceinline__ bool node_inside(const Dims &g, int i, int j, int k) {
+    return i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+}
+
+// Steps (cd0, cr0) and (cd1, cr1) from x into x_out; grid (tiles, right-hand sides), tx, ty: tiles along x, y.
+// PRO: the staged iterate is x + P xcv on the unconstrained DOFs (inv > 0), the coarse correction of level gc
+// prolonged as k_gmg_prolong forms it (one launch fewer; x itself is left as it was).
+template <bool PRO>
+__global__ __launch_bounds__(kC2Threads) void k_gmg_cheb2_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+                                                            const float *__restrict__ x, const float *__restrict__ b,
+                                                            const float *__restrict__ d,
+                                                            const float *__restrict__ binv, float cd0, float cr0,
+                                                            float cd1, float cr1, float *__restrict__ x_out, int tx,
+                                                            int ty, const float *__restrict__ inv, Dims gc,
+                                                            const float *__restrict__ xcv) {
+    __shared__ float xs[3 * kC2X * kC2X * kC2X];
+    __shared__ float x1[3 * kC2A * kC2A * kC2A], d1[3 * kC2A * kC2A * kC2A];
+    const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
+    x += o;
+    b += o;
+    d += o;
+    x_out += o;
+    if (PRO) xcv += static_cast<long long>(blockIdx.y) * gc.n_dofs;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = kC2T * (tile % tx), j0 = kC2T * ((tile / tx) % ty), k0 = kC2T * (tile / (tx * ty));
+    const int t = threadIdx.x, lane = t & 63, half = lane >> 5, slot = (t >> 6) * 32 + (lane & 31);
+
+    // Staged frame: node (i0 - 2 + sx, j0 - 2 + sy, k0 - 2 + sz) at sx + kC2X (sy + kC2X sz).
+    const int sx = t % kC2X, sy = (t / kC2X) % kC2X, sz = t / (kC2X * kC2X);
+    const bool vs = t < kC2X * kC2X * kC2X && node_inside(g, i0 - 2 + sx, j0 - 2 + sy, k0 - 2 + sz);
+    float xv[3] = {0.0f, 0.0f, 0.0f};
+    if (vs) {
+        const int m = 3 * ((i0 - 2 + sx) + g.nx1 * ((j0 - 2 + sy) + g.ny1 * (k0 - 2 + sz)));
+#pragma unroll
+        for (int c = 0; c < 3; c++) xv[c] = x[m + c];
+        if (PRO) {
+            float iv[3], sv[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) iv[c] = inv[m + c];
+            prolong_sums(gc, xcv, i0 - 2 + sx, j0 - 2 + sy, k0 - 2 + sz, sv);
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                if (iv[c] > 0.0f) xv[c] = xv[c] + sv[c];
+        }
+    }
+    // First step: lane pair `slot` of the frame around the tile, node (i0 - 1 + ax, ...).
+    const int ax = slot % kC2A, ay = (slot / kC2A) % kC2A, az = slot / (kC2A * kC2A);
+    const int ai = i0 - 1 + ax, aj = j0 - 1 + ay, ak = k0 - 1 + az;
+    const bool va = slot < kC2A * kC2A * kC2A && node_inside(g, ai, aj, ak);
+    const int an = ai + g.nx1 * (aj + g.ny1 * ak);
+    float ea[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ba[3] = {0.0f, 0.0f, 0.0f}, da[3] = {0.0f, 0.0f, 0.0f}, bia[kBinv] = {};
+    if (va) {
+        half_moduli(g, ep, ai, aj, ak, half, ea);
+        if (half == 0) {
+            bload(binv, an, bia);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ba[c] = b[3 * an + c];
+                da[c] = d[3 * an + c];
+            }
+        }
+    }
+    // Second step: lane pair `slot` of the tile, node (i0 + bx_, ...).
+    const int bx_ = slot % kC2T, by_ = (slot / kC2T) % kC2T, bz_ = slot / (kC2T * kC2T);
+    const int bi_ = i0 + bx_, bj_ = j0 + by_, bk_ = k0 + bz_;
+    const bool vb = slot < kC2T * kC2T * kC2T && node_inside(g, bi_, bj_, bk_);
+    const int bn = bi_ + g.nx1 * (bj_ + g.ny1 * bk_);
+    float eb[4] = {0.0f, 0.0f, 0.0f, 0.0f}, bb[3] = {0.0f, 0.0f, 0.0f}, bib[kBinv] = {};
+    if (vb) {
+        half_moduli(g, ep, bi_, bj_, bk_, half, eb);
+        if (half == 0) {
+            bload(binv, bn, bib);
+#pragma unroll
+            for (int c = 0; c < 3; c++) bb[c] = b[3 * bn + c];
+        }
+    }
+    if (t < kC2X * kC2X * kC2X) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) xs[3 * t + c] = xv[c];
+    }
+    __syncthreads();
+
+    float kx[3] = {0.0f, 0.0f, 0.0f}, xc[3] = {0.0f, 0.0f, 0.0f};
+    if (va) gather_half_lds<kC2X>(g, ks, xs, ea, ai, aj, ak, ax + 1, ay + 1, az + 1, half, kx, xc);
+#pragma unroll
+    for (int c = 0; c < 3; c++) kx[c] += __shfl_down(kx[c], 32, 64);  // every lane takes part
+    if (va && half == 0) {
+        float r[3], z[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[c] = ba[c] - kx[c];
+        bapply(bia, r, z);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float dn = cheb_dn(cd0, da[c], cr0, z[c]);
+            x1[3 * slot + c] = xc[c] + dn;
+            d1[3 * slot + c] = dn;
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int c = 0; c < 3; c++) kx[c] = xc[c] = 0.0f;
+    const int as = (bx_ + 1) + kC2A * ((by_ + 1) + kC2A * (bz_ + 1));  // the tile node's first-step slot
+    if (vb) gather_half_lds<kC2A>(g, ks, x1, eb, bi_, bj_, bk_, bx_ + 1, by_ + 1, bz_ + 1, half, kx, xc);
+#pragma unroll
+    for (int c = 0; c < 3; c++) kx[c] += __shfl_down(kx[c], 32, 64);
+    if (vb && half == 0) {
+        float r[3], z[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[c] = bb[c] - kx[c];
+        bapply(bib, r, z);
+#pragma unroll
+        for (int c = 0; c < 3; c++) x_out[3 * bn + c] = xc[c] + cheb_dn(cd1, d1[3 * as + c], cr1, z[c]);
+    }
+}
+
+// Whether the post-smoothing pair of a launched coarse level (and the prolongation before it) runs as one
+// k_gmg_cheb2_s (MYCEL_GMG_CHEB2=0: k_gmg_prolong and two k_gmg_cheb_s launches, the same bits).
+bool gmg_cheb2() {
+    static const bool v = [] {
+        const char *e = std::getenv("MYCEL_GMG_CHEB2");
+        return !(e && *e == '0');
+    }();
+    return v;
 }
 
 __global__ __launch_bounds__(256) void k_gmg_residual_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
@@ -2367,7 +2567,23 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
-        if (l != t0 - 1)  // level t0 - 1 was prolonged into by the tail
+        // Level t0 - 1 was prolonged into by the tail; a launched coarse level takes its prolongation and its
+        // two post-smoothing steps in one launch.
+        if (l > 0 && split && gmg_cheb2()) {
+            const int tx = cdiv(L.d.nx1, kC2T), ty = cdiv(L.d.ny1, kC2T), tz = cdiv(L.d.nz1, kC2T);
+            const dim3 grid(tx * ty * tz, R);
+            if (l != t0 - 1)
+                hipLaunchKernelGGL(k_gmg_cheb2_s<true>, grid, dim3(kC2Threads), 0, s, L.d, L.ks, L.ep, cur[l],
+                                   L.b.get(), L.dv.get(), L.binv, 0.0f, g->theta_inv, g->cd[1], g->cr[1], nxt[l],
+                                   tx, ty, L.inv, C.d, cur[l + 1]);
+            else
+                hipLaunchKernelGGL(k_gmg_cheb2_s<false>, grid, dim3(kC2Threads), 0, s, L.d, L.ks, L.ep, cur[l],
+                                   L.b.get(), L.dv.get(), L.binv, 0.0f, g->theta_inv, g->cd[1], g->cr[1], nxt[l],
+                                   tx, ty, L.inv, C.d, cur[l + 1]);
+            std::swap(cur[l], nxt[l]);
+            continue;
+        }
+        if (l != t0 - 1)
             hipLaunchKernelGGL(k_gmg_prolong, dim3(L.blocks, R), b256, 0, s, L.d, cur[l], L.inv, C.d,
                                cur[l + 1]);
         cheb(l, 0.0f, g->theta_inv);

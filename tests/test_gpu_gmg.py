@@ -151,6 +151,40 @@ def test_gmg_tail_gathers_switch(gpu, tmp_path):
     assert abs(it_ec - it_gather) <= 2
 
 
+def _cheb2_solution(nx, ny, nz, path):
+    p = W.cantilever_simp(nx, ny, nz)
+    p.density = random_field(43, p.n_elem, 0.01, 1.0)
+    r = _check_against_oracle(p, max_gmg_iters=60)
+    op = FemOperator(p.fe_model())
+    try:
+        b = np.where(oracle_model(p).constrained, 0.0, oracle_forces(p))
+        u = np.zeros(p.n_dofs)
+        PcgSolver().Solve(op, b, u, MultigridPreconditioner(), tol=1e-8, maxIterations=1000)
+    finally:
+        op.close()
+    np.save(path, u)
+    return r.Iterations
+
+
+def test_gmg_post_smoothing_pair_same_bits(gpu, tmp_path):
+    """The launched coarse levels' two post-smoothing steps (with level 1's prolongation from level 2) run as one
+    LDS-tiled launch by default and as separate launches with MYCEL_GMG_CHEB2=0 (read once per process): the same
+    solution bits. 48x24x24: levels 1 (25x13x13 nodes, prolonged from the launched level 2 in the fused launch)
+    and 2 (13x7x7, prolonged into by the tail) are launched, odd tile remainders on every axis."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    ref = tmp_path / "two_launches.npy"
+    code = ("import sys; sys.path[:0] = [%r, %r]; import test_gpu_gmg as T; T._cheb2_solution(48, 24, 24, %r)"
+            % (root, here, str(ref)))
+    env = dict(os.environ, MYCEL_GMG_CHEB2="0")
+    print("[gmg] child process with MYCEL_GMG_CHEB2=0", flush=True)
+    subprocess.run([sys.executable, "-c", code], env=env, cwd=root, check=True, timeout=240)
+    _cheb2_solution(48, 24, 24, tmp_path / "one_launch.npy")
+    u1, u2 = np.load(tmp_path / "one_launch.npy"), np.load(ref)
+    print(f"[gmg] post-smoothing pair: max |diff| {np.abs(u1 - u2).max():.3e}", flush=True)
+    assert np.array_equal(u1, u2)
+
+
 def test_gmg_refresh_after_density_update_and_warm_start(gpu):
     """The hierarchy follows UpdateDensities (the same answer as a fresh operator), and a warm start from the
     converged solution stops at once."""
