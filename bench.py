@@ -268,12 +268,16 @@ def mg_iteration_model(nx, ny, nz, active_nodes, active_elements, tail_nodes=512
       k_gmg_cheb_ec (x4)     read x, b, d, B^-1; write x', d; the moduli: 28 D0 + 4 E0
       k_gmg_residual_ec      read x, b, D^-1; write r: 16 D0 + 4 E0
       k_gmg_restrict_dof     read r; read the coarse D^-1 and B^-1 and write its b, x, d: 4 D0 + 24 D1
-      k_gmg_cheb_s (x3/level) levels 1 .. t0-1, as the Chebyshev step above: 28 D + 4 E
+      k_gmg_cheb_s           levels 1 .. t0-1, the second pre-smoothing step, as the step above: 28 D + 4 E
+      k_gmg_cheb2_s          levels 1 .. t0-1, both post-smoothing steps: read x, b, d, B^-1, write x'; the
+                             moduli: 24 D + 4 E; below level t0-1 also the prolongation's D^-1 and coarse x:
+                             + 4 D_l + 4 D_l+1
       k_gmg_residual_restrict read x, b, D^-1 of level l and write level l+1's entry: 12 D_l + 4 E_l + 24 D_l+1
       k_gmg_tail             stage level t0's b, x, d and every tail level's moduli, D^-1 and B^-1, the coarsest
                              inverse (4 n^2 B), prolong into level t0-1 (read D^-1, read and write x): 12 D_t0 +
                              sum(12 D + 4 E) + 4 n^2 + 12 D_t0-1
-      k_gmg_prolong          levels t0-2 .. 0: read the coarse x, read D^-1 and x, write x: 12 D_l + 4 D_l+1
+      k_gmg_prolong          the finest level (when t0 > 1): read the coarse x, read D^-1 and x, write x:
+                             12 D_0 + 4 D_1
       k_gmg_cheb_final_ec    read x, b, d, B^-1 (FP32), r and Ap (FP64); write z (FP64); moduli: 44 D0 + 4 E0
       k_pcg_step4_ap<double, true  read z, p; write p', Ap' (FP64); the FP64 moduli: 32 D0 + 8 E0
     Levels below t0 (the first level from which every level has <= 512 nodes) run inside the tail."""
@@ -291,11 +295,13 @@ def mg_iteration_model(nx, ny, nz, active_nodes, active_elements, tail_nodes=512
             ("k_gmg_residual_ec<", 1, 16 * D[0] + 4 * E[0]),
             ("k_gmg_restrict_dof", 1, 4 * D[0] + 24 * D[1])]
     mid = range(1, t0)
-    rows.append(("k_gmg_cheb_s", 3 * len(mid), sum(3 * (28 * D[l] + 4 * E[l]) for l in mid)))
+    rows.append(("k_gmg_cheb_s", len(mid), sum(28 * D[l] + 4 * E[l] for l in mid)))
+    rows.append(("k_gmg_cheb2_s", len(mid), sum(24 * D[l] + 4 * E[l] + (4 * D[l] + 4 * D[l + 1] if l < t0 - 1 else 0)
+                                                for l in mid)))
     rows.append(("k_gmg_residual_restrict", len(mid), sum(12 * D[l] + 4 * E[l] + 24 * D[l + 1] for l in mid)))
     rows.append(("k_gmg_tail", 1, 12 * D[t0] + sum(12 * D[l] + 4 * E[l] for l in range(t0, top + 1)) + 4 * nc * nc
                  + 12 * D[t0 - 1]))
-    rows.append(("k_gmg_prolong", t0 - 1, sum(12 * D[l] + 4 * D[l + 1] for l in range(0, t0 - 1))))
+    rows.append(("k_gmg_prolong", 1 if t0 > 1 else 0, 12 * D[0] + 4 * D[1] if t0 > 1 else 0))
     rows.append(("k_gmg_cheb_final_ec<", 1, 44 * D[0] + 4 * E[0]))
     rows.append(("k_pcg_step4_ap<double, true", 1, 32 * D[0] + 8 * E[0]))
     return rows

@@ -2567,9 +2567,11 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
     if (ta.count > 1) std::swap(cur[t0], nxt[t0]);  // the tail leaves its first level's iterate in x2
     for (int l = t0 - 1; l >= 0; l--) {
         GmgLevel &L = g->lv[l], &C = g->lv[l + 1];
-        // Level t0 - 1 was prolonged into by the tail; a launched coarse level takes its prolongation and its
-        // two post-smoothing steps in one launch.
-        if (l > 0 && split && gmg_cheb2()) {
+        // Level t0 - 1 was prolonged into by the tail; in a single solve a launched coarse level takes its
+        // prolongation and its two post-smoothing steps in one launch. With several right-hand sides these
+        // launches are no longer bound by their fixed cost, and the tiles' recomputed halo costs more than the
+        // launches save (16 load cases of a 100x50x50 box: 3.97 vs 3.68 ms per case-solve): separate launches.
+        if (l > 0 && split && R == 1 && gmg_cheb2()) {
             const int tx = cdiv(L.d.nx1, kC2T), ty = cdiv(L.d.ny1, kC2T), tz = cdiv(L.d.nz1, kC2T);
             const dim3 grid(tx * ty * tz, R);
             if (l != t0 - 1)
