@@ -703,7 +703,7 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, con
 }
 
 // ---- Two Chebyshev steps of a launched coarse level in one launch (the post-smoothing pair) ----
-// A 512-thread block owns a kC2T^3 tile of nodes. It stages the iterate on the tile and two node layers around
+// A block owns a T^3 tile of nodes (T = 4: 512 threads; T = 2: 256). It stages the iterate on the tile and two node layers around
 // it in LDS (the prolongation from the level below, when that level is launched too, applied on the staged
 // copy), forms the first step on the tile and one layer around it (the neighbours the second step reads; the
 // halo's nodes are formed again by the neighbouring tiles) and the second step on the tile. Every global load
@@ -711,10 +711,14 @@ __global__ __launch_bounds__(256) void k_gmg_cheb_s(Dims g, KeSym<float> ks, con
 // Each node's prolongation sums are k_gmg_prolong's, its half sums ku_gather_half's (from LDS) and its updates
 // cheb_s_body's: the same bits as k_gmg_prolong and two k_gmg_cheb_s launches. The level's d is not written:
 // after the post-smoothing it is dead until the next cycle's restriction sets it again.
-constexpr int kC2T = 4, kC2A = kC2T + 2, kC2X = kC2T + 4;  // tile, first-step frame and staged frame edges
-constexpr int kC2Threads = 512;
-static_assert(kC2X * kC2X * kC2X <= kC2Threads, "one staged node per thread");
-static_assert(kC2A * kC2A * kC2A <= kC2Threads / 2, "one first-step node per lane pair");
+// Tile edge T: first-step frame A = T + 2 and staged frame X = T + 4 node edges; NT threads.
+template <int T>
+struct Cheb2Tile {
+    static constexpr int A = T + 2, X = T + 4;
+    static constexpr int NT = ((X * X * X > 2 * A * A * A ? X * X * X : 2 * A * A * A) + 63) / 64 * 64;
+    static_assert(X * X * X <= NT && A * A * A <= NT / 2 && NT <= 1024, "one staged node per thread, one "
+                  "first-step node per lane pair");
+};
 
 // The moduli of the four elements of half `half` at node (i, j, k), as ku_gather_half reads them.
 __device__ __forceinline__ void half_moduli(const Dims &g, const float *__restrict__ ep, int i, int j, int k,
@@ -770,14 +774,15 @@ __device__ __forceinline__ bool node_inside(const Dims &g, int i, int j, int k) 
 // Steps (cd0, cr0) and (cd1, cr1) from x into x_out; grid (tiles, right-hand sides), tx, ty: tiles along x, y.
 // PRO: the staged iterate is x + P xcv on the unconstrained DOFs (inv > 0), the coarse correction of level gc
 // prolonged as k_gmg_prolong forms it (one launch fewer; x itself is left as it was).
-template <bool PRO>
-__global__ __launch_bounds__(kC2Threads) void k_gmg_cheb2_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
+template <int T, bool PRO>
+__global__ __launch_bounds__(Cheb2Tile<T>::NT) void k_gmg_cheb2_s(Dims g, KeSym<float> ks, const float *__restrict__ ep,
                                                             const float *__restrict__ x, const float *__restrict__ b,
                                                             const float *__restrict__ d,
                                                             const float *__restrict__ binv, float cd0, float cr0,
                                                             float cd1, float cr1, float *__restrict__ x_out, int tx,
                                                             int ty, const float *__restrict__ inv, Dims gc,
                                                             const float *__restrict__ xcv) {
+    constexpr int kC2T = T, kC2A = Cheb2Tile<T>::A, kC2X = Cheb2Tile<T>::X;
     __shared__ float xs[3 * kC2X * kC2X * kC2X];
     __shared__ float x1[3 * kC2A * kC2A * kC2A], d1[3 * kC2A * kC2A * kC2A];
     const long long o = static_cast<long long>(blockIdx.y) * g.n_dofs;
@@ -885,6 +890,17 @@ bool gmg_cheb2() {
     static const bool v = [] {
         const char *e = std::getenv("MYCEL_GMG_CHEB2");
         return !(e && *e == '0');
+    }();
+    return v;
+}
+
+// Levels with fewer nodes than this take 2x2x2 tiles in k_gmg_cheb2_s (more blocks), the others 4x4x4 (fewer
+// recomputed halo nodes) (MYCEL_GMG_CHEB2_SMALL=n). Holed box, paired runs: 2x2x2 tiles on its 2,541-node
+// level 165.7-165.8 vs 165.3-165.5 solves/s; on the 18,081-node level as well 159.7.
+int gmg_cheb2_small() {
+    static const int v = [] {
+        const char *e = std::getenv("MYCEL_GMG_CHEB2_SMALL");
+        return e && *e ? std::atoi(e) : 8192;
     }();
     return v;
 }
@@ -2572,16 +2588,15 @@ void gmg_enqueue_vcycle(mycel_op *op, const double *r64, const double *ap64, dou
         // launches are no longer bound by their fixed cost, and the tiles' recomputed halo costs more than the
         // launches save (16 load cases of a 100x50x50 box: 3.97 vs 3.68 ms per case-solve): separate launches.
         if (l > 0 && split && R == 1 && gmg_cheb2()) {
-            const int tx = cdiv(L.d.nx1, kC2T), ty = cdiv(L.d.ny1, kC2T), tz = cdiv(L.d.nz1, kC2T);
-            const dim3 grid(tx * ty * tz, R);
-            if (l != t0 - 1)
-                hipLaunchKernelGGL(k_gmg_cheb2_s<true>, grid, dim3(kC2Threads), 0, s, L.d, L.ks, L.ep, cur[l],
-                                   L.b.get(), L.dv.get(), L.binv, 0.0f, g->theta_inv, g->cd[1], g->cr[1], nxt[l],
-                                   tx, ty, L.inv, C.d, cur[l + 1]);
-            else
-                hipLaunchKernelGGL(k_gmg_cheb2_s<false>, grid, dim3(kC2Threads), 0, s, L.d, L.ks, L.ep, cur[l],
-                                   L.b.get(), L.dv.get(), L.binv, 0.0f, g->theta_inv, g->cd[1], g->cr[1], nxt[l],
-                                   tx, ty, L.inv, C.d, cur[l + 1]);
+            const bool small = L.d.n_nodes < gmg_cheb2_small();  // 2x2x2 tiles: more blocks on a small level
+            const int T = small ? 2 : 4;
+            const int tx = cdiv(L.d.nx1, T), ty = cdiv(L.d.ny1, T), tz = cdiv(L.d.nz1, T);
+            const dim3 grid(tx * ty * tz, R), blk(small ? Cheb2Tile<2>::NT : Cheb2Tile<4>::NT);
+            const bool pro = l != t0 - 1;
+            auto kern = small ? (pro ? k_gmg_cheb2_s<2, true> : k_gmg_cheb2_s<2, false>)
+                              : (pro ? k_gmg_cheb2_s<4, true> : k_gmg_cheb2_s<4, false>);
+            hipLaunchKernelGGL(kern, grid, blk, 0, s, L.d, L.ks, L.ep, cur[l], L.b.get(), L.dv.get(), L.binv, 0.0f,
+                               g->theta_inv, g->cd[1], g->cr[1], nxt[l], tx, ty, L.inv, C.d, cur[l + 1]);
             std::swap(cur[l], nxt[l]);
             continue;
         }
